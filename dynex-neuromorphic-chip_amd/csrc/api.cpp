@@ -1,0 +1,672 @@
+// api.cpp -- the C ABI of libdnx (include/dnx.h).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "api_internal.hpp"
+
+namespace dnx {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+int hip_fail(hipError_t e, const char *what)
+{
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return DNX_E_HIP;
+}
+
+DevProb dev_prob(const dnx_problem *p)
+{
+    DevProb d;
+    d.n = p->hp.n;
+    d.m = p->hp.m;
+    d.max_deg = p->hp.max_deg;
+    d.cp = p->hp.uniform3 ? nullptr : p->d_cp;
+    d.lits = p->d_lits;
+    d.lpos = p->d_lpos;
+    d.row_ptr = p->d_row_ptr;
+    d.unit = p->hp.has_unit ? p->d_unit : nullptr;
+    return d;
+}
+
+DevParams dev_params(const dnx_params *p)
+{
+    DevParams d;
+    d.alpha = p->alpha;
+    d.beta = p->beta;
+    d.gamma = p->gamma;
+    d.delta = p->delta;
+    d.epsilon = p->epsilon;
+    d.zeta = p->zeta;
+    d.xl_max = (double)p->xl_max;
+    return d;
+}
+
+// Device buffer that frees itself.
+DevBuf::~DevBuf()
+{
+    if (ptr) (void)hipFree(ptr);
+}
+int DevBuf::alloc(size_t bytes)
+{
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    size = bytes;
+    if (bytes == 0) return DNX_OK;
+    hipError_t e = hipMalloc(&ptr, bytes);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        set_error(std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+        return DNX_E_NOMEM;
+    }
+    return DNX_OK;
+}
+
+int upload_problem(dnx_problem *p)
+{
+    const HostProblem &h = p->hp;
+    int rc;
+    if ((rc = p->b_cp.alloc(sizeof(int32_t) * h.cp.size())) ||
+        (rc = p->b_lits.alloc(sizeof(int32_t) * h.lits.size())) ||
+        (rc = p->b_lpos.alloc(sizeof(int32_t) * h.lpos.size())) ||
+        (rc = p->b_row.alloc(sizeof(int32_t) * h.row_ptr.size())) ||
+        (rc = p->b_unit.alloc(h.unit.size())))
+        return rc;
+    hipStream_t st = p->ctx->stream;
+    hipError_t e;
+#define UP(buf, vec)                                                                                  \
+    if (!vec.empty() &&                                                                               \
+        (e = hipMemcpyAsync(buf.ptr, vec.data(), buf.size, hipMemcpyHostToDevice, st)) != hipSuccess) \
+        return hip_fail(e, "upload");
+    UP(p->b_cp, h.cp)
+    UP(p->b_lits, h.lits)
+    UP(p->b_lpos, h.lpos)
+    UP(p->b_row, h.row_ptr)
+    UP(p->b_unit, h.unit)
+#undef UP
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "upload sync");
+    p->d_cp = (int32_t *)p->b_cp.ptr;
+    p->d_lits = (int32_t *)p->b_lits.ptr;
+    p->d_lpos = (int32_t *)p->b_lpos.ptr;
+    p->d_row_ptr = (int32_t *)p->b_row.ptr;
+    p->d_unit = (int8_t *)p->b_unit.ptr;
+    return DNX_OK;
+}
+
+size_t prec_size(int prec) { return prec == DNX_FP64 ? 8 : 4; }
+
+// host double <-> device precision staging
+void pack_host(int prec, const double *src, size_t count, std::vector<char> &out)
+{
+    out.resize(count * prec_size(prec));
+    if (prec == DNX_FP64) {
+        memcpy(out.data(), src, count * 8);
+    } else {
+        float *f = (float *)out.data();
+        for (size_t i = 0; i < count; i++) f[i] = (float)src[i];
+    }
+}
+void unpack_host(int prec, const std::vector<char> &in, size_t count, double *dst)
+{
+    if (prec == DNX_FP64) {
+        memcpy(dst, in.data(), count * 8);
+    } else {
+        const float *f = (const float *)in.data();
+        for (size_t i = 0; i < count; i++) dst[i] = (double)f[i];
+    }
+}
+
+static const int kChunk = 30;  // steps per captured graph (multiple of 3)
+
+static void init_ctl(Ctl &c)
+{
+    memset(&c, 0, sizeof(c));
+    c.status = DNX_RUNNING;
+    c.stop_on_solve = 1;
+    c.steps = 0;
+    c.cap = LLONG_MAX;
+    c.t = 0.0;
+    for (int i = 0; i < 3; i++) {
+        c.loc[i] = 0;
+        c.vbad[i] = 0;
+        c.level[i] = -1;
+        c.done[i] = 0;
+    }
+}
+
+static int state_enqueue_steps(dnx_state *s, long long count)
+{
+    hipStream_t st = s->p->ctx->stream;
+    DevProb P = dev_prob(s->p);
+    // align to the graph's slot phase, then replay whole chunks
+    while (count > 0 && (s->enq % 3) != 0) {
+        launch_step(s->prec, P, s->prm, s->b_vd.ptr, s->b_xs.ptr, s->b_xl.ptr, s->b_e.ptr, s->ctl,
+                    (int)(s->enq % 3), st);
+        s->enq++;
+        count--;
+    }
+    while (count >= kChunk) {
+        if (!s->graph) {
+            hipGraph_t g;
+            hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+            for (int i = 0; i < kChunk; i++)
+                launch_step(s->prec, P, s->prm, s->b_vd.ptr, s->b_xs.ptr, s->b_xl.ptr, s->b_e.ptr, s->ctl, i % 3,
+                            st);
+            if ((e = hipStreamEndCapture(st, &g)) != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+            e = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                s->graph = nullptr;
+                return hip_fail(e, "hipGraphInstantiate");
+            }
+        }
+        hipError_t e = hipGraphLaunch(s->graph, st);
+        if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+        s->enq += kChunk;
+        count -= kChunk;
+    }
+    while (count > 0) {
+        launch_step(s->prec, P, s->prm, s->b_vd.ptr, s->b_xs.ptr, s->b_xl.ptr, s->b_e.ptr, s->ctl,
+                    (int)(s->enq % 3), st);
+        s->enq++;
+        count--;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "step launch");
+    return DNX_OK;
+}
+
+// status of a control-block snapshot: the stored status, or the `done`
+// verdict of the last evaluated state when no later pass has run
+static int ctl_status(const Ctl &c)
+{
+    if (c.status != DNX_RUNNING) return c.status;
+    return c.done[c.steps % 3];
+}
+
+}  // namespace dnx
+
+using namespace dnx;
+
+static int host_only(const dnx_problem *)
+{
+    set_error("problem was created without a context and has no device copy");
+    return DNX_E_STATE;
+}
+
+extern "C" {
+
+const char *dnx_last_error(void) { return g_err.c_str(); }
+
+int dnx_params_default(int m, int xl_max_flag, dnx_params *p)
+{
+    if (!p || m < 0) return DNX_E_ARG;
+    p->alpha = 2.5;                 // dynex.cc:132
+    p->beta = 20.0;                 // dynex.cc:134
+    p->gamma = 25.0 / 100.0;        // dynex.cc:135
+    p->delta = 5.0 / 100.0;         // dynex.cc:136
+    p->epsilon = 1.0 / 10.0;        // dynex.cc:137
+    p->zeta = 1.0 / 10.0;           // dynex.cc:138
+    // dynex.cc:2358-2359: int product (wrapping), INT_MAX if not positive
+    int32_t w = (int32_t)((uint32_t)xl_max_flag * (uint32_t)m);
+    if (w <= 0) w = INT_MAX;
+    p->xl_max = w;
+    return DNX_OK;
+}
+
+int dnx_open(int device, dnx_ctx **out)
+{
+    if (!out) return DNX_E_ARG;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
+    if (device < 0 || device >= count) {
+        set_error("device " + std::to_string(device) + " not present (" + std::to_string(count) + " visible)");
+        return DNX_E_ARG;
+    }
+    if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+    dnx_ctx *c = new dnx_ctx();
+    c->device = device;
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
+        delete c;
+        return hip_fail(e, "hipStreamCreate");
+    }
+    *out = c;
+    return DNX_OK;
+}
+
+int dnx_close(dnx_ctx *ctx)
+{
+    if (!ctx) return DNX_E_ARG;
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return DNX_OK;
+}
+
+static int finish_problem(dnx_ctx *ctx, dnx_problem *p, dnx_problem **out)
+{
+    if (!ctx) {  // host-only problem: parsing, inspection, checks
+        *out = p;
+        return DNX_OK;
+    }
+    (void)hipSetDevice(ctx->device);
+    int rc = upload_problem(p);
+    if (rc != DNX_OK) {
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return DNX_OK;
+}
+
+int dnx_problem_from_dimacs(dnx_ctx *ctx, const char *path, dnx_problem **out)
+{
+    if (!path || !out) return DNX_E_ARG;
+    dnx_problem *p = new dnx_problem();
+    p->ctx = ctx;
+    std::string err;
+    int rc = parse_dimacs(path, p->hp, err);
+    if (rc != DNX_OK) {
+        set_error(err);
+        delete p;
+        return rc;
+    }
+    return finish_problem(ctx, p, out);
+}
+
+int dnx_problem_from_lits(dnx_ctx *ctx, int n, int m, const int32_t *clause_ptr, const int32_t *lits,
+                          dnx_problem **out)
+{
+    if (!out || n < 0 || m < 0 || (m > 0 && !clause_ptr)) return DNX_E_ARG;
+    dnx_problem *p = new dnx_problem();
+    p->ctx = ctx;
+    p->hp.n = n;
+    p->hp.m = m;
+    if (clause_ptr) {
+        p->hp.cp.assign(clause_ptr, clause_ptr + m + 1);
+    } else {
+        p->hp.cp.assign(1, 0);
+    }
+    int64_t nnz = p->hp.cp.back();
+    if (nnz < 0 || (nnz > 0 && !lits)) {
+        delete p;
+        return DNX_E_ARG;
+    }
+    for (int c = 0; c < m; c++)
+        if (p->hp.cp[(size_t)c + 1] < p->hp.cp[(size_t)c]) {
+            delete p;
+            set_error("clause_ptr is not non-decreasing");
+            return DNX_E_ARG;
+        }
+    if (nnz > 0) p->hp.lits.assign(lits, lits + nnz);
+    int rc = finalize_problem(p->hp);
+    if (rc != DNX_OK) {
+        set_error(rc == DNX_E_TOOLONG ? "clause longer than DNX_MAX_LITS" : "literal 0 or out of range");
+        delete p;
+        return rc;
+    }
+    return finish_problem(ctx, p, out);
+}
+
+int dnx_problem_info(const dnx_problem *p, int *n, int *m, int64_t *nnz)
+{
+    if (!p) return DNX_E_ARG;
+    if (n) *n = p->hp.n;
+    if (m) *m = p->hp.m;
+    if (nnz) *nnz = p->hp.nnz();
+    return DNX_OK;
+}
+
+int dnx_problem_clauses(const dnx_problem *p, int32_t *clause_ptr, int32_t *lits)
+{
+    if (!p) return DNX_E_ARG;
+    if (clause_ptr) memcpy(clause_ptr, p->hp.cp.data(), sizeof(int32_t) * p->hp.cp.size());
+    if (lits && !p->hp.lits.empty()) memcpy(lits, p->hp.lits.data(), sizeof(int32_t) * p->hp.lits.size());
+    return DNX_OK;
+}
+
+int dnx_problem_free(dnx_problem *p)
+{
+    if (!p) return DNX_E_ARG;
+    if (p->ctx) {
+        (void)hipSetDevice(p->ctx->device);
+        (void)hipStreamSynchronize(p->ctx->stream);
+    }
+    delete p;
+    return DNX_OK;
+}
+
+int dnx_rhs(dnx_problem *p, int prec, const dnx_params *prm, const double *x, double *dxdt, int *loc,
+            double *energy)
+{
+    if (!p || !prm || !x || !dxdt || (prec != DNX_FP64 && prec != DNX_FP32)) return DNX_E_ARG;
+    if (!p->ctx) return host_only(p);
+    (void)hipSetDevice(p->ctx->device);
+    const HostProblem &h = p->hp;
+    const size_t N = (size_t)h.n + 2 * (size_t)h.m, es = prec_size(prec);
+    const int nb = rhs_clause_blocks(h.m);
+    DevBuf bx, bd, be, bloc, bep;
+    int rc;
+    if ((rc = bx.alloc(N * es)) || (rc = bd.alloc(N * es)) || (rc = be.alloc((size_t)h.nnz() * es + es)) ||
+        (rc = bloc.alloc(sizeof(int))) || (rc = bep.alloc(sizeof(double) * (size_t)nb)))
+        return rc;
+    hipStream_t st = p->ctx->stream;
+    std::vector<char> stage;
+    pack_host(prec, x, N, stage);
+    hipError_t e;
+    if (N && (e = hipMemcpyAsync(bx.ptr, stage.data(), N * es, hipMemcpyHostToDevice, st)) != hipSuccess)
+        return hip_fail(e, "rhs upload");
+    if ((e = hipMemsetAsync(bloc.ptr, 0, sizeof(int), st)) != hipSuccess) return hip_fail(e, "memset");
+    if ((e = hipMemsetAsync(bep.ptr, 0, sizeof(double) * (size_t)nb, st)) != hipSuccess)
+        return hip_fail(e, "memset");
+    launch_rhs(prec, dev_prob(p), dev_params(prm), bx.ptr, bd.ptr, be.ptr, (int *)bloc.ptr, (double *)bep.ptr, nb,
+               st);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "rhs launch");
+    std::vector<double> ep((size_t)nb);
+    int hloc = 0;
+    if (N && (e = hipMemcpyAsync(stage.data(), bd.ptr, N * es, hipMemcpyDeviceToHost, st)) != hipSuccess)
+        return hip_fail(e, "rhs download");
+    if ((e = hipMemcpyAsync(&hloc, bloc.ptr, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess ||
+        (e = hipMemcpyAsync(ep.data(), bep.ptr, sizeof(double) * (size_t)nb, hipMemcpyDeviceToHost, st)) !=
+            hipSuccess)
+        return hip_fail(e, "rhs download");
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "rhs sync");
+    unpack_host(prec, stage, N, dxdt);
+    if (loc) *loc = hloc;
+    if (energy) {
+        double s = 0.0;
+        for (double v : ep) s += v;
+        *energy = s;
+    }
+    return DNX_OK;
+}
+
+int dnx_euler(dnx_problem *p, int prec, const dnx_params *prm, const double *x, const double *dxdt, double h,
+              double *x_out)
+{
+    if (!p || !prm || !x || !dxdt || !x_out || (prec != DNX_FP64 && prec != DNX_FP32)) return DNX_E_ARG;
+    if (!p->ctx) return host_only(p);
+    (void)hipSetDevice(p->ctx->device);
+    const size_t N = (size_t)p->hp.n + 2 * (size_t)p->hp.m, es = prec_size(prec);
+    if (N == 0) return DNX_OK;
+    DevBuf bx, bd, bo;
+    int rc;
+    if ((rc = bx.alloc(N * es)) || (rc = bd.alloc(N * es)) || (rc = bo.alloc(N * es))) return rc;
+    hipStream_t st = p->ctx->stream;
+    std::vector<char> s1, s2;
+    pack_host(prec, x, N, s1);
+    pack_host(prec, dxdt, N, s2);
+    hipError_t e;
+    if ((e = hipMemcpyAsync(bx.ptr, s1.data(), N * es, hipMemcpyHostToDevice, st)) != hipSuccess ||
+        (e = hipMemcpyAsync(bd.ptr, s2.data(), N * es, hipMemcpyHostToDevice, st)) != hipSuccess)
+        return hip_fail(e, "euler upload");
+    launch_euler(prec, dev_prob(p), dev_params(prm), bx.ptr, bd.ptr, h, bo.ptr, st);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "euler launch");
+    if ((e = hipMemcpyAsync(s1.data(), bo.ptr, N * es, hipMemcpyDeviceToHost, st)) != hipSuccess)
+        return hip_fail(e, "euler download");
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "euler sync");
+    unpack_host(prec, s1, N, x_out);
+    return DNX_OK;
+}
+
+int dnx_state_create(dnx_problem *p, int prec, const dnx_params *prm, const double *x0, dnx_state **out)
+{
+    if (!p || !prm || !x0 || !out || (prec != DNX_FP64 && prec != DNX_FP32)) return DNX_E_ARG;
+    if (!p->ctx) return host_only(p);
+    (void)hipSetDevice(p->ctx->device);
+    const HostProblem &h = p->hp;
+    const size_t es = prec_size(prec), N = (size_t)h.n + 2 * (size_t)h.m;
+    dnx_state *s = new dnx_state();
+    s->p = p;
+    s->prec = prec;
+    s->prm = dev_params(prm);
+    int rc;
+    DevBuf bx;
+    if ((rc = s->b_vd.alloc(2 * es * (size_t)(h.n > 0 ? h.n : 1))) ||
+        (rc = s->b_xs.alloc(es * (size_t)(h.m > 0 ? h.m : 1))) ||
+        (rc = s->b_xl.alloc(es * (size_t)(h.m > 0 ? h.m : 1))) ||
+        (rc = s->b_e.alloc(es * (size_t)(h.nnz() > 0 ? h.nnz() : 1))) || (rc = s->b_ctl.alloc(sizeof(Ctl))) ||
+        (rc = bx.alloc(es * (N ? N : 1)))) {
+        delete s;
+        return rc;
+    }
+    s->ctl = (Ctl *)s->b_ctl.ptr;
+    hipError_t e;
+    if ((e = hipHostMalloc((void **)&s->h_ctl, 2 * sizeof(Ctl), hipHostMallocDefault)) != hipSuccess) {
+        s->h_ctl = nullptr;
+        delete s;
+        return hip_fail(e, "hipHostMalloc");
+    }
+    Ctl c;
+    init_ctl(c);
+    std::vector<char> stage;
+    pack_host(prec, x0, N, stage);
+    hipStream_t st = p->ctx->stream;
+    if ((N && (e = hipMemcpyAsync(bx.ptr, stage.data(), N * es, hipMemcpyHostToDevice, st)) != hipSuccess) ||
+        (e = hipMemcpyAsync(s->ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice, st)) != hipSuccess) {
+        delete s;
+        return hip_fail(e, "state upload");
+    }
+    launch_load_state(prec, dev_prob(p), bx.ptr, s->b_vd.ptr, s->b_xs.ptr, s->b_xl.ptr, st);
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) {
+        delete s;
+        return hip_fail(e, "state load");
+    }
+    *out = s;
+    return DNX_OK;
+}
+
+int dnx_state_free(dnx_state *s)
+{
+    if (!s) return DNX_E_ARG;
+    (void)hipSetDevice(s->p->ctx->device);
+    (void)hipStreamSynchronize(s->p->ctx->stream);
+    delete s;
+    return DNX_OK;
+}
+
+static int set_mode(dnx_state *s, int stop, long long cap)
+{
+    // the two fields sit next to each other only in part: write them apart
+    hipStream_t st = s->p->ctx->stream;
+    hipError_t e;
+    int v = stop;
+    if ((e = hipMemcpyAsync(&s->ctl->stop_on_solve, &v, sizeof(int), hipMemcpyHostToDevice, st)) != hipSuccess ||
+        (e = hipMemcpyAsync(&s->ctl->cap, &cap, sizeof(long long), hipMemcpyHostToDevice, st)) != hipSuccess)
+        return hip_fail(e, "mode upload");
+    // the source values live on this stack frame: wait before returning
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "mode sync");
+    return DNX_OK;
+}
+
+int dnx_state_step(dnx_state *s, int nsteps)
+{
+    if (!s || nsteps < 0) return DNX_E_ARG;
+    (void)hipSetDevice(s->p->ctx->device);
+    int rc = set_mode(s, 0, LLONG_MAX);
+    if (rc) return rc;
+    return state_enqueue_steps(s, nsteps);
+}
+
+int dnx_state_read(dnx_state *s, double *x, double *t, int64_t *steps, int *last_level, int *loc)
+{
+    if (!s) return DNX_E_ARG;
+    (void)hipSetDevice(s->p->ctx->device);
+    const HostProblem &h = s->p->hp;
+    const size_t es = prec_size(s->prec), N = (size_t)h.n + 2 * (size_t)h.m;
+    hipStream_t st = s->p->ctx->stream;
+    hipError_t e;
+    DevBuf bo;
+    std::vector<char> stage;
+    if (x && N) {
+        int rc = bo.alloc(N * es);
+        if (rc) return rc;
+        launch_materialize(s->prec, dev_prob(s->p), s->prm, s->b_vd.ptr, s->b_xs.ptr, s->b_xl.ptr, s->ctl, bo.ptr,
+                           st);
+        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "materialize");
+        stage.resize(N * es);
+        if ((e = hipMemcpyAsync(stage.data(), bo.ptr, N * es, hipMemcpyDeviceToHost, st)) != hipSuccess)
+            return hip_fail(e, "state download");
+    }
+    Ctl c;
+    if ((e = hipMemcpyAsync(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st)) != hipSuccess)
+        return hip_fail(e, "ctl download");
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "state sync");
+    if (x && N) unpack_host(s->prec, stage, N, x);
+    const int sl = (int)(c.steps % 3);
+    const int plev = (c.status == DNX_RUNNING && !c.done[sl]) ? c.level[sl] : -1;
+    static const double dts[5] = {0.125, 0.0625, 0.03125, 0.015625, 0.0078125};
+    if (t) *t = c.t + (plev >= 0 ? dts[plev] : 0.0);
+    if (steps) *steps = c.steps + (plev >= 0 ? 1 : 0);
+    if (last_level) *last_level = plev;
+    if (loc) *loc = c.loc[sl];
+    return DNX_OK;
+}
+
+int dnx_state_solve(dnx_state *s, int64_t step_cap, double walltime_s, int8_t *assign_out, int64_t *steps_out,
+                    double *t_out, int *status)
+{
+    if (!s || step_cap < 0 || !status) return DNX_E_ARG;
+    (void)hipSetDevice(s->p->ctx->device);
+    const HostProblem &h = s->p->hp;
+    hipStream_t st = s->p->ctx->stream;
+    hipError_t e;
+    // the device step index equals the number of enqueued steps while running
+    int rc = set_mode(s, 1, (long long)s->enq + step_cap);
+    if (rc) return rc;
+    auto t0 = std::chrono::steady_clock::now();
+    hipEvent_t ev[2];
+    if ((e = hipEventCreateWithFlags(&ev[0], hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming)) != hipSuccess)
+        return hip_fail(e, "hipEventCreate");
+    auto issue = [&](int b) -> int {
+        int r = state_enqueue_steps(s, kChunk);
+        if (r) return r;
+        hipError_t ee;
+        if ((ee = hipMemcpyAsync(&s->h_ctl[b], s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st)) != hipSuccess)
+            return hip_fail(ee, "ctl poll");
+        if ((ee = hipEventRecord(ev[b], st)) != hipSuccess) return hip_fail(ee, "event");
+        return DNX_OK;
+    };
+    int st_final = DNX_RUNNING;
+    // states 0..step_cap are evaluated: step_cap+1 step pairs reach the cap
+    const long long need = (long long)step_cap + 1;
+    long long issued = 0;
+    int inflight[2], nin = 0, nextb = 0;
+    while (rc == DNX_OK) {
+        // keep up to two chunks queued so the device never waits on a poll
+        while (nin < 2 && issued < need) {
+            rc = issue(nextb);
+            if (rc) break;
+            inflight[nin++] = nextb;
+            nextb ^= 1;
+            issued += kChunk;
+        }
+        if (rc || nin == 0) break;
+        const int b = inflight[0];
+        inflight[0] = inflight[1];
+        nin--;
+        if ((e = hipEventSynchronize(ev[b])) != hipSuccess) {
+            rc = hip_fail(e, "poll sync");
+            break;
+        }
+        int sv = ctl_status(s->h_ctl[b]);
+        if (sv != DNX_RUNNING) {
+            st_final = sv;
+            break;
+        }
+        if (walltime_s > 0) {
+            double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (el > walltime_s) {
+                st_final = DNX_UNKNOWN;
+                break;
+            }
+        }
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "solve drain");
+    if (rc) return rc;
+    // final snapshot after the queue drained
+    Ctl c;
+    if ((e = hipMemcpy(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost)) != hipSuccess)
+        return hip_fail(e, "ctl read");
+    int sv = ctl_status(c);
+    if (sv != DNX_RUNNING) st_final = sv;
+    if (steps_out) *steps_out = c.steps;
+    if (t_out) *t_out = c.t;
+    if (st_final == DNX_SAT) {
+        // exact integer check of the verified state's signs on the host
+        std::vector<double> x((size_t)h.n + 2 * (size_t)h.m);
+        rc = dnx_state_read(s, x.data(), nullptr, nullptr, nullptr, nullptr);
+        if (rc) return rc;
+        std::vector<int8_t> a((size_t)h.n);
+        for (int v = 0; v < h.n; v++) a[(size_t)v] = x[(size_t)v] >= 0.0 ? 1 : -1;
+        if (count_unsat(h, a.data()) != 0) {
+            set_error("device reported a solution that fails the clause check");
+            *status = DNX_FAILED;
+            return DNX_E_STATE;
+        }
+        if (assign_out) memcpy(assign_out, a.data(), (size_t)h.n);
+    }
+    *status = st_final == DNX_RUNNING ? DNX_UNKNOWN : st_final;
+    return DNX_OK;
+}
+
+int dnx_solve(dnx_problem *p, int prec, const dnx_params *prm, const double *x0, int64_t step_cap,
+              double walltime_s, int8_t *assign_out, int64_t *steps_out, double *t_out, int *status)
+{
+    dnx_state *s = nullptr;
+    int rc = dnx_state_create(p, prec, prm, x0, &s);
+    if (rc) return rc;
+    rc = dnx_state_solve(s, step_cap, walltime_s, assign_out, steps_out, t_out, status);
+    dnx_state_free(s);
+    return rc;
+}
+
+int dnx_init_reference(const dnx_problem *p, int seed, double *x0)
+{
+    if (!p || !x0) return DNX_E_ARG;
+    init_reference(p->hp, seed, x0);
+    return DNX_OK;
+}
+
+int dnx_generate_planted(uint64_t seed, int n, int m, double p1, double p2, int32_t *lits_out, int8_t *planted_out)
+{
+    if (n < 3 || m < 0 || !lits_out || p1 < 0 || p2 < 0 || p1 + p2 > 1.0) return DNX_E_ARG;
+    generate_planted(seed, n, m, p1, p2, lits_out, planted_out);
+    return DNX_OK;
+}
+
+int dnx_init_uniform(uint64_t seed, int n, int m, double xs0, double xl0, double *x0)
+{
+    if (n < 0 || m < 0 || !x0) return DNX_E_ARG;
+    init_uniform(seed, n, m, xs0, xl0, x0);
+    return DNX_OK;
+}
+
+int dnx_write_dimacs(const char *path, int n, int m, const int32_t *clause_ptr, const int32_t *lits,
+                     const char *comment)
+{
+    if (!path || n < 0 || m < 0 || (m > 0 && (!clause_ptr || !lits))) return DNX_E_ARG;
+    int rc = write_dimacs(path, n, m, clause_ptr, lits, comment);
+    if (rc) set_error(std::string("cannot write ") + path);
+    return rc;
+}
+
+int dnx_check_assignment(const dnx_problem *p, const int8_t *assign, int *unsat)
+{
+    if (!p || !assign || !unsat) return DNX_E_ARG;
+    *unsat = count_unsat(p->hp, assign);
+    return DNX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,66 @@
+// api_internal.hpp -- definitions of the opaque ABI objects and helpers
+// shared by the C ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "dnx.h"
+#include "host.hpp"
+#include "kernels.hpp"
+
+namespace dnx {
+
+// Owning device allocation.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t size = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf();
+    int alloc(size_t bytes);
+};
+
+int hip_fail(hipError_t e, const char *what);
+size_t prec_size(int prec);
+void pack_host(int prec, const double *src, size_t count, std::vector<char> &out);
+void unpack_host(int prec, const std::vector<char> &in, size_t count, double *dst);
+DevParams dev_params(const dnx_params *p);
+
+}  // namespace dnx
+
+struct dnx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+};
+
+struct dnx_problem {
+    dnx_ctx *ctx = nullptr;
+    dnx::HostProblem hp;
+    dnx::DevBuf b_cp, b_lits, b_lpos, b_row, b_unit;
+    int32_t *d_cp = nullptr, *d_lits = nullptr, *d_lpos = nullptr, *d_row_ptr = nullptr;
+    int8_t *d_unit = nullptr;
+};
+
+struct dnx_state {
+    dnx_problem *p = nullptr;
+    int prec = DNX_FP64;
+    dnx::DevParams prm{};
+    dnx::DevBuf b_vd, b_xs, b_xl, b_e, b_ctl;
+    dnx::Ctl *ctl = nullptr;     // device control block
+    dnx::Ctl *h_ctl = nullptr;   // two pinned host snapshots for polling
+    long long enq = 0;           // steps enqueued since creation
+    hipGraphExec_t graph = nullptr;
+    ~dnx_state()
+    {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (h_ctl) (void)hipHostFree(h_ctl);
+    }
+};
+
+namespace dnx {
+DevProb dev_prob(const dnx_problem *p);
+}

@@ -1,0 +1,244 @@
+"""Python view of libdnx (the C ABI in include/dnx.h) over ctypes.
+
+Used by the test suite and the benchmark to drive the HIP path.  There is no
+fallback: when ``lib/libdnx.so`` is missing, or no GPU is visible, the calls
+raise instead of computing anything on the CPU.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libdnx.so")
+INCLUDE_H = os.path.join(os.path.dirname(HERE), "include", "dnx.h")
+
+OK, E_ARG, E_IO, E_PARSE, E_TOOLONG, E_HIP, E_NOMEM, E_BUDGET, E_STATE = 0, -1, -2, -3, -4, -5, -6, -7, -8
+FP64, FP32 = 0, 1
+RUNNING, SAT, UNKNOWN, FAILED = 0, 1, 2, 3
+
+I32P = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+I8P = np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")
+I64P = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+F64P = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+class Params(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("beta", C.c_double), ("gamma", C.c_double),
+                ("delta", C.c_double), ("epsilon", C.c_double), ("zeta", C.c_double),
+                ("xl_max", C.c_int64)]
+
+
+class DnxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"dnx error {code}: {msg}")
+        self.code = code
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise DnxError(E_STATE, f"{LIB_PATH} is not built (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp, ip, dp = C.c_void_p, C.c_int, C.c_double
+    pp = C.POINTER(C.c_void_p)
+    L.dnx_last_error.restype = C.c_char_p
+    L.dnx_params_default.argtypes = [ip, ip, C.POINTER(Params)]
+    L.dnx_open.argtypes = [ip, pp]
+    L.dnx_close.argtypes = [vp]
+    L.dnx_problem_from_dimacs.argtypes = [vp, C.c_char_p, pp]
+    L.dnx_problem_from_lits.argtypes = [vp, ip, ip, I32P, I32P, pp]
+    L.dnx_problem_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)]
+    L.dnx_problem_clauses.argtypes = [vp, I32P, I32P]
+    L.dnx_problem_free.argtypes = [vp]
+    L.dnx_rhs.argtypes = [vp, ip, C.POINTER(Params), F64P, F64P, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.dnx_euler.argtypes = [vp, ip, C.POINTER(Params), F64P, F64P, dp, F64P]
+    L.dnx_state_create.argtypes = [vp, ip, C.POINTER(Params), F64P, pp]
+    L.dnx_state_free.argtypes = [vp]
+    L.dnx_state_step.argtypes = [vp, ip]
+    L.dnx_state_read.argtypes = [vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_int),
+                                 C.POINTER(C.c_int)]
+    L.dnx_state_solve.argtypes = [vp, C.c_int64, dp, vp, C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_int)]
+    L.dnx_solve.argtypes = [vp, ip, C.POINTER(Params), F64P, C.c_int64, dp, vp, C.POINTER(C.c_int64),
+                            C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.dnx_init_reference.argtypes = [vp, ip, F64P]
+    L.dnx_generate_planted.argtypes = [C.c_uint64, ip, ip, dp, dp, I32P, vp]
+    L.dnx_init_uniform.argtypes = [C.c_uint64, ip, ip, dp, dp, F64P]
+    L.dnx_write_dimacs.argtypes = [C.c_char_p, ip, ip, I32P, I32P, C.c_char_p]
+    L.dnx_check_assignment.argtypes = [vp, I8P, C.POINTER(C.c_int)]
+    _LIB = L
+    return L
+
+
+def check(rc):
+    if rc != OK:
+        raise DnxError(rc, lib().dnx_last_error().decode(errors="replace"))
+    return rc
+
+
+def declared_symbols():
+    """Names of the functions include/dnx.h declares."""
+    import re
+    txt = open(INCLUDE_H).read()
+    return sorted(set(re.findall(r"\b(dnx_[a-z0-9_]+)\s*\(", txt)))
+
+
+def params(m, xl_max_flag=10000, **kw):
+    p = Params()
+    check(lib().dnx_params_default(m, xl_max_flag, C.byref(p)))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+# ---------------------------------------------------------------------------
+# host utilities (no GPU needed)
+# ---------------------------------------------------------------------------
+def generate_planted(seed, n, m, p1=0.46, p2=0.46, with_planted=False):
+    lits = np.zeros(3 * m, np.int32)
+    planted = np.zeros(n, np.int8)
+    check(lib().dnx_generate_planted(seed, n, m, p1, p2, lits, planted.ctypes.data_as(C.c_void_p)))
+    return (lits, planted) if with_planted else lits
+
+
+def init_uniform(seed, n, m, xs0=0.5, xl0=1.0):
+    x = np.zeros(n + 2 * m, np.float64)
+    check(lib().dnx_init_uniform(seed, n, m, xs0, xl0, x))
+    return x
+
+
+def write_dimacs(path, n, cp, lits, comment=""):
+    m = len(cp) - 1
+    check(lib().dnx_write_dimacs(path.encode(), n, m, np.ascontiguousarray(cp, np.int32),
+                                 np.ascontiguousarray(lits, np.int32), comment.encode()))
+
+
+# ---------------------------------------------------------------------------
+# GPU objects
+# ---------------------------------------------------------------------------
+class Context:
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().dnx_open(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().dnx_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def problem_from_lits(self, n, cp, lits):
+        return Problem(self, n, cp, lits)
+
+    def problem_from_dimacs(self, path):
+        h = C.c_void_p()
+        check(lib().dnx_problem_from_dimacs(self.h, path.encode(), C.byref(h)))
+        return Problem._wrap(self, h)
+
+
+class Problem:
+    def __init__(self, ctx, n, cp, lits):
+        cp = np.ascontiguousarray(cp, np.int32)
+        lits = np.ascontiguousarray(lits, np.int32)
+        if lits.size == 0:
+            lits = np.zeros(1, np.int32)
+        h = C.c_void_p()
+        check(lib().dnx_problem_from_lits(ctx.h, int(n), len(cp) - 1, cp, lits, C.byref(h)))
+        self._init(ctx, h)
+
+    @classmethod
+    def _wrap(cls, ctx, h):
+        o = cls.__new__(cls)
+        o._init(ctx, h)
+        return o
+
+    def _init(self, ctx, h):
+        self.ctx, self.h = ctx, h
+        n, m, nnz = C.c_int(), C.c_int(), C.c_int64()
+        check(lib().dnx_problem_info(h, C.byref(n), C.byref(m), C.byref(nnz)))
+        self.n, self.m, self.nnz = n.value, m.value, nnz.value
+
+    def clauses(self):
+        cp = np.zeros(self.m + 1, np.int32)
+        lits = np.zeros(max(self.nnz, 1), np.int32)
+        check(lib().dnx_problem_clauses(self.h, cp, lits))
+        return cp, lits[:self.nnz]
+
+    def free(self):
+        if self.h:
+            lib().dnx_problem_free(self.h)
+            self.h = None
+
+    def rhs(self, prm, x, prec=FP64):
+        x = np.ascontiguousarray(x, np.float64)
+        d = np.zeros_like(x)
+        loc, en = C.c_int(), C.c_double()
+        check(lib().dnx_rhs(self.h, prec, C.byref(prm), x, d, C.byref(loc), C.byref(en)))
+        return d, loc.value, en.value
+
+    def euler(self, prm, x, dxdt, h, prec=FP64):
+        x = np.ascontiguousarray(x, np.float64)
+        out = np.zeros_like(x)
+        check(lib().dnx_euler(self.h, prec, C.byref(prm), x, np.ascontiguousarray(dxdt, np.float64), h, out))
+        return out
+
+    def init_reference(self, seed=0):
+        x = np.zeros(self.n + 2 * self.m, np.float64)
+        check(lib().dnx_init_reference(self.h, seed, x))
+        return x
+
+    def check_assignment(self, assign):
+        u = C.c_int()
+        check(lib().dnx_check_assignment(self.h, np.ascontiguousarray(assign, np.int8), C.byref(u)))
+        return u.value
+
+    def solve(self, prm, x0, prec=FP64, cap=100000, walltime=0.0):
+        a = np.zeros(max(self.n, 1), np.int8)
+        steps, t, st = C.c_int64(), C.c_double(), C.c_int()
+        check(lib().dnx_solve(self.h, prec, C.byref(prm), np.ascontiguousarray(x0, np.float64), cap, walltime,
+                              a.ctypes.data_as(C.c_void_p), C.byref(steps), C.byref(t), C.byref(st)))
+        return dict(status=st.value, steps=steps.value, t=t.value, assign=a[:self.n])
+
+    def state(self, prm, x0, prec=FP64):
+        return State(self, prm, x0, prec)
+
+
+class State:
+    def __init__(self, prob, prm, x0, prec):
+        h = C.c_void_p()
+        check(lib().dnx_state_create(prob.h, prec, C.byref(prm), np.ascontiguousarray(x0, np.float64), C.byref(h)))
+        self.prob, self.h, self.prec = prob, h, prec
+
+    def free(self):
+        if self.h:
+            lib().dnx_state_free(self.h)
+            self.h = None
+
+    def step(self, nsteps):
+        check(lib().dnx_state_step(self.h, nsteps))
+
+    def read(self):
+        x = np.zeros(self.prob.n + 2 * self.prob.m, np.float64)
+        t, steps, lev, loc = C.c_double(), C.c_int64(), C.c_int(), C.c_int()
+        check(lib().dnx_state_read(self.h, x.ctypes.data_as(C.c_void_p), C.byref(t), C.byref(steps), C.byref(lev),
+                                   C.byref(loc)))
+        return dict(x=x, t=t.value, steps=steps.value, level=lev.value, loc=loc.value)
+
+    def solve(self, cap=100000, walltime=0.0):
+        a = np.zeros(max(self.prob.n, 1), np.int8)
+        steps, t, st = C.c_int64(), C.c_double(), C.c_int()
+        check(lib().dnx_state_solve(self.h, cap, walltime, a.ctypes.data_as(C.c_void_p), C.byref(steps),
+                                    C.byref(t), C.byref(st)))
+        return dict(status=st.value, steps=steps.value, t=t.value, assign=a[:self.prob.n])

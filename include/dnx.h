@@ -1,0 +1,177 @@
+/*
+ * dnx.h -- C ABI of libdnx, the MI355X-native memcomputing SAT integrator.
+ *
+ * The reference has no plugin or FFI layer: its in-process boundary is two
+ * C++ functions that pass std::vector<double> by value and publish results
+ * through globals, and its process boundary is the `dynex -i file.cnf` CLI.
+ * Each entry point below names the reference code it replaces.  Common rules:
+ *   - every function returns an int status: DNX_OK (0) or a negative DNX_E_*
+ *     code; dnx_last_error() returns a message for the calling thread's last
+ *     failure.  Nothing calls exit() or kills a thread (the reference calls
+ *     pthread_exit from inside its RHS, dynex.cc:340-345).
+ *   - buffers are caller-owned host memory; sizes are explicit.
+ *   - state vectors at this boundary use the reference layout
+ *     x = [V(0..n-1) | Xs(0..m-1) | Xl(0..m-1)] in double (dynex.cc:1205-1210);
+ *     the device keeps its own structure-of-arrays layout.
+ *   - a dnx_ctx owns one HIP stream on one device; a context and the objects
+ *     made from it must not be used from two threads at once.
+ */
+#ifndef DNX_H
+#define DNX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define DNX_OK 0
+#define DNX_E_ARG -1     /* bad argument */
+#define DNX_E_IO -2      /* file cannot be opened or written */
+#define DNX_E_PARSE -3   /* malformed DIMACS */
+#define DNX_E_TOOLONG -4 /* clause longer than DNX_MAX_LITS (dynex.cc:2388) */
+#define DNX_E_HIP -5     /* HIP runtime failure */
+#define DNX_E_NOMEM -6   /* device or host allocation failed */
+#define DNX_E_BUDGET -7  /* request exceeds the configured memory budget */
+#define DNX_E_STATE -8   /* call not valid in the object's current state */
+
+#define DNX_MAX_LITS 25 /* MAX_LITS_SYSTEM, dynex.cc:69 */
+
+/* ---- precision ------------------------------------------------------- */
+#define DNX_FP64 0 /* reference arithmetic: double, sorted per-variable fold */
+#define DNX_FP32 1 /* float state and RHS, clause-order fold, double time   */
+
+/* ---- solve status ---------------------------------------------------- */
+#define DNX_RUNNING 0
+#define DNX_SAT 1     /* assignment passed the exact integer clause check */
+#define DNX_UNKNOWN 2 /* step cap or wall-clock limit reached (dynex.cc:1519-1521) */
+#define DNX_FAILED 3
+
+/* ---- adaptive step levels -------------------------------------------- */
+/* level L in 0..3: the step was accepted at h = 2^-(3+L);
+ * level 4: the 1/64 trial also failed; its state is kept and t advances by
+ * 1/128 (dynex.cc:1394-1417). */
+
+typedef struct dnx_ctx dnx_ctx;
+typedef struct dnx_problem dnx_problem;
+typedef struct dnx_state dnx_state;
+
+/* Equation constants.  Defaults (dnx_params_default) are the reference's
+ * (dynex.cc:132-140): alpha 2.5, beta 20, gamma 0.25, delta 0.05,
+ * epsilon 0.1, zeta 0.1.  xl_max is the already-scaled bound on Xl; the
+ * reference computes it as 10000*m in wrapping int arithmetic and uses
+ * INT_MAX when that is not positive (dynex.cc:2358-2359). */
+typedef struct {
+    double alpha, beta, gamma, delta, epsilon, zeta;
+    int64_t xl_max;
+} dnx_params;
+
+/* Thread-local message for the last failing call. */
+const char *dnx_last_error(void);
+
+/* Fill *p with the reference defaults for a problem with m clauses;
+ * xl_max_flag is the -m value (10000 by default, dynex.cc:141). */
+int dnx_params_default(int m, int xl_max_flag, dnx_params *p);
+
+/* ---- context --------------------------------------------------------- */
+/* Opens device `device` and creates the context's stream. */
+int dnx_open(int device, dnx_ctx **out);
+int dnx_close(dnx_ctx *ctx);
+
+/* ---- problems -------------------------------------------------------- */
+/* A NULL ctx makes a host-only problem: it can be inspected, checked and
+ * used for initial states, and the device entry points reject it with
+ * DNX_E_STATE.
+ *
+ * DIMACS reader with the reference's token rules (dynex.cc:2306-2392):
+ * tokens up to "p", then " cnf n m", clauses terminated by "0", "c" skips a
+ * line.  A missing file is DNX_E_IO (the reference crashes, dynex.cc:2308). */
+int dnx_problem_from_dimacs(dnx_ctx *ctx, const char *path, dnx_problem **out);
+
+/* From a clause table in CSR form: clause c has literals
+ * lits[clause_ptr[c] .. clause_ptr[c+1]), signed, 1-based
+ * (the reference's cls/clauseSizes, dynex.cc:178-183). */
+int dnx_problem_from_lits(dnx_ctx *ctx, int n, int m, const int32_t *clause_ptr,
+                          const int32_t *lits, dnx_problem **out);
+int dnx_problem_info(const dnx_problem *p, int *n, int *m, int64_t *nnz);
+/* Copies the clause table back out (clause_ptr[m+1], lits[nnz]). */
+int dnx_problem_clauses(const dnx_problem *p, int32_t *clause_ptr, int32_t *lits);
+int dnx_problem_free(dnx_problem *p);
+
+/* ---- function-level entry points (parity surface) -------------------- */
+/* dmm_generate_dxdt (dynex.cc:332-530): dxdt = RHS(x); *loc = number of
+ * clauses with C >= 1/2; *energy = sum of C.  In DNX_FP64 the result is
+ * bit-identical to the reference's strict-IEEE evaluation. */
+int dnx_rhs(dnx_problem *p, int prec, const dnx_params *prm, const double *x,
+            double *dxdt, int *loc, double *energy);
+
+/* dmm_dostep (dynex.cc:301-327): x_out = clamp(x + h*dxdt) with unit pins. */
+int dnx_euler(dnx_problem *p, int prec, const dnx_params *prm, const double *x,
+              const double *dxdt, double h, double *x_out);
+
+/* ---- resident state (the mode-6 loop of dynex.cc:1381-1427) ---------- */
+/* Uploads x0 (reference layout; unit-clause pins are applied to V as the
+ * reference does at dynex.cc:2694-2699) and keeps the state on the device. */
+int dnx_state_create(dnx_problem *p, int prec, const dnx_params *prm,
+                     const double *x0, dnx_state **out);
+int dnx_state_free(dnx_state *s);
+
+/* Advances nsteps adaptive steps without stopping at a solution (fixed-step
+ * trajectory mode).  No host synchronisation inside. */
+int dnx_state_step(dnx_state *s, int nsteps);
+
+/* Reads the current state: x (reference layout, may be NULL), simulated
+ * time, accepted-step count, the level of the last step, and loc of the
+ * last evaluated state. */
+int dnx_state_read(dnx_state *s, double *x, double *t, int64_t *steps,
+                   int *last_level, int *loc);
+
+/* Runs the mode-6 loop until the evaluated state passes the exact clause
+ * check (DNX_SAT), step_cap states have been stepped past, or walltime_s
+ * seconds pass (DNX_UNKNOWN; walltime_s <= 0 means no limit).  assign_out[n]
+ * receives +1/-1 from the sign of the verified V (x >= 0 positive, as the
+ * reference's solution file, dynex.cc:693-699).  *steps_out counts the
+ * accepted steps before the verified state (the counter the reference
+ * prints at that RHS, dynex.cc:549). */
+int dnx_state_solve(dnx_state *s, int64_t step_cap, double walltime_s,
+                    int8_t *assign_out, int64_t *steps_out, double *t_out,
+                    int *status);
+
+/* One-call form of create + solve + free (the reference's dmm(),
+ * dynex.cc:1182). */
+int dnx_solve(dnx_problem *p, int prec, const dnx_params *prm, const double *x0,
+              int64_t step_cap, double walltime_s, int8_t *assign_out,
+              int64_t *steps_out, double *t_out, int *status);
+
+/* ---- host utilities -------------------------------------------------- */
+/* Initial state of the reference's thread 0 for seed `seed`
+ * (dynex.cc:2643-2700): seed 0 sets every entry to 1.0; other seeds draw V
+ * from std::uniform_int_distribution<int>(-1,1) over std::mt19937(seed) with
+ * Xs = 0 and Xl = 1; seed -1 / -2 set V to 0 / -1; unit clauses pin V. */
+int dnx_init_reference(const dnx_problem *p, int seed, double *x0);
+
+/* Planted random k=3 instance (no counterpart in the reference, which reads
+ * files only): sigma uniform +-1, three distinct variables per clause, the
+ * number t of literals true under sigma drawn with P(1)=p1, P(2)=p2,
+ * P(3)=1-p1-p2, the true ones a uniform subset.  lits_out[3m];
+ * planted_out[n] (may be NULL) gets sigma. */
+int dnx_generate_planted(uint64_t seed, int n, int m, double p1, double p2,
+                         int32_t *lits_out, int8_t *planted_out);
+
+/* Initial state v ~ U[-1,1), Xs = xs0, Xl = xl0 from a seeded stream. */
+int dnx_init_uniform(uint64_t seed, int n, int m, double xs0, double xl0,
+                     double *x0);
+
+/* Writes a DIMACS file for a clause table. */
+int dnx_write_dimacs(const char *path, int n, int m, const int32_t *clause_ptr,
+                     const int32_t *lits, const char *comment);
+
+/* Exact integer clause check of an assignment (+1/-1 per variable):
+ * returns the number of unsatisfied clauses through *unsat. */
+int dnx_check_assignment(const dnx_problem *p, const int8_t *assign, int *unsat);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DNX_H */

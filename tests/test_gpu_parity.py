@@ -1,0 +1,269 @@
+"""GPU parity of the HIP path (through the C ABI) against the CPU oracle.
+
+FP64 is held to bit-identity with the oracle, which restates the reference's
+strict-IEEE arithmetic (dynex.cc).  FP32 is held to bit-identity with the
+oracle's float instance (same operation order, clause-order fold), and its
+deviation from the FP64 trajectory is reported."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import dnx
+import oracle as O
+from conftest import CNF_NAMES, cnf_path, planted_cp, random_state
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = dnx.Context(0)
+    yield c
+    c.close()
+
+
+def gen(n, m, seed):
+    lits = dnx.generate_planted(seed, n, m)
+    return planted_cp(m), lits
+
+
+def mixed_problem(seed=4):
+    """k in {1,2,3,4,5}, a duplicated literal, an unused variable, unit pins."""
+    rng = np.random.default_rng(seed)
+    n = 60
+    cls = []
+    for c in range(300):
+        k = int(rng.choice([1, 2, 3, 3, 3, 4, 5], p=[0.02, 0.2, 0.2, 0.2, 0.18, 0.1, 0.1]))
+        vs = rng.choice(np.arange(1, n), k, replace=False)  # variable n never used
+        cls.append(list(vs * rng.choice([-1, 1], k)))
+    cls.append([5, 5, -7])
+    cp = np.cumsum([0] + [len(c) for c in cls]).astype(np.int32)
+    return n, cp, np.array(sum(cls, []), np.int32)
+
+
+def both(ctx, n, cp, lits):
+    return ctx.problem_from_lits(n, cp, lits), O.Problem(n, len(cp) - 1, cp, lits)
+
+
+def bits_equal(a, b):
+    return np.array_equal(np.asarray(a).view(np.uint64) if a.dtype == np.float64 else a,
+                          np.asarray(b).view(np.uint64) if b.dtype == np.float64 else b)
+
+
+# ---------------------------------------------------------------------------
+# function-level parity: RHS (dynex.cc:332-530) and Euler (dynex.cc:301-327)
+# ---------------------------------------------------------------------------
+def rhs_cases(ctx, cnf_dir):
+    for nv in (100, 1000, 10000):
+        path = cnf_path(cnf_dir, nv)
+        yield f"fixture{nv}", ctx.problem_from_dimacs(path), O.Problem.from_dimacs(path)
+    n, cp, lits = mixed_problem()
+    yield ("mixed",) + both(ctx, n, cp, lits)
+    cp, lits = gen(3000, 24000, 2)
+    yield ("planted3000",) + both(ctx, 3000, cp, lits)
+
+
+def test_rhs_fp64_bitexact(ctx, cnf_dir):
+    rng = np.random.default_rng(0)
+    for name, gp, op in rhs_cases(ctx, cnf_dir):
+        prm = dnx.params(gp.m)
+        oprm = O.params(op.m)
+        states = [random_state(rng, gp.n, gp.m, xl_hi=50.0) for _ in range(3)]
+        wild = random_state(rng, gp.n, gp.m)
+        wild[:gp.n] *= 1.5  # out-of-range values exercise the read clamps
+        wild[gp.n:gp.n + gp.m] = rng.uniform(-0.5, 1.5, gp.m)
+        wild[gp.n + gp.m:] = rng.uniform(0.0, 3.0, gp.m)
+        states.append(wild)
+        for x in states:
+            d, loc, en = gp.rhs(prm, x, dnx.FP64)
+            d0, loc0, en0 = O.rhs(op, oprm, x)
+            assert loc == loc0, name
+            assert bits_equal(d, d0), (name, np.flatnonzero(d != d0)[:10])
+            assert abs(en - en0) <= 1e-9 * max(1.0, abs(en0))
+        gp.free()
+
+
+def test_rhs_fp32_bitexact(ctx, cnf_dir):
+    rng = np.random.default_rng(1)
+    for name, gp, op in rhs_cases(ctx, cnf_dir):
+        prm = dnx.params(gp.m)
+        oprm = O.params(op.m)
+        for _ in range(3):
+            x = random_state(rng, gp.n, gp.m, xl_hi=50.0).astype(np.float32)
+            d, loc, en = gp.rhs(prm, x.astype(np.float64), dnx.FP32)
+            d0, loc0, _ = O.rhs(op, oprm, x, prec=32)
+            assert loc == loc0, name
+            assert np.array_equal(d.astype(np.float32).view(np.uint32), d0.view(np.uint32)), name
+        gp.free()
+
+
+def test_euler_bitexact(ctx):
+    n, cp, lits = mixed_problem(9)
+    gp, op = both(ctx, n, cp, lits)
+    m = gp.m
+    prm, oprm = dnx.params(m), O.params(m)
+    rng = np.random.default_rng(2)
+    for h in (0.125, 0.0625, 0.03125, 0.015625):
+        x = random_state(rng, n, m)
+        d = rng.normal(0, 8, n + 2 * m)
+        assert bits_equal(gp.euler(prm, x, d, h, dnx.FP64), O.dostep(op, oprm, x, d, h))
+        x32, d32 = x.astype(np.float32), d.astype(np.float32)
+        out = gp.euler(prm, x32.astype(np.float64), d32.astype(np.float64), h, dnx.FP32).astype(np.float32)
+        assert np.array_equal(out, O.dostep(op, oprm, x32, d32, h, prec=32))
+    gp.free()
+
+
+# ---------------------------------------------------------------------------
+# resident trajectories: the mode-6 loop (dynex.cc:1387-1417), fixed steps
+# ---------------------------------------------------------------------------
+def run_traj(gp, op, x0, checkpoints, prec):
+    """Steps the GPU state and the oracle to each checkpoint and compares."""
+    m = gp.m
+    prm, oprm = dnx.params(m), O.params(m)
+    st = gp.state(prm, x0, dnx.FP64 if prec == 64 else dnx.FP32)
+    xo = x0.copy() if prec == 64 else x0.astype(np.float32)
+    if op.has_unit:  # the reference pins V at initialisation (dynex.cc:2694-2699)
+        xo[:op.n] = np.where(op.unit != 0, op.unit, xo[:op.n])
+    t_o, done = 0.0, 0
+    dev = []
+    try:
+        for cpnt in checkpoints:
+            k = cpnt - done
+            st.step(k)
+            tt, lev, loc = O.trajectory(op, oprm, xo, k, prec=prec)
+            t_o += tt
+            r = st.read()
+            assert r["steps"] == cpnt
+            assert r["t"] == t_o
+            assert r["level"] == lev[-1]
+            if prec == 64:
+                assert bits_equal(r["x"], xo), (cpnt, np.flatnonzero(r["x"] != xo)[:8])
+            else:
+                assert np.array_equal(r["x"].astype(np.float32), xo), cpnt
+            done = cpnt
+            dev.append(r["x"])
+    finally:
+        st.free()
+    return dev
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_trajectory_c1_1000_steps(ctx, prec):
+    cp, lits = gen(100, 800, 11)
+    gp, op = both(ctx, 100, cp, lits)
+    x0 = dnx.init_uniform(11, 100, 800)
+    run_traj(gp, op, x0, [1, 2, 10, 100, 500, 1000], prec)
+    gp.free()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_trajectory_mixed_k(ctx, prec):
+    n, cp, lits = mixed_problem(5)
+    gp, op = both(ctx, n, cp, lits)
+    x0 = random_state(np.random.default_rng(6), n, len(cp) - 1)
+    run_traj(gp, op, x0, [1, 3, 50, 300], prec)
+    gp.free()
+
+
+def test_trajectory_fixture_seed0(ctx, cnf_dir):
+    path = cnf_path(cnf_dir, 1000)
+    gp, op = ctx.problem_from_dimacs(path), O.Problem.from_dimacs(path)
+    x0 = O.init_ref(op, 0)
+    run_traj(gp, op, x0, [1, 5, 60, 200], 64)
+    gp.free()
+
+
+def test_trajectory_c2_fp64_1000_steps(ctx):
+    """n=10,000, m=80,000: 1000 steps bit-identical to the oracle (well inside
+    the 1e-9 relative bound the path is specified to)."""
+    cp, lits = gen(10000, 80000, 21)
+    gp, op = both(ctx, 10000, cp, lits)
+    x0 = dnx.init_uniform(21, 10000, 80000)
+    run_traj(gp, op, x0, [1, 10, 100, 1000], 64)
+    gp.free()
+
+
+def test_trajectory_c2_fp32_vs_fp64(ctx):
+    """FP32 is bit-identical to the float oracle; its distance from the FP64
+    trajectory is reported (not a parity bound)."""
+    cp, lits = gen(10000, 80000, 22)
+    gp, op = both(ctx, 10000, cp, lits)
+    x0 = dnx.init_uniform(22, 10000, 80000)
+    x32 = run_traj(gp, op, x0, [10, 100, 200], 32)
+    s64 = gp.state(dnx.params(80000), x0, dnx.FP64)
+    try:
+        for k, x in zip([10, 100, 200], x32):
+            s64.step(k - (0 if k == 10 else {100: 10, 200: 100}[k]))
+            ref = s64.read()["x"]
+            dv = np.max(np.abs(x[:10000] - ref[:10000]))
+            print(f"step {k}: max |V32 - V64| = {dv:.3e}")
+    finally:
+        s64.free()
+    gp.free()
+
+
+# ---------------------------------------------------------------------------
+# solving: stop at the first verified state (dynex.cc:662-703)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("nv", sorted(CNF_NAMES))
+def test_solve_fixtures_fp64_matches_oracle(ctx, cnf_dir, nv):
+    path = cnf_path(cnf_dir, nv)
+    gp, op = ctx.problem_from_dimacs(path), O.Problem.from_dimacs(path)
+    x0 = O.init_ref(op, 0)
+    r = gp.solve(dnx.params(gp.m), x0, dnx.FP64, cap=100000)
+    assert r["status"] == dnx.SAT
+    assert gp.check_assignment(r["assign"]) == 0
+    if nv <= 1000:
+        xo = x0.copy()
+        steps, t, status = O.solve(op, O.params(op.m), xo, 100000)
+        assert status == 1 and r["steps"] == steps and r["t"] == t
+        assert np.array_equal(r["assign"], np.where(xo[:op.n] >= 0, 1, -1))
+    gp.free()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_solve_generated_matches_oracle(ctx, prec):
+    for seed, (n, m) in enumerate([(100, 800), (500, 2150), (2000, 12000)]):
+        cp, lits = gen(n, m, 100 + seed)
+        gp, op = both(ctx, n, cp, lits)
+        x0 = dnx.init_uniform(100 + seed, n, m)
+        r = gp.solve(dnx.params(m), x0, dnx.FP64 if prec == 64 else dnx.FP32, cap=20000)
+        xo = x0.copy() if prec == 64 else x0.astype(np.float32)
+        steps, t, status = O.solve(op, O.params(m), xo, 20000, prec=prec)
+        assert r["status"] == (dnx.SAT if status == 1 else dnx.UNKNOWN)
+        assert r["steps"] == steps and r["t"] == t
+        if status == 1:
+            assert gp.check_assignment(r["assign"]) == 0
+        gp.free()
+
+
+def test_step_cap_reports_unknown(ctx):
+    cp, lits = gen(10000, 80000, 5)
+    gp = ctx.problem_from_lits(10000, cp, lits)
+    x0 = dnx.init_uniform(5, 10000, 80000)
+    for cap in (0, 1, 4, 31):
+        r = gp.solve(dnx.params(80000), x0, dnx.FP64, cap=cap)
+        assert r["status"] == dnx.UNKNOWN and r["steps"] == cap
+    gp.free()
+
+
+def test_cli_solves_fixture(cnf_dir, tmp_path):
+    src = cnf_path(cnf_dir, 200)
+    path = str(tmp_path / os.path.basename(src))
+    with open(src) as f, open(path, "w") as g:
+        g.write(f.read())
+    exe = os.path.join(dnx.HERE, "lib", "dynex")
+    r = subprocess.run([exe, "-i", path], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "s [0] SATISFIABLE" in r.stdout and "SAT (VERIFIED)" in r.stdout
+    vals = [int(t) for ln in r.stdout.splitlines() if ln.startswith("v ") for t in ln[2:].split()]
+    assert vals[-1] == 0 and len(vals) == 201
+    sol = open(path + ".solution.txt").read().replace(",", " ").split()
+    op = O.Problem.from_dimacs(path)
+    a = np.zeros(op.n)
+    for t in sol:
+        a[abs(int(t)) - 1] = 1.0 if int(t) > 0 else -1.0
+    assert O.verify(op, np.concatenate([a, np.zeros(2 * op.m)])) == 0
+    assert r.stdout.rstrip().endswith("0")
