@@ -1,0 +1,251 @@
+#!/usr/bin/env python3
+"""Benchmark of the memcomputing SAT integrator on one MI355X.
+
+Workload (default): 4,096 independent planted 3-SAT instances, n log-uniform
+in [100, 5000], clause ratio uniform in [4.3, 8.0], P(t true literals) =
+0.46/0.46/0.08, initial v ~ U[-1,1), Xs = 0.5, Xl = 1, step cap 100,000.
+A bench "step" is one batched FP32 solve of the whole set on the device with
+its inputs already resident; `value` is verified solved instances per second
+of device run time.  The same JSON line carries p50/p99 time-to-solution,
+the plain FP64 GPU solver (one instance at a time) on a sample of the same
+instances, the n=100,000 single-instance step against the HBM roofline, the
+fused kernel's roofline figures and the CPU oracle as baseline.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dynex-neuromorphic-chip_amd"))
+import dnx  # noqa: E402
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "solved instances/s + p50/p99 time-to-solution (batched FP32 vs plain FP64, 1 GPU)"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def bytes_alg(n, m):
+    """Compulsory bytes of one FP32 step: int32 literals read once (12m), Xs
+    and Xl read and written (16m), V read and written (8n)."""
+    return 28 * m + 8 * n
+
+
+def make_workload(count, seed):
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(math.log(100), math.log(5000), count)).round().astype(int)
+    rs = rng.uniform(4.3, 8.0, count)
+    out = []
+    for i in range(count):
+        n = int(ns[i])
+        m = int(round(rs[i] * n))
+        s = seed * 100003 + i
+        lits = dnx.generate_planted(s, n, m)
+        cp = np.arange(0, 3 * m + 1, 3, dtype=np.int32)
+        out.append(dict(n=n, m=m, cp=cp, lits=lits, x0=dnx.init_uniform(s, n, m)))
+    return out
+
+
+def percentile_ms(vals, q):
+    return float(np.percentile(np.asarray(vals), q) * 1e3) if len(vals) else None
+
+
+def batched_solve(ctx, work, hps, cap, spl, timing):
+    b = dnx.Batch(ctx, slots=len(work))
+    try:
+        slots = [b.admit(hp, dnx.params(w["m"]), w["x0"], cap=cap) for w, hp in zip(work, hps)]
+        if timing:
+            b.timing(1)
+        t_start, rate = b.stamp()
+        t0 = time.perf_counter()
+        b.run(steps_per_launch=spl)
+        wall = time.perf_counter() - t0
+        ms, nl = b.timing(0 if timing else -1)
+        res = b.poll()
+        st = res["status"][slots]
+        steps = res["steps"][slots]
+        tts = (res["t_done"][slots].astype(np.float64) - float(t_start)) / rate
+        verified = np.zeros(len(slots), bool)
+        for k, s in enumerate(slots):
+            if st[k] == dnx.SAT:
+                _, unsat = b.assignment(s, work[k]["n"])
+                verified[k] = unsat == 0
+        modes = np.array([b.info(s)[2] for s in slots])
+    finally:
+        b.close()
+    return dict(wall=wall, status=st, steps=steps, tts=tts, verified=verified, ms=ms, launches=nl, modes=modes)
+
+
+def plain_fp64(ctx, work, idx, cap, budget_s):
+    done, solved, t_used, steps = 0, 0, 0.0, []
+    for i in idx:
+        w = work[i]
+        gp = ctx.problem_from_lits(w["n"], w["cp"], w["lits"])
+        t0 = time.perf_counter()
+        r = gp.solve(dnx.params(w["m"]), w["x0"], dnx.FP64, cap=cap)
+        t_used += time.perf_counter() - t0
+        if r["status"] == dnx.SAT and gp.check_assignment(r["assign"]) == 0:
+            solved += 1
+        gp.free()
+        done += 1
+        steps.append(r["steps"])
+        if t_used > budget_s:
+            break
+    return dict(instances=done, solved=solved, seconds=t_used, ips=solved / t_used if t_used else 0.0,
+                idx=list(idx[:done]), steps=steps)
+
+
+def single_instance(ctx, n, m, prec, nsteps):
+    lits = dnx.generate_planted(90001, n, m)
+    cp = np.arange(0, 3 * m + 1, 3, dtype=np.int32)
+    gp = ctx.problem_from_lits(n, cp, lits)
+    st = gp.state(dnx.params(m), dnx.init_uniform(90001, n, m), prec)
+    st.time_steps(30)
+    ms = st.time_steps(nsteps)
+    st.free()
+    gp.free()
+    us = ms * 1e3 / nsteps
+    bpe = bytes_alg(n, m) if prec == dnx.FP32 else 44 * m + 16 * n
+    return dict(n=n, m=m, us_per_step=us, achieved_GBps=bpe / (us * 1e-6) / 1e9,
+                frac=bpe / (us * 1e-6) / 1e9 / HBM_PEAK_GBPS, bytes_per_step=bpe)
+
+
+def cpu_baseline(work, idx, cap, budget_s):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    t_used, done, solved = 0.0, 0, 0
+    for i in idx:
+        w = work[i]
+        op = O.Problem(w["n"], w["m"], w["cp"], w["lits"])
+        x = w["x0"].copy()
+        t0 = time.perf_counter()
+        steps, t, status = O.solve(op, O.params(w["m"]), x, cap)
+        t_used += time.perf_counter() - t0
+        done += 1
+        solved += status == 1
+        if t_used > budget_s:
+            break
+    return dict(value=solved / t_used if t_used else 0.0, instances=done, solved=solved, seconds=t_used)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--instances", type=int, default=4096)
+    ap.add_argument("--cap", type=int, default=100000)
+    ap.add_argument("--steps-per-launch", type=int, default=512)
+    ap.add_argument("--fp64-sample", type=int, default=64)
+    ap.add_argument("--fp64-seconds", type=float, default=25.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--seed", type=int, default=4096)
+    ap.add_argument("--skip-single", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only; the batched instances would shard by instance "
+              "across GPUs with no per-step communication (see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    t0 = time.perf_counter()
+    work = make_workload(args.instances, args.seed)
+    hps = [dnx.host_problem(w["n"], w["cp"], w["lits"]) for w in work]
+    log(f"workload: {len(work)} instances, sum n={sum(w['n'] for w in work)}, "
+        f"sum m={sum(w['m'] for w in work)} ({time.perf_counter() - t0:.1f}s to build)")
+
+    ctx = dnx.Context(0)
+    runs = []
+    for it in range(args.warmup + args.steps):
+        r = batched_solve(ctx, work, hps, args.cap, args.steps_per_launch, timing=it >= args.warmup)
+        tag = "warmup" if it < args.warmup else "timed"
+        log(f"{tag} batch {it}: {r['wall'] * 1e3:.1f} ms, solved {int((r['status'] == dnx.SAT).sum())}, "
+            f"verified {int(r['verified'].sum())}")
+        if it >= args.warmup:
+            runs.append(r)
+    wall = np.array([r["wall"] for r in runs])
+    last = runs[-1]
+    solved_verified = int(last["verified"].sum())
+    value = float(np.mean([r["verified"].sum() / r["wall"] for r in runs]))
+    tts = last["tts"][last["verified"]]
+
+    # dominant kernel: the execution shape with the most device time
+    ms_sum = np.sum([r["ms"] for r in runs], axis=0)
+    nl_sum = np.sum([r["launches"] for r in runs], axis=0)
+    dom = int(np.argmax(ms_sum))
+    alg = 0.0
+    for r in runs:
+        sel = r["modes"] == dom
+        ns = np.array([w["n"] for w in work])[sel]
+        ms_ = np.array([w["m"] for w in work])[sel]
+        alg += float(np.sum((r["steps"][sel] + 1) * (28 * ms_ + 8 * ns)))
+    kern_s = ms_sum[dom] / 1e3
+    achieved = alg / kern_s / 1e9 if kern_s > 0 else 0.0
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBPS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBPS, 4), traffic=None,
+                    kernel=f"k_batch<mode {dom}>", launches=int(nl_sum[dom]),
+                    avg_launch_ms=round(ms_sum[dom] / max(1, nl_sum[dom]), 3),
+                    alg_bytes_per_launch=round(alg / max(1, nl_sum[dom])))
+
+    rng = np.random.default_rng(7)
+    sample = rng.permutation(len(work))[:args.fp64_sample]
+    log("plain FP64 GPU solver on a sample ...")
+    p64 = plain_fp64(ctx, work, sample, args.cap, args.fp64_seconds)
+    fp32_rate = float(np.mean(last["verified"][p64["idx"]])) if p64["idx"] else None
+    fp64_rate = p64["solved"] / p64["instances"] if p64["instances"] else None
+
+    single = None
+    if not args.skip_single:
+        log("single instance n=100k ...")
+        single = dict(fp32=single_instance(ctx, 100000, 800000, dnx.FP32, 300),
+                      fp64=single_instance(ctx, 100000, 800000, dnx.FP64, 100))
+    ctx.close()
+
+    log("CPU oracle baseline ...")
+    cpu = cpu_baseline(work, sample, args.cap, args.cpu_seconds)
+    cpu_line = dict(value=round(cpu["value"], 3), unit="instances/s", cores=1, kind="port",
+                    sample=f"{cpu['instances']} instances of the same workload (random sample), FP64 oracle, "
+                           f"1 thread, {cpu['seconds']:.1f}s")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "instances/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(float(wall.mean() * 1e3), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "mixed planted 3-SAT, 4096 instances: n~LogU[100,5000], m=round(U[4.3,8]*n), "
+                        "P(t)=.46/.46/.08, v~U[-1,1), Xs=0.5, Xl=1, step cap 100k, batched FP32",
+            "instances": len(work),
+            "steps_per_launch": args.steps_per_launch,
+        },
+        "solved": int((last["status"] == dnx.SAT).sum()),
+        "verified": solved_verified,
+        "tts_p50_ms": percentile_ms(tts, 50),
+        "tts_p99_ms": percentile_ms(tts, 99),
+        "plain_fp64": dict(instances=p64["instances"], solved=p64["solved"], seconds=round(p64["seconds"], 3),
+                           instances_per_s=round(p64["ips"], 2)),
+        "speedup_vs_plain_fp64": round(value / p64["ips"], 1) if p64["ips"] else None,
+        "solve_rate_fp32_vs_fp64_on_sample": [fp32_rate, fp64_rate],
+        "single_instance_n100k": single,
+        "roofline": roofline,
+        "cpu_baseline": cpu_line,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

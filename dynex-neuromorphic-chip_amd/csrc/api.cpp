@@ -29,7 +29,7 @@ DevProb dev_prob(const dnx_problem *p)
     d.max_deg = p->hp.max_deg;
     d.cp = p->hp.uniform3 ? nullptr : p->d_cp;
     d.lits = p->d_lits;
-    d.lpos = p->d_lpos;
+    d.eidx = p->d_eidx;
     d.row_ptr = p->d_row_ptr;
     d.unit = p->hp.has_unit ? p->d_unit : nullptr;
     return d;
@@ -74,7 +74,7 @@ int upload_problem(dnx_problem *p)
     int rc;
     if ((rc = p->b_cp.alloc(sizeof(int32_t) * h.cp.size())) ||
         (rc = p->b_lits.alloc(sizeof(int32_t) * h.lits.size())) ||
-        (rc = p->b_lpos.alloc(sizeof(int32_t) * h.lpos.size())) ||
+        (rc = p->b_eidx.alloc(sizeof(int32_t) * h.eidx.size())) ||
         (rc = p->b_row.alloc(sizeof(int32_t) * h.row_ptr.size())) ||
         (rc = p->b_unit.alloc(h.unit.size())))
         return rc;
@@ -86,14 +86,14 @@ int upload_problem(dnx_problem *p)
         return hip_fail(e, "upload");
     UP(p->b_cp, h.cp)
     UP(p->b_lits, h.lits)
-    UP(p->b_lpos, h.lpos)
+    UP(p->b_eidx, h.eidx)
     UP(p->b_row, h.row_ptr)
     UP(p->b_unit, h.unit)
 #undef UP
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "upload sync");
     p->d_cp = (int32_t *)p->b_cp.ptr;
     p->d_lits = (int32_t *)p->b_lits.ptr;
-    p->d_lpos = (int32_t *)p->b_lpos.ptr;
+    p->d_eidx = (int32_t *)p->b_eidx.ptr;
     p->d_row_ptr = (int32_t *)p->b_row.ptr;
     p->d_unit = (int8_t *)p->b_unit.ptr;
     return DNX_OK;
@@ -129,15 +129,7 @@ static void init_ctl(Ctl &c)
     memset(&c, 0, sizeof(c));
     c.status = DNX_RUNNING;
     c.stop_on_solve = 1;
-    c.steps = 0;
     c.cap = LLONG_MAX;
-    c.t = 0.0;
-    for (int i = 0; i < 3; i++) {
-        c.loc[i] = 0;
-        c.vbad[i] = 0;
-        c.level[i] = -1;
-        c.done[i] = 0;
-    }
 }
 
 static int state_enqueue_steps(dnx_state *s, long long count)
@@ -181,6 +173,25 @@ static int state_enqueue_steps(dnx_state *s, long long count)
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "step launch");
     return DNX_OK;
+}
+
+// clauses of the state x (reference layout) with C >= 1/2, i.e. whose
+// literals all have q*clamp(V) <= 0 (dynex.cc:384-387, 499)
+static int count_loc(const HostProblem &h, const double *x)
+{
+    int loc = 0;
+    for (int c = 0; c < h.m; c++) {
+        bool unsat = true;
+        if (h.cp[(size_t)c + 1] - h.cp[(size_t)c] == 1) unsat = false;  // C = 0 (dynex.cc:517)
+        for (int q = h.cp[(size_t)c]; unsat && q < h.cp[(size_t)c + 1]; q++) {
+            int32_t lit = h.lits[(size_t)q];
+            double v = x[std::abs(lit) - 1];
+            v = v < -1.0 ? -1.0 : (v > 1.0 ? 1.0 : v);
+            if ((lit > 0 ? v : -v) > 0.0) unsat = false;
+        }
+        loc += unsat;
+    }
+    return loc;
 }
 
 // status of a control-block snapshot: the stored status, or the `done`
@@ -356,7 +367,7 @@ int dnx_rhs(dnx_problem *p, int prec, const dnx_params *prm, const double *x, do
     DevBuf bx, bd, be, bloc, bep;
     int rc;
     if ((rc = bx.alloc(N * es)) || (rc = bd.alloc(N * es)) || (rc = be.alloc((size_t)h.nnz() * es + es)) ||
-        (rc = bloc.alloc(sizeof(int))) || (rc = bep.alloc(sizeof(double) * (size_t)nb)))
+        (rc = bloc.alloc(sizeof(int) * (size_t)nb)) || (rc = bep.alloc(sizeof(double) * (size_t)nb)))
         return rc;
     hipStream_t st = p->ctx->stream;
     std::vector<char> stage;
@@ -364,23 +375,26 @@ int dnx_rhs(dnx_problem *p, int prec, const dnx_params *prm, const double *x, do
     hipError_t e;
     if (N && (e = hipMemcpyAsync(bx.ptr, stage.data(), N * es, hipMemcpyHostToDevice, st)) != hipSuccess)
         return hip_fail(e, "rhs upload");
-    if ((e = hipMemsetAsync(bloc.ptr, 0, sizeof(int), st)) != hipSuccess) return hip_fail(e, "memset");
+    if ((e = hipMemsetAsync(bloc.ptr, 0, sizeof(int) * (size_t)nb, st)) != hipSuccess) return hip_fail(e, "memset");
     if ((e = hipMemsetAsync(bep.ptr, 0, sizeof(double) * (size_t)nb, st)) != hipSuccess)
         return hip_fail(e, "memset");
-    launch_rhs(prec, dev_prob(p), dev_params(prm), bx.ptr, bd.ptr, be.ptr, (int *)bloc.ptr, (double *)bep.ptr, nb,
-               st);
+    launch_rhs(prec, dev_prob(p), dev_params(prm), bx.ptr, bd.ptr, be.ptr, (int *)bloc.ptr, (double *)bep.ptr, st);
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "rhs launch");
     std::vector<double> ep((size_t)nb);
-    int hloc = 0;
+    std::vector<int> lp((size_t)nb);
     if (N && (e = hipMemcpyAsync(stage.data(), bd.ptr, N * es, hipMemcpyDeviceToHost, st)) != hipSuccess)
         return hip_fail(e, "rhs download");
-    if ((e = hipMemcpyAsync(&hloc, bloc.ptr, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess ||
+    if ((e = hipMemcpyAsync(lp.data(), bloc.ptr, sizeof(int) * (size_t)nb, hipMemcpyDeviceToHost, st)) != hipSuccess ||
         (e = hipMemcpyAsync(ep.data(), bep.ptr, sizeof(double) * (size_t)nb, hipMemcpyDeviceToHost, st)) !=
             hipSuccess)
         return hip_fail(e, "rhs download");
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "rhs sync");
     unpack_host(prec, stage, N, dxdt);
-    if (loc) *loc = hloc;
+    if (loc) {
+        int tot = 0;
+        for (int v : lp) tot += v;
+        *loc = tot;
+    }
     if (energy) {
         double s = 0.0;
         for (double v : ep) s += v;
@@ -496,6 +510,34 @@ int dnx_state_step(dnx_state *s, int nsteps)
     return state_enqueue_steps(s, nsteps);
 }
 
+int dnx_state_time(dnx_state *s, int nsteps, double *ms)
+{
+    if (!s || nsteps <= 0 || !ms) return DNX_E_ARG;
+    (void)hipSetDevice(s->p->ctx->device);
+    int rc = set_mode(s, 0, LLONG_MAX);
+    if (rc) return rc;
+    hipStream_t st = s->p->ctx->stream;
+    hipEvent_t a, b;
+    hipError_t e;
+    if ((e = hipEventCreate(&a)) != hipSuccess) return hip_fail(e, "event");
+    if ((e = hipEventCreate(&b)) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return hip_fail(e, "event");
+    }
+    e = hipEventRecord(a, st);
+    if (e == hipSuccess) rc = state_enqueue_steps(s, nsteps);
+    if (e == hipSuccess && rc == DNX_OK) e = hipEventRecord(b, st);
+    if (e == hipSuccess && rc == DNX_OK) e = hipEventSynchronize(b);
+    float f = 0.0f;
+    if (e == hipSuccess && rc == DNX_OK) e = hipEventElapsedTime(&f, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "state timing");
+    *ms = (double)f;
+    return DNX_OK;
+}
+
 int dnx_state_read(dnx_state *s, double *x, double *t, int64_t *steps, int *last_level, int *loc)
 {
     if (!s) return DNX_E_ARG;
@@ -522,12 +564,12 @@ int dnx_state_read(dnx_state *s, double *x, double *t, int64_t *steps, int *last
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "state sync");
     if (x && N) unpack_host(s->prec, stage, N, x);
     const int sl = (int)(c.steps % 3);
-    const int plev = (c.status == DNX_RUNNING && !c.done[sl]) ? c.level[sl] : -1;
+    const int plev = (c.status == DNX_RUNNING && !c.done[sl] && c.stepped[sl]) ? ctl_level(c, sl) : -1;
     static const double dts[5] = {0.125, 0.0625, 0.03125, 0.015625, 0.0078125};
     if (t) *t = c.t + (plev >= 0 ? dts[plev] : 0.0);
     if (steps) *steps = c.steps + (plev >= 0 ? 1 : 0);
     if (last_level) *last_level = plev;
-    if (loc) *loc = c.loc[sl];
+    if (loc) *loc = (x && N) ? count_loc(h, x) : -1;
     return DNX_OK;
 }
 

@@ -40,8 +40,8 @@ struct dnx_ctx {
 struct dnx_problem {
     dnx_ctx *ctx = nullptr;
     dnx::HostProblem hp;
-    dnx::DevBuf b_cp, b_lits, b_lpos, b_row, b_unit;
-    int32_t *d_cp = nullptr, *d_lits = nullptr, *d_lpos = nullptr, *d_row_ptr = nullptr;
+    dnx::DevBuf b_cp, b_lits, b_eidx, b_row, b_unit;
+    int32_t *d_cp = nullptr, *d_lits = nullptr, *d_eidx = nullptr, *d_row_ptr = nullptr;
     int8_t *d_unit = nullptr;
 };
 

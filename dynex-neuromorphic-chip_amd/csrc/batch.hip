@@ -1,0 +1,291 @@
+// batch.hip -- the fused step kernel of the batched engine (FP32).
+//
+// One workgroup owns one instance at a time and advances it up to K adaptive
+// steps entirely inside the kernel: clause pass, workgroup barrier, variable
+// pass, workgroup max of the ladder level, barrier, next step.  No grid-wide
+// synchronisation exists, so thousands of uneven instances run side by side
+// and each keeps its own step size, time and step count.  Workgroups take
+// instances from a list (largest first) through one atomic ticket per
+// instance.  The arithmetic is the single-instance FP32 arithmetic
+// (dmm_math.hpp) with the same clause-order fold, so an instance computes the
+// same bits alone, in a batch of any composition, and on the single-instance
+// path.
+#include <hip/hip_runtime.h>
+
+#include "batch.hpp"
+#include "dmm_math.hpp"
+
+#pragma clang fp contract(off)
+
+namespace dnx {
+
+template <int MODE, int BS>
+__global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, BDyn *__restrict__ dyn,
+                                              const int *__restrict__ work, int nwork, int *__restrict__ head,
+                                              int K)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_item;
+    __shared__ int s_lev[2];
+    const int tid = threadIdx.x;
+    for (;;) {
+        __syncthreads();  // previous item finished with LDS and s_item
+        if (tid == 0) {
+            s_item = atomicAdd(head, 1);
+            s_lev[0] = 0;
+            s_lev[1] = 0;
+        }
+        __syncthreads();
+        const int it = s_item;
+        if (it >= nwork) return;
+        const int slot = __builtin_amdgcn_readfirstlane(work[it]);
+        BDyn d = dyn[slot];
+        if (d.status != DNX_RUNNING) continue;
+        const BInst I = insts[slot];
+        const int n = I.n, m = I.m;
+        const bool general = (I.flags & BF_GENERAL) != 0;
+        PT<float> p;
+        p.alpha = I.alpha;
+        p.beta = I.beta;
+        p.gamma = I.gamma;
+        p.delta = I.delta;
+        p.eps = I.eps;
+        p.zeta = I.zeta;
+        p.xlmax = I.xlmax;
+
+        BF2 *pv = (MODE == BM_GLOBAL) ? I.vd : (BF2 *)smem;
+        float *se = (float *)(smem + ((size_t)n * sizeof(BF2) + 15) / 16 * 16);
+        if (MODE != BM_GLOBAL) {
+            for (int v = tid; v < n; v += BS) pv[v] = I.vd[v];
+            __syncthreads();
+        }
+        bool pend = d.plevel >= 0;
+        int plev = d.plevel;
+        long long steps = d.steps;
+        double t = d.t;
+        int done = 0;
+        for (int k = 0; k < K; k++) {
+            const float h = pend ? level_h<float>(plev) : 0.0f;
+            if (pend) {
+                steps += 1;
+                t += level_dt(plev);
+            }
+            // ---- clause pass --------------------------------------------
+            int unsolved = 0;
+            auto prevV = [&](int v) { return pv[v].v; };
+            auto curV = [&](int v) {
+                BF2 a = pv[v];
+                return pend ? clampV(a.v + h * a.d) : a.v;
+            };
+            for (int c = tid; c < m; c += BS) {
+                const int b = general ? I.cp[c] : 3 * c;
+                const int kk = general ? I.cp[c + 1] - b : 3;
+                const int32_t *cl = I.lits + b;
+                float xs = I.xs[c], xl = I.xl[c];
+                if (pend) {
+                    float Cp = general ? clauseK_C<float>(cl, kk, prevV) : clause3_C<float>(cl, prevV);
+                    memory_update(p, Cp, h, xs, xl);
+                    I.xs[c] = xs;
+                    I.xl[c] = xl;
+                }
+                const float cxs = clamp01(xs), cxl = clampXl(xl, p.xlmax);
+                bool sat;
+                float C;
+                if (MODE == BM_LDS_ALL) {
+                    const int32_t *lp = I.lpos + b;
+                    auto emit = [&](int j, float val) { se[lp[j]] = val; };
+                    C = general ? clauseK<float>(p, cl, kk, cxs, cxl, curV, emit, sat)
+                                : clause3<float>(p, cl, cxs, cxl, curV, emit, sat);
+                } else {
+                    float *ep = I.e + b;
+                    auto emit = [&](int j, float val) { ep[j] = val; };
+                    C = general ? clauseK<float>(p, cl, kk, cxs, cxl, curV, emit, sat)
+                                : clause3<float>(p, cl, cxs, cxl, curV, emit, sat);
+                }
+                unsolved |= !(C < 0.5f) || !sat;
+            }
+            const bool any_unsolved = __syncthreads_or(unsolved);
+            if (d.stop_on_solve && !any_unsolved)
+                done = DNX_SAT;
+            else if (steps >= d.cap)
+                done = DNX_UNKNOWN;
+            // ---- variable pass ------------------------------------------
+            int fmax = 0;
+            for (int v = tid; v < n; v += BS) {
+                BF2 a = pv[v];
+                const float V = pend ? clampV(a.v + h * a.d) : a.v;
+                float dV = 0.0f;
+                if (!done) {
+                    const int rb = I.row_ptr[v], re = I.row_ptr[v + 1];
+                    if (MODE == BM_LDS_ALL) {
+                        for (int q = rb; q < re; q++) dV = dV + se[q];
+                    } else {
+                        dV = fold_gather<float>(I.e, I.eidx, rb, re);
+                    }
+                    if ((I.flags & BF_UNIT) && I.unit[v] != 0) dV = 0.0f;  // dynex.cc:306
+                    fmax = max(fmax, fail_level(V, dV));
+                }
+                BF2 o;
+                o.v = V;
+                o.d = dV;
+                pv[v] = o;
+            }
+            if (done) break;
+            const int wm = wave_max(fmax);
+            if ((tid & 63) == 0 && wm > 0) atomicMax(&s_lev[k & 1], wm);
+            __syncthreads();
+            plev = s_lev[k & 1];
+            pend = true;
+            if (tid == 0) s_lev[(k + 1) & 1] = 0;
+        }
+        __syncthreads();
+        if (MODE != BM_GLOBAL)
+            for (int v = tid; v < n; v += BS) I.vd[v] = pv[v];
+        if (tid == 0) {
+            BDyn o = d;
+            o.steps = steps;
+            o.t = t;
+            if (done) {
+                o.status = done;
+                o.plevel = -1;
+                o.t_done = wall_clock64();
+            } else {
+                o.plevel = plev;
+            }
+            dyn[slot] = o;
+        }
+    }
+}
+
+__global__ void k_batch_zero(int *p, int count)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = 0;
+}
+
+__global__ void k_batch_stamp(unsigned long long *p)
+{
+    if (threadIdx.x == 0) *p = wall_clock64();
+}
+
+// reference-layout state of one slot with its pending step applied
+__global__ __launch_bounds__(256) void k_batch_materialize(const BInst *__restrict__ ip, const BDyn *__restrict__ dp,
+                                                           float *__restrict__ xo)
+{
+    const BInst I = *ip;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int n = I.n, m = I.m;
+    const int plev = dp->status == DNX_RUNNING ? dp->plevel : -1;
+    const bool pend = plev >= 0;
+    const float h = pend ? level_h<float>(plev) : 0.0f;
+    if (i < n) {
+        BF2 a = I.vd[i];
+        xo[i] = pend ? clampV(a.v + h * a.d) : a.v;
+    }
+    if (i < m) {
+        float xs = I.xs[i], xl = I.xl[i];
+        if (pend) {
+            PT<float> p;
+            p.alpha = I.alpha;
+            p.beta = I.beta;
+            p.gamma = I.gamma;
+            p.delta = I.delta;
+            p.eps = I.eps;
+            p.zeta = I.zeta;
+            p.xlmax = I.xlmax;
+            const bool general = (I.flags & BF_GENERAL) != 0;
+            const int b = general ? I.cp[i] : 3 * i;
+            auto getV = [&](int v) { return I.vd[v].v; };
+            float Cp = general ? clauseK_C<float>(I.lits + b, I.cp[i + 1] - b, getV)
+                               : clause3_C<float>(I.lits + b, getV);
+            memory_update(p, Cp, h, xs, xl);
+        }
+        xo[n + i] = xs;
+        xo[n + m + i] = xl;
+    }
+}
+
+template <int MODE, int BS>
+static void launch_one(int lds, int grid, const BInst *insts, BDyn *dyn, const int *work, int nwork, int *head,
+                       int K, hipStream_t st)
+{
+    static int lds_set = 0;
+    if (lds > 65536 && lds > lds_set) {
+        (void)hipFuncSetAttribute((const void *)k_batch<MODE, BS>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        lds_set = lds;
+    }
+    k_batch<MODE, BS><<<grid, BS, lds, st>>>(insts, dyn, work, nwork, head, K);
+}
+
+template <int MODE, int BS>
+static int occ_one(int lds)
+{
+    if (lds > 65536)
+        (void)hipFuncSetAttribute((const void *)k_batch<MODE, BS>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<MODE, BS>, BS, lds) != hipSuccess) return 0;
+    return nb;
+}
+
+template <int MODE>
+static void launch_mode(int bs, int lds, int grid, const BInst *insts, BDyn *dyn, const int *work, int nwork,
+                        int *head, int K, hipStream_t st)
+{
+    switch (bs) {
+        case 128: launch_one<MODE, 128>(lds, grid, insts, dyn, work, nwork, head, K, st); break;
+        case 256: launch_one<MODE, 256>(lds, grid, insts, dyn, work, nwork, head, K, st); break;
+        case 512: launch_one<MODE, 512>(lds, grid, insts, dyn, work, nwork, head, K, st); break;
+        default: launch_one<MODE, 1024>(lds, grid, insts, dyn, work, nwork, head, K, st); break;
+    }
+}
+
+template <int MODE>
+static int occ_mode(int bs, int lds)
+{
+    switch (bs) {
+        case 128: return occ_one<MODE, 128>(lds);
+        case 256: return occ_one<MODE, 256>(lds);
+        case 512: return occ_one<MODE, 512>(lds);
+        default: return occ_one<MODE, 1024>(lds);
+    }
+}
+
+int batch_max_threads() { return 1024; }
+
+int batch_occupancy(int mode, int bs, int lds)
+{
+    if (mode == BM_LDS_ALL) return occ_mode<BM_LDS_ALL>(bs, lds);
+    if (mode == BM_LDS_V) return occ_mode<BM_LDS_V>(bs, lds);
+    return occ_mode<BM_GLOBAL>(bs, 0);
+}
+
+void launch_batch(int mode, int bs, int lds_bytes, int grid, const BInst *insts, BDyn *dyn, const int *work,
+                  int nwork, int *head, int ksteps, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    if (nwork <= 0 || grid <= 0) return;
+    if (mode == BM_LDS_ALL)
+        launch_mode<BM_LDS_ALL>(bs, lds_bytes, grid, insts, dyn, work, nwork, head, ksteps, st);
+    else if (mode == BM_LDS_V)
+        launch_mode<BM_LDS_V>(bs, lds_bytes, grid, insts, dyn, work, nwork, head, ksteps, st);
+    else
+        launch_mode<BM_GLOBAL>(bs, 0, grid, insts, dyn, work, nwork, head, ksteps, st);
+}
+
+void launch_batch_zero(int *p, int count, void *stream)
+{
+    if (count > 0) k_batch_zero<<<(count + 255) / 256, 256, 0, (hipStream_t)stream>>>(p, count);
+}
+
+void launch_batch_stamp(unsigned long long *p, void *stream)
+{
+    k_batch_stamp<<<1, 64, 0, (hipStream_t)stream>>>(p);
+}
+
+void launch_batch_materialize(const BInst *inst, const BDyn *dyn, int n, int m, float *x_out, void *stream)
+{
+    int mx = n > m ? n : m;
+    if (mx > 0) k_batch_materialize<<<(mx + 255) / 256, 256, 0, (hipStream_t)stream>>>(inst, dyn, x_out);
+}
+
+}  // namespace dnx

@@ -1,0 +1,63 @@
+// batch.hpp -- device structures of the batched engine, shared by the
+// engine (batch_engine.cpp) and its kernels (batch.hip).  No HIP headers.
+#pragma once
+
+#include <cstdint>
+
+namespace dnx {
+
+// Execution shape of one instance inside the fused step kernel.
+//   0  V/dV pairs and the contribution array live in LDS
+//   1  V/dV pairs live in LDS, contributions in device memory
+//   2  both in device memory (instances too large for LDS)
+enum BatchMode { BM_LDS_ALL = 0, BM_LDS_V = 1, BM_GLOBAL = 2, BM_COUNT = 3 };
+
+struct BF2 {
+    float v, d;
+};
+
+// Read-only descriptor of an admitted instance (one per slot).
+struct BInst {
+    int n, m;
+    int mode;
+    int flags;                // BF_GENERAL: clause_ptr valid; BF_UNIT: pins
+    const int32_t *lits;      // signed 1-based literals, clause order
+    const int32_t *cp;        // m+1 offsets (general clause lengths only)
+    const int32_t *lpos;      // variable-major slot of each literal (LDS e)
+    const int32_t *eidx;      // literal of each variable-major slot (global e)
+    const int32_t *row_ptr;   // n+1
+    const int8_t *unit;       // n pins
+    float *xs, *xl;           // m each
+    BF2 *vd;                  // n resident [V, dV] pairs
+    float *e;                 // nnz contributions (modes 1, 2)
+    float alpha, beta, gamma, delta, eps, zeta, xlmax, pad;
+};
+enum { BF_GENERAL = 1, BF_UNIT = 2 };
+
+// Mutable per-slot state.  `steps`, `t` and `plevel` follow the same lazy
+// convention as the single-instance control block: the stored V/Xs/Xl are
+// state `steps`; when plevel >= 0 the step from it at that ladder level has
+// been decided and is applied at the start of the next launch.
+struct BDyn {
+    int status;               // DNX_RUNNING / DNX_SAT / DNX_UNKNOWN, -1 free slot
+    int plevel;
+    int stop_on_solve;
+    int loc;                  // unused by the kernel, kept for alignment
+    long long steps;
+    long long cap;
+    double t;
+    unsigned long long t_done;  // device wall clock when the instance ended
+};
+
+// one launch of the fused kernel over `nwork` slots listed in `work`
+void launch_batch(int mode, int bs, int lds_bytes, int grid, const BInst *insts, BDyn *dyn,
+                  const int *work, int nwork, int *head, int ksteps, void *stream);
+void launch_batch_zero(int *p, int count, void *stream);
+void launch_batch_stamp(unsigned long long *p, void *stream);
+// materialised state of one slot in the reference layout (float)
+void launch_batch_materialize(const BInst *inst, const BDyn *dyn, int n, int m, float *x_out, void *stream);
+int batch_max_threads();
+// resident workgroups per CU for a shape (0 if it cannot launch)
+int batch_occupancy(int mode, int bs, int lds_bytes);
+
+}  // namespace dnx

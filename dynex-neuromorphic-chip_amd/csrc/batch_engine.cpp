@@ -1,0 +1,689 @@
+// batch_engine.cpp -- the batched engine behind dnx_batch_* (include/dnx.h).
+//
+// Instances live in slots.  Each admitted instance gets one contiguous range
+// of a device arena (first-fit, coalescing free list) holding its clause
+// table, transpose, state and contribution buffer, and a descriptor in the
+// slot table.  An advance launches the fused step kernel once per execution
+// shape over the slots that are still running, largest first; a poll copies
+// the per-slot status table back through pinned memory.  Solved instances
+// stay in their slot until retired, which frees the arena range at once.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "api_internal.hpp"
+#include "batch.hpp"
+
+namespace dnx {
+
+static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// First-fit allocator over [0, cap) with coalescing on release.
+struct Arena {
+    size_t cap = 0;
+    std::map<size_t, size_t> free_;  // offset -> size
+    void reset(size_t c)
+    {
+        cap = c;
+        free_.clear();
+        if (c) free_[0] = c;
+    }
+    bool alloc(size_t sz, size_t &off)
+    {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= sz) {
+                off = it->first;
+                size_t rest = it->second - sz;
+                free_.erase(it);
+                if (rest) free_[off + sz] = rest;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t sz)
+    {
+        auto it = free_.emplace(off, sz).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    size_t free_bytes() const
+    {
+        size_t s = 0;
+        for (auto &kv : free_) s += kv.second;
+        return s;
+    }
+};
+
+struct SlotHost {
+    bool used = false;
+    int n = 0, m = 0;
+    int64_t nnz = 0;
+    int mode = 0;
+    size_t off = 0, size = 0;
+    std::vector<int32_t> cp, lits;  // kept for the host-side clause check
+};
+
+struct Shape {
+    int bs;
+    int lds_cap;  // bytes of LDS the shape may use per workgroup
+};
+
+}  // namespace dnx
+
+using namespace dnx;
+
+struct dnx_batch {
+    dnx_ctx *ctx = nullptr;
+    int nslots = 0;
+    Arena arena;
+    void *arena_base = nullptr;
+    DevBuf b_inst, b_dyn, b_work, b_head, b_stamp;
+    std::vector<BInst> h_inst;
+    std::vector<SlotHost> slots;
+    std::vector<int> free_slots;
+    BDyn *h_dyn = nullptr;  // pinned: [2][nslots]
+    int *h_work = nullptr;  // pinned: [2][BM_COUNT * nslots]
+    Shape shape[BM_COUNT] = {{256, 48 * 1024}, {256, 40 * 1024}, {1024, 0}};
+    int cus = 0, lds_max = 0;
+    double clock_hz = 1e8;
+    int buf = 0;            // alternate pinned buffer
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool ev_pending[2] = {false, false};
+    hipStream_t mstream[BM_COUNT] = {nullptr, nullptr, nullptr};
+    hipEvent_t fork = nullptr, join[BM_COUNT] = {nullptr, nullptr, nullptr};
+    // optional per-shape launch timing (start/stop events around each launch)
+    bool timing = false;
+    hipEvent_t t0[BM_COUNT] = {nullptr, nullptr, nullptr}, t1[BM_COUNT] = {nullptr, nullptr, nullptr};
+    bool t_pending[BM_COUNT] = {false, false, false};
+    double t_ms[BM_COUNT] = {0, 0, 0};
+    long long t_launches[BM_COUNT] = {0, 0, 0};
+    ~dnx_batch()
+    {
+        if (arena_base) (void)hipFree(arena_base);
+        if (h_dyn) (void)hipHostFree(h_dyn);
+        if (h_work) (void)hipHostFree(h_work);
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (int i = 0; i < BM_COUNT; i++) {
+            if (mstream[i]) (void)hipStreamDestroy(mstream[i]);
+            if (join[i]) (void)hipEventDestroy(join[i]);
+            if (t0[i]) (void)hipEventDestroy(t0[i]);
+            if (t1[i]) (void)hipEventDestroy(t1[i]);
+        }
+        if (fork) (void)hipEventDestroy(fork);
+    }
+};
+
+namespace dnx {
+
+static int choose_mode(const dnx_batch *b, int n, int64_t nnz)
+{
+    size_t pv = ((size_t)n * sizeof(BF2) + 15) / 16 * 16;
+    if (pv + (size_t)nnz * 4 <= (size_t)b->shape[BM_LDS_ALL].lds_cap) return BM_LDS_ALL;
+    if (pv <= (size_t)b->shape[BM_LDS_V].lds_cap) return BM_LDS_V;
+    return BM_GLOBAL;
+}
+
+// Byte layout of one instance inside its arena range.
+struct Layout {
+    size_t lits, cp, lpos, eidx, row, unit, xs, xl, vd, e, total;
+};
+static Layout layout_of(int n, int m, int64_t nnz, bool general, bool has_unit, int mode)
+{
+    Layout L{};
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        size_t at = o;
+        o += al256(bytes);
+        return at;
+    };
+    L.lits = take((size_t)nnz * 4);
+    L.cp = general ? take(((size_t)m + 1) * 4) : 0;
+    L.lpos = mode == BM_LDS_ALL ? take((size_t)nnz * 4) : 0;
+    L.eidx = mode != BM_LDS_ALL ? take((size_t)nnz * 4) : 0;
+    L.row = take(((size_t)n + 1) * 4);
+    L.unit = has_unit ? take((size_t)n) : 0;
+    L.xs = take((size_t)m * 4);
+    L.xl = take((size_t)m * 4);
+    L.vd = take((size_t)n * sizeof(BF2));
+    L.e = mode != BM_LDS_ALL ? take((size_t)nnz * 4) : 0;
+    L.total = o ? o : 256;
+    return L;
+}
+
+// Adds the elapsed time of the last timed launch of each shape (the caller
+// has waited for the work).
+static void batch_collect_timing(dnx_batch *b)
+{
+    for (int md = 0; md < BM_COUNT; md++) {
+        if (!b->t_pending[md]) continue;
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, b->t0[md], b->t1[md]) == hipSuccess) {
+            b->t_ms[md] += ms;
+            b->t_launches[md] += 1;
+        }
+        b->t_pending[md] = false;
+    }
+}
+
+static int batch_sync(dnx_batch *b)
+{
+    hipError_t e = hipStreamSynchronize(b->ctx->stream);
+    return e == hipSuccess ? DNX_OK : hip_fail(e, "batch sync");
+}
+
+// Enqueue one advance of every running slot by up to ksteps steps.  The work
+// lists come from `known` (the last polled status); slots that ended since
+// are skipped by the kernel itself.
+static int batch_enqueue(dnx_batch *b, int ksteps, const BDyn *known)
+{
+    hipStream_t st = b->ctx->stream;
+    const int S = b->nslots;
+    int *hw = b->h_work + (size_t)b->buf * BM_COUNT * S;
+    int cnt[BM_COUNT] = {0, 0, 0};
+    std::vector<std::pair<int, int>> order;
+    order.reserve((size_t)S);
+    for (int s = 0; s < S; s++)
+        if (b->slots[(size_t)s].used && known[s].status == DNX_RUNNING) order.push_back({b->slots[(size_t)s].m, s});
+    std::sort(order.begin(), order.end(), [](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+        return x.first != y.first ? x.first > y.first : x.second < y.second;
+    });
+    for (auto &o : order) {
+        int md = b->slots[(size_t)o.second].mode;
+        hw[(size_t)md * S + cnt[md]++] = o.second;
+    }
+    if (order.empty()) return DNX_OK;
+    // a previous advance that used this pinned work buffer must have consumed it
+    int *dw = (int *)b->b_work.ptr + (size_t)b->buf * BM_COUNT * S;
+    int *dh = (int *)b->b_head.ptr + b->buf * BM_COUNT;
+    hipError_t e;
+    for (int md = 0; md < BM_COUNT; md++) {
+        if (!cnt[md]) continue;
+        if ((e = hipMemcpyAsync(dw + (size_t)md * S, hw + (size_t)md * S, sizeof(int) * cnt[md],
+                                hipMemcpyHostToDevice, st)) != hipSuccess)
+            return hip_fail(e, "work list upload");
+    }
+    launch_batch_zero(dh, BM_COUNT, st);
+    if ((e = hipEventRecord(b->fork, st)) != hipSuccess) return hip_fail(e, "fork");
+    for (int md = 0; md < BM_COUNT; md++) {
+        if (!cnt[md]) continue;
+        hipStream_t ms = b->mstream[md];
+        if ((e = hipStreamWaitEvent(ms, b->fork, 0)) != hipSuccess) return hip_fail(e, "fork wait");
+        const Shape &sh = b->shape[md];
+        int lds = 0;
+        if (md != BM_GLOBAL) {
+            // the largest LDS need among this shape's instances
+            for (int i = 0; i < cnt[md]; i++) {
+                const SlotHost &sl = b->slots[(size_t)hw[(size_t)md * S + i]];
+                size_t need = ((size_t)sl.n * sizeof(BF2) + 15) / 16 * 16 + (md == BM_LDS_ALL ? (size_t)sl.nnz * 4 : 0);
+                lds = std::max(lds, (int)need);
+            }
+        }
+        int per_cu = batch_occupancy(md, sh.bs, lds);
+        if (per_cu < 1) per_cu = 1;
+        int grid = std::min(cnt[md], b->cus * per_cu);
+        const bool timed = b->timing && !b->t_pending[md];
+        if (timed && (e = hipEventRecord(b->t0[md], ms)) != hipSuccess) return hip_fail(e, "timing");
+        launch_batch(md, sh.bs, lds, grid, (const BInst *)b->b_inst.ptr, (BDyn *)b->b_dyn.ptr, dw + (size_t)md * S,
+                     cnt[md], dh + md, ksteps, ms);
+        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "batch launch");
+        if (timed) {
+            if ((e = hipEventRecord(b->t1[md], ms)) != hipSuccess) return hip_fail(e, "timing");
+            b->t_pending[md] = true;
+        }
+        if ((e = hipEventRecord(b->join[md], ms)) != hipSuccess) return hip_fail(e, "join");
+        if ((e = hipStreamWaitEvent(st, b->join[md], 0)) != hipSuccess) return hip_fail(e, "join wait");
+    }
+    return DNX_OK;
+}
+
+}  // namespace dnx
+
+extern "C" {
+
+int dnx_batch_open(dnx_ctx *ctx, int prec, int slots, int64_t arena_bytes, dnx_batch **out)
+{
+    if (!ctx || !out || slots <= 0) return DNX_E_ARG;
+    if (prec != DNX_FP32) {
+        set_error("the batched engine runs in DNX_FP32");
+        return DNX_E_ARG;
+    }
+    (void)hipSetDevice(ctx->device);
+    hipError_t e;
+    size_t freeb = 0, totalb = 0;
+    if ((e = hipMemGetInfo(&freeb, &totalb)) != hipSuccess) return hip_fail(e, "hipMemGetInfo");
+    if (arena_bytes <= 0) arena_bytes = (int64_t)(0.8 * (double)totalb);
+    if ((size_t)arena_bytes > freeb) arena_bytes = (int64_t)(freeb - (freeb >> 4));
+    dnx_batch *b = new dnx_batch();
+    b->ctx = ctx;
+    b->nslots = slots;
+    int rc;
+    if ((rc = b->b_inst.alloc(sizeof(BInst) * (size_t)slots)) || (rc = b->b_dyn.alloc(sizeof(BDyn) * (size_t)slots)) ||
+        (rc = b->b_work.alloc(sizeof(int) * 2 * BM_COUNT * (size_t)slots)) ||
+        (rc = b->b_head.alloc(sizeof(int) * 2 * BM_COUNT)) || (rc = b->b_stamp.alloc(sizeof(unsigned long long)))) {
+        delete b;
+        return rc;
+    }
+    if ((e = hipMalloc(&b->arena_base, (size_t)arena_bytes)) != hipSuccess) {
+        b->arena_base = nullptr;
+        delete b;
+        set_error("arena hipMalloc(" + std::to_string(arena_bytes) + ") failed");
+        return DNX_E_NOMEM;
+    }
+    b->arena.reset((size_t)arena_bytes);
+    if ((e = hipHostMalloc((void **)&b->h_dyn, sizeof(BDyn) * 2 * (size_t)slots, hipHostMallocDefault)) !=
+            hipSuccess ||
+        (e = hipHostMalloc((void **)&b->h_work, sizeof(int) * 2 * BM_COUNT * (size_t)slots, hipHostMallocDefault)) !=
+            hipSuccess) {
+        delete b;
+        return hip_fail(e, "hipHostMalloc");
+    }
+    for (int i = 0; i < 2; i++)
+        if ((e = hipEventCreateWithFlags(&b->ev[i], hipEventDisableTiming)) != hipSuccess) {
+            delete b;
+            return hip_fail(e, "event");
+        }
+    if ((e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) != hipSuccess) {
+        delete b;
+        return hip_fail(e, "event");
+    }
+    for (int i = 0; i < BM_COUNT; i++)
+        if ((e = hipStreamCreateWithFlags(&b->mstream[i], hipStreamNonBlocking)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming)) != hipSuccess ||
+            (e = hipEventCreate(&b->t0[i])) != hipSuccess || (e = hipEventCreate(&b->t1[i])) != hipSuccess) {
+            delete b;
+            return hip_fail(e, "stream");
+        }
+    hipDeviceProp_t prop;
+    if ((e = hipGetDeviceProperties(&prop, ctx->device)) != hipSuccess) {
+        delete b;
+        return hip_fail(e, "hipGetDeviceProperties");
+    }
+    b->cus = prop.multiProcessorCount;
+    int lds_max = 0;
+    (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
+    b->lds_max = lds_max > 0 ? lds_max : 65536;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device) == hipSuccess && khz > 0)
+        b->clock_hz = (double)khz * 1e3;
+    for (int md = 0; md < BM_COUNT; md++)
+        if (b->shape[md].lds_cap > b->lds_max) b->shape[md].lds_cap = b->lds_max;
+    b->h_inst.assign((size_t)slots, BInst{});
+    b->slots.assign((size_t)slots, SlotHost{});
+    for (int s = slots - 1; s >= 0; s--) b->free_slots.push_back(s);
+    std::vector<BDyn> init((size_t)slots);
+    for (auto &d : init) {
+        memset(&d, 0, sizeof(d));
+        d.status = -1;
+        d.plevel = -1;
+    }
+    memcpy(b->h_dyn, init.data(), sizeof(BDyn) * (size_t)slots);
+    memcpy(b->h_dyn + slots, init.data(), sizeof(BDyn) * (size_t)slots);
+    if ((e = hipMemcpy(b->b_dyn.ptr, init.data(), sizeof(BDyn) * (size_t)slots, hipMemcpyHostToDevice)) !=
+        hipSuccess) {
+        delete b;
+        return hip_fail(e, "dyn init");
+    }
+    *out = b;
+    return DNX_OK;
+}
+
+int dnx_batch_close(dnx_batch *b)
+{
+    if (!b) return DNX_E_ARG;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipDeviceSynchronize();
+    delete b;
+    return DNX_OK;
+}
+
+int dnx_batch_set_shape(dnx_batch *b, int mode, int block, int lds_cap)
+{
+    if (!b || mode < 0 || mode >= BM_COUNT || (block != 128 && block != 256 && block != 512 && block != 1024))
+        return DNX_E_ARG;
+    for (auto &sl : b->slots)
+        if (sl.used) {
+            set_error("shapes can only change while the batch is empty");
+            return DNX_E_STATE;
+        }
+    b->shape[mode].bs = block;
+    if (mode != BM_GLOBAL) b->shape[mode].lds_cap = std::max(0, std::min(lds_cap, b->lds_max));
+    return DNX_OK;
+}
+
+int dnx_batch_admit(dnx_batch *b, const dnx_problem *p, const dnx_params *prm, const double *x0, int64_t step_cap,
+                    int stop_on_solve, int *slot_out)
+{
+    if (!b || !p || !prm || !x0 || !slot_out || step_cap < 0) return DNX_E_ARG;
+    if (b->free_slots.empty()) {
+        set_error("no free slot");
+        return DNX_E_BUDGET;
+    }
+    (void)hipSetDevice(b->ctx->device);
+    const HostProblem &h = p->hp;
+    const int n = h.n, m = h.m;
+    const int64_t nnz = h.nnz();
+    const bool general = !h.uniform3;
+    const int mode = choose_mode(b, n, nnz);
+    const Layout L = layout_of(n, m, nnz, general, h.has_unit, mode);
+    size_t off;
+    if (!b->arena.alloc(L.total, off)) {
+        set_error("instance needs " + std::to_string(L.total) + " bytes; arena has " +
+                  std::to_string(b->arena.free_bytes()) + " free in fragments");
+        return DNX_E_BUDGET;
+    }
+    const int slot = b->free_slots.back();
+    // stage the whole range on the host, then one copy
+    std::vector<unsigned char> stage(L.total, 0);
+    auto put = [&](size_t at, const void *src, size_t bytes) {
+        if (bytes) memcpy(stage.data() + at, src, bytes);
+    };
+    put(L.lits, h.lits.data(), (size_t)nnz * 4);
+    if (general) put(L.cp, h.cp.data(), ((size_t)m + 1) * 4);
+    if (mode == BM_LDS_ALL)
+        put(L.lpos, h.lpos.data(), (size_t)nnz * 4);
+    else
+        put(L.eidx, h.eidx.data(), (size_t)nnz * 4);
+    put(L.row, h.row_ptr.data(), ((size_t)n + 1) * 4);
+    if (h.has_unit) put(L.unit, h.unit.data(), (size_t)n);
+    float *xs = (float *)(stage.data() + L.xs), *xl = (float *)(stage.data() + L.xl);
+    for (int c = 0; c < m; c++) {
+        xs[c] = (float)x0[(size_t)n + c];
+        xl[c] = (float)x0[(size_t)n + m + c];
+    }
+    BF2 *vd = (BF2 *)(stage.data() + L.vd);
+    for (int v = 0; v < n; v++) {  // unit pins at admission (dynex.cc:2694-2699)
+        vd[v].v = h.unit[(size_t)v] != 0 ? (float)h.unit[(size_t)v] : (float)x0[v];
+        vd[v].d = 0.0f;
+    }
+    char *base = (char *)b->arena_base + off;
+    hipError_t e;
+    if ((e = hipMemcpyAsync(base, stage.data(), L.total, hipMemcpyHostToDevice, b->ctx->stream)) != hipSuccess) {
+        b->arena.release(off, L.total);
+        return hip_fail(e, "admit upload");
+    }
+    BInst I{};
+    I.n = n;
+    I.m = m;
+    I.mode = mode;
+    I.flags = (general ? BF_GENERAL : 0) | (h.has_unit ? BF_UNIT : 0);
+    I.lits = (const int32_t *)(base + L.lits);
+    I.cp = general ? (const int32_t *)(base + L.cp) : nullptr;
+    I.lpos = mode == BM_LDS_ALL ? (const int32_t *)(base + L.lpos) : nullptr;
+    I.eidx = mode != BM_LDS_ALL ? (const int32_t *)(base + L.eidx) : nullptr;
+    I.row_ptr = (const int32_t *)(base + L.row);
+    I.unit = h.has_unit ? (const int8_t *)(base + L.unit) : nullptr;
+    I.xs = (float *)(base + L.xs);
+    I.xl = (float *)(base + L.xl);
+    I.vd = (BF2 *)(base + L.vd);
+    I.e = mode != BM_LDS_ALL ? (float *)(base + L.e) : nullptr;
+    I.alpha = (float)prm->alpha;
+    I.beta = (float)prm->beta;
+    I.gamma = (float)prm->gamma;
+    I.delta = (float)prm->delta;
+    I.eps = (float)prm->epsilon;
+    I.zeta = (float)prm->zeta;
+    I.xlmax = (float)(double)prm->xl_max;
+    BDyn D;
+    memset(&D, 0, sizeof(D));
+    D.status = DNX_RUNNING;
+    D.plevel = -1;
+    D.stop_on_solve = stop_on_solve ? 1 : 0;
+    D.cap = step_cap;
+    b->h_inst[(size_t)slot] = I;
+    // the descriptor and status copies read from host memory that outlives
+    // the stream: h_inst is owned by the batch; D is staged in pinned memory
+    BDyn *pd0 = b->h_dyn + slot, *pd1 = b->h_dyn + b->nslots + slot;
+    *pd0 = D;
+    *pd1 = D;
+    if ((e = hipMemcpyAsync((BInst *)b->b_inst.ptr + slot, &b->h_inst[(size_t)slot], sizeof(BInst),
+                            hipMemcpyHostToDevice, b->ctx->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync((BDyn *)b->b_dyn.ptr + slot, pd0, sizeof(BDyn), hipMemcpyHostToDevice,
+                            b->ctx->stream)) != hipSuccess) {
+        b->arena.release(off, L.total);
+        return hip_fail(e, "admit descriptor");
+    }
+    // the staging vector is released on return: wait for its copy
+    if ((e = hipStreamSynchronize(b->ctx->stream)) != hipSuccess) return hip_fail(e, "admit sync");
+    SlotHost &sl = b->slots[(size_t)slot];
+    sl.used = true;
+    sl.n = n;
+    sl.m = m;
+    sl.nnz = nnz;
+    sl.mode = mode;
+    sl.off = off;
+    sl.size = L.total;
+    sl.cp = h.cp;
+    sl.lits = h.lits;
+    b->free_slots.pop_back();
+    *slot_out = slot;
+    return DNX_OK;
+}
+
+int dnx_batch_advance(dnx_batch *b, int steps)
+{
+    if (!b || steps <= 0) return DNX_E_ARG;
+    (void)hipSetDevice(b->ctx->device);
+    // the host-side view of which slots run: the newest completed snapshot
+    int rc = batch_enqueue(b, steps, b->h_dyn + (size_t)b->buf * b->nslots);
+    if (rc) return rc;
+    return DNX_OK;
+}
+
+// Copies the status table into pinned buffer `buf` after the queued work.
+static int batch_snapshot(dnx_batch *b, int which)
+{
+    hipError_t e;
+    if ((e = hipMemcpyAsync(b->h_dyn + (size_t)which * b->nslots, b->b_dyn.ptr, sizeof(BDyn) * (size_t)b->nslots,
+                            hipMemcpyDeviceToHost, b->ctx->stream)) != hipSuccess)
+        return hip_fail(e, "status copy");
+    if ((e = hipEventRecord(b->ev[which], b->ctx->stream)) != hipSuccess) return hip_fail(e, "event");
+    b->ev_pending[which] = true;
+    return DNX_OK;
+}
+
+static int batch_wait(dnx_batch *b, int which)
+{
+    if (!b->ev_pending[which]) return DNX_OK;
+    hipError_t e = hipEventSynchronize(b->ev[which]);
+    b->ev_pending[which] = false;
+    return e == hipSuccess ? DNX_OK : hip_fail(e, "status wait");
+}
+
+int dnx_batch_poll(dnx_batch *b, int *status, int64_t *steps, double *t, uint64_t *t_done_ticks)
+{
+    if (!b) return DNX_E_ARG;
+    (void)hipSetDevice(b->ctx->device);
+    int rc = batch_snapshot(b, b->buf);
+    if (!rc) rc = batch_wait(b, b->buf);
+    if (rc) return rc;
+    const BDyn *d = b->h_dyn + (size_t)b->buf * b->nslots;
+    for (int s = 0; s < b->nslots; s++) {
+        if (status) status[s] = b->slots[(size_t)s].used ? d[s].status : -1;
+        if (steps) steps[s] = d[s].steps;
+        if (t) t[s] = d[s].t;
+        if (t_done_ticks) t_done_ticks[s] = d[s].t_done;
+    }
+    return DNX_OK;
+}
+
+int dnx_batch_run(dnx_batch *b, int steps_per_launch, double walltime_s, int *remaining)
+{
+    if (!b || steps_per_launch <= 0) return DNX_E_ARG;
+    (void)hipSetDevice(b->ctx->device);
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = batch_snapshot(b, 0);
+    if (!rc) rc = batch_wait(b, 0);
+    if (rc) return rc;
+    b->buf = 0;
+    int running = 0;
+    for (;;) {
+        const BDyn *known = b->h_dyn;
+        running = 0;
+        for (int s = 0; s < b->nslots; s++) running += b->slots[(size_t)s].used && known[s].status == DNX_RUNNING;
+        if (!running) break;
+        if (walltime_s > 0 &&
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > walltime_s)
+            break;
+        if ((rc = batch_enqueue(b, steps_per_launch, known))) return rc;
+        if ((rc = batch_snapshot(b, 0))) return rc;
+        if ((rc = batch_wait(b, 0))) return rc;
+        batch_collect_timing(b);
+    }
+    if (remaining) *remaining = running;
+    return DNX_OK;
+}
+
+int dnx_batch_stamp(dnx_batch *b, uint64_t *ticks, double *ticks_per_s)
+{
+    if (!b) return DNX_E_ARG;
+    (void)hipSetDevice(b->ctx->device);
+    launch_batch_stamp((unsigned long long *)b->b_stamp.ptr, b->ctx->stream);
+    unsigned long long v = 0;
+    hipError_t e = hipMemcpyAsync(&v, b->b_stamp.ptr, sizeof(v), hipMemcpyDeviceToHost, b->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "stamp");
+    if (ticks) *ticks = v;
+    if (ticks_per_s) *ticks_per_s = b->clock_hz;
+    return DNX_OK;
+}
+
+int dnx_batch_read_state(dnx_batch *b, int slot, double *x, int *status, int64_t *steps, double *t)
+{
+    if (!b || slot < 0 || slot >= b->nslots || !b->slots[(size_t)slot].used) return DNX_E_ARG;
+    (void)hipSetDevice(b->ctx->device);
+    const SlotHost &sl = b->slots[(size_t)slot];
+    const size_t N = (size_t)sl.n + 2 * (size_t)sl.m;
+    hipStream_t st = b->ctx->stream;
+    DevBuf bo;
+    int rc = bo.alloc(sizeof(float) * (N ? N : 1));
+    if (rc) return rc;
+    launch_batch_materialize((const BInst *)b->b_inst.ptr + slot, (const BDyn *)b->b_dyn.ptr + slot, sl.n, sl.m,
+                             (float *)bo.ptr, st);
+    std::vector<float> xf(N ? N : 1);
+    BDyn d;
+    hipError_t e;
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "materialize");
+    if ((N && (e = hipMemcpyAsync(xf.data(), bo.ptr, sizeof(float) * N, hipMemcpyDeviceToHost, st)) != hipSuccess) ||
+        (e = hipMemcpyAsync(&d, (BDyn *)b->b_dyn.ptr + slot, sizeof(BDyn), hipMemcpyDeviceToHost, st)) !=
+            hipSuccess)
+        return hip_fail(e, "state copy");
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "state sync");
+    const bool pend = d.status == DNX_RUNNING && d.plevel >= 0;
+    static const double dts[5] = {0.125, 0.0625, 0.03125, 0.015625, 0.0078125};
+    if (x)
+        for (size_t i = 0; i < N; i++) x[i] = (double)xf[i];
+    if (status) *status = d.status;
+    if (steps) *steps = d.steps + (pend ? 1 : 0);
+    if (t) *t = d.t + (pend ? dts[d.plevel] : 0.0);
+    return DNX_OK;
+}
+
+int dnx_batch_assignment(dnx_batch *b, int slot, int8_t *assign, int *unsat)
+{
+    if (!b || slot < 0 || slot >= b->nslots || !b->slots[(size_t)slot].used || !assign) return DNX_E_ARG;
+    (void)hipSetDevice(b->ctx->device);
+    const SlotHost &sl = b->slots[(size_t)slot];
+    std::vector<BF2> vd((size_t)(sl.n ? sl.n : 1));
+    hipError_t e;
+    if (sl.n &&
+        (e = hipMemcpyAsync(vd.data(), b->h_inst[(size_t)slot].vd, sizeof(BF2) * (size_t)sl.n, hipMemcpyDeviceToHost,
+                            b->ctx->stream)) != hipSuccess)
+        return hip_fail(e, "assignment copy");
+    if ((e = hipStreamSynchronize(b->ctx->stream)) != hipSuccess) return hip_fail(e, "assignment sync");
+    for (int v = 0; v < sl.n; v++) assign[v] = vd[(size_t)v].v >= 0.0f ? 1 : -1;
+    if (unsat) {
+        int bad = 0;
+        for (int c = 0; c < sl.m; c++) {
+            bool ok = false;
+            for (int q = sl.cp[(size_t)c]; q < sl.cp[(size_t)c + 1] && !ok; q++) {
+                int32_t lit = sl.lits[(size_t)q];
+                int8_t a = assign[std::abs(lit) - 1];
+                ok = (lit > 0 && a > 0) || (lit < 0 && a < 0);
+            }
+            bad += !ok;
+        }
+        *unsat = bad;
+    }
+    return DNX_OK;
+}
+
+int dnx_batch_retire(dnx_batch *b, int slot)
+{
+    if (!b || slot < 0 || slot >= b->nslots || !b->slots[(size_t)slot].used) return DNX_E_ARG;
+    (void)hipSetDevice(b->ctx->device);
+    // kernels still queued may read this slot: retire after they finish
+    int rc = batch_sync(b);
+    if (rc) return rc;
+    SlotHost &sl = b->slots[(size_t)slot];
+    b->arena.release(sl.off, sl.size);
+    BDyn D;
+    memset(&D, 0, sizeof(D));
+    D.status = -1;
+    D.plevel = -1;
+    b->h_dyn[slot] = D;
+    b->h_dyn[(size_t)b->nslots + slot] = D;
+    hipError_t e = hipMemcpy((BDyn *)b->b_dyn.ptr + slot, &D, sizeof(BDyn), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "retire");
+    sl = SlotHost();
+    b->free_slots.push_back(slot);
+    return DNX_OK;
+}
+
+int dnx_batch_info(dnx_batch *b, int slot, int *n, int *m, int *mode)
+{
+    if (!b || slot < 0 || slot >= b->nslots) return DNX_E_ARG;
+    const SlotHost &sl = b->slots[(size_t)slot];
+    if (n) *n = sl.n;
+    if (m) *m = sl.m;
+    if (mode) *mode = sl.used ? sl.mode : -1;
+    return DNX_OK;
+}
+
+int dnx_batch_timing(dnx_batch *b, int enable, double *ms, int64_t *launches)
+{
+    if (!b) return DNX_E_ARG;
+    (void)hipSetDevice(b->ctx->device);
+    int rc = batch_sync(b);
+    if (rc) return rc;
+    batch_collect_timing(b);
+    for (int md = 0; md < BM_COUNT; md++) {
+        if (ms) ms[md] = b->t_ms[md];
+        if (launches) launches[md] = b->t_launches[md];
+    }
+    if (enable >= 0) {
+        b->timing = enable != 0;
+        for (int md = 0; md < BM_COUNT; md++) {
+            b->t_ms[md] = 0.0;
+            b->t_launches[md] = 0;
+        }
+    }
+    return DNX_OK;
+}
+
+int dnx_batch_memory(dnx_batch *b, int64_t *arena_bytes, int64_t *free_bytes)
+{
+    if (!b) return DNX_E_ARG;
+    if (arena_bytes) *arena_bytes = (int64_t)b->arena.cap;
+    if (free_bytes) *free_bytes = (int64_t)b->arena.free_bytes();
+    return DNX_OK;
+}
+
+}  // extern "C"

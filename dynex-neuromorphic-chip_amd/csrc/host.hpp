@@ -25,6 +25,7 @@ struct HostProblem {
     // literal-position order; lpos[p] is the slot of literal position p.
     std::vector<int32_t> row_ptr;  // n+1
     std::vector<int32_t> lpos;     // nnz
+    std::vector<int32_t> eidx;     // nnz: literal position of each slot (inverse of lpos)
 
     int64_t nnz() const { return (int64_t)lits.size(); }
 };

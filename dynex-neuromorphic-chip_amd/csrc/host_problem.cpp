@@ -49,6 +49,8 @@ int finalize_problem(HostProblem &p)
         int v = std::abs(p.lits[(size_t)q]) - 1;
         p.lpos[(size_t)q] = fill[(size_t)v]++;
     }
+    p.eidx.assign((size_t)p.nnz(), 0);
+    for (int64_t q = 0; q < p.nnz(); q++) p.eidx[(size_t)p.lpos[(size_t)q]] = (int32_t)q;
     return DNX_OK;
 }
 

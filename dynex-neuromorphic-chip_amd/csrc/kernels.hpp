@@ -14,7 +14,7 @@ struct DevProb {
     int max_deg;
     const int32_t *cp;
     const int32_t *lits;     // signed 1-based literal per position
-    const int32_t *lpos;     // variable-major slot of each literal position
+    const int32_t *eidx;     // literal position of each variable-major slot
     const int32_t *row_ptr;  // n+1
     const int8_t *unit;      // n pins, or nullptr when there are none
 };
@@ -24,28 +24,53 @@ struct DevParams {
 };
 
 // Per-instance control block.  Three rotating slots carry one step's
-// reductions: step s writes slot s%3 (loc and vbad from the clause pass,
-// level from the variable pass); the clause pass of step s+1 reads slot s%3
-// to apply step s's update lazily; the variable pass of step s clears slot
-// (s+1)%3 for step s+1.
+// verdicts: step s writes slot s%3 (the clause pass raises the `unsolved`
+// flag group, the variable pass the level groups and `stepped`); the clause
+// pass of step s+1 reads slot s%3 to apply step s's update lazily; the
+// variable pass of step s clears slot (s+1)%3 for step s+1.
+//
+// A flag group is 8 cache lines of 32 ints.  Workgroup b raises entry
+// (b & 7) * 32 + ((b >> 3) & 7) with a plain store: workgroups are dealt to
+// the 8 XCDs round-robin, so each line is written only from one XCD's L2 and
+// no store contends across the chip.  A reader ORs the 64 used entries with
+// one load per lane and a wave ballot.
+constexpr int FLAG_GROUP = 256;   // ints per group (8 lines x 32)
+constexpr int FLAG_USED = 64;     // entries a group actually uses
+constexpr int FG_UNSOLVED = 0;    // some clause has C >= 1/2 or no true literal
+constexpr int FG_LEVEL = 1;       // groups 1..4: some variable needs level >= L
+constexpr int FG_COUNT = 5;
+
 struct Ctl {
     int status;         // 0 running, else DNX_SAT / DNX_UNKNOWN
     int stop_on_solve;  // 1: stop at the first verified state
     long long steps;    // index of the last state the clause pass produced
     long long cap;      // step cap: state `cap` is the last one evaluated
     double t;           // simulated time of that state
-    int loc[3];         // clauses with C >= 1/2
-    int vbad[3];        // clauses with no strictly true literal
-    int level[3];       // max fail level, -1 when no step was taken
+    int stepped[3];     // the variable pass produced a level (a step is pending)
     int done[3];        // DNX_SAT / DNX_UNKNOWN when the state ended the run
-    int pad[4];
+    int pad[2];
+    int flags[3][FG_COUNT][FLAG_GROUP];
 };
+
+inline int flag_entry(int i) { return (i & 7) * 32 + (i >> 3); }  // i in [0, 64)
+inline bool ctl_any(const Ctl &c, int s, int g)
+{
+    for (int i = 0; i < FLAG_USED; i++)
+        if (c.flags[s][g][flag_entry(i)]) return true;
+    return false;
+}
+// level of a slot: the level groups are prefix-closed
+inline int ctl_level(const Ctl &c, int s)
+{
+    int L = 0;
+    for (int g = 0; g < 4; g++) L += ctl_any(c, s, FG_LEVEL + g) ? 1 : 0;
+    return L;
+}
 
 // ---- launchers (kernels.hip) ---------------------------------------------
 // prec: 0 = FP64 (sorted fold), 1 = FP32 (clause-order fold)
 void launch_rhs(int prec, const DevProb &P, const DevParams &prm, const void *x,
-                void *dxdt, void *e, int *loc, double *energy_part, int nblk_clause,
-                void *stream);
+                void *dxdt, void *e, int *loc_part, double *energy_part, void *stream);
 int rhs_clause_blocks(int m);
 
 void launch_euler(int prec, const DevProb &P, const DevParams &prm, const void *x,

@@ -72,6 +72,21 @@ def lib():
     L.dnx_init_uniform.argtypes = [C.c_uint64, ip, ip, dp, dp, F64P]
     L.dnx_write_dimacs.argtypes = [C.c_char_p, ip, ip, I32P, I32P, C.c_char_p]
     L.dnx_check_assignment.argtypes = [vp, I8P, C.POINTER(C.c_int)]
+    L.dnx_batch_open.argtypes = [vp, ip, ip, C.c_int64, pp]
+    L.dnx_batch_close.argtypes = [vp]
+    L.dnx_batch_set_shape.argtypes = [vp, ip, ip, ip]
+    L.dnx_batch_admit.argtypes = [vp, vp, C.POINTER(Params), F64P, C.c_int64, ip, C.POINTER(C.c_int)]
+    L.dnx_batch_advance.argtypes = [vp, ip]
+    L.dnx_batch_poll.argtypes = [vp, vp, vp, vp, vp]
+    L.dnx_batch_run.argtypes = [vp, ip, dp, C.POINTER(C.c_int)]
+    L.dnx_batch_stamp.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]
+    L.dnx_batch_read_state.argtypes = [vp, ip, vp, C.POINTER(C.c_int), C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+    L.dnx_batch_assignment.argtypes = [vp, ip, I8P, C.POINTER(C.c_int)]
+    L.dnx_batch_retire.argtypes = [vp, ip]
+    L.dnx_batch_info.argtypes = [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.dnx_batch_memory.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.dnx_batch_timing.argtypes = [vp, ip, vp, vp]
+    L.dnx_state_time.argtypes = [vp, ip, C.POINTER(C.c_double)]
     _LIB = L
     return L
 
@@ -111,6 +126,17 @@ def init_uniform(seed, n, m, xs0=0.5, xl0=1.0):
     x = np.zeros(n + 2 * m, np.float64)
     check(lib().dnx_init_uniform(seed, n, m, xs0, xl0, x))
     return x
+
+
+def host_problem(n, cp, lits):
+    """A problem without a device copy (for batch admission and checks)."""
+    h = C.c_void_p()
+    cp = np.ascontiguousarray(cp, np.int32)
+    lits = np.ascontiguousarray(lits, np.int32)
+    if lits.size == 0:
+        lits = np.zeros(1, np.int32)
+    check(lib().dnx_problem_from_lits(None, int(n), len(cp) - 1, cp, lits, C.byref(h)))
+    return Problem._wrap(None, h)
 
 
 def write_dimacs(path, n, cp, lits, comment=""):
@@ -236,9 +262,98 @@ class State:
                                    C.byref(loc)))
         return dict(x=x, t=t.value, steps=steps.value, level=lev.value, loc=loc.value)
 
+    def time_steps(self, nsteps):
+        ms = C.c_double()
+        check(lib().dnx_state_time(self.h, int(nsteps), C.byref(ms)))
+        return ms.value
+
     def solve(self, cap=100000, walltime=0.0):
         a = np.zeros(max(self.prob.n, 1), np.int8)
         steps, t, st = C.c_int64(), C.c_double(), C.c_int()
         check(lib().dnx_state_solve(self.h, cap, walltime, a.ctypes.data_as(C.c_void_p), C.byref(steps),
                                     C.byref(t), C.byref(st)))
         return dict(status=st.value, steps=steps.value, t=t.value, assign=a[:self.prob.n])
+
+
+class Batch:
+    """Slots of independent instances advanced together (FP32)."""
+
+    def __init__(self, ctx, slots, arena_bytes=0):
+        h = C.c_void_p()
+        check(lib().dnx_batch_open(ctx.h, FP32, int(slots), int(arena_bytes), C.byref(h)))
+        self.h, self.slots = h, int(slots)
+
+    def close(self):
+        if self.h:
+            lib().dnx_batch_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_shape(self, mode, block, lds_cap=0):
+        check(lib().dnx_batch_set_shape(self.h, mode, block, lds_cap))
+
+    def admit(self, prob, prm, x0, cap=100000, stop_on_solve=True):
+        s = C.c_int()
+        check(lib().dnx_batch_admit(self.h, prob.h, C.byref(prm), np.ascontiguousarray(x0, np.float64), int(cap),
+                                    int(bool(stop_on_solve)), C.byref(s)))
+        return s.value
+
+    def advance(self, steps):
+        check(lib().dnx_batch_advance(self.h, int(steps)))
+
+    def run(self, steps_per_launch=256, walltime=0.0):
+        r = C.c_int()
+        check(lib().dnx_batch_run(self.h, int(steps_per_launch), float(walltime), C.byref(r)))
+        return r.value
+
+    def poll(self):
+        S = self.slots
+        st = np.zeros(S, np.int32)
+        steps = np.zeros(S, np.int64)
+        t = np.zeros(S, np.float64)
+        td = np.zeros(S, np.uint64)
+        check(lib().dnx_batch_poll(self.h, st.ctypes.data_as(C.c_void_p), steps.ctypes.data_as(C.c_void_p),
+                                   t.ctypes.data_as(C.c_void_p), td.ctypes.data_as(C.c_void_p)))
+        return dict(status=st, steps=steps, t=t, t_done=td)
+
+    def stamp(self):
+        tk, rate = C.c_uint64(), C.c_double()
+        check(lib().dnx_batch_stamp(self.h, C.byref(tk), C.byref(rate)))
+        return tk.value, rate.value
+
+    def read_state(self, slot, n, m):
+        x = np.zeros(n + 2 * m, np.float64)
+        st, steps, t = C.c_int(), C.c_int64(), C.c_double()
+        check(lib().dnx_batch_read_state(self.h, slot, x.ctypes.data_as(C.c_void_p), C.byref(st), C.byref(steps),
+                                         C.byref(t)))
+        return dict(x=x, status=st.value, steps=steps.value, t=t.value)
+
+    def assignment(self, slot, n):
+        a = np.zeros(max(n, 1), np.int8)
+        u = C.c_int()
+        check(lib().dnx_batch_assignment(self.h, slot, a, C.byref(u)))
+        return a[:n], u.value
+
+    def retire(self, slot):
+        check(lib().dnx_batch_retire(self.h, slot))
+
+    def info(self, slot):
+        n, m, md = C.c_int(), C.c_int(), C.c_int()
+        check(lib().dnx_batch_info(self.h, slot, C.byref(n), C.byref(m), C.byref(md)))
+        return n.value, m.value, md.value
+
+    def timing(self, enable=-1):
+        ms = np.zeros(3, np.float64)
+        nl = np.zeros(3, np.int64)
+        check(lib().dnx_batch_timing(self.h, int(enable), ms.ctypes.data_as(C.c_void_p), nl.ctypes.data_as(C.c_void_p)))
+        return ms, nl
+
+    def memory(self):
+        a, f = C.c_int64(), C.c_int64()
+        check(lib().dnx_batch_memory(self.h, C.byref(a), C.byref(f)))
+        return a.value, f.value

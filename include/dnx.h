@@ -121,6 +121,10 @@ int dnx_state_free(dnx_state *s);
  * trajectory mode).  No host synchronisation inside. */
 int dnx_state_step(dnx_state *s, int nsteps);
 
+/* Advances nsteps steps in fixed-step mode like dnx_state_step and reports
+ * the device time they took (HIP events on the context stream), in ms. */
+int dnx_state_time(dnx_state *s, int nsteps, double *ms);
+
 /* Reads the current state: x (reference layout, may be NULL), simulated
  * time, accepted-step count, the level of the last step, and loc of the
  * last evaluated state. */
@@ -143,6 +147,67 @@ int dnx_state_solve(dnx_state *s, int64_t step_cap, double walltime_s,
 int dnx_solve(dnx_problem *p, int prec, const dnx_params *prm, const double *x0,
               int64_t step_cap, double walltime_s, int8_t *assign_out,
               int64_t *steps_out, double *t_out, int *status);
+
+/* ---- batched engine -------------------------------------------------- */
+/* Many independent instances advanced together on one device (no
+ * counterpart in the reference, which runs one instance per thread; its
+ * thread portfolio, dynex.cc:2731-2801, is one use).  Each instance sits in a
+ * slot and keeps its own step size, simulated time and step count; its
+ * results are bit-identical to solving it alone (dnx_solve with DNX_FP32).
+ * Instances are packed into one device arena sized by a memory budget. */
+typedef struct dnx_batch dnx_batch;
+
+/* Opens an engine with `slots` instance slots and a device arena of
+ * arena_bytes (<= 0: 80% of device memory).  prec must be DNX_FP32. */
+int dnx_batch_open(dnx_ctx *ctx, int prec, int slots, int64_t arena_bytes, dnx_batch **out);
+int dnx_batch_close(dnx_batch *b);
+
+/* Execution shape per mode (0: V/dV and contributions in LDS, 1: V/dV in
+ * LDS, 2: device memory): workgroup size (128/256/512/1024) and the LDS
+ * bytes an instance may use to qualify for the mode.  Only while empty. */
+int dnx_batch_set_shape(dnx_batch *b, int mode, int block, int lds_cap);
+
+/* Uploads an instance (a host-only problem is fine) with initial state x0
+ * (reference layout; unit pins applied) into a free slot.  The run ends at
+ * the first verified state (stop_on_solve != 0) or at state step_cap.
+ * DNX_E_BUDGET when no slot or arena range is free. */
+int dnx_batch_admit(dnx_batch *b, const dnx_problem *p, const dnx_params *prm, const double *x0,
+                    int64_t step_cap, int stop_on_solve, int *slot_out);
+
+/* Enqueues one launch per execution shape that advances every running slot
+ * by up to `steps` steps.  Returns without waiting. */
+int dnx_batch_advance(dnx_batch *b, int steps);
+
+/* Waits for queued work and copies the per-slot table (arrays of length
+ * `slots`; any may be NULL): status (-1 free slot), steps of the evaluated
+ * state, simulated time, and the device wall-clock tick at which the
+ * instance ended (see dnx_batch_stamp). */
+int dnx_batch_poll(dnx_batch *b, int *status, int64_t *steps, double *t, uint64_t *t_done_ticks);
+
+/* Advances until no slot runs (or walltime_s passes when > 0), launching
+ * steps_per_launch steps at a time.  *remaining = slots still running. */
+int dnx_batch_run(dnx_batch *b, int steps_per_launch, double walltime_s, int *remaining);
+
+/* Current device wall-clock tick and its rate. */
+int dnx_batch_stamp(dnx_batch *b, uint64_t *ticks, double *ticks_per_s);
+
+/* State of a slot in the reference layout (pending step applied). */
+int dnx_batch_read_state(dnx_batch *b, int slot, double *x, int *status, int64_t *steps, double *t);
+
+/* Sign assignment (+1/-1, x >= 0 positive) of a slot and, when unsat is
+ * not NULL, the number of clauses it leaves unsatisfied (exact check). */
+int dnx_batch_assignment(dnx_batch *b, int slot, int8_t *assign, int *unsat);
+
+/* Frees a slot and its arena range. */
+int dnx_batch_retire(dnx_batch *b, int slot);
+
+int dnx_batch_info(dnx_batch *b, int slot, int *n, int *m, int *mode);
+
+/* Waits for queued work, then reports per mode the summed device time (ms)
+ * and count of the launches timed since the last reset; enable = 1/0 turns
+ * launch timing on/off and resets the sums (-1: report only). */
+int dnx_batch_timing(dnx_batch *b, int enable, double *ms, int64_t *launches);
+int dnx_batch_memory(dnx_batch *b, int64_t *arena_bytes, int64_t *free_bytes);
 
 /* ---- host utilities -------------------------------------------------- */
 /* Initial state of the reference's thread 0 for seed `seed`

@@ -1,0 +1,172 @@
+"""Batched engine (FP32) on the GPU: every instance must compute the same bits
+as the float oracle alone, whatever the batch around it, the execution shape
+or the number of steps per launch."""
+import numpy as np
+import pytest
+
+import dnx
+import oracle as O
+from conftest import planted_cp
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = dnx.Context(0)
+    yield c
+    c.close()
+
+
+def make_set(seed, count, nmin=60, nmax=2500, rmin=4.6, rmax=8.0, with_general=True):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+        m = int(round(rng.uniform(rmin, rmax) * n))
+        lits = dnx.generate_planted(seed * 1000 + i, n, m)
+        cp = planted_cp(m)
+        if with_general and i % 7 == 3:
+            # turn a few clauses into k=2, k=1 and k=4 clauses
+            cl = [list(lits[3 * c:3 * c + 3]) for c in range(m)]
+            cl[0] = cl[0][:2]
+            cl[1] = cl[1][:1]
+            cl[2] = cl[2] + [-cl[3][0]]
+            cp = np.cumsum([0] + [len(c) for c in cl]).astype(np.int32)
+            lits = np.array(sum(cl, []), np.int32)
+        x0 = dnx.init_uniform(seed * 1000 + i, n, m)
+        out.append((n, cp, lits, x0))
+    return out
+
+
+def oracle_solve(n, cp, lits, x0, cap):
+    op = O.Problem(n, len(cp) - 1, cp, lits)
+    x = x0.astype(np.float32)
+    if op.has_unit:
+        x[:n] = np.where(op.unit != 0, op.unit, x[:n])
+    steps, t, status = O.solve(op, O.params(op.m), x, cap, prec=32)
+    return steps, t, status, x
+
+
+def run_batch(ctx, inst, cap, spl, shapes=None, stop=True):
+    b = dnx.Batch(ctx, slots=len(inst) + 2, arena_bytes=1 << 30)
+    if shapes:
+        for md, (bs, lds) in shapes.items():
+            b.set_shape(md, bs, lds)
+    slots = []
+    for n, cp, lits, x0 in inst:
+        hp = dnx.host_problem(n, cp, lits)
+        slots.append(b.admit(hp, dnx.params(len(cp) - 1), x0, cap=cap, stop_on_solve=stop))
+        hp.free()
+    b.run(steps_per_launch=spl)
+    res = b.poll()
+    out = []
+    for (n, cp, lits, x0), s in zip(inst, slots):
+        m = len(cp) - 1
+        st = b.read_state(s, n, m)
+        a, unsat = b.assignment(s, n)
+        out.append(dict(status=int(res["status"][s]), steps=int(res["steps"][s]), t=float(res["t"][s]),
+                        x=st["x"].astype(np.float32), assign=a, unsat=unsat, mode=b.info(s)[2]))
+    b.close()
+    return out
+
+
+@pytest.fixture(scope="module")
+def inst_set():
+    return make_set(7, 36)
+
+
+@pytest.fixture(scope="module")
+def oracle_results(inst_set):
+    return [oracle_solve(n, cp, lits, x0, 1500) for n, cp, lits, x0 in inst_set]
+
+
+def test_batch_matches_oracle(ctx, inst_set, oracle_results):
+    out = run_batch(ctx, inst_set, 1500, spl=37)
+    modes = set()
+    for r, (steps, t, status, xo), (n, cp, lits, x0) in zip(out, oracle_results, inst_set):
+        modes.add(r["mode"])
+        assert r["status"] == (dnx.SAT if status == 1 else dnx.UNKNOWN)
+        assert r["steps"] == steps and r["t"] == t
+        assert np.array_equal(r["x"], xo)
+        if status == 1:
+            assert r["unsat"] == 0
+    assert modes >= {0, 1}
+
+
+@pytest.mark.parametrize("shapes", [
+    {0: (128, 0), 1: (128, 0), 2: (256, 0)},        # everything in device memory
+    {0: (512, 0), 1: (1024, 65536), 2: (1024, 0)},  # V/dV in LDS
+    {0: (1024, 160 * 1024), 1: (256, 0), 2: (512, 0)},  # as much in LDS as fits
+])
+def test_shapes_give_identical_bits(ctx, inst_set, oracle_results, shapes):
+    out = run_batch(ctx, inst_set, 1500, spl=64, shapes=shapes)
+    for r, (steps, t, status, xo) in zip(out, oracle_results):
+        assert r["steps"] == steps and r["t"] == t
+        assert np.array_equal(r["x"], xo)
+
+
+def test_alone_equals_in_the_mix(ctx, inst_set):
+    mix = run_batch(ctx, inst_set, 1500, spl=101)
+    for i in (0, 5, 17, 35):
+        alone = run_batch(ctx, [inst_set[i]], 1500, spl=9)[0]
+        assert alone["steps"] == mix[i]["steps"] and alone["t"] == mix[i]["t"]
+        assert np.array_equal(alone["x"], mix[i]["x"])
+
+
+def test_fixed_step_trajectory(ctx):
+    inst = make_set(11, 6, nmin=100, nmax=3000, with_general=True)
+    out = run_batch(ctx, inst, 60, spl=25, stop=False)
+    for r, (n, cp, lits, x0) in zip(out, inst):
+        op = O.Problem(n, len(cp) - 1, cp, lits)
+        x = x0.astype(np.float32)
+        if op.has_unit:
+            x[:n] = np.where(op.unit != 0, op.unit, x[:n])
+        O.trajectory(op, O.params(op.m), x, 60, prec=32)
+        assert r["status"] == dnx.UNKNOWN and r["steps"] == 60
+        assert np.array_equal(r["x"], x)
+
+
+def test_single_instance_path_agrees(ctx, inst_set):
+    """The batched engine and dnx_solve(FP32) are the same computation."""
+    out = run_batch(ctx, inst_set[:8], 1500, spl=50)
+    for r, (n, cp, lits, x0) in zip(out, inst_set[:8]):
+        gp = ctx.problem_from_lits(n, cp, lits)
+        s = gp.solve(dnx.params(len(cp) - 1), x0, dnx.FP32, cap=1500)
+        gp.free()
+        assert s["steps"] == r["steps"] and s["t"] == r["t"]
+        if s["status"] == dnx.SAT:
+            assert np.array_equal(s["assign"], r["assign"])
+
+
+def test_retire_readmit_and_budget(ctx):
+    inst = make_set(13, 10, nmin=200, nmax=2000)
+    b = dnx.Batch(ctx, slots=4, arena_bytes=8 << 20)
+    arena, free0 = b.memory()
+    assert free0 == arena
+    slots = []
+    for n, cp, lits, x0 in inst[:4]:
+        hp = dnx.host_problem(n, cp, lits)
+        slots.append(b.admit(hp, dnx.params(len(cp) - 1), x0, cap=1500))
+        hp.free()
+    n, cp, lits, x0 = inst[4]
+    hp = dnx.host_problem(n, cp, lits)
+    with pytest.raises(dnx.DnxError) as e:
+        b.admit(hp, dnx.params(len(cp) - 1), x0)
+    assert e.value.code == dnx.E_BUDGET  # no free slot
+    b.run(64)
+    for s in slots:
+        b.retire(s)
+    assert b.memory()[1] == arena
+    s = b.admit(hp, dnx.params(len(cp) - 1), x0, cap=1500)
+    b.run(64)
+    st = b.poll()["status"][s]
+    assert st in (dnx.SAT, dnx.UNKNOWN)
+    hp.free()
+    big = dnx.generate_planted(1, 50000, 400000)
+    hp = dnx.host_problem(50000, planted_cp(400000), big)
+    with pytest.raises(dnx.DnxError) as e:
+        b.admit(hp, dnx.params(400000), dnx.init_uniform(1, 50000, 400000))
+    assert e.value.code == dnx.E_BUDGET  # larger than the arena
+    hp.free()
+    b.close()
