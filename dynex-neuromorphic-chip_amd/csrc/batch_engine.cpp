@@ -10,11 +10,20 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <climits>
+#include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
+#include <functional>
 #include <map>
+#include <memory>
+#include <mutex>
+#include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "api_internal.hpp"
@@ -133,6 +142,8 @@ struct dnx_batch {
 
 namespace dnx {
 
+int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *prm, const double *x0,
+                         int64_t step_cap, int stop_on_solve, int *slot_out);
 static int choose_mode(const dnx_batch *b, int n, int64_t nnz)
 {
     size_t pv = ((size_t)n * sizeof(BF2) + 15) / 16 * 16;
@@ -372,12 +383,21 @@ int dnx_batch_admit(dnx_batch *b, const dnx_problem *p, const dnx_params *prm, c
                     int stop_on_solve, int *slot_out)
 {
     if (!b || !p || !prm || !x0 || !slot_out || step_cap < 0) return DNX_E_ARG;
+    return dnx_batch_admit_host(b, p->hp, prm, x0, step_cap, stop_on_solve, slot_out);
+}
+
+}  // extern "C"
+
+namespace dnx {
+
+int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *prm, const double *x0,
+                         int64_t step_cap, int stop_on_solve, int *slot_out)
+{
     if (b->free_slots.empty()) {
         set_error("no free slot");
         return DNX_E_BUDGET;
     }
     (void)hipSetDevice(b->ctx->device);
-    const HostProblem &h = p->hp;
     const int n = h.n, m = h.m;
     const int64_t nnz = h.nnz();
     const bool general = !h.uniform3;
@@ -476,6 +496,10 @@ int dnx_batch_admit(dnx_batch *b, const dnx_problem *p, const dnx_params *prm, c
     *slot_out = slot;
     return DNX_OK;
 }
+
+}  // namespace dnx
+
+extern "C" {
 
 int dnx_batch_advance(dnx_batch *b, int steps)
 {
@@ -687,3 +711,237 @@ int dnx_batch_memory(dnx_batch *b, int64_t *arena_bytes, int64_t *free_bytes)
 }
 
 }  // extern "C"
+
+namespace dnx {
+
+// One request of a served stream.
+struct ServeReq {
+    const HostProblem *hp = nullptr;
+    const double *x0 = nullptr;
+    dnx_params prm{};
+};
+
+struct ServeOut {
+    int *status;
+    int64_t *steps;
+    double *tts, *wait;
+};
+
+// The continuous-batching loop.  Requests become visible at their arrival
+// time; each is admitted as soon as a slot and an arena range are free
+// (first come, first served), advanced together with everything resident in
+// launches of `spl` steps, and retired the moment its state verifies or
+// reaches the cap, which frees its memory for the queue.  `get(i)` returns
+// request i (it may block while the request is still being prepared) and
+// `release(i)` says its host copy is no longer needed.
+static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t cap, int spl,
+                      const std::function<ServeReq(int)> &get, const std::function<void(int)> &release,
+                      const ServeOut &out, double *makespan)
+{
+    using clk = std::chrono::steady_clock;
+    const clk::time_point T0 = clk::now();
+    auto now_s = [&]() { return std::chrono::duration<double>(clk::now() - T0).count(); };
+    // device clock -> seconds since T0
+    uint64_t tick0 = 0;
+    double rate = 1e8;
+    const double hb = now_s();
+    int rc = dnx_batch_stamp(b, &tick0, &rate);
+    if (rc) return rc;
+    const double host0 = 0.5 * (hb + now_s());
+    std::vector<int> slot_req((size_t)b->nslots, -1);
+    std::vector<double> arrived_at((size_t)count, 0.0);
+    std::deque<int> queue;
+    int next = 0, finished = 0, running = 0;
+    std::vector<int8_t> assign;
+    while (finished < count) {
+        const double now = now_s();
+        while (next < count && arrival[next] <= now) queue.push_back(next++);
+        while (!queue.empty()) {
+            const int r = queue.front();
+            ServeReq q = get(r);
+            int slot = -1;
+            rc = dnx_batch_admit_host(b, *q.hp, &q.prm, q.x0, cap, 1, &slot);
+            if (rc == DNX_E_BUDGET) break;
+            if (rc) return rc;
+            slot_req[(size_t)slot] = r;
+            out.wait[r] = now_s() - arrival[r];
+            running++;
+            queue.pop_front();
+        }
+        if (running == 0) {
+            if (!queue.empty()) {
+                // larger than the whole arena: it can never run
+                const int r = queue.front();
+                queue.pop_front();
+                out.status[r] = DNX_FAILED;
+                out.steps[r] = 0;
+                out.tts[r] = -1.0;
+                release(r);
+                finished++;
+                continue;
+            }
+            if (next < count) {
+                const double dt = arrival[next] - now_s();
+                if (dt > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(dt, 0.01)));
+            }
+            continue;
+        }
+        if ((rc = dnx_batch_advance(b, spl))) return rc;
+        if ((rc = batch_snapshot(b, 0)) || (rc = batch_wait(b, 0))) return rc;
+        batch_collect_timing(b);
+        const BDyn *d = b->h_dyn;
+        for (int s = 0; s < b->nslots; s++) {
+            const int r = slot_req[(size_t)s];
+            if (r < 0 || d[s].status == DNX_RUNNING) continue;
+            int st = d[s].status;
+            out.steps[r] = d[s].steps;
+            out.tts[r] = host0 + (double)(d[s].t_done - tick0) / rate - arrival[r];
+            if (st == DNX_SAT) {
+                const SlotHost &sl = b->slots[(size_t)s];
+                assign.resize((size_t)(sl.n ? sl.n : 1));
+                int unsat = -1;
+                if ((rc = dnx_batch_assignment(b, s, assign.data(), &unsat))) return rc;
+                if (unsat != 0) st = DNX_FAILED;
+            }
+            out.status[r] = st;
+            if ((rc = dnx_batch_retire(b, s))) return rc;
+            slot_req[(size_t)s] = -1;
+            release(r);
+            running--;
+            finished++;
+        }
+    }
+    if (makespan) *makespan = now_s();
+    return DNX_OK;
+}
+
+}  // namespace dnx
+
+extern "C" {
+
+int dnx_serve_trace(dnx_batch *b, int count, dnx_problem *const *probs, const dnx_params *prms,
+                    const double *const *x0s, const double *arrival_s, int64_t step_cap, int steps_per_launch,
+                    int *status, int64_t *steps, double *tts_s, double *wait_s, double *makespan_s)
+{
+    if (!b || count < 0 || (count && (!probs || !x0s || !arrival_s || !status || !steps || !tts_s || !wait_s)) ||
+        steps_per_launch <= 0)
+        return DNX_E_ARG;
+    for (int i = 1; i < count; i++)
+        if (arrival_s[i] < arrival_s[i - 1]) {
+            set_error("arrival times must be non-decreasing");
+            return DNX_E_ARG;
+        }
+    auto get = [&](int i) {
+        ServeReq q;
+        q.hp = &probs[i]->hp;
+        q.x0 = x0s[i];
+        if (prms)
+            q.prm = prms[i];
+        else
+            dnx_params_default(probs[i]->hp.m, 10000, &q.prm);
+        return q;
+    };
+    ServeOut out{status, steps, tts_s, wait_s};
+    return serve_loop(b, count, arrival_s, step_cap, steps_per_launch, get, [](int) {}, out, makespan_s);
+}
+
+// Served planted stream: Poisson arrivals at rate_per_s over duration_s
+// (at most max_requests), n log-uniform in [n_min, n_max], ratio uniform in
+// [r_min, r_max].  Instances are generated by worker threads ahead of their
+// arrival time and freed at retirement.
+int dnx_serve_planted(dnx_batch *b, uint64_t seed, double rate_per_s, double duration_s, int n_min, int n_max,
+                      double r_min, double r_max, double p1, double p2, int64_t step_cap, int steps_per_launch,
+                      int max_requests, int gen_threads, int *count_out, double *arrival_out, int *n_out, int *m_out,
+                      int *status, int64_t *steps, double *tts_s, double *wait_s, double *makespan_s)
+{
+    if (!b || rate_per_s <= 0 || duration_s <= 0 || n_min < 3 || n_max < n_min || r_min <= 0 || r_max < r_min ||
+        max_requests <= 0 || !count_out || !arrival_out || !n_out || !m_out || !status || !steps || !tts_s ||
+        !wait_s || steps_per_launch <= 0)
+        return DNX_E_ARG;
+    // the trace: arrival times and instance sizes from one seeded stream
+    SplitMix64 rng(seed);
+    std::vector<uint64_t> iseed;
+    int count = 0;
+    double t = 0.0;
+    for (;;) {
+        t += -std::log(1.0 - rng.unit()) / rate_per_s;
+        if (t > duration_s || count >= max_requests) break;
+        const double ln = std::log((double)n_min) + rng.unit() * (std::log((double)n_max) - std::log((double)n_min));
+        const int n = std::max(n_min, std::min(n_max, (int)std::lround(std::exp(ln))));
+        const double r = r_min + rng.unit() * (r_max - r_min);
+        arrival_out[count] = t;
+        n_out[count] = n;
+        m_out[count] = (int)std::lround(r * n);
+        iseed.push_back(rng.next());
+        count++;
+    }
+    *count_out = count;
+    struct Slot {
+        std::unique_ptr<HostProblem> hp;
+        std::vector<double> x0;
+        bool ready = false;
+    };
+    std::vector<Slot> reqs((size_t)count);
+    std::mutex mu;
+    std::condition_variable cv;
+    std::atomic<int> next_gen{0};
+    std::atomic<int> released{0};
+    std::atomic<bool> stop{false};
+    const double lookahead_s = 2.0;  // generate at most this far ahead of arrival
+    using clk = std::chrono::steady_clock;
+    const clk::time_point T0 = clk::now();
+    auto worker = [&]() {
+        for (;;) {
+            const int i = next_gen.fetch_add(1);
+            if (i >= count || stop.load()) return;
+            for (;;) {  // keep host memory bounded: stay near real time
+                const double now = std::chrono::duration<double>(clk::now() - T0).count();
+                if (arrival_out[i] - now < lookahead_s || stop.load()) break;
+                std::this_thread::sleep_for(std::chrono::milliseconds(5));
+            }
+            auto hp = std::make_unique<HostProblem>();
+            hp->n = n_out[i];
+            hp->m = m_out[i];
+            hp->lits.resize((size_t)hp->m * 3);
+            generate_planted(iseed[(size_t)i], hp->n, hp->m, p1, p2, hp->lits.data(), nullptr);
+            hp->cp.resize((size_t)hp->m + 1);
+            for (int c = 0; c <= hp->m; c++) hp->cp[(size_t)c] = 3 * c;
+            finalize_problem(*hp);
+            std::vector<double> x0((size_t)hp->n + 2 * (size_t)hp->m);
+            init_uniform(iseed[(size_t)i], hp->n, hp->m, 0.5, 1.0, x0.data());
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                reqs[(size_t)i].hp = std::move(hp);
+                reqs[(size_t)i].x0 = std::move(x0);
+                reqs[(size_t)i].ready = true;
+            }
+            cv.notify_all();
+        }
+    };
+    const int nt = std::max(1, std::min(gen_threads > 0 ? gen_threads : 4, 32));
+    std::vector<std::thread> th;
+    for (int k = 0; k < nt; k++) th.emplace_back(worker);
+    auto get = [&](int i) {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return reqs[(size_t)i].ready; });
+        ServeReq q;
+        q.hp = reqs[(size_t)i].hp.get();
+        q.x0 = reqs[(size_t)i].x0.data();
+        dnx_params_default(q.hp->m, 10000, &q.prm);
+        return q;
+    };
+    auto release = [&](int i) {
+        std::lock_guard<std::mutex> lk(mu);
+        reqs[(size_t)i].hp.reset();
+        std::vector<double>().swap(reqs[(size_t)i].x0);
+        released++;
+    };
+    ServeOut out{status, steps, tts_s, wait_s};
+    int rc = serve_loop(b, count, arrival_out, step_cap, steps_per_launch, get, release, out, makespan_s);
+    stop = true;
+    for (auto &x : th) x.join();
+    return rc;
+}
+
+}  // extern "C"
+

@@ -87,6 +87,9 @@ def lib():
     L.dnx_batch_memory.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.dnx_batch_timing.argtypes = [vp, ip, vp, vp]
     L.dnx_state_time.argtypes = [vp, ip, C.POINTER(C.c_double)]
+    L.dnx_serve_trace.argtypes = [vp, ip, vp, vp, vp, F64P, C.c_int64, ip, vp, vp, vp, vp, C.POINTER(C.c_double)]
+    L.dnx_serve_planted.argtypes = [vp, C.c_uint64, dp, dp, ip, ip, dp, dp, dp, dp, C.c_int64, ip, ip, ip,
+                                    C.POINTER(C.c_int), vp, vp, vp, vp, vp, vp, vp, C.POINTER(C.c_double)]
     _LIB = L
     return L
 
@@ -357,3 +360,41 @@ class Batch:
         a, f = C.c_int64(), C.c_int64()
         check(lib().dnx_batch_memory(self.h, C.byref(a), C.byref(f)))
         return a.value, f.value
+
+    def serve(self, probs, x0s, arrivals, cap=100000, steps_per_launch=64):
+        """Continuous batching over a request trace; see dnx_serve_trace."""
+        k = len(probs)
+        ph = (C.c_void_p * max(k, 1))(*[p.h for p in probs])
+        xs = [np.ascontiguousarray(x, np.float64) for x in x0s]
+        xp = (C.c_void_p * max(k, 1))(*[x.ctypes.data for x in xs])
+        arr = np.ascontiguousarray(arrivals, np.float64)
+        st = np.zeros(max(k, 1), np.int32)
+        steps = np.zeros(max(k, 1), np.int64)
+        tts = np.zeros(max(k, 1), np.float64)
+        wait = np.zeros(max(k, 1), np.float64)
+        mk = C.c_double()
+        check(lib().dnx_serve_trace(self.h, k, ph, None, xp, arr if k else np.zeros(1), int(cap), int(steps_per_launch),
+                                    st.ctypes.data_as(C.c_void_p), steps.ctypes.data_as(C.c_void_p),
+                                    tts.ctypes.data_as(C.c_void_p), wait.ctypes.data_as(C.c_void_p), C.byref(mk)))
+        return dict(status=st[:k], steps=steps[:k], tts=tts[:k], wait=wait[:k], makespan=mk.value)
+
+    def serve_planted(self, seed, rate, duration, n_min=100, n_max=100000, r_min=4.3, r_max=8.0, p1=0.46, p2=0.46,
+                      cap=100000, steps_per_launch=64, max_requests=100000, gen_threads=8):
+        M = int(max_requests)
+        cnt = C.c_int()
+        arr = np.zeros(M, np.float64)
+        ns = np.zeros(M, np.int32)
+        ms = np.zeros(M, np.int32)
+        st = np.zeros(M, np.int32)
+        steps = np.zeros(M, np.int64)
+        tts = np.zeros(M, np.float64)
+        wait = np.zeros(M, np.float64)
+        mk = C.c_double()
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+        check(lib().dnx_serve_planted(self.h, int(seed), float(rate), float(duration), int(n_min), int(n_max),
+                                      float(r_min), float(r_max), float(p1), float(p2), int(cap),
+                                      int(steps_per_launch), M, int(gen_threads), C.byref(cnt), ptr(arr), ptr(ns),
+                                      ptr(ms), ptr(st), ptr(steps), ptr(tts), ptr(wait), C.byref(mk)))
+        k = cnt.value
+        return dict(count=k, arrival=arr[:k], n=ns[:k], m=ms[:k], status=st[:k], steps=steps[:k], tts=tts[:k],
+                    wait=wait[:k], makespan=mk.value)

@@ -209,6 +209,34 @@ int dnx_batch_info(dnx_batch *b, int slot, int *n, int *m, int *mode);
 int dnx_batch_timing(dnx_batch *b, int enable, double *ms, int64_t *launches);
 int dnx_batch_memory(dnx_batch *b, int64_t *arena_bytes, int64_t *free_bytes);
 
+/* ---- serving ---------------------------------------------------------- */
+/* Continuous batching over a batch engine (no counterpart in the reference).
+ * Request i becomes visible arrival_s[i] seconds after the call starts
+ * (non-decreasing); it is admitted as soon as a slot and an arena range are
+ * free (first come, first served), advanced alongside everything resident in
+ * launches of steps_per_launch steps, and retired the moment its state
+ * verifies or reaches step_cap, freeing its memory for the queue.  Per
+ * request: status (DNX_SAT only after the exact clause check; DNX_FAILED for
+ * a request larger than the arena), steps, time-to-solution from arrival to
+ * retirement by the device clock, and queueing delay before admission.
+ * prms may be NULL (reference defaults). */
+int dnx_serve_trace(dnx_batch *b, int count, dnx_problem *const *probs, const dnx_params *prms,
+                    const double *const *x0s, const double *arrival_s, int64_t step_cap,
+                    int steps_per_launch, int *status, int64_t *steps, double *tts_s,
+                    double *wait_s, double *makespan_s);
+
+/* The served planted stream: Poisson arrivals at rate_per_s for duration_s
+ * (at most max_requests), n log-uniform in [n_min, n_max], clause ratio
+ * uniform in [r_min, r_max], literal truth counts as dnx_generate_planted,
+ * v ~ U[-1,1), Xs = 0.5, Xl = 1.  gen_threads host threads build instances
+ * ahead of their arrival.  Outputs are arrays of max_requests entries;
+ * *count_out requests were issued. */
+int dnx_serve_planted(dnx_batch *b, uint64_t seed, double rate_per_s, double duration_s, int n_min,
+                      int n_max, double r_min, double r_max, double p1, double p2, int64_t step_cap,
+                      int steps_per_launch, int max_requests, int gen_threads, int *count_out,
+                      double *arrival_out, int *n_out, int *m_out, int *status, int64_t *steps,
+                      double *tts_s, double *wait_s, double *makespan_s);
+
 /* ---- host utilities -------------------------------------------------- */
 /* Initial state of the reference's thread 0 for seed `seed`
  * (dynex.cc:2643-2700): seed 0 sets every entry to 1.0; other seeds draw V

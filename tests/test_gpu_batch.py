@@ -170,3 +170,38 @@ def test_retire_readmit_and_budget(ctx):
     assert e.value.code == dnx.E_BUDGET  # larger than the arena
     hp.free()
     b.close()
+
+
+def test_serving_results_equal_solo_runs(ctx, inst_set, oracle_results):
+    """Requests arriving while others run are admitted as slots and memory
+    free up; every served result equals the instance solved alone."""
+    k = 24
+    probs = [dnx.host_problem(n, cp, lits) for n, cp, lits, _ in inst_set[:k]]
+    x0s = [x0 for _, _, _, x0 in inst_set[:k]]
+    arrivals = np.sort(np.random.default_rng(3).uniform(0.0, 0.25, k))
+    # few slots and a small arena force queueing and slot reuse
+    b = dnx.Batch(ctx, slots=6, arena_bytes=6 << 20)
+    r = b.serve(probs, x0s, arrivals, cap=1500, steps_per_launch=16)
+    b.close()
+    for p in probs:
+        p.free()
+    for i in range(k):
+        steps, t, status, _ = oracle_results[i]
+        assert r["status"][i] == (dnx.SAT if status == 1 else dnx.UNKNOWN)
+        assert r["steps"][i] == steps
+        assert r["tts"][i] > 0 and r["wait"][i] >= 0
+    assert r["makespan"] >= arrivals[-1]
+    assert np.max(r["wait"]) > 0  # some requests had to queue
+
+
+def test_serve_planted_stream(ctx):
+    b = dnx.Batch(ctx, slots=64)
+    r = b.serve_planted(seed=5, rate=200.0, duration=0.5, n_min=100, n_max=3000, cap=20000, steps_per_launch=32,
+                        max_requests=1000, gen_threads=4)
+    b.close()
+    assert 60 <= r["count"] <= 150
+    assert np.all(np.diff(r["arrival"]) >= 0)
+    assert np.all((r["status"] == dnx.SAT) | (r["status"] == dnx.UNKNOWN))
+    assert np.all(r["tts"] > 0)
+    print("served", r["count"], "p50 tts ms", np.percentile(r["tts"], 50) * 1e3,
+          "p99", np.percentile(r["tts"], 99) * 1e3)
