@@ -633,9 +633,11 @@ int dnx_state_solve(dnx_state *s, int64_t step_cap, double walltime_s, int8_t *a
             }
         }
     }
+    // drain the queued chunk before its event is destroyed
+    e = hipStreamSynchronize(st);
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
-    if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "solve drain");
+    if (e != hipSuccess) return hip_fail(e, "solve drain");
     if (rc) return rc;
     // final snapshot after the queue drained
     Ctl c;

@@ -77,32 +77,76 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                 BF2 a = pv[v];
                 return pend ? clampV(a.v + h * a.d) : a.v;
             };
-            for (int c = tid; c < m; c += BS) {
-                const int b = general ? I.cp[c] : 3 * c;
-                const int kk = general ? I.cp[c + 1] - b : 3;
-                const int32_t *cl = I.lits + b;
-                float xs = I.xs[c], xl = I.xl[c];
-                if (pend) {
-                    float Cp = general ? clauseK_C<float>(cl, kk, prevV) : clause3_C<float>(cl, prevV);
-                    memory_update(p, Cp, h, xs, xl);
-                    I.xs[c] = xs;
-                    I.xl[c] = xl;
+            if (!general) {
+                // 3-SAT: the next clause's literals and memories are loaded
+                // while the current one is evaluated
+                int c = tid;
+                int32_t n0 = 0, n1 = 0, n2 = 0;
+                float nxs = 0.0f, nxl = 0.0f;
+                if (c < m) {
+                    n0 = I.lits[3 * c];
+                    n1 = I.lits[3 * c + 1];
+                    n2 = I.lits[3 * c + 2];
+                    nxs = I.xs[c];
+                    nxl = I.xl[c];
                 }
-                const float cxs = clamp01(xs), cxl = clampXl(xl, p.xlmax);
-                bool sat;
-                float C;
-                if (MODE == BM_LDS_ALL) {
-                    const int32_t *lp = I.lpos + b;
-                    auto emit = [&](int j, float val) { se[lp[j]] = val; };
-                    C = general ? clauseK<float>(p, cl, kk, cxs, cxl, curV, emit, sat)
-                                : clause3<float>(p, cl, cxs, cxl, curV, emit, sat);
-                } else {
-                    float *ep = I.e + b;
-                    auto emit = [&](int j, float val) { ep[j] = val; };
-                    C = general ? clauseK<float>(p, cl, kk, cxs, cxl, curV, emit, sat)
-                                : clause3<float>(p, cl, cxs, cxl, curV, emit, sat);
+                for (; c < m; c += BS) {
+                    const int32_t l0 = n0, l1 = n1, l2 = n2;
+                    float xs = nxs, xl = nxl;
+                    const int cn = c + BS;
+                    if (cn < m) {
+                        n0 = I.lits[3 * cn];
+                        n1 = I.lits[3 * cn + 1];
+                        n2 = I.lits[3 * cn + 2];
+                        nxs = I.xs[cn];
+                        nxl = I.xl[cn];
+                    }
+                    if (pend) {
+                        float Cp = clause3v_C<float>(l0, l1, l2, prevV);
+                        memory_update(p, Cp, h, xs, xl);
+                        I.xs[c] = xs;
+                        I.xl[c] = xl;
+                    }
+                    const float cxs = clamp01(xs), cxl = clampXl(xl, p.xlmax);
+                    bool sat;
+                    float C;
+                    if (MODE == BM_LDS_ALL) {
+                        const int32_t *lp = I.lpos + 3 * c;
+                        auto emit = [&](int j, float val) { se[lp[j]] = val; };
+                        C = clause3v<float>(p, l0, l1, l2, cxs, cxl, curV, emit, sat);
+                    } else {
+                        float *ep = I.e + 3 * c;
+                        auto emit = [&](int j, float val) { ep[j] = val; };
+                        C = clause3v<float>(p, l0, l1, l2, cxs, cxl, curV, emit, sat);
+                    }
+                    unsolved |= !(C < 0.5f) || !sat;
                 }
-                unsolved |= !(C < 0.5f) || !sat;
+            } else {
+                for (int c = tid; c < m; c += BS) {
+                    const int b = I.cp[c];
+                    const int kk = I.cp[c + 1] - b;
+                    const int32_t *cl = I.lits + b;
+                    float xs = I.xs[c], xl = I.xl[c];
+                    if (pend) {
+                        float Cp = clauseK_C<float>(cl, kk, prevV);
+                        memory_update(p, Cp, h, xs, xl);
+                        I.xs[c] = xs;
+                        I.xl[c] = xl;
+                    }
+                    const float cxs = clamp01(xs), cxl = clampXl(xl, p.xlmax);
+                    bool sat;
+                    float C;
+                    if (MODE == BM_LDS_ALL) {
+                        const int32_t *lp = I.lpos + b;
+                        auto emit = [&](int j, float val) { se[lp[j]] = val; };
+                        C = clauseK<float>(p, cl, kk, cxs, cxl, curV, emit, sat);
+                    } else {
+                        float *ep = I.e + b;
+                        auto emit = [&](int j, float val) { ep[j] = val; };
+                        C = clauseK<float>(p, cl, kk, cxs, cxl, curV, emit, sat);
+                    }
+                    unsolved |= !(C < 0.5f) || !sat;
+                }
             }
             const bool any_unsolved = __syncthreads_or(unsolved);
             if (d.stop_on_solve && !any_unsolved)

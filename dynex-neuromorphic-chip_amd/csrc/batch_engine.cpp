@@ -247,12 +247,19 @@ static int batch_enqueue(dnx_batch *b, int ksteps, const BDyn *known)
                 lds = std::max(lds, (int)need);
             }
         }
-        int per_cu = batch_occupancy(md, sh.bs, lds);
+        // When few instances remain, each gets a larger workgroup: the aim is
+        // about four waves per SIMD across the chip, so lone long-running
+        // instances (the tail of a batch) are not left latency-bound on a
+        // small workgroup.
+        int bs = sh.bs;
+        const long long target_waves = 16LL * b->cus;
+        while (bs < 1024 && (long long)order.size() * (bs / 64) < target_waves) bs *= 2;
+        int per_cu = batch_occupancy(md, bs, lds);
         if (per_cu < 1) per_cu = 1;
         int grid = std::min(cnt[md], b->cus * per_cu);
         const bool timed = b->timing && !b->t_pending[md];
         if (timed && (e = hipEventRecord(b->t0[md], ms)) != hipSuccess) return hip_fail(e, "timing");
-        launch_batch(md, sh.bs, lds, grid, (const BInst *)b->b_inst.ptr, (BDyn *)b->b_dyn.ptr, dw + (size_t)md * S,
+        launch_batch(md, bs, lds, grid, (const BInst *)b->b_inst.ptr, (BDyn *)b->b_dyn.ptr, dw + (size_t)md * S,
                      cnt[md], dh + md, ksteps, ms);
         if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "batch launch");
         if (timed) {

@@ -128,12 +128,11 @@ __device__ __forceinline__ bool lit_true(int32_t lit, double x)
 // emit(j, value) receives literal j's contribution.  Returns C and sets
 // `sat` to the exact sign check of the raw values (dynex.cc:674-687).
 // ---------------------------------------------------------------------------
-template <typename T, typename GetV, typename Emit>
-__device__ __forceinline__ T clause3(const PT<T> &p, const int32_t *cl, T Xs, T Xl, GetV getV,
-                                     Emit emit, bool &sat)
+// r0..r2 are the raw state values of the clause's variables.
+template <typename T, typename Emit>
+__device__ __forceinline__ T clause3_vals(const PT<T> &p, int32_t l0, int32_t l1, int32_t l2, T r0, T r1, T r2, T Xs,
+                                          T Xl, Emit emit, bool &sat)
 {
-    int32_t l0 = cl[0], l1 = cl[1], l2 = cl[2];
-    T r0 = getV(abs(l0) - 1), r1 = getV(abs(l1) - 1), r2 = getV(abs(l2) - 1);
     sat = lit_true(l0, (double)r0) || lit_true(l1, (double)r1) || lit_true(l2, (double)r2);
     T q0 = l0 > 0 ? T(1) : T(-1), q1 = l1 > 0 ? T(1) : T(-1), q2 = l2 > 0 ? T(1) : T(-1);
     T V0 = clampV(r0), V1 = clampV(r1), V2 = clampV(r2);
@@ -153,16 +152,39 @@ __device__ __forceinline__ T clause3(const PT<T> &p, const int32_t *cl, T Xs, T 
     return C;
 }
 
+template <typename T>
+__device__ __forceinline__ T clause3_C_vals(int32_t l0, int32_t l1, int32_t l2, T r0, T r1, T r2)
+{
+    T i0 = T(1.0) - (l0 > 0 ? T(1) : T(-1)) * clampV(r0);
+    T i1 = T(1.0) - (l1 > 0 ? T(1) : T(-1)) * clampV(r1);
+    T i2 = T(1.0) - (l2 > 0 ? T(1) : T(-1)) * clampV(r2);
+    return clamp01(tmin(i0, tmin(i1, i2)) / T(2.0));
+}
+
+template <typename T, typename GetV, typename Emit>
+__device__ __forceinline__ T clause3v(const PT<T> &p, int32_t l0, int32_t l1, int32_t l2, T Xs, T Xl, GetV getV,
+                                      Emit emit, bool &sat)
+{
+    return clause3_vals<T>(p, l0, l1, l2, getV(abs(l0) - 1), getV(abs(l1) - 1), getV(abs(l2) - 1), Xs, Xl, emit,
+                           sat);
+}
+
+template <typename T, typename GetV, typename Emit>
+__device__ __forceinline__ T clause3(const PT<T> &p, const int32_t *cl, T Xs, T Xl, GetV getV, Emit emit, bool &sat)
+{
+    return clause3v<T>(p, cl[0], cl[1], cl[2], Xs, Xl, getV, emit, sat);
+}
+
 // C of a 3-literal clause only (for the lazy memory update).
+template <typename T, typename GetV>
+__device__ __forceinline__ T clause3v_C(int32_t l0, int32_t l1, int32_t l2, GetV getV)
+{
+    return clause3_C_vals<T>(l0, l1, l2, getV(abs(l0) - 1), getV(abs(l1) - 1), getV(abs(l2) - 1));
+}
 template <typename T, typename GetV>
 __device__ __forceinline__ T clause3_C(const int32_t *cl, GetV getV)
 {
-    T V0 = clampV(getV(abs(cl[0]) - 1)), V1 = clampV(getV(abs(cl[1]) - 1)),
-      V2 = clampV(getV(abs(cl[2]) - 1));
-    T i0 = T(1.0) - (cl[0] > 0 ? T(1) : T(-1)) * V0;
-    T i1 = T(1.0) - (cl[1] > 0 ? T(1) : T(-1)) * V1;
-    T i2 = T(1.0) - (cl[2] > 0 ? T(1) : T(-1)) * V2;
-    return clamp01(tmin(i0, tmin(i1, i2)) / T(2.0));
+    return clause3v_C<T>(cl[0], cl[1], cl[2], getV);
 }
 
 // Any clause length (dynex.cc:420-494 for k=2 and k>=4, 517 for k=1; the

@@ -49,7 +49,7 @@ __device__ __forceinline__ T fold_var(const T *__restrict__ e, const int32_t *__
     // contributions (which the reference never pushes) do not change an
     // ascending fold from +0, so every slot is folded.
     const int t = threadIdx.x;
-    if (deg <= CAP) {
+    if (CAP > 0 && deg <= CAP) {
         // stage the gathered values (eight loads in flight), then sort
         int j = 0;
         for (; j + 8 <= deg; j += 8) {
@@ -98,8 +98,6 @@ __device__ __forceinline__ T fold_var(const T *__restrict__ e, const int32_t *__
 }
 
 constexpr int CBS = 256;  // clause-pass block size
-constexpr int VBS = 64;   // variable-pass block size
-constexpr int SORT_CAP = 64;
 
 // ---------------------------------------------------------------------------
 // Resident-state step: clause pass.
@@ -157,9 +155,29 @@ __global__ __launch_bounds__(CBS) void k_step_clause(DevProb P, PT<T> p, const P
     }
     const int c = blockIdx.x * CBS + threadIdx.x;
     int unsolved = 0;
-    if (c < P.m) {
-        const int b = U3 ? 3 * c : P.cp[c];
-        const int k = U3 ? 3 : P.cp[c + 1] - b;
+    if (U3 && c < P.m) {
+        // one gather of each [V, dV] pair serves both the previous state's C
+        // (lazy memory update) and the new state's V
+        const int32_t l0 = P.lits[3 * c], l1 = P.lits[3 * c + 1], l2 = P.lits[3 * c + 2];
+        const Pair<T> a0 = vd[abs(l0) - 1], a1 = vd[abs(l1) - 1], a2 = vd[abs(l2) - 1];
+        T xs = Xs[c], xl = Xl[c];
+        T r0 = a0.v, r1 = a1.v, r2 = a2.v;
+        if (pend) {
+            memory_update(p, clause3_C_vals<T>(l0, l1, l2, r0, r1, r2), h, xs, xl);
+            Xs[c] = xs;
+            Xl[c] = xl;
+            r0 = clampV(a0.v + h * a0.d);
+            r1 = clampV(a1.v + h * a1.d);
+            r2 = clampV(a2.v + h * a2.d);
+        }
+        T *ep = e + 3 * c;
+        auto emit = [&](int j, T val) { ep[j] = val; };
+        bool sat;
+        T C = clause3_vals<T>(p, l0, l1, l2, r0, r1, r2, clamp01(xs), clampXl(xl, p.xlmax), emit, sat);
+        unsolved = !(C < T(0.5)) || !sat;
+    } else if (!U3 && c < P.m) {
+        const int b = P.cp[c];
+        const int k = P.cp[c + 1] - b;
         const int32_t *cl = P.lits + b;
         T xs = Xs[c], xl = Xl[c];
         auto prevV = [&](int v) { return vd[v].v; };
@@ -168,7 +186,7 @@ __global__ __launch_bounds__(CBS) void k_step_clause(DevProb P, PT<T> p, const P
             return pend ? clampV(a.v + h * a.d) : a.v;
         };
         if (pend) {
-            T Cp = U3 ? clause3_C<T>(cl, prevV) : clauseK_C<T>(cl, k, prevV);
+            T Cp = clauseK_C<T>(cl, k, prevV);
             memory_update(p, Cp, h, xs, xl);
             Xs[c] = xs;
             Xl[c] = xl;
@@ -177,8 +195,7 @@ __global__ __launch_bounds__(CBS) void k_step_clause(DevProb P, PT<T> p, const P
         T *ep = e + b;
         auto emit = [&](int j, T val) { ep[j] = val; };
         bool sat;
-        T C = U3 ? clause3<T>(p, cl, cxs, cxl, curV, emit, sat)
-                 : clauseK<T>(p, cl, k, cxs, cxl, curV, emit, sat);
+        T C = clauseK<T>(p, cl, k, cxs, cxl, curV, emit, sat);
         unsolved = !(C < T(0.5)) || !sat;
     }
     if (__syncthreads_or(unsolved) && threadIdx.x == 0) raise_flag(ctl->flags[slot][FG_UNSOLVED]);
@@ -187,11 +204,70 @@ __global__ __launch_bounds__(CBS) void k_step_clause(DevProb P, PT<T> p, const P
 // ---------------------------------------------------------------------------
 // Resident-state step: variable pass.
 // ---------------------------------------------------------------------------
-template <typename T, bool SORT, int BS>
+// Wave-cooperative fold: a wave owns 64 consecutive variables, whose
+// transpose slots are contiguous.  The wave stages all their contributions
+// in LDS (coalesced index loads, independent gathers, eight per lane in
+// flight), then each lane folds its own segment in slot order -- sorted
+// first in FP64 -- exactly as a lane-serial fold would.  A wave whose
+// segment exceeds the staging area folds from device memory instead.
+// Must be reached by every thread of the block (it synchronises).
+template <typename T, bool SORT, int CAPW>
+__device__ __forceinline__ T wave_fold(const T *__restrict__ e, const int32_t *__restrict__ eidx,
+                                       const int32_t *__restrict__ row_ptr, int n, int v0, bool active, T *stage,
+                                       T *sortbuf_unused)
+{
+    (void)sortbuf_unused;
+    const int lane = threadIdx.x & 63;
+    const int v = v0 + lane;
+    const int vend = min(v0 + 64, n);
+    const int sb = v0 < n ? row_ptr[v0] : 0;
+    const int total = v0 < n ? row_ptr[vend] - sb : 0;
+    const bool fits = total <= CAPW;
+    if (active && fits) {
+        int q = lane;
+        for (; q + 7 * 64 < total; q += 8 * 64) {
+            int i[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) i[u] = eidx[sb + q + u * 64];
+#pragma unroll
+            for (int u = 0; u < 8; u++) stage[q + u * 64] = e[i[u]];
+        }
+        for (; q < total; q += 64) stage[q] = e[eidx[sb + q]];
+    }
+    __syncthreads();
+    T acc = T(0.0);
+    if (active && v < n) {
+        const int rb = row_ptr[v], re = row_ptr[v + 1];
+        if (fits) {
+            T *s = stage + (rb - sb);
+            const int deg = re - rb;
+            if (SORT) {
+                // dynex.cc:527-530: ascending sort, left fold from +0
+                for (int j = 1; j < deg; j++) {
+                    T x = s[j];
+                    int k = j;
+                    while (k > 0 && s[k - 1] > x) {
+                        s[k] = s[k - 1];
+                        k--;
+                    }
+                    s[k] = x;
+                }
+            }
+            for (int j = 0; j < deg; j++) acc = acc + s[j];
+        } else if (SORT) {
+            acc = fold_var<T, true, 64, 0>(e, eidx, rb, re - rb, nullptr);
+        } else {
+            acc = fold_gather<T>(e, eidx, rb, re);
+        }
+    }
+    return acc;
+}
+
+template <typename T, bool SORT, int BS, int CAPW>
 __global__ __launch_bounds__(BS) void k_step_var(DevProb P, Pair<T> *__restrict__ vd, const T *__restrict__ e,
                                                  Ctl *__restrict__ ctl, int slot)
 {
-    __shared__ T buf[SORT ? SORT_CAP * BS : 1];
+    __shared__ T stage[BS / 64][CAPW];
     __shared__ int sh[BS / 64];
     if (ctl->status != DNX_RUNNING) return;
     const int ps = (slot + 2) % 3;
@@ -200,15 +276,17 @@ __global__ __launch_bounds__(BS) void k_step_var(DevProb P, Pair<T> *__restrict_
     const T h = pend ? level_h<T>(dev_level(ctl, ps)) : T(0.0);
     const bool solved = ctl->stop_on_solve && !any_flag(ctl->flags[slot][FG_UNSOLVED]);
     const int done = solved ? DNX_SAT : (ctl->steps >= ctl->cap ? DNX_UNKNOWN : 0);
-    const int v = blockIdx.x * BS + threadIdx.x;
+    const int w = threadIdx.x >> 6;
+    const int v0 = (blockIdx.x * (BS / 64) + w) * 64;
+    const int v = v0 + (threadIdx.x & 63);
+    const T dVf = wave_fold<T, SORT, CAPW>(e, P.eidx, P.row_ptr, P.n, v0, !done, stage[w], nullptr);
     int f = 0;
     if (v < P.n) {
         Pair<T> a = vd[v];
         T V = pend ? clampV(a.v + h * a.d) : a.v;
         T dV = T(0.0);
         if (!done) {
-            const int b = P.row_ptr[v];
-            dV = fold_var<T, SORT, BS, SORT_CAP>(e, P.eidx, b, P.row_ptr[v + 1] - b, buf);
+            dV = dVf;
             if (P.unit && P.unit[v] != 0) dV = T(0.0);  // pinned (dynex.cc:306)
             f = fail_level(V, dV);
         }
@@ -278,14 +356,15 @@ __global__ __launch_bounds__(CBS) void k_rhs_clause(DevProb P, PT<T> p, const T 
     }
 }
 
-template <typename T, bool SORT>
-__global__ __launch_bounds__(VBS) void k_rhs_var(DevProb P, const T *__restrict__ e, T *__restrict__ dxdt)
+template <typename T, bool SORT, int BS, int CAPW>
+__global__ __launch_bounds__(BS) void k_rhs_var(DevProb P, const T *__restrict__ e, T *__restrict__ dxdt)
 {
-    __shared__ T buf[SORT ? SORT_CAP * VBS : 1];
-    const int v = blockIdx.x * VBS + threadIdx.x;
-    if (v >= P.n) return;
-    const int b = P.row_ptr[v];
-    dxdt[v] = fold_var<T, SORT, VBS, SORT_CAP>(e, P.eidx, b, P.row_ptr[v + 1] - b, buf);
+    __shared__ T stage[BS / 64][CAPW];
+    const int w = threadIdx.x >> 6;
+    const int v0 = (blockIdx.x * (BS / 64) + w) * 64;
+    const int v = v0 + (threadIdx.x & 63);
+    const T d = wave_fold<T, SORT, CAPW>(e, P.eidx, P.row_ptr, P.n, v0, true, stage[w], nullptr);
+    if (v < P.n) dxdt[v] = d;
 }
 
 template <typename T>
@@ -376,7 +455,8 @@ static void rhs_impl(const DevProb &P, const DevParams &prm, const void *x, void
         k_rhs_clause<T, true><<<gc, CBS, 0, st>>>(P, p, (const T *)x, (T *)dxdt, (T *)e, lpart, epart);
     else
         k_rhs_clause<T, false><<<gc, CBS, 0, st>>>(P, p, (const T *)x, (T *)dxdt, (T *)e, lpart, epart);
-    if (P.n > 0) k_rhs_var<T, SORT><<<nblk(P.n, VBS), VBS, 0, st>>>(P, (const T *)e, (T *)dxdt);
+    constexpr int BS = SORT ? 128 : 256;
+    if (P.n > 0) k_rhs_var<T, SORT, BS, 2048><<<nblk(P.n, BS), BS, 0, st>>>(P, (const T *)e, (T *)dxdt);
 }
 
 void launch_rhs(int prec, const DevProb &P, const DevParams &prm, const void *x, void *dxdt, void *e, int *loc_part,
@@ -416,12 +496,9 @@ static void step_impl(const DevProb &P, const DevParams &prm, void *vd, void *xs
     else
         k_step_clause<T, false><<<gc, CBS, 0, st>>>(P, p, (const Pair<T> *)vd, (T *)xs, (T *)xl, (T *)e, ctl,
                                                     slot);
-    if (SORT)
-        k_step_var<T, SORT, VBS><<<nblk(P.n > 0 ? P.n : 1, VBS), VBS, 0, st>>>(P, (Pair<T> *)vd, (const T *)e, ctl,
-                                                                          slot);
-    else
-        k_step_var<T, SORT, 256><<<nblk(P.n > 0 ? P.n : 1, 256), 256, 0, st>>>(P, (Pair<T> *)vd, (const T *)e, ctl,
-                                                                          slot);
+    constexpr int BS = SORT ? 128 : 256;
+    k_step_var<T, SORT, BS, 2048><<<nblk(P.n > 0 ? P.n : 1, BS), BS, 0, st>>>(P, (Pair<T> *)vd, (const T *)e, ctl,
+                                                                             slot);
 }
 
 void launch_step(int prec, const DevProb &P, const DevParams &prm, void *vd, void *xs, void *xl, void *e, Ctl *ctl,
