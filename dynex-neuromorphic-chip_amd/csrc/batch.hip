@@ -201,6 +201,228 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
     }
 }
 
+// ---------------------------------------------------------------------------
+// Wide mode.  When few instances remain, one workgroup per instance leaves
+// most of the chip idle and each instance latency-bound.  Wide mode advances
+// the remaining instances together with one clause kernel and one variable
+// kernel per step, spread over the union of their clauses and variables: a
+// block map names each block's instance and its local block index.  Each
+// instance keeps its own control block (the single-instance Ctl protocol,
+// indexed by a rotation r shared by all) and so its own h, t and step
+// count; the arithmetic is the single-instance FP32 arithmetic.
+// ---------------------------------------------------------------------------
+constexpr int WBS = 256;
+constexpr int WCAPW = 2048;
+
+// BDyn -> Ctl for the listed slots; pending levels go to rotation (r0+2)%3.
+__global__ __launch_bounds__(256) void k_wide_enter(const BDyn *__restrict__ dyn, Ctl *__restrict__ wctl,
+                                                    const int *__restrict__ list, int count, int r0)
+{
+    const int i = blockIdx.x;
+    if (i >= count) return;
+    const int slot = list[i];
+    Ctl *c = &wctl[slot];
+    int *f = &c->flags[0][0][0];
+    for (int j = threadIdx.x; j < 3 * FG_COUNT * FLAG_GROUP; j += blockDim.x) f[j] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const BDyn d = dyn[slot];
+        c->status = d.status;
+        c->stop_on_solve = d.stop_on_solve;
+        c->steps = d.steps;
+        c->cap = d.cap;
+        c->t = d.t;
+        c->t_done = 0;
+        for (int s = 0; s < 3; s++) {
+            c->stepped[s] = 0;
+            c->done[s] = 0;
+        }
+        if (d.plevel >= 0) {
+            const int ps = (r0 + 2) % 3;
+            c->stepped[ps] = 1;
+            for (int L = 0; L < d.plevel; L++) c->flags[ps][FG_LEVEL + L][0] = 1;
+        }
+    }
+}
+
+// Ctl -> BDyn after a chunk whose last step used rotation rl.
+__global__ __launch_bounds__(64) void k_wide_sync(BDyn *__restrict__ dyn, Ctl *__restrict__ wctl,
+                                                  const int *__restrict__ list, int count, int rl)
+{
+    const int i = blockIdx.x;
+    if (i >= count) return;
+    const int slot = list[i];
+    BDyn o = dyn[slot];
+    if (o.status != DNX_RUNNING) return;
+    Ctl *c = &wctl[slot];
+    const int lev = dev_level(c, rl);
+    if (threadIdx.x == 0) {
+        int st = c->status;
+        if (st == DNX_RUNNING && c->done[rl]) st = c->done[rl];
+        o.steps = c->steps;
+        o.t = c->t;
+        if (st != DNX_RUNNING) {
+            c->status = st;
+            o.status = st;
+            o.plevel = -1;
+            o.t_done = c->t_done;
+        } else {
+            o.plevel = c->stepped[rl] ? lev : -1;
+        }
+        dyn[slot] = o;
+    }
+}
+
+__global__ __launch_bounds__(WBS) void k_wide_clause(const BInst *__restrict__ insts, Ctl *__restrict__ wctl,
+                                                     const int2 *__restrict__ bmap, int nblk, int r)
+{
+    const int b = blockIdx.x;
+    if (b >= nblk) return;
+    const int2 bm = bmap[b];
+    const int slot = __builtin_amdgcn_readfirstlane(bm.x), lb = __builtin_amdgcn_readfirstlane(bm.y);
+    Ctl *__restrict__ ctl = &wctl[slot];
+    if (ctl->status != DNX_RUNNING) return;
+    const int ps = (r + 2) % 3;
+    const int done = ctl->done[ps];
+    if (done) {
+        if (lb == 0 && threadIdx.x == 0) ctl->status = done;
+        return;
+    }
+    const BInst I = insts[slot];
+    const bool pend = ctl->stepped[ps] != 0;
+    const int plev = dev_level(ctl, ps);
+    const float h = pend ? level_h<float>(plev) : 0.0f;
+    if (pend && lb == 0 && threadIdx.x == 0) {
+        ctl->t += level_dt(plev);
+        ctl->steps += 1;
+    }
+    PT<float> p;
+    p.alpha = I.alpha;
+    p.beta = I.beta;
+    p.gamma = I.gamma;
+    p.delta = I.delta;
+    p.eps = I.eps;
+    p.zeta = I.zeta;
+    p.xlmax = I.xlmax;
+    const int c = lb * WBS + threadIdx.x;
+    int unsolved = 0;
+    if (!(I.flags & BF_GENERAL) && c < I.m) {
+        const int32_t l0 = I.lits[3 * c], l1 = I.lits[3 * c + 1], l2 = I.lits[3 * c + 2];
+        const BF2 a0 = I.vd[abs(l0) - 1], a1 = I.vd[abs(l1) - 1], a2 = I.vd[abs(l2) - 1];
+        float xs = I.xs[c], xl = I.xl[c];
+        float r0 = a0.v, r1 = a1.v, r2 = a2.v;
+        if (pend) {
+            memory_update(p, clause3_C_vals<float>(l0, l1, l2, r0, r1, r2), h, xs, xl);
+            I.xs[c] = xs;
+            I.xl[c] = xl;
+            r0 = clampV(a0.v + h * a0.d);
+            r1 = clampV(a1.v + h * a1.d);
+            r2 = clampV(a2.v + h * a2.d);
+        }
+        float *ep = I.e + 3 * c;
+        auto emit = [&](int j, float val) { ep[j] = val; };
+        bool sat;
+        float C = clause3_vals<float>(p, l0, l1, l2, r0, r1, r2, clamp01(xs), clampXl(xl, p.xlmax), emit, sat);
+        unsolved = !(C < 0.5f) || !sat;
+    } else if ((I.flags & BF_GENERAL) && c < I.m) {
+        const int bb = I.cp[c];
+        const int k = I.cp[c + 1] - bb;
+        const int32_t *cl = I.lits + bb;
+        float xs = I.xs[c], xl = I.xl[c];
+        auto prevV = [&](int v) { return I.vd[v].v; };
+        auto curV = [&](int v) {
+            BF2 a = I.vd[v];
+            return pend ? clampV(a.v + h * a.d) : a.v;
+        };
+        if (pend) {
+            memory_update(p, clauseK_C<float>(cl, k, prevV), h, xs, xl);
+            I.xs[c] = xs;
+            I.xl[c] = xl;
+        }
+        float *ep = I.e + bb;
+        auto emit = [&](int j, float val) { ep[j] = val; };
+        bool sat;
+        float C = clauseK<float>(p, cl, k, clamp01(xs), clampXl(xl, p.xlmax), curV, emit, sat);
+        unsolved = !(C < 0.5f) || !sat;
+    }
+    if (__syncthreads_or(unsolved) && threadIdx.x == 0) raise_flag(ctl->flags[r][FG_UNSOLVED]);
+}
+
+__global__ __launch_bounds__(WBS) void k_wide_var(const BInst *__restrict__ insts, Ctl *__restrict__ wctl,
+                                                  const int2 *__restrict__ vmap, int nblk, int r)
+{
+    __shared__ float stage[WBS / 64][WCAPW];
+    __shared__ int sh[WBS / 64];
+    const int b = blockIdx.x;
+    if (b >= nblk) return;
+    const int2 bm = vmap[b];
+    const int slot = __builtin_amdgcn_readfirstlane(bm.x), lb = __builtin_amdgcn_readfirstlane(bm.y);
+    Ctl *__restrict__ ctl = &wctl[slot];
+    if (ctl->status != DNX_RUNNING) return;
+    const int ps = (r + 2) % 3;
+    if (ctl->done[ps]) return;
+    const BInst I = insts[slot];
+    const bool pend = ctl->stepped[ps] != 0;
+    const float h = pend ? level_h<float>(dev_level(ctl, ps)) : 0.0f;
+    const bool solved = ctl->stop_on_solve && !any_flag(ctl->flags[r][FG_UNSOLVED]);
+    const int done = solved ? DNX_SAT : (ctl->steps >= ctl->cap ? DNX_UNKNOWN : 0);
+    const int w = threadIdx.x >> 6;
+    const int v0 = (lb * (WBS / 64) + w) * 64;
+    const int v = v0 + (threadIdx.x & 63);
+    const float dVf = wave_fold<float, false, WCAPW>(I.e, I.eidx, I.row_ptr, I.n, v0, !done, stage[w], nullptr);
+    int f = 0;
+    if (v < I.n) {
+        BF2 a = I.vd[v];
+        const float V = pend ? clampV(a.v + h * a.d) : a.v;
+        float dV = 0.0f;
+        if (!done) {
+            dV = dVf;
+            if ((I.flags & BF_UNIT) && I.unit[v] != 0) dV = 0.0f;  // dynex.cc:306
+            f = fail_level(V, dV);
+        }
+        BF2 o;
+        o.v = V;
+        o.d = dV;
+        I.vd[v] = o;
+    }
+    const int ns = (r + 1) % 3;
+    if (done) {
+        if (lb == 0) {
+            if (threadIdx.x == 0) {
+                ctl->done[r] = done;
+                ctl->t_done = wall_clock64();
+            }
+            clear_slot(ctl, ns);
+        }
+        return;
+    }
+    const int bmx = block_max_int<WBS>(f, sh);
+    if (threadIdx.x == 0)
+        for (int L = 0; L < bmx; L++) raise_flag(ctl->flags[r][FG_LEVEL + L]);
+    if (lb == 0) {
+        if (threadIdx.x == 0) ctl->stepped[r] = 1;
+        clear_slot(ctl, ns);
+    }
+}
+
+void launch_wide_enter(const BDyn *dyn, Ctl *wctl, const int *list, int count, int r0, void *stream)
+{
+    if (count > 0) k_wide_enter<<<count, 256, 0, (hipStream_t)stream>>>(dyn, wctl, list, count, r0);
+}
+
+void launch_wide_sync(BDyn *dyn, Ctl *wctl, const int *list, int count, int rl, void *stream)
+{
+    if (count > 0) k_wide_sync<<<count, 64, 0, (hipStream_t)stream>>>(dyn, wctl, list, count, rl);
+}
+
+void launch_wide_step(const BInst *insts, Ctl *wctl, const int *bmap, int nbc, const int *vmap, int nbv, int r,
+                      void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    if (nbc > 0) k_wide_clause<<<nbc, WBS, 0, st>>>(insts, wctl, (const int2 *)bmap, nbc, r);
+    if (nbv > 0) k_wide_var<<<nbv, WBS, 0, st>>>(insts, wctl, (const int2 *)vmap, nbv, r);
+}
+
 __global__ void k_batch_zero(int *p, int count)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;

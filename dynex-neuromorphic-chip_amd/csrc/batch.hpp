@@ -4,6 +4,8 @@
 
 #include <cstdint>
 
+#include "kernels.hpp"
+
 namespace dnx {
 
 // Execution shape of one instance inside the fused step kernel.
@@ -57,6 +59,13 @@ void launch_batch_stamp(unsigned long long *p, void *stream);
 // materialised state of one slot in the reference layout (float)
 void launch_batch_materialize(const BInst *inst, const BDyn *dyn, int n, int m, float *x_out, void *stream);
 int batch_max_threads();
+
+// wide mode (see batch.hip); maps hold (slot, local block) pairs
+constexpr int WIDE_BLOCK = 256;  // clauses or variables per wide block
+void launch_wide_enter(const BDyn *dyn, Ctl *wctl, const int *list, int count, int r0, void *stream);
+void launch_wide_sync(BDyn *dyn, Ctl *wctl, const int *list, int count, int rl, void *stream);
+void launch_wide_step(const BInst *insts, Ctl *wctl, const int *bmap, int nbc, const int *vmap, int nbv, int r,
+                      void *stream);
 // resident workgroups per CU for a shape (0 if it cannot launch)
 int batch_occupancy(int mode, int bs, int lds_bytes);
 

@@ -107,7 +107,8 @@ struct dnx_batch {
     std::vector<BInst> h_inst;
     std::vector<SlotHost> slots;
     std::vector<int> free_slots;
-    BDyn *h_dyn = nullptr;  // pinned: [2][nslots]
+    BDyn *h_dyn = nullptr;    // pinned: [2][nslots] status snapshots
+    BDyn *h_stage = nullptr;  // pinned: [nslots] source of status uploads
     int *h_work = nullptr;  // pinned: [2][BM_COUNT * nslots]
     Shape shape[BM_COUNT] = {{256, 48 * 1024}, {256, 40 * 1024}, {1024, 0}};
     int cus = 0, lds_max = 0;
@@ -117,26 +118,42 @@ struct dnx_batch {
     bool ev_pending[2] = {false, false};
     hipStream_t mstream[BM_COUNT] = {nullptr, nullptr, nullptr};
     hipEvent_t fork = nullptr, join[BM_COUNT] = {nullptr, nullptr, nullptr};
+    // admission uploads run on their own stream and complete before admit
+    // returns, so they never wait behind queued kernels
+    hipStream_t ustream = nullptr;
     // optional per-shape launch timing (start/stop events around each launch)
     bool timing = false;
-    hipEvent_t t0[BM_COUNT] = {nullptr, nullptr, nullptr}, t1[BM_COUNT] = {nullptr, nullptr, nullptr};
-    bool t_pending[BM_COUNT] = {false, false, false};
-    double t_ms[BM_COUNT] = {0, 0, 0};
-    long long t_launches[BM_COUNT] = {0, 0, 0};
+    // timing buckets: the three fused-kernel modes, then wide mode
+    hipEvent_t t0[BM_COUNT + 1] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t t1[BM_COUNT + 1] = {nullptr, nullptr, nullptr, nullptr};
+    bool t_pending[BM_COUNT + 1] = {false, false, false, false};
+    double t_ms[BM_COUNT + 1] = {0, 0, 0, 0};
+    long long t_launches[BM_COUNT + 1] = {0, 0, 0, 0};
+    // wide mode: entered when the running instances hold at most
+    // wide_clauses clauses in total; wide_chunk steps between status syncs
+    long long wide_clauses = 1500000;
+    int wide_chunk = 30;
+    // serving: requests with at least this many clauses run in the wide lane
+    long long serve_big = 60000;
+    DevBuf b_wctl, b_wlist, b_bmap, b_vmap;
     ~dnx_batch()
     {
         if (arena_base) (void)hipFree(arena_base);
         if (h_dyn) (void)hipHostFree(h_dyn);
+        if (h_stage) (void)hipHostFree(h_stage);
         if (h_work) (void)hipHostFree(h_work);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
         for (int i = 0; i < BM_COUNT; i++) {
             if (mstream[i]) (void)hipStreamDestroy(mstream[i]);
             if (join[i]) (void)hipEventDestroy(join[i]);
+        }
+        for (int i = 0; i <= BM_COUNT; i++) {
             if (t0[i]) (void)hipEventDestroy(t0[i]);
             if (t1[i]) (void)hipEventDestroy(t1[i]);
         }
         if (fork) (void)hipEventDestroy(fork);
+        if (ustream) (void)hipStreamDestroy(ustream);
     }
 };
 
@@ -167,14 +184,16 @@ static Layout layout_of(int n, int m, int64_t nnz, bool general, bool has_unit, 
     };
     L.lits = take((size_t)nnz * 4);
     L.cp = general ? take(((size_t)m + 1) * 4) : 0;
-    L.lpos = mode == BM_LDS_ALL ? take((size_t)nnz * 4) : 0;
-    L.eidx = mode != BM_LDS_ALL ? take((size_t)nnz * 4) : 0;
+    // both transposes and the device contribution array exist for every
+    // instance, so any instance can also run in wide mode
+    L.lpos = take((size_t)nnz * 4);
+    L.eidx = take((size_t)nnz * 4);
     L.row = take(((size_t)n + 1) * 4);
     L.unit = has_unit ? take((size_t)n) : 0;
     L.xs = take((size_t)m * 4);
     L.xl = take((size_t)m * 4);
     L.vd = take((size_t)n * sizeof(BF2));
-    L.e = mode != BM_LDS_ALL ? take((size_t)nnz * 4) : 0;
+    L.e = take((size_t)nnz * 4);
     L.total = o ? o : 256;
     return L;
 }
@@ -183,7 +202,7 @@ static Layout layout_of(int n, int m, int64_t nnz, bool general, bool has_unit, 
 // has waited for the work).
 static void batch_collect_timing(dnx_batch *b)
 {
-    for (int md = 0; md < BM_COUNT; md++) {
+    for (int md = 0; md <= BM_COUNT; md++) {
         if (!b->t_pending[md]) continue;
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, b->t0[md], b->t1[md]) == hipSuccess) {
@@ -203,16 +222,28 @@ static int batch_sync(dnx_batch *b)
 // Enqueue one advance of every running slot by up to ksteps steps.  The work
 // lists come from `known` (the last polled status); slots that ended since
 // are skipped by the kernel itself.
+static int batch_enqueue_slots(dnx_batch *b, int ksteps, const std::vector<int> &slots);
+
 static int batch_enqueue(dnx_batch *b, int ksteps, const BDyn *known)
+{
+    std::vector<int> run;
+    for (int s = 0; s < b->nslots; s++)
+        if (b->slots[(size_t)s].used && known[s].status == DNX_RUNNING) run.push_back(s);
+    return batch_enqueue_slots(b, ksteps, run);
+}
+
+// Enqueue one advance of the listed slots (fused kernel, one launch per
+// execution shape, each on its own stream, joined back into the context
+// stream).
+static int batch_enqueue_slots(dnx_batch *b, int ksteps, const std::vector<int> &slots)
 {
     hipStream_t st = b->ctx->stream;
     const int S = b->nslots;
     int *hw = b->h_work + (size_t)b->buf * BM_COUNT * S;
     int cnt[BM_COUNT] = {0, 0, 0};
     std::vector<std::pair<int, int>> order;
-    order.reserve((size_t)S);
-    for (int s = 0; s < S; s++)
-        if (b->slots[(size_t)s].used && known[s].status == DNX_RUNNING) order.push_back({b->slots[(size_t)s].m, s});
+    order.reserve(slots.size());
+    for (int s : slots) order.push_back({b->slots[(size_t)s].m, s});
     std::sort(order.begin(), order.end(), [](const std::pair<int, int> &x, const std::pair<int, int> &y) {
         return x.first != y.first ? x.first > y.first : x.second < y.second;
     });
@@ -308,6 +339,7 @@ int dnx_batch_open(dnx_ctx *ctx, int prec, int slots, int64_t arena_bytes, dnx_b
     b->arena.reset((size_t)arena_bytes);
     if ((e = hipHostMalloc((void **)&b->h_dyn, sizeof(BDyn) * 2 * (size_t)slots, hipHostMallocDefault)) !=
             hipSuccess ||
+        (e = hipHostMalloc((void **)&b->h_stage, sizeof(BDyn) * (size_t)slots, hipHostMallocDefault)) != hipSuccess ||
         (e = hipHostMalloc((void **)&b->h_work, sizeof(int) * 2 * BM_COUNT * (size_t)slots, hipHostMallocDefault)) !=
             hipSuccess) {
         delete b;
@@ -318,17 +350,26 @@ int dnx_batch_open(dnx_ctx *ctx, int prec, int slots, int64_t arena_bytes, dnx_b
             delete b;
             return hip_fail(e, "event");
         }
-    if ((e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) != hipSuccess) {
+    if ((e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&b->ustream, hipStreamNonBlocking)) != hipSuccess) {
         delete b;
         return hip_fail(e, "event");
     }
     for (int i = 0; i < BM_COUNT; i++)
         if ((e = hipStreamCreateWithFlags(&b->mstream[i], hipStreamNonBlocking)) != hipSuccess ||
-            (e = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming)) != hipSuccess ||
-            (e = hipEventCreate(&b->t0[i])) != hipSuccess || (e = hipEventCreate(&b->t1[i])) != hipSuccess) {
+            (e = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming)) != hipSuccess) {
             delete b;
             return hip_fail(e, "stream");
         }
+    for (int i = 0; i <= BM_COUNT; i++)
+        if ((e = hipEventCreate(&b->t0[i])) != hipSuccess || (e = hipEventCreate(&b->t1[i])) != hipSuccess) {
+            delete b;
+            return hip_fail(e, "event");
+        }
+    if ((rc = b->b_wctl.alloc(sizeof(Ctl) * (size_t)slots))) {
+        delete b;
+        return rc;
+    }
     hipDeviceProp_t prop;
     if ((e = hipGetDeviceProperties(&prop, ctx->device)) != hipSuccess) {
         delete b;
@@ -424,10 +465,8 @@ int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *p
     };
     put(L.lits, h.lits.data(), (size_t)nnz * 4);
     if (general) put(L.cp, h.cp.data(), ((size_t)m + 1) * 4);
-    if (mode == BM_LDS_ALL)
-        put(L.lpos, h.lpos.data(), (size_t)nnz * 4);
-    else
-        put(L.eidx, h.eidx.data(), (size_t)nnz * 4);
+    put(L.lpos, h.lpos.data(), (size_t)nnz * 4);
+    put(L.eidx, h.eidx.data(), (size_t)nnz * 4);
     put(L.row, h.row_ptr.data(), ((size_t)n + 1) * 4);
     if (h.has_unit) put(L.unit, h.unit.data(), (size_t)n);
     float *xs = (float *)(stage.data() + L.xs), *xl = (float *)(stage.data() + L.xl);
@@ -442,7 +481,7 @@ int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *p
     }
     char *base = (char *)b->arena_base + off;
     hipError_t e;
-    if ((e = hipMemcpyAsync(base, stage.data(), L.total, hipMemcpyHostToDevice, b->ctx->stream)) != hipSuccess) {
+    if ((e = hipMemcpyAsync(base, stage.data(), L.total, hipMemcpyHostToDevice, b->ustream)) != hipSuccess) {
         b->arena.release(off, L.total);
         return hip_fail(e, "admit upload");
     }
@@ -453,14 +492,14 @@ int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *p
     I.flags = (general ? BF_GENERAL : 0) | (h.has_unit ? BF_UNIT : 0);
     I.lits = (const int32_t *)(base + L.lits);
     I.cp = general ? (const int32_t *)(base + L.cp) : nullptr;
-    I.lpos = mode == BM_LDS_ALL ? (const int32_t *)(base + L.lpos) : nullptr;
-    I.eidx = mode != BM_LDS_ALL ? (const int32_t *)(base + L.eidx) : nullptr;
+    I.lpos = (const int32_t *)(base + L.lpos);
+    I.eidx = (const int32_t *)(base + L.eidx);
     I.row_ptr = (const int32_t *)(base + L.row);
     I.unit = h.has_unit ? (const int8_t *)(base + L.unit) : nullptr;
     I.xs = (float *)(base + L.xs);
     I.xl = (float *)(base + L.xl);
     I.vd = (BF2 *)(base + L.vd);
-    I.e = mode != BM_LDS_ALL ? (float *)(base + L.e) : nullptr;
+    I.e = (float *)(base + L.e);
     I.alpha = (float)prm->alpha;
     I.beta = (float)prm->beta;
     I.gamma = (float)prm->gamma;
@@ -477,18 +516,20 @@ int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *p
     b->h_inst[(size_t)slot] = I;
     // the descriptor and status copies read from host memory that outlives
     // the stream: h_inst is owned by the batch; D is staged in pinned memory
-    BDyn *pd0 = b->h_dyn + slot, *pd1 = b->h_dyn + b->nslots + slot;
-    *pd0 = D;
-    *pd1 = D;
+    // the upload's source is the slot's staging record: the snapshot buffers
+    // may be the target of status copies still in flight
+    b->h_stage[slot] = D;
+    b->h_dyn[slot] = D;
+    b->h_dyn[(size_t)b->nslots + slot] = D;
     if ((e = hipMemcpyAsync((BInst *)b->b_inst.ptr + slot, &b->h_inst[(size_t)slot], sizeof(BInst),
-                            hipMemcpyHostToDevice, b->ctx->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync((BDyn *)b->b_dyn.ptr + slot, pd0, sizeof(BDyn), hipMemcpyHostToDevice,
-                            b->ctx->stream)) != hipSuccess) {
+                            hipMemcpyHostToDevice, b->ustream)) != hipSuccess ||
+        (e = hipMemcpyAsync((BDyn *)b->b_dyn.ptr + slot, &b->h_stage[slot], sizeof(BDyn), hipMemcpyHostToDevice,
+                            b->ustream)) != hipSuccess) {
         b->arena.release(off, L.total);
         return hip_fail(e, "admit descriptor");
     }
     // the staging vector is released on return: wait for its copy
-    if ((e = hipStreamSynchronize(b->ctx->stream)) != hipSuccess) return hip_fail(e, "admit sync");
+    if ((e = hipStreamSynchronize(b->ustream)) != hipSuccess) return hip_fail(e, "admit sync");
     SlotHost &sl = b->slots[(size_t)slot];
     sl.used = true;
     sl.n = n;
@@ -555,6 +596,101 @@ int dnx_batch_poll(dnx_batch *b, int *status, int64_t *steps, double *t, uint64_
     return DNX_OK;
 }
 
+// Runs every running slot in wide mode until none runs or the wall-clock
+// limit passes.  The status table in h_dyn (buffer 0) is current on return.
+static int batch_wide_run(dnx_batch *b, double walltime_s, std::chrono::steady_clock::time_point t0)
+{
+    hipStream_t st = b->ctx->stream;
+    std::vector<int> list;
+    for (int s = 0; s < b->nslots; s++)
+        if (b->slots[(size_t)s].used && b->h_dyn[s].status == DNX_RUNNING) list.push_back(s);
+    if (list.empty()) return DNX_OK;
+    std::sort(list.begin(), list.end(), [&](int x, int y) {
+        return b->slots[(size_t)x].m != b->slots[(size_t)y].m ? b->slots[(size_t)x].m > b->slots[(size_t)y].m
+                                                                : x < y;
+    });
+    std::vector<int> bmap, vmap;
+    for (int s : list) {
+        const SlotHost &sl = b->slots[(size_t)s];
+        for (int j = 0; j * WIDE_BLOCK < sl.m; j++) {
+            bmap.push_back(s);
+            bmap.push_back(j);
+        }
+        for (int j = 0; j * WIDE_BLOCK < sl.n; j++) {
+            vmap.push_back(s);
+            vmap.push_back(j);
+        }
+    }
+    const int count = (int)list.size(), nbc = (int)bmap.size() / 2, nbv = (int)vmap.size() / 2;
+    int rc;
+    auto grow = [&](DevBuf &buf, size_t bytes) { return buf.size >= bytes ? DNX_OK : buf.alloc(bytes); };
+    if ((rc = grow(b->b_wlist, sizeof(int) * list.size())) || (rc = grow(b->b_bmap, sizeof(int) * (bmap.size() + 2))) ||
+        (rc = grow(b->b_vmap, sizeof(int) * (vmap.size() + 2))))
+        return rc;
+    hipError_t e;
+    if ((e = hipMemcpy(b->b_wlist.ptr, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (nbc && (e = hipMemcpy(b->b_bmap.ptr, bmap.data(), sizeof(int) * bmap.size(), hipMemcpyHostToDevice)) !=
+                    hipSuccess) ||
+        (nbv && (e = hipMemcpy(b->b_vmap.ptr, vmap.data(), sizeof(int) * vmap.size(), hipMemcpyHostToDevice)) !=
+                    hipSuccess))
+        return hip_fail(e, "wide map upload");
+    Ctl *wctl = (Ctl *)b->b_wctl.ptr;
+    BDyn *dyn = (BDyn *)b->b_dyn.ptr;
+    launch_wide_enter(dyn, wctl, (const int *)b->b_wlist.ptr, count, 0, st);
+    // one chunk of wide_chunk steps (a multiple of 3, so the rotation
+    // returns to 0) is captured once and replayed
+    const int chunk = (b->wide_chunk + 2) / 3 * 3;
+    hipGraphExec_t gexec = nullptr;
+    {
+        hipGraph_t g;
+        if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+            return hip_fail(e, "wide capture");
+        for (int k = 0; k < chunk; k++)
+            launch_wide_step((const BInst *)b->b_inst.ptr, wctl, (const int *)b->b_bmap.ptr, nbc,
+                             (const int *)b->b_vmap.ptr, nbv, k % 3, st);
+        if ((e = hipStreamEndCapture(st, &g)) != hipSuccess) return hip_fail(e, "wide capture end");
+        e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return hip_fail(e, "wide graph");
+    }
+    struct GraphGuard {
+        hipGraphExec_t &g;
+        ~GraphGuard()
+        {
+            if (g) (void)hipGraphExecDestroy(g);
+        }
+    } guard{gexec};
+    const int r = 0;
+    for (;;) {
+        const bool timed = b->timing && !b->t_pending[BM_COUNT];
+        if (timed && (e = hipEventRecord(b->t0[BM_COUNT], st)) != hipSuccess) return hip_fail(e, "timing");
+        if ((e = hipGraphLaunch(gexec, st)) != hipSuccess) return hip_fail(e, "wide graph launch");
+        if (timed) {
+            if ((e = hipEventRecord(b->t1[BM_COUNT], st)) != hipSuccess) return hip_fail(e, "timing");
+            b->t_pending[BM_COUNT] = true;
+        }
+        launch_wide_sync(dyn, wctl, (const int *)b->b_wlist.ptr, count, (r + 2) % 3, st);
+        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "wide launch");
+        if ((rc = batch_snapshot(b, 0)) || (rc = batch_wait(b, 0))) return rc;
+        batch_collect_timing(b);
+        int running = 0;
+        long long live = 0;
+        for (int s : list)
+            if (b->h_dyn[s].status == DNX_RUNNING) {
+                running++;
+                live += b->slots[(size_t)s].m;
+            }
+        if (!running) break;
+        // finished instances still cost their blocks; rebuild the maps once
+        // most of the clause blocks belong to finished instances
+        if (2 * (live / WIDE_BLOCK + running) < nbc) break;
+        if (walltime_s > 0 &&
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > walltime_s)
+            break;
+    }
+    return DNX_OK;
+}
+
 int dnx_batch_run(dnx_batch *b, int steps_per_launch, double walltime_s, int *remaining)
 {
     if (!b || steps_per_launch <= 0) return DNX_E_ARG;
@@ -573,6 +709,13 @@ int dnx_batch_run(dnx_batch *b, int steps_per_launch, double walltime_s, int *re
         if (walltime_s > 0 &&
             std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > walltime_s)
             break;
+        long long clauses = 0;
+        for (int s = 0; s < b->nslots; s++)
+            if (b->slots[(size_t)s].used && known[s].status == DNX_RUNNING) clauses += b->slots[(size_t)s].m;
+        if (clauses <= b->wide_clauses) {
+            if ((rc = batch_wide_run(b, walltime_s, t0))) return rc;
+            continue;
+        }
         if ((rc = batch_enqueue(b, steps_per_launch, known))) return rc;
         if ((rc = batch_snapshot(b, 0))) return rc;
         if ((rc = batch_wait(b, 0))) return rc;
@@ -671,7 +814,8 @@ int dnx_batch_retire(dnx_batch *b, int slot)
     D.plevel = -1;
     b->h_dyn[slot] = D;
     b->h_dyn[(size_t)b->nslots + slot] = D;
-    hipError_t e = hipMemcpy((BDyn *)b->b_dyn.ptr + slot, &D, sizeof(BDyn), hipMemcpyHostToDevice);
+    b->h_stage[slot] = D;
+    hipError_t e = hipMemcpy((BDyn *)b->b_dyn.ptr + slot, &b->h_stage[slot], sizeof(BDyn), hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(e, "retire");
     sl = SlotHost();
     b->free_slots.push_back(slot);
@@ -688,6 +832,21 @@ int dnx_batch_info(dnx_batch *b, int slot, int *n, int *m, int *mode)
     return DNX_OK;
 }
 
+int dnx_batch_set_wide(dnx_batch *b, int64_t max_clauses, int steps_per_sync)
+{
+    if (!b || max_clauses < 0 || steps_per_sync <= 0) return DNX_E_ARG;
+    b->wide_clauses = max_clauses;
+    b->wide_chunk = steps_per_sync;
+    return DNX_OK;
+}
+
+int dnx_batch_set_serve_big(dnx_batch *b, int64_t clauses)
+{
+    if (!b || clauses < 0) return DNX_E_ARG;
+    b->serve_big = clauses;
+    return DNX_OK;
+}
+
 int dnx_batch_timing(dnx_batch *b, int enable, double *ms, int64_t *launches)
 {
     if (!b) return DNX_E_ARG;
@@ -695,13 +854,13 @@ int dnx_batch_timing(dnx_batch *b, int enable, double *ms, int64_t *launches)
     int rc = batch_sync(b);
     if (rc) return rc;
     batch_collect_timing(b);
-    for (int md = 0; md < BM_COUNT; md++) {
+    for (int md = 0; md <= BM_COUNT; md++) {
         if (ms) ms[md] = b->t_ms[md];
         if (launches) launches[md] = b->t_launches[md];
     }
     if (enable >= 0) {
         b->timing = enable != 0;
-        for (int md = 0; md < BM_COUNT; md++) {
+        for (int md = 0; md <= BM_COUNT; md++) {
             b->t_ms[md] = 0.0;
             b->t_launches[md] = 0;
         }
@@ -735,89 +894,273 @@ struct ServeOut {
 };
 
 // The continuous-batching loop.  Requests become visible at their arrival
-// time; each is admitted as soon as a slot and an arena range are free
-// (first come, first served), advanced together with everything resident in
-// launches of `spl` steps, and retired the moment its state verifies or
-// reaches the cap, which frees its memory for the queue.  `get(i)` returns
-// request i (it may block while the request is still being prepared) and
-// `release(i)` says its host copy is no longer needed.
-static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t cap, int spl,
+// time and are admitted first come, first served as soon as a slot and an
+// arena range are free.  Two lanes advance what is resident, each on its own
+// streams and polled through events, so admission and retirement never wait
+// for the other lane:
+//   narrow  instances below `big` clauses, fused-kernel launches of `spl`
+//           steps (one workgroup per instance);
+//   wide    larger instances, graph-captured chunks of wide steps (their
+//           clauses and variables spread over many workgroups).
+// An instance is retired the moment a lane observes that it ended, which
+// frees its memory for the queue.  `get(i)` returns request i (it may block
+// while the request is still being prepared); `release(i)` says its host
+// copy is no longer needed.
+static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t cap, int spl, double deadline,
                       const std::function<ServeReq(int)> &get, const std::function<void(int)> &release,
                       const ServeOut &out, double *makespan)
 {
     using clk = std::chrono::steady_clock;
     const clk::time_point T0 = clk::now();
     auto now_s = [&]() { return std::chrono::duration<double>(clk::now() - T0).count(); };
-    // device clock -> seconds since T0
     uint64_t tick0 = 0;
     double rate = 1e8;
     const double hb = now_s();
     int rc = dnx_batch_stamp(b, &tick0, &rate);
     if (rc) return rc;
     const double host0 = 0.5 * (hb + now_s());
+    const long long big = b->wide_clauses > 0 ? std::max<long long>(b->serve_big, 1) : LLONG_MAX;
+    hipStream_t st = b->ctx->stream;
+    hipError_t e;
+    // lane bookkeeping
     std::vector<int> slot_req((size_t)b->nslots, -1);
-    std::vector<double> arrived_at((size_t)count, 0.0);
+    std::vector<char> slot_wide((size_t)b->nslots, 0);
+    std::vector<int> narrow_run, wide_set;  // members of each lane
+    std::vector<int> narrow_launched;       // slots in the narrow launch in flight
+    bool narrow_busy = false, wide_busy = false, wide_dirty = false;
+    hipStream_t wst = nullptr;
+    hipEvent_t ev_n = nullptr, ev_w = nullptr, ev_adm = nullptr;
+    hipGraphExec_t wgraph = nullptr;
+    int nbc = 0, nbv = 0;
+    struct Cleanup {
+        hipStream_t &wst;
+        hipEvent_t &a, &b2, &c;
+        hipGraphExec_t &g;
+        ~Cleanup()
+        {
+            if (wst) (void)hipStreamSynchronize(wst);
+            if (g) (void)hipGraphExecDestroy(g);
+            if (a) (void)hipEventDestroy(a);
+            if (b2) (void)hipEventDestroy(b2);
+            if (c) (void)hipEventDestroy(c);
+            if (wst) (void)hipStreamDestroy(wst);
+        }
+    } cleanup{wst, ev_n, ev_w, ev_adm, wgraph};
+    if ((e = hipStreamCreateWithFlags(&wst, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&ev_n, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&ev_w, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&ev_adm, hipEventDisableTiming)) != hipSuccess)
+        return hip_fail(e, "serve setup");
+    BDyn *hn = b->h_dyn, *hw = b->h_dyn + b->nslots;  // lane snapshots
     std::deque<int> queue;
-    int next = 0, finished = 0, running = 0;
+    int next = 0, finished = 0;
     std::vector<int8_t> assign;
+
+    // ends slot s's request: time-to-solution from the device tick, the
+    // exact clause check of its sign assignment (copied on the lane's own
+    // stream, which is idle), then the slot and its arena range are freed.
+    // Free slots are in no work list, so the device status record is left
+    // as it is.
+    std::vector<BF2> vbuf;
+    auto finish = [&](int s, const BDyn &d, hipStream_t ls) -> int {
+        const int r = slot_req[(size_t)s];
+        int stt = d.status;
+        out.steps[r] = d.steps;
+        out.tts[r] = host0 + (double)(d.t_done - tick0) / rate - arrival[r];
+        SlotHost &sl = b->slots[(size_t)s];
+        if (stt == DNX_SAT) {
+            vbuf.resize((size_t)(sl.n ? sl.n : 1));
+            hipError_t ee = sl.n ? hipMemcpyAsync(vbuf.data(), b->h_inst[(size_t)s].vd, sizeof(BF2) * (size_t)sl.n,
+                                                  hipMemcpyDeviceToHost, ls)
+                                 : hipSuccess;
+            if (ee == hipSuccess) ee = hipStreamSynchronize(ls);
+            if (ee != hipSuccess) return hip_fail(ee, "assignment copy");
+            int bad = 0;
+            for (int c = 0; c < sl.m; c++) {
+                bool ok = false;
+                for (int q = sl.cp[(size_t)c]; q < sl.cp[(size_t)c + 1] && !ok; q++) {
+                    const int32_t lit = sl.lits[(size_t)q];
+                    const bool pos = vbuf[(size_t)std::abs(lit) - 1].v >= 0.0f;
+                    ok = (lit > 0) == pos;
+                }
+                bad += !ok;
+            }
+            if (bad != 0) stt = DNX_FAILED;
+        }
+        out.status[r] = stt;
+        b->arena.release(sl.off, sl.size);
+        sl = SlotHost();
+        b->free_slots.push_back(s);
+        slot_req[(size_t)s] = -1;
+        release(r);
+        finished++;
+        return DNX_OK;
+    };
+
+    for (int i = 0; i < count; i++) {
+        out.status[i] = DNX_UNKNOWN;
+        out.steps[i] = 0;
+        out.tts[i] = -1.0;
+        out.wait[i] = -1.0;
+    }
     while (finished < count) {
+        bool progressed = false;
         const double now = now_s();
+        if (deadline > 0 && now > deadline) {
+            // requests still queued, running or not yet arrived stay
+            // DNX_UNKNOWN with no time-to-solution
+            if (wst) (void)hipStreamSynchronize(wst);
+            (void)hipStreamSynchronize(st);
+            for (int s = 0; s < b->nslots; s++) {
+                if (slot_req[(size_t)s] < 0) continue;
+                const int r = slot_req[(size_t)s];
+                SlotHost &sl = b->slots[(size_t)s];
+                b->arena.release(sl.off, sl.size);
+                sl = SlotHost();
+                b->free_slots.push_back(s);
+                slot_req[(size_t)s] = -1;
+                release(r);
+            }
+            for (int r : queue) release(r);
+            for (int r = next; r < count; r++) release(r);
+            break;
+        }
         while (next < count && arrival[next] <= now) queue.push_back(next++);
+        // ---- admission -------------------------------------------------
         while (!queue.empty()) {
             const int r = queue.front();
             ServeReq q = get(r);
             int slot = -1;
             rc = dnx_batch_admit_host(b, *q.hp, &q.prm, q.x0, cap, 1, &slot);
-            if (rc == DNX_E_BUDGET) break;
+            if (rc == DNX_E_BUDGET) {
+                if (narrow_run.empty() && wide_set.empty()) {
+                    // larger than the whole arena: it can never run
+                    queue.pop_front();
+                    out.status[r] = DNX_FAILED;
+                    out.steps[r] = 0;
+                    out.tts[r] = -1.0;
+                    out.wait[r] = now_s() - arrival[r];
+                    release(r);
+                    finished++;
+                    continue;
+                }
+                break;
+            }
             if (rc) return rc;
             slot_req[(size_t)slot] = r;
             out.wait[r] = now_s() - arrival[r];
-            running++;
+            if (q.hp->m >= big) {
+                slot_wide[(size_t)slot] = 1;
+                wide_set.push_back(slot);
+                wide_dirty = true;
+            } else {
+                slot_wide[(size_t)slot] = 0;
+                narrow_run.push_back(slot);
+            }
             queue.pop_front();
+            progressed = true;
         }
-        if (running == 0) {
-            if (!queue.empty()) {
-                // larger than the whole arena: it can never run
-                const int r = queue.front();
-                queue.pop_front();
-                out.status[r] = DNX_FAILED;
-                out.steps[r] = 0;
-                out.tts[r] = -1.0;
-                release(r);
-                finished++;
-                continue;
+        // ---- narrow lane -----------------------------------------------
+        if (narrow_busy && hipEventQuery(ev_n) == hipSuccess) {
+            narrow_busy = false;
+            progressed = true;
+            for (int s : narrow_launched) {
+                if (slot_req[(size_t)s] < 0 || hn[s].status == DNX_RUNNING) continue;
+                if ((rc = finish(s, hn[s], st))) return rc;
+                narrow_run.erase(std::find(narrow_run.begin(), narrow_run.end(), s));
             }
-            if (next < count) {
-                const double dt = arrival[next] - now_s();
-                if (dt > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(dt, 0.01)));
-            }
-            continue;
         }
-        if ((rc = dnx_batch_advance(b, spl))) return rc;
-        if ((rc = batch_snapshot(b, 0)) || (rc = batch_wait(b, 0))) return rc;
-        batch_collect_timing(b);
-        const BDyn *d = b->h_dyn;
-        for (int s = 0; s < b->nslots; s++) {
-            const int r = slot_req[(size_t)s];
-            if (r < 0 || d[s].status == DNX_RUNNING) continue;
-            int st = d[s].status;
-            out.steps[r] = d[s].steps;
-            out.tts[r] = host0 + (double)(d[s].t_done - tick0) / rate - arrival[r];
-            if (st == DNX_SAT) {
-                const SlotHost &sl = b->slots[(size_t)s];
-                assign.resize((size_t)(sl.n ? sl.n : 1));
-                int unsat = -1;
-                if ((rc = dnx_batch_assignment(b, s, assign.data(), &unsat))) return rc;
-                if (unsat != 0) st = DNX_FAILED;
+        if (!narrow_busy && !narrow_run.empty()) {
+            narrow_launched = narrow_run;
+            b->buf = 0;
+            if ((rc = batch_enqueue_slots(b, spl, narrow_launched))) return rc;
+            if ((e = hipMemcpyAsync(hn, b->b_dyn.ptr, sizeof(BDyn) * (size_t)b->nslots, hipMemcpyDeviceToHost, st)) !=
+                    hipSuccess ||
+                (e = hipEventRecord(ev_n, st)) != hipSuccess)
+                return hip_fail(e, "narrow poll");
+            narrow_busy = progressed = true;
+        }
+        // ---- wide lane -------------------------------------------------
+        if (wide_busy && hipEventQuery(ev_w) == hipSuccess) {
+            wide_busy = false;
+            progressed = true;
+            std::vector<int> keep;
+            for (int s : wide_set) {
+                if (hw[s].status == DNX_RUNNING || hw[s].status == -1) {
+                    keep.push_back(s);  // still running, or admitted after the snapshot
+                    continue;
+                }
+                if ((rc = finish(s, hw[s], wst))) return rc;
+                wide_dirty = true;
             }
-            out.status[r] = st;
-            if ((rc = dnx_batch_retire(b, s))) return rc;
-            slot_req[(size_t)s] = -1;
-            release(r);
-            running--;
-            finished++;
+            wide_set.swap(keep);
+        }
+        if (!wide_busy && !wide_set.empty()) {
+            if (wide_dirty || !wgraph) {
+                // new maps and graph for the current members, entered from
+                // their status records (current after every chunk)
+                std::vector<int> bmap, vmap;
+                for (int s : wide_set) {
+                    const SlotHost &sl = b->slots[(size_t)s];
+                    for (int j = 0; j * WIDE_BLOCK < sl.m; j++) {
+                        bmap.push_back(s);
+                        bmap.push_back(j);
+                    }
+                    for (int j = 0; j * WIDE_BLOCK < sl.n; j++) {
+                        vmap.push_back(s);
+                        vmap.push_back(j);
+                    }
+                }
+                nbc = (int)bmap.size() / 2;
+                nbv = (int)vmap.size() / 2;
+                auto grow = [&](DevBuf &buf, size_t bytes) { return buf.size >= bytes ? DNX_OK : buf.alloc(bytes); };
+                if ((rc = grow(b->b_wlist, sizeof(int) * wide_set.size())) ||
+                    (rc = grow(b->b_bmap, sizeof(int) * (bmap.size() + 2))) ||
+                    (rc = grow(b->b_vmap, sizeof(int) * (vmap.size() + 2))))
+                    return rc;
+                if ((e = hipMemcpy(b->b_wlist.ptr, wide_set.data(), sizeof(int) * wide_set.size(),
+                                   hipMemcpyHostToDevice)) != hipSuccess ||
+                    (nbc && (e = hipMemcpy(b->b_bmap.ptr, bmap.data(), sizeof(int) * bmap.size(),
+                                           hipMemcpyHostToDevice)) != hipSuccess) ||
+                    (nbv && (e = hipMemcpy(b->b_vmap.ptr, vmap.data(), sizeof(int) * vmap.size(),
+                                           hipMemcpyHostToDevice)) != hipSuccess))
+                    return hip_fail(e, "wide map upload");
+                if (wgraph) {
+                    (void)hipGraphExecDestroy(wgraph);
+                    wgraph = nullptr;
+                }
+                const int chunk = (b->wide_chunk + 2) / 3 * 3;
+                hipGraph_t g;
+                if ((e = hipStreamBeginCapture(wst, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+                    return hip_fail(e, "wide capture");
+                for (int k = 0; k < chunk; k++)
+                    launch_wide_step((const BInst *)b->b_inst.ptr, (Ctl *)b->b_wctl.ptr, (const int *)b->b_bmap.ptr,
+                                     nbc, (const int *)b->b_vmap.ptr, nbv, k % 3, wst);
+                if ((e = hipStreamEndCapture(wst, &g)) != hipSuccess) return hip_fail(e, "wide capture end");
+                e = hipGraphInstantiate(&wgraph, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (e != hipSuccess) return hip_fail(e, "wide graph");
+                launch_wide_enter((const BDyn *)b->b_dyn.ptr, (Ctl *)b->b_wctl.ptr, (const int *)b->b_wlist.ptr,
+                                  (int)wide_set.size(), 0, wst);
+                wide_dirty = false;
+            }
+            if ((e = hipGraphLaunch(wgraph, wst)) != hipSuccess) return hip_fail(e, "wide launch");
+            launch_wide_sync((BDyn *)b->b_dyn.ptr, (Ctl *)b->b_wctl.ptr, (const int *)b->b_wlist.ptr,
+                             (int)wide_set.size(), 2, wst);
+            if ((e = hipMemcpyAsync(hw, b->b_dyn.ptr, sizeof(BDyn) * (size_t)b->nslots, hipMemcpyDeviceToHost, wst)) !=
+                    hipSuccess ||
+                (e = hipEventRecord(ev_w, wst)) != hipSuccess)
+                return hip_fail(e, "wide poll");
+            wide_busy = progressed = true;
+        }
+        if (!progressed) {
+            // nothing completed: sleep a little, or until the next arrival
+            double dt = 50e-6;
+            if (narrow_run.empty() && wide_set.empty() && next < count) dt = std::min(0.01, arrival[next] - now_s());
+            if (dt > 0) std::this_thread::sleep_for(std::chrono::duration<double>(dt));
         }
     }
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "serve drain");
     if (makespan) *makespan = now_s();
     return DNX_OK;
 }
@@ -828,7 +1171,7 @@ extern "C" {
 
 int dnx_serve_trace(dnx_batch *b, int count, dnx_problem *const *probs, const dnx_params *prms,
                     const double *const *x0s, const double *arrival_s, int64_t step_cap, int steps_per_launch,
-                    int *status, int64_t *steps, double *tts_s, double *wait_s, double *makespan_s)
+                    double deadline_s, int *status, int64_t *steps, double *tts_s, double *wait_s, double *makespan_s)
 {
     if (!b || count < 0 || (count && (!probs || !x0s || !arrival_s || !status || !steps || !tts_s || !wait_s)) ||
         steps_per_launch <= 0)
@@ -849,7 +1192,7 @@ int dnx_serve_trace(dnx_batch *b, int count, dnx_problem *const *probs, const dn
         return q;
     };
     ServeOut out{status, steps, tts_s, wait_s};
-    return serve_loop(b, count, arrival_s, step_cap, steps_per_launch, get, [](int) {}, out, makespan_s);
+    return serve_loop(b, count, arrival_s, step_cap, steps_per_launch, deadline_s, get, [](int) {}, out, makespan_s);
 }
 
 // Served planted stream: Poisson arrivals at rate_per_s over duration_s
@@ -858,8 +1201,9 @@ int dnx_serve_trace(dnx_batch *b, int count, dnx_problem *const *probs, const dn
 // arrival time and freed at retirement.
 int dnx_serve_planted(dnx_batch *b, uint64_t seed, double rate_per_s, double duration_s, int n_min, int n_max,
                       double r_min, double r_max, double p1, double p2, int64_t step_cap, int steps_per_launch,
-                      int max_requests, int gen_threads, int *count_out, double *arrival_out, int *n_out, int *m_out,
-                      int *status, int64_t *steps, double *tts_s, double *wait_s, double *makespan_s)
+                      double deadline_s, int max_requests, int gen_threads, int *count_out, double *arrival_out,
+                      int *n_out, int *m_out, int *status, int64_t *steps, double *tts_s, double *wait_s,
+                      double *makespan_s)
 {
     if (!b || rate_per_s <= 0 || duration_s <= 0 || n_min < 3 || n_max < n_min || r_min <= 0 || r_max < r_min ||
         max_requests <= 0 || !count_out || !arrival_out || !n_out || !m_out || !status || !steps || !tts_s ||
@@ -944,11 +1288,184 @@ int dnx_serve_planted(dnx_batch *b, uint64_t seed, double rate_per_s, double dur
         released++;
     };
     ServeOut out{status, steps, tts_s, wait_s};
-    int rc = serve_loop(b, count, arrival_out, step_cap, steps_per_launch, get, release, out, makespan_s);
+    int rc = serve_loop(b, count, arrival_out, step_cap, steps_per_launch, deadline_s, get, release, out, makespan_s);
     stop = true;
     for (auto &x : th) x.join();
     return rc;
 }
 
+
 }  // extern "C"
 
+// ---------------------------------------------------------------------------
+// Autotuner.  Candidates are timed on synthetic planted instances drawn from
+// the caller's size distribution, each in fixed-step mode (every instance
+// takes exactly `steps` steps, so every candidate does the same work):
+//   1. fused-kernel shapes: workgroup size of modes 0 and 1 and the LDS
+//      budget that admits an instance to mode 0, scored by compulsory bytes
+//      per second over the whole sample;
+//   2. the wide-mode threshold: the k largest instances (k = 1, 4, 16, 64)
+//      are stepped narrow and wide; the threshold is the largest total
+//      clause count at which wide was faster.
+// The winners are applied to `b`, which must be empty.
+// ---------------------------------------------------------------------------
+namespace dnx {
+
+struct TuneInst {
+    HostProblem hp;
+    std::vector<double> x0;
+};
+
+static int tune_run(dnx_ctx *ctx, const std::vector<TuneInst> &inst, const std::vector<int> &pick, int bs0, int cap0,
+                    int bs1, long long wide, int steps, double *ms_out)
+{
+    size_t bytes = 1 << 20;
+    for (int i : pick) {
+        const HostProblem &h = inst[(size_t)i].hp;
+        bytes += layout_of(h.n, h.m, h.nnz(), !h.uniform3, h.has_unit, BM_GLOBAL).total;
+    }
+    dnx_batch *t = nullptr;
+    int rc = dnx_batch_open(ctx, DNX_FP32, (int)pick.size(), (int64_t)bytes, &t);
+    if (rc) return rc;
+    struct Close {
+        dnx_batch *t;
+        ~Close() { dnx_batch_close(t); }
+    } closer{t};
+    if ((rc = dnx_batch_set_shape(t, BM_LDS_ALL, bs0, cap0)) || (rc = dnx_batch_set_shape(t, BM_LDS_V, bs1, 1 << 30)) ||
+        (rc = dnx_batch_set_wide(t, wide, 30)))
+        return rc;
+    for (int i : pick) {
+        dnx_params prm;
+        dnx_params_default(inst[(size_t)i].hp.m, 10000, &prm);
+        int slot;
+        if ((rc = dnx_batch_admit_host(t, inst[(size_t)i].hp, &prm, inst[(size_t)i].x0.data(), steps, 0, &slot)))
+            return rc;
+    }
+    double ms[BM_COUNT + 1];
+    int64_t nl[BM_COUNT + 1];
+    if ((rc = dnx_batch_timing(t, 1, nullptr, nullptr))) return rc;
+    // one launch runs the fixed steps (steps is a multiple of the wide chunk)
+    if ((rc = dnx_batch_run(t, steps, 0.0, nullptr))) return rc;
+    if ((rc = dnx_batch_timing(t, 0, ms, nl))) return rc;
+    double tot = 0.0;
+    for (int md = 0; md <= BM_COUNT; md++) tot += ms[md];
+    *ms_out = tot;
+    return DNX_OK;
+}
+
+}  // namespace dnx
+
+extern "C" {
+
+int dnx_batch_tune(dnx_batch *b, int n_min, int n_max, double r_min, double r_max, int sample, int steps,
+                   uint64_t seed, double *report, int report_len, int *report_count)
+{
+    if (!b || n_min < 3 || n_max < n_min || r_min <= 0 || r_max < r_min || sample <= 0 || steps <= 0)
+        return DNX_E_ARG;
+    for (auto &sl : b->slots)
+        if (sl.used) {
+            set_error("tune an empty batch");
+            return DNX_E_STATE;
+        }
+    steps = (steps + 29) / 30 * 30;
+    // the sample
+    SplitMix64 rng(seed);
+    std::vector<TuneInst> inst((size_t)sample);
+    for (auto &ti : inst) {
+        const double ln = std::log((double)n_min) + rng.unit() * (std::log((double)n_max) - std::log((double)n_min));
+        const int n = std::max(n_min, std::min(n_max, (int)std::lround(std::exp(ln))));
+        const int m = (int)std::lround((r_min + rng.unit() * (r_max - r_min)) * n);
+        const uint64_t s = rng.next();
+        ti.hp.n = n;
+        ti.hp.m = m;
+        ti.hp.lits.resize((size_t)m * 3);
+        generate_planted(s, n, m, 0.46, 0.46, ti.hp.lits.data(), nullptr);
+        ti.hp.cp.resize((size_t)m + 1);
+        for (int c = 0; c <= m; c++) ti.hp.cp[(size_t)c] = 3 * c;
+        finalize_problem(ti.hp);
+        ti.x0.resize((size_t)n + 2 * (size_t)m);
+        init_uniform(s, n, m, 0.5, 1.0, ti.x0.data());
+    }
+    double alg = 0.0;
+    for (auto &ti : inst) alg += (double)steps * (28.0 * ti.hp.m + 8.0 * ti.hp.n);
+    std::vector<int> all((size_t)sample);
+    for (int i = 0; i < sample; i++) all[(size_t)i] = i;
+    int nrep = 0;
+    auto rep = [&](double a, double c, double d, double e, double f) {
+        if (report && nrep + 5 <= report_len) {
+            report[nrep] = a;
+            report[nrep + 1] = c;
+            report[nrep + 2] = d;
+            report[nrep + 3] = e;
+            report[nrep + 4] = f;
+        }
+        nrep += 5;
+    };
+    // 1. fused-kernel shapes
+    const int bss[3] = {256, 512, 1024};
+    const int caps[4] = {0, 48 * 1024, 96 * 1024, 160 * 1024};
+    double best = -1.0;
+    int b0 = b->shape[BM_LDS_ALL].bs, c0 = b->shape[BM_LDS_ALL].lds_cap, b1 = b->shape[BM_LDS_V].bs;
+    for (int cap : caps) {
+        if (cap > b->lds_max) continue;
+        for (int bs : bss) {
+            double ms = 0.0;
+            int rc = tune_run(b->ctx, inst, all, bs, cap, bs, 0, steps, &ms);
+            if (rc) return rc;
+            const double gbps = alg / (ms * 1e-3) / 1e9;
+            rep(0, bs, cap, ms, gbps);
+            if (gbps > best) {
+                best = gbps;
+                b0 = bs;
+                c0 = cap;
+                b1 = bs;
+            }
+        }
+    }
+    // mode-1 workgroup size on its own, with the chosen mode-0 shape
+    for (int bs : bss) {
+        if (bs == b1) continue;
+        double ms = 0.0;
+        int rc = tune_run(b->ctx, inst, all, b0, c0, bs, 0, steps, &ms);
+        if (rc) return rc;
+        const double gbps = alg / (ms * 1e-3) / 1e9;
+        rep(1, bs, c0, ms, gbps);
+        if (gbps > best) {
+            best = gbps;
+            b1 = bs;
+        }
+    }
+    // 2. wide-mode threshold on the largest instances
+    std::vector<int> bym = all;
+    std::sort(bym.begin(), bym.end(), [&](int x, int y) { return inst[(size_t)x].hp.m > inst[(size_t)y].hp.m; });
+    long long threshold = 0;
+    for (int k : {1, 4, 16, 64}) {
+        if (k > sample) break;
+        std::vector<int> pick(bym.begin(), bym.begin() + k);
+        long long clauses = 0;
+        for (int i : pick) clauses += inst[(size_t)i].hp.m;
+        double ms_n = 0.0, ms_w = 0.0;
+        int rc = tune_run(b->ctx, inst, pick, b0, c0, b1, 0, steps, &ms_n);
+        if (!rc) rc = tune_run(b->ctx, inst, pick, b0, c0, b1, 1LL << 50, steps, &ms_w);
+        if (rc) return rc;
+        rep(2, k, (double)clauses, ms_n, ms_w);
+        if (ms_w < ms_n) threshold = std::max(threshold, 2 * clauses);
+    }
+    b->shape[BM_LDS_ALL].bs = b0;
+    b->shape[BM_LDS_ALL].lds_cap = c0;
+    b->shape[BM_LDS_V].bs = b1;
+    b->wide_clauses = threshold;
+    if (report_count) *report_count = nrep;
+    return DNX_OK;
+}
+
+int dnx_batch_get_shape(dnx_batch *b, int mode, int *block, int *lds_cap, int64_t *wide_clauses)
+{
+    if (!b || mode < 0 || mode >= BM_COUNT) return DNX_E_ARG;
+    if (block) *block = b->shape[mode].bs;
+    if (lds_cap) *lds_cap = b->shape[mode].lds_cap;
+    if (wide_clauses) *wide_clauses = b->wide_clauses;
+    return DNX_OK;
+}
+
+}  // extern "C"

@@ -309,4 +309,158 @@ __device__ __forceinline__ int block_max_int(int v, int *sh)
     return r;  // valid on thread 0
 }
 
+// Flag groups (see Ctl): a workgroup raises its own entry with a plain
+// store; every wave reads all 64 entries and ballots.
+__device__ __forceinline__ void raise_flag(int *group)
+{
+    const int b = blockIdx.x;
+    group[(b & 7) * 32 + ((b >> 3) & 7)] = 1;
+}
+__device__ __forceinline__ bool any_flag(const int *group)
+{
+    const int l = threadIdx.x & 63;
+    return __ballot(group[(l & 7) * 32 + (l >> 3)] != 0) != 0ull;
+}
+__device__ __forceinline__ int dev_level(const Ctl *c, int s)
+{
+    int L = 0;
+#pragma unroll
+    for (int g = 0; g < 4; g++) L += any_flag(c->flags[s][FG_LEVEL + g]) ? 1 : 0;
+    return L;
+}
+// clears slot s (called by all threads of one workgroup)
+__device__ __forceinline__ void clear_slot(Ctl *c, int s)
+{
+    for (int i = threadIdx.x; i < FG_COUNT * FLAG_USED; i += blockDim.x) {
+        const int g = i / FLAG_USED, l = i % FLAG_USED;
+        c->flags[s][g][(l & 7) * 32 + (l >> 3)] = 0;
+    }
+    if (threadIdx.x == 0) {
+        c->stepped[s] = 0;
+        c->done[s] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-variable fold.  The variable owns transpose slots [b, b+deg); slot q
+// holds the literal position eidx[q] whose contribution e[] carries (the
+// clause pass writes e in literal order, coalesced).
+// ---------------------------------------------------------------------------
+template <typename T, bool SORT, int BS, int CAP>
+__device__ __forceinline__ T fold_var(const T *__restrict__ e, const int32_t *__restrict__ eidx, int b, int deg,
+                                      T *buf)
+{
+    T acc = T(0.0);
+    if (!SORT) return fold_gather<T>(e, eidx, b, b + deg);
+    // dynex.cc:527-530: ascending std::sort then left fold from 0.  Zero
+    // contributions (which the reference never pushes) do not change an
+    // ascending fold from +0, so every slot is folded.
+    const int t = threadIdx.x;
+    if (CAP > 0 && deg <= CAP) {
+        // stage the gathered values (eight loads in flight), then sort
+        int j = 0;
+        for (; j + 8 <= deg; j += 8) {
+            int i[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) i[u] = eidx[b + j + u];
+#pragma unroll
+            for (int u = 0; u < 8; u++) buf[(j + u) * BS + t] = e[i[u]];
+        }
+        for (; j < deg; j++) buf[j * BS + t] = e[eidx[b + j]];
+        for (int j = 1; j < deg; j++) {
+            T x = buf[j * BS + t];
+            int i = j;
+            while (i > 0 && buf[(i - 1) * BS + t] > x) {
+                buf[i * BS + t] = buf[(i - 1) * BS + t];
+                i--;
+            }
+            buf[i * BS + t] = x;
+        }
+        for (int j = 0; j < deg; j++) acc = acc + buf[j * BS + t];
+    } else {
+        // rare high-degree variable: selection of the next larger value by
+        // repeated scans (ties counted), no scratch needed
+        T last = T(0.0);
+        int taken = 0;
+        bool first = true;
+        while (taken < deg) {
+            T best = T(0.0);
+            bool have = false;
+            for (int j = 0; j < deg; j++) {
+                T x = e[eidx[b + j]];
+                if ((first || x > last) && (!have || x < best)) {
+                    best = x;
+                    have = true;
+                }
+            }
+            int cnt = 0;
+            for (int j = 0; j < deg; j++) cnt += (e[eidx[b + j]] == best);
+            for (int r = 0; r < cnt; r++) acc = acc + best;
+            taken += cnt;
+            last = best;
+            first = false;
+        }
+    }
+    return acc;
+}
+
+// Wave-cooperative fold: a wave owns 64 consecutive variables, whose
+// transpose slots are contiguous.  The wave stages all their contributions
+// in LDS (coalesced index loads, independent gathers, eight per lane in
+// flight), then each lane folds its own segment in slot order -- sorted
+// first in FP64 -- exactly as a lane-serial fold would.  A wave whose
+// segment exceeds the staging area folds from device memory instead.
+// Must be reached by every thread of the block (it synchronises).
+template <typename T, bool SORT, int CAPW>
+__device__ __forceinline__ T wave_fold(const T *__restrict__ e, const int32_t *__restrict__ eidx,
+                                       const int32_t *__restrict__ row_ptr, int n, int v0, bool active, T *stage,
+                                       T *sortbuf_unused)
+{
+    (void)sortbuf_unused;
+    const int lane = threadIdx.x & 63;
+    const int v = v0 + lane;
+    const int vend = min(v0 + 64, n);
+    const int sb = v0 < n ? row_ptr[v0] : 0;
+    const int total = v0 < n ? row_ptr[vend] - sb : 0;
+    const bool fits = total <= CAPW;
+    if (active && fits) {
+        int q = lane;
+        for (; q + 7 * 64 < total; q += 8 * 64) {
+            int i[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) i[u] = eidx[sb + q + u * 64];
+#pragma unroll
+            for (int u = 0; u < 8; u++) stage[q + u * 64] = e[i[u]];
+        }
+        for (; q < total; q += 64) stage[q] = e[eidx[sb + q]];
+    }
+    __syncthreads();
+    T acc = T(0.0);
+    if (active && v < n) {
+        const int rb = row_ptr[v], re = row_ptr[v + 1];
+        if (fits) {
+            T *s = stage + (rb - sb);
+            const int deg = re - rb;
+            if (SORT) {
+                // dynex.cc:527-530: ascending sort, left fold from +0
+                for (int j = 1; j < deg; j++) {
+                    T x = s[j];
+                    int k = j;
+                    while (k > 0 && s[k - 1] > x) {
+                        s[k] = s[k - 1];
+                        k--;
+                    }
+                    s[k] = x;
+                }
+            }
+            for (int j = 0; j < deg; j++) acc = acc + s[j];
+        } else if (SORT) {
+            acc = fold_var<T, true, 64, 0>(e, eidx, rb, re - rb, nullptr);
+        } else {
+            acc = fold_gather<T>(e, eidx, rb, re);
+        }
+    }
+    return acc;
+}
+
 }  // namespace dnx

@@ -48,7 +48,7 @@ struct Ctl {
     double t;           // simulated time of that state
     int stepped[3];     // the variable pass produced a level (a step is pending)
     int done[3];        // DNX_SAT / DNX_UNKNOWN when the state ended the run
-    int pad[2];
+    unsigned long long t_done;  // device clock when a run ended (batched wide mode)
     int flags[3][FG_COUNT][FLAG_GROUP];
 };
 

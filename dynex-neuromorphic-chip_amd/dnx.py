@@ -86,9 +86,14 @@ def lib():
     L.dnx_batch_info.argtypes = [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.dnx_batch_memory.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.dnx_batch_timing.argtypes = [vp, ip, vp, vp]
+    L.dnx_batch_set_wide.argtypes = [vp, C.c_int64, ip]
+    L.dnx_batch_set_serve_big.argtypes = [vp, C.c_int64]
+    L.dnx_batch_tune.argtypes = [vp, ip, ip, dp, dp, ip, ip, C.c_uint64, vp, ip, C.POINTER(C.c_int)]
+    L.dnx_batch_get_shape.argtypes = [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)]
     L.dnx_state_time.argtypes = [vp, ip, C.POINTER(C.c_double)]
-    L.dnx_serve_trace.argtypes = [vp, ip, vp, vp, vp, F64P, C.c_int64, ip, vp, vp, vp, vp, C.POINTER(C.c_double)]
-    L.dnx_serve_planted.argtypes = [vp, C.c_uint64, dp, dp, ip, ip, dp, dp, dp, dp, C.c_int64, ip, ip, ip,
+    L.dnx_serve_trace.argtypes = [vp, ip, vp, vp, vp, F64P, C.c_int64, ip, dp, vp, vp, vp, vp,
+                                  C.POINTER(C.c_double)]
+    L.dnx_serve_planted.argtypes = [vp, C.c_uint64, dp, dp, ip, ip, dp, dp, dp, dp, C.c_int64, ip, dp, ip, ip,
                                     C.POINTER(C.c_int), vp, vp, vp, vp, vp, vp, vp, C.POINTER(C.c_double)]
     _LIB = L
     return L
@@ -350,9 +355,29 @@ class Batch:
         check(lib().dnx_batch_info(self.h, slot, C.byref(n), C.byref(m), C.byref(md)))
         return n.value, m.value, md.value
 
+    def tune(self, n_min=100, n_max=5000, r_min=4.3, r_max=8.0, sample=512, steps=60, seed=1):
+        """Autotune execution shapes; returns the measurements as rows."""
+        rep = np.zeros(512, np.float64)
+        cnt = C.c_int()
+        check(lib().dnx_batch_tune(self.h, int(n_min), int(n_max), float(r_min), float(r_max), int(sample),
+                                   int(steps), int(seed), rep.ctypes.data_as(C.c_void_p), rep.size, C.byref(cnt)))
+        return rep[:min(cnt.value, rep.size)].reshape(-1, 5)
+
+    def get_shape(self, mode):
+        bs, cap, wide = C.c_int(), C.c_int(), C.c_int64()
+        check(lib().dnx_batch_get_shape(self.h, int(mode), C.byref(bs), C.byref(cap), C.byref(wide)))
+        return bs.value, cap.value, wide.value
+
+    def set_serve_big(self, clauses):
+        check(lib().dnx_batch_set_serve_big(self.h, int(clauses)))
+
+    def set_wide(self, max_clauses, steps_per_sync=30):
+        check(lib().dnx_batch_set_wide(self.h, int(max_clauses), int(steps_per_sync)))
+
     def timing(self, enable=-1):
-        ms = np.zeros(3, np.float64)
-        nl = np.zeros(3, np.int64)
+        """Summed device ms and launch counts per fused-kernel mode (0-2) and wide mode (3)."""
+        ms = np.zeros(4, np.float64)
+        nl = np.zeros(4, np.int64)
         check(lib().dnx_batch_timing(self.h, int(enable), ms.ctypes.data_as(C.c_void_p), nl.ctypes.data_as(C.c_void_p)))
         return ms, nl
 
@@ -361,7 +386,7 @@ class Batch:
         check(lib().dnx_batch_memory(self.h, C.byref(a), C.byref(f)))
         return a.value, f.value
 
-    def serve(self, probs, x0s, arrivals, cap=100000, steps_per_launch=64):
+    def serve(self, probs, x0s, arrivals, cap=100000, steps_per_launch=64, deadline=0.0):
         """Continuous batching over a request trace; see dnx_serve_trace."""
         k = len(probs)
         ph = (C.c_void_p * max(k, 1))(*[p.h for p in probs])
@@ -374,12 +399,12 @@ class Batch:
         wait = np.zeros(max(k, 1), np.float64)
         mk = C.c_double()
         check(lib().dnx_serve_trace(self.h, k, ph, None, xp, arr if k else np.zeros(1), int(cap), int(steps_per_launch),
-                                    st.ctypes.data_as(C.c_void_p), steps.ctypes.data_as(C.c_void_p),
+                                    float(deadline), st.ctypes.data_as(C.c_void_p), steps.ctypes.data_as(C.c_void_p),
                                     tts.ctypes.data_as(C.c_void_p), wait.ctypes.data_as(C.c_void_p), C.byref(mk)))
         return dict(status=st[:k], steps=steps[:k], tts=tts[:k], wait=wait[:k], makespan=mk.value)
 
     def serve_planted(self, seed, rate, duration, n_min=100, n_max=100000, r_min=4.3, r_max=8.0, p1=0.46, p2=0.46,
-                      cap=100000, steps_per_launch=64, max_requests=100000, gen_threads=8):
+                      cap=100000, steps_per_launch=64, max_requests=100000, gen_threads=8, deadline=0.0):
         M = int(max_requests)
         cnt = C.c_int()
         arr = np.zeros(M, np.float64)
@@ -393,7 +418,8 @@ class Batch:
         ptr = lambda a: a.ctypes.data_as(C.c_void_p)
         check(lib().dnx_serve_planted(self.h, int(seed), float(rate), float(duration), int(n_min), int(n_max),
                                       float(r_min), float(r_max), float(p1), float(p2), int(cap),
-                                      int(steps_per_launch), M, int(gen_threads), C.byref(cnt), ptr(arr), ptr(ns),
+                                      int(steps_per_launch), float(deadline), M, int(gen_threads), C.byref(cnt),
+                                      ptr(arr), ptr(ns),
                                       ptr(ms), ptr(st), ptr(steps), ptr(tts), ptr(wait), C.byref(mk)))
         k = cnt.value
         return dict(count=k, arrival=arr[:k], n=ns[:k], m=ms[:k], status=st[:k], steps=steps[:k], tts=tts[:k],

@@ -203,8 +203,35 @@ int dnx_batch_retire(dnx_batch *b, int slot);
 
 int dnx_batch_info(dnx_batch *b, int slot, int *n, int *m, int *mode);
 
-/* Waits for queued work, then reports per mode the summed device time (ms)
- * and count of the launches timed since the last reset; enable = 1/0 turns
+/* Wide mode: when the running instances hold at most max_clauses clauses
+ * in total, dnx_batch_run advances them with per-step kernels spread over
+ * all of their clauses and variables (many workgroups per instance) instead
+ * of one workgroup per instance, syncing status every steps_per_sync steps.
+ * 0 disables it.  Results are bit-identical either way. */
+int dnx_batch_set_wide(dnx_batch *b, int64_t max_clauses, int steps_per_sync);
+
+/* Serving: requests with at least `clauses` clauses run in the wide lane
+ * (default 60,000; needs wide mode enabled). */
+int dnx_batch_set_serve_big(dnx_batch *b, int64_t clauses);
+
+/* Autotuner: times candidate execution shapes on `sample` synthetic planted
+ * instances (n log-uniform in [n_min, n_max], clause ratio uniform in
+ * [r_min, r_max]) for `steps` fixed steps each -- workgroup size of modes 0
+ * and 1, the LDS budget admitting an instance to mode 0, and the wide-mode
+ * threshold (narrow against wide on the 1, 4, 16, 64 largest instances) --
+ * and applies the fastest to the empty batch b.  report (may be NULL)
+ * receives report_len doubles: one 5-tuple per measurement
+ * (kind, block|k, lds_cap|clauses, ms, GB/s|wide ms); *report_count is the
+ * number of doubles written. */
+int dnx_batch_tune(dnx_batch *b, int n_min, int n_max, double r_min, double r_max, int sample,
+                   int steps, uint64_t seed, double *report, int report_len, int *report_count);
+
+/* The current shape of a mode and the wide-mode threshold. */
+int dnx_batch_get_shape(dnx_batch *b, int mode, int *block, int *lds_cap, int64_t *wide_clauses);
+
+/* Waits for queued work, then reports the summed device time (ms) and the
+ * count of timed launches since the last reset, in four buckets: modes 0-2
+ * of the fused kernel, then wide mode (arrays of 4).  enable = 1/0 turns
  * launch timing on/off and resets the sums (-1: report only). */
 int dnx_batch_timing(dnx_batch *b, int enable, double *ms, int64_t *launches);
 int dnx_batch_memory(dnx_batch *b, int64_t *arena_bytes, int64_t *free_bytes);
@@ -219,11 +246,13 @@ int dnx_batch_memory(dnx_batch *b, int64_t *arena_bytes, int64_t *free_bytes);
  * request: status (DNX_SAT only after the exact clause check; DNX_FAILED for
  * a request larger than the arena), steps, time-to-solution from arrival to
  * retirement by the device clock, and queueing delay before admission.
- * prms may be NULL (reference defaults). */
+ * prms may be NULL (reference defaults).  With deadline_s > 0 the loop
+ * stops deadline_s seconds after the call starts; requests not finished by
+ * then stay DNX_UNKNOWN with time-to-solution -1. */
 int dnx_serve_trace(dnx_batch *b, int count, dnx_problem *const *probs, const dnx_params *prms,
                     const double *const *x0s, const double *arrival_s, int64_t step_cap,
-                    int steps_per_launch, int *status, int64_t *steps, double *tts_s,
-                    double *wait_s, double *makespan_s);
+                    int steps_per_launch, double deadline_s, int *status, int64_t *steps,
+                    double *tts_s, double *wait_s, double *makespan_s);
 
 /* The served planted stream: Poisson arrivals at rate_per_s for duration_s
  * (at most max_requests), n log-uniform in [n_min, n_max], clause ratio
@@ -233,9 +262,9 @@ int dnx_serve_trace(dnx_batch *b, int count, dnx_problem *const *probs, const dn
  * *count_out requests were issued. */
 int dnx_serve_planted(dnx_batch *b, uint64_t seed, double rate_per_s, double duration_s, int n_min,
                       int n_max, double r_min, double r_max, double p1, double p2, int64_t step_cap,
-                      int steps_per_launch, int max_requests, int gen_threads, int *count_out,
-                      double *arrival_out, int *n_out, int *m_out, int *status, int64_t *steps,
-                      double *tts_s, double *wait_s, double *makespan_s);
+                      int steps_per_launch, double deadline_s, int max_requests, int gen_threads,
+                      int *count_out, double *arrival_out, int *n_out, int *m_out, int *status,
+                      int64_t *steps, double *tts_s, double *wait_s, double *makespan_s);
 
 /* ---- host utilities -------------------------------------------------- */
 /* Initial state of the reference's thread 0 for seed `seed`

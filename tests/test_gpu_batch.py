@@ -48,8 +48,9 @@ def oracle_solve(n, cp, lits, x0, cap):
     return steps, t, status, x
 
 
-def run_batch(ctx, inst, cap, spl, shapes=None, stop=True):
+def run_batch(ctx, inst, cap, spl, shapes=None, stop=True, wide=0):
     b = dnx.Batch(ctx, slots=len(inst) + 2, arena_bytes=1 << 30)
+    b.set_wide(wide, 30 if wide else 30)
     if shapes:
         for md, (bs, lds) in shapes.items():
             b.set_shape(md, bs, lds)
@@ -104,6 +105,32 @@ def test_shapes_give_identical_bits(ctx, inst_set, oracle_results, shapes):
     for r, (steps, t, status, xo) in zip(out, oracle_results):
         assert r["steps"] == steps and r["t"] == t
         assert np.array_equal(r["x"], xo)
+
+
+@pytest.mark.parametrize("wide", [1 << 40, 60000])
+def test_wide_mode_gives_identical_bits(ctx, inst_set, oracle_results, wide):
+    """Wide mode from the start (all instances), and switching to it once the
+    running instances hold at most 60,000 clauses."""
+    out = run_batch(ctx, inst_set, 1500, spl=40, wide=wide)
+    for r, (steps, t, status, xo) in zip(out, oracle_results):
+        assert r["status"] == (dnx.SAT if status == 1 else dnx.UNKNOWN)
+        assert r["steps"] == steps and r["t"] == t
+        assert np.array_equal(r["x"], xo)
+        if status == 1:
+            assert r["unsat"] == 0
+
+
+def test_wide_fixed_step_trajectory(ctx):
+    inst = make_set(12, 5, nmin=100, nmax=3000, with_general=True)
+    out = run_batch(ctx, inst, 45, spl=25, stop=False, wide=1 << 40)
+    for r, (n, cp, lits, x0) in zip(out, inst):
+        op = O.Problem(n, len(cp) - 1, cp, lits)
+        x = x0.astype(np.float32)
+        if op.has_unit:
+            x[:n] = np.where(op.unit != 0, op.unit, x[:n])
+        O.trajectory(op, O.params(op.m), x, 45, prec=32)
+        assert r["status"] == dnx.UNKNOWN and r["steps"] == 45
+        assert np.array_equal(r["x"], x)
 
 
 def test_alone_equals_in_the_mix(ctx, inst_set):
@@ -172,15 +199,18 @@ def test_retire_readmit_and_budget(ctx):
     b.close()
 
 
-def test_serving_results_equal_solo_runs(ctx, inst_set, oracle_results):
+@pytest.mark.parametrize("big", [1 << 40, 8000])
+def test_serving_results_equal_solo_runs(ctx, inst_set, oracle_results, big):
     """Requests arriving while others run are admitted as slots and memory
-    free up; every served result equals the instance solved alone."""
+    free up; every served result equals the instance solved alone, whether it
+    ran in the narrow lane or (clauses >= big) in the wide lane."""
     k = 24
     probs = [dnx.host_problem(n, cp, lits) for n, cp, lits, _ in inst_set[:k]]
     x0s = [x0 for _, _, _, x0 in inst_set[:k]]
     arrivals = np.sort(np.random.default_rng(3).uniform(0.0, 0.25, k))
     # few slots and a small arena force queueing and slot reuse
     b = dnx.Batch(ctx, slots=6, arena_bytes=6 << 20)
+    b.set_serve_big(big)
     r = b.serve(probs, x0s, arrivals, cap=1500, steps_per_launch=16)
     b.close()
     for p in probs:
@@ -205,3 +235,25 @@ def test_serve_planted_stream(ctx):
     assert np.all(r["tts"] > 0)
     print("served", r["count"], "p50 tts ms", np.percentile(r["tts"], 50) * 1e3,
           "p99", np.percentile(r["tts"], 99) * 1e3)
+
+
+def test_autotuner_applies_a_shape(ctx, inst_set, oracle_results):
+    b = dnx.Batch(ctx, slots=64, arena_bytes=1 << 30)
+    rep = b.tune(n_min=100, n_max=2000, sample=48, steps=30)
+    assert rep.shape[1] == 5 and len(rep) >= 12
+    assert np.all(rep[:, 3] > 0)
+    bs0, cap0, wide = b.get_shape(0)
+    assert bs0 in (256, 512, 1024) and wide >= 0
+    print("tuned: mode0", b.get_shape(0), "mode1", b.get_shape(1))
+    # the tuned batch computes the same bits
+    slots = []
+    for n, cp, lits, x0 in inst_set[:10]:
+        hp = dnx.host_problem(n, cp, lits)
+        slots.append(b.admit(hp, dnx.params(len(cp) - 1), x0, cap=1500))
+        hp.free()
+    b.run(64)
+    res = b.poll()
+    for s, (steps, t, status, xo), (n, cp, lits, x0) in zip(slots, oracle_results[:10], inst_set[:10]):
+        assert res["steps"][s] == steps and res["t"][s] == t
+        assert np.array_equal(b.read_state(s, n, len(cp) - 1)["x"].astype(np.float32), xo)
+    b.close()
