@@ -196,6 +196,8 @@ def main():
     ap.add_argument("--served-rate", type=float, default=200.0)
     ap.add_argument("--served-fp64-sample", type=int, default=12)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--profile-pass", action="store_true",
+                    help="only the copy-kernel calibration and one batched solve (for rocprofv3 runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; the batched instances would shard by instance "
@@ -209,6 +211,19 @@ def main():
         f"sum m={sum(w['m'] for w in work)} ({time.perf_counter() - t0:.1f}s to build)")
 
     ctx = dnx.Context(0)
+    if args.profile_pass:
+        copy_bytes = 256 << 20
+        ms = ctx.device_copy(copy_bytes, repeats=4)
+        log(f"copy kernel: {copy_bytes} B in {ms:.3f} ms ({2 * copy_bytes / ms / 1e6:.0f} GB/s read+write)")
+        r = batched_solve(ctx, work, hps, args.cap, args.steps_per_launch, timing=True)
+        ns_all = np.array([w["n"] for w in work])
+        ms_all = np.array([w["m"] for w in work])
+        alg = float(np.sum((r["steps"] + 1) * (28 * ms_all + 8 * ns_all)))
+        print(json.dumps(dict(profile_pass=True, copy_bytes=copy_bytes, copy_ms=ms, batch_ms=r["wall"] * 1e3,
+                              alg_bytes=alg, device_ms=r["ms"].tolist(), launches=r["launches"].tolist(),
+                              verified=int(r["verified"].sum()))), flush=True)
+        ctx.close()
+        return
     shape, tune_rows = None, None
     if not args.no_tune:
         t0 = time.perf_counter()

@@ -676,6 +676,39 @@ int dnx_solve(dnx_problem *p, int prec, const dnx_params *prm, const double *x0,
     return rc;
 }
 
+int dnx_device_copy(dnx_ctx *ctx, int64_t bytes, int repeats, double *ms)
+{
+    if (!ctx || bytes < 16 || repeats <= 0 || !ms) return DNX_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    bytes &= ~(int64_t)15;
+    DevBuf a, b;
+    int rc;
+    if ((rc = a.alloc((size_t)bytes)) || (rc = b.alloc((size_t)bytes))) return rc;
+    hipStream_t st = ctx->stream;
+    hipError_t e;
+    if ((e = hipMemsetAsync(a.ptr, 1, (size_t)bytes, st)) != hipSuccess) return hip_fail(e, "memset");
+    hipEvent_t e0, e1;
+    if ((e = hipEventCreate(&e0)) != hipSuccess) return hip_fail(e, "event");
+    if ((e = hipEventCreate(&e1)) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return hip_fail(e, "event");
+    }
+    e = hipEventRecord(e0, st);
+    for (int i = 0; i < repeats && e == hipSuccess; i++) {
+        launch_copy(a.ptr, b.ptr, bytes, st);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float f = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return hip_fail(e, "copy");
+    *ms = (double)f / repeats;
+    return DNX_OK;
+}
+
 int dnx_init_reference(const dnx_problem *p, int seed, double *x0)
 {
     if (!p || !x0) return DNX_E_ARG;

@@ -135,6 +135,8 @@ struct dnx_batch {
     int wide_chunk = 30;
     // serving: requests with at least this many clauses run in the wide lane
     long long serve_big = 60000;
+    // serving: clause budget of the wide lane's active set per chunk
+    long long serve_active_clauses = 2000000;
     DevBuf b_wctl, b_wlist, b_bmap, b_vmap;
     ~dnx_batch()
     {
@@ -840,10 +842,11 @@ int dnx_batch_set_wide(dnx_batch *b, int64_t max_clauses, int steps_per_sync)
     return DNX_OK;
 }
 
-int dnx_batch_set_serve_big(dnx_batch *b, int64_t clauses)
+int dnx_batch_set_serve_big(dnx_batch *b, int64_t clauses, int64_t active_clauses)
 {
-    if (!b || clauses < 0) return DNX_E_ARG;
+    if (!b || clauses < 0 || active_clauses <= 0) return DNX_E_ARG;
     b->serve_big = clauses;
+    b->serve_active_clauses = active_clauses;
     return DNX_OK;
 }
 
@@ -926,7 +929,12 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
     std::vector<int> slot_req((size_t)b->nslots, -1);
     std::vector<char> slot_wide((size_t)b->nslots, 0);
     std::vector<int> narrow_run, wide_set;  // members of each lane
-    std::vector<int> narrow_launched;       // slots in the narrow launch in flight
+    // the slots each lane's launch in flight covers: a lane judges only
+    // these from its snapshot, because a slot admitted later may still show
+    // its previous occupant's final status there
+    std::vector<int> narrow_launched, wide_launched;
+    std::vector<int> wide_active;                          // members the wide graph covers
+    std::vector<long long> attained((size_t)b->nslots, 0);  // steps done per slot
     bool narrow_busy = false, wide_busy = false, wide_dirty = false;
     hipStream_t wst = nullptr;
     hipEvent_t ev_n = nullptr, ev_w = nullptr, ev_adm = nullptr;
@@ -1048,6 +1056,7 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
             }
             if (rc) return rc;
             slot_req[(size_t)slot] = r;
+            attained[(size_t)slot] = 0;
             out.wait[r] = now_s() - arrival[r];
             if (q.hp->m >= big) {
                 slot_wide[(size_t)slot] = 1;
@@ -1084,23 +1093,38 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
         if (wide_busy && hipEventQuery(ev_w) == hipSuccess) {
             wide_busy = false;
             progressed = true;
-            std::vector<int> keep;
-            for (int s : wide_set) {
-                if (hw[s].status == DNX_RUNNING || hw[s].status == -1) {
-                    keep.push_back(s);  // still running, or admitted after the snapshot
-                    continue;
-                }
+            for (int s : wide_launched) {
+                attained[(size_t)s] = hw[s].steps;
+                if (hw[s].status == DNX_RUNNING) continue;
                 if ((rc = finish(s, hw[s], wst))) return rc;
+                wide_set.erase(std::find(wide_set.begin(), wide_set.end(), s));
                 wide_dirty = true;
             }
-            wide_set.swap(keep);
         }
         if (!wide_busy && !wide_set.empty()) {
+            // least attained service: the members with the fewest steps so
+            // far run, up to the lane's clause budget; the others keep
+            // their state and wait.  New arrivals start at once, and under
+            // overload the longest-running instances yield.
+            std::vector<int> active = wide_set;
+            std::stable_sort(active.begin(), active.end(),
+                             [&](int x, int y) { return attained[(size_t)x] < attained[(size_t)y]; });
+            long long budget = 0;
+            size_t keep = 0;
+            while (keep < active.size() &&
+                   (keep == 0 || budget + b->slots[(size_t)active[keep]].m <= b->serve_active_clauses)) {
+                budget += b->slots[(size_t)active[keep]].m;
+                keep++;
+            }
+            active.resize(keep);
+            std::sort(active.begin(), active.end());
+            if (active != wide_active) wide_dirty = true;
             if (wide_dirty || !wgraph) {
-                // new maps and graph for the current members, entered from
+                wide_active = active;
+                // new maps and graph for the active members, entered from
                 // their status records (current after every chunk)
                 std::vector<int> bmap, vmap;
-                for (int s : wide_set) {
+                for (int s : wide_active) {
                     const SlotHost &sl = b->slots[(size_t)s];
                     for (int j = 0; j * WIDE_BLOCK < sl.m; j++) {
                         bmap.push_back(s);
@@ -1114,11 +1138,11 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
                 nbc = (int)bmap.size() / 2;
                 nbv = (int)vmap.size() / 2;
                 auto grow = [&](DevBuf &buf, size_t bytes) { return buf.size >= bytes ? DNX_OK : buf.alloc(bytes); };
-                if ((rc = grow(b->b_wlist, sizeof(int) * wide_set.size())) ||
+                if ((rc = grow(b->b_wlist, sizeof(int) * wide_active.size())) ||
                     (rc = grow(b->b_bmap, sizeof(int) * (bmap.size() + 2))) ||
                     (rc = grow(b->b_vmap, sizeof(int) * (vmap.size() + 2))))
                     return rc;
-                if ((e = hipMemcpy(b->b_wlist.ptr, wide_set.data(), sizeof(int) * wide_set.size(),
+                if ((e = hipMemcpy(b->b_wlist.ptr, wide_active.data(), sizeof(int) * wide_active.size(),
                                    hipMemcpyHostToDevice)) != hipSuccess ||
                     (nbc && (e = hipMemcpy(b->b_bmap.ptr, bmap.data(), sizeof(int) * bmap.size(),
                                            hipMemcpyHostToDevice)) != hipSuccess) ||
@@ -1141,12 +1165,13 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
                 (void)hipGraphDestroy(g);
                 if (e != hipSuccess) return hip_fail(e, "wide graph");
                 launch_wide_enter((const BDyn *)b->b_dyn.ptr, (Ctl *)b->b_wctl.ptr, (const int *)b->b_wlist.ptr,
-                                  (int)wide_set.size(), 0, wst);
+                                  (int)wide_active.size(), 0, wst);
                 wide_dirty = false;
             }
+            wide_launched = wide_active;
             if ((e = hipGraphLaunch(wgraph, wst)) != hipSuccess) return hip_fail(e, "wide launch");
             launch_wide_sync((BDyn *)b->b_dyn.ptr, (Ctl *)b->b_wctl.ptr, (const int *)b->b_wlist.ptr,
-                             (int)wide_set.size(), 2, wst);
+                             (int)wide_active.size(), 2, wst);
             if ((e = hipMemcpyAsync(hw, b->b_dyn.ptr, sizeof(BDyn) * (size_t)b->nslots, hipMemcpyDeviceToHost, wst)) !=
                     hipSuccess ||
                 (e = hipEventRecord(ev_w, wst)) != hipSuccess)

@@ -283,12 +283,26 @@ __global__ __launch_bounds__(256) void k_load_state(DevProb P, const T *__restri
     }
 }
 
+// Plain streaming copy (16 bytes per lane per iteration): the known-size
+// reference for calibrating memory-traffic counters.
+__global__ __launch_bounds__(256) void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, long long count)
+{
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < count; i += (long long)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 static inline unsigned nblk(long long count, int bs) { return (unsigned)((count + bs - 1) / bs); }
 
 int rhs_clause_blocks(int m) { return (int)nblk(m > 0 ? m : 1, CBS); }
+
+void launch_copy(const void *src, void *dst, long long bytes, void *stream)
+{
+    const long long count = bytes / 16;
+    if (count > 0) k_copy<<<4096, 256, 0, (hipStream_t)stream>>>((const uint4 *)src, (uint4 *)dst, count);
+}
 
 template <typename T>
 static void rhs_impl(const DevProb &P, const DevParams &prm, const void *x, void *dxdt, void *e, int *lpart,

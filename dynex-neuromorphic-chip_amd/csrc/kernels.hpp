@@ -73,6 +73,9 @@ void launch_rhs(int prec, const DevProb &P, const DevParams &prm, const void *x,
                 void *dxdt, void *e, int *loc_part, double *energy_part, void *stream);
 int rhs_clause_blocks(int m);
 
+// streaming copy of `bytes` (multiple of 16) from src to dst
+void launch_copy(const void *src, void *dst, long long bytes, void *stream);
+
 void launch_euler(int prec, const DevProb &P, const DevParams &prm, const void *x,
                   const void *dxdt, double h, void *x_out, void *stream);
 

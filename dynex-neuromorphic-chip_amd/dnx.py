@@ -72,6 +72,7 @@ def lib():
     L.dnx_init_uniform.argtypes = [C.c_uint64, ip, ip, dp, dp, F64P]
     L.dnx_write_dimacs.argtypes = [C.c_char_p, ip, ip, I32P, I32P, C.c_char_p]
     L.dnx_check_assignment.argtypes = [vp, I8P, C.POINTER(C.c_int)]
+    L.dnx_device_copy.argtypes = [vp, C.c_int64, ip, C.POINTER(C.c_double)]
     L.dnx_batch_open.argtypes = [vp, ip, ip, C.c_int64, pp]
     L.dnx_batch_close.argtypes = [vp]
     L.dnx_batch_set_shape.argtypes = [vp, ip, ip, ip]
@@ -87,7 +88,7 @@ def lib():
     L.dnx_batch_memory.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.dnx_batch_timing.argtypes = [vp, ip, vp, vp]
     L.dnx_batch_set_wide.argtypes = [vp, C.c_int64, ip]
-    L.dnx_batch_set_serve_big.argtypes = [vp, C.c_int64]
+    L.dnx_batch_set_serve_big.argtypes = [vp, C.c_int64, C.c_int64]
     L.dnx_batch_tune.argtypes = [vp, ip, ip, dp, dp, ip, ip, C.c_uint64, vp, ip, C.POINTER(C.c_int)]
     L.dnx_batch_get_shape.argtypes = [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)]
     L.dnx_state_time.argtypes = [vp, ip, C.POINTER(C.c_double)]
@@ -172,6 +173,12 @@ class Context:
 
     def __exit__(self, *a):
         self.close()
+
+    def device_copy(self, nbytes, repeats=1):
+        """Mean device ms of one plain copy of nbytes (known-size reference)."""
+        ms = C.c_double()
+        check(lib().dnx_device_copy(self.h, int(nbytes), int(repeats), C.byref(ms)))
+        return ms.value
 
     def problem_from_lits(self, n, cp, lits):
         return Problem(self, n, cp, lits)
@@ -368,8 +375,8 @@ class Batch:
         check(lib().dnx_batch_get_shape(self.h, int(mode), C.byref(bs), C.byref(cap), C.byref(wide)))
         return bs.value, cap.value, wide.value
 
-    def set_serve_big(self, clauses):
-        check(lib().dnx_batch_set_serve_big(self.h, int(clauses)))
+    def set_serve_big(self, clauses, active_clauses=2000000):
+        check(lib().dnx_batch_set_serve_big(self.h, int(clauses), int(active_clauses)))
 
     def set_wide(self, max_clauses, steps_per_sync=30):
         check(lib().dnx_batch_set_wide(self.h, int(max_clauses), int(steps_per_sync)))

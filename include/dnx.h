@@ -211,8 +211,11 @@ int dnx_batch_info(dnx_batch *b, int slot, int *n, int *m, int *mode);
 int dnx_batch_set_wide(dnx_batch *b, int64_t max_clauses, int steps_per_sync);
 
 /* Serving: requests with at least `clauses` clauses run in the wide lane
- * (default 60,000; needs wide mode enabled). */
-int dnx_batch_set_serve_big(dnx_batch *b, int64_t clauses);
+ * (default 60,000; needs wide mode enabled).  Each wide chunk advances the
+ * members with the fewest steps so far, up to active_clauses clauses in
+ * total (default 2,000,000): least-attained-service order, so new requests
+ * start at once and, under overload, the longest-running ones wait. */
+int dnx_batch_set_serve_big(dnx_batch *b, int64_t clauses, int64_t active_clauses);
 
 /* Autotuner: times candidate execution shapes on `sample` synthetic planted
  * instances (n log-uniform in [n_min, n_max], clause ratio uniform in
@@ -265,6 +268,13 @@ int dnx_serve_planted(dnx_batch *b, uint64_t seed, double rate_per_s, double dur
                       int steps_per_launch, double deadline_s, int max_requests, int gen_threads,
                       int *count_out, double *arrival_out, int *n_out, int *m_out, int *status,
                       int64_t *steps, double *tts_s, double *wait_s, double *makespan_s);
+
+/* ---- measurement ------------------------------------------------------ */
+/* Streams `bytes` (rounded down to a multiple of 16) from one device buffer
+ * to another `repeats` times with a plain copy kernel and reports the mean
+ * device time per copy (ms).  The known-size reference for memory-traffic
+ * counters and for the achievable streaming bandwidth. */
+int dnx_device_copy(dnx_ctx *ctx, int64_t bytes, int repeats, double *ms);
 
 /* ---- host utilities -------------------------------------------------- */
 /* Initial state of the reference's thread 0 for seed `seed`
