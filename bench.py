@@ -25,6 +25,10 @@ sys.path.insert(0, os.path.join(ROOT, "dynex-neuromorphic-chip_amd"))
 import dnx  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0
+# HBM bytes per launch of each bucket's kernels from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE, scaled by the counters' reading on the copy
+# kernel (None where not measured).
+TRAFFIC = {}
 METRIC = "solved instances/s + p50/p99 time-to-solution (batched FP32 vs plain FP64, 1 GPU)"
 
 
@@ -75,7 +79,7 @@ def batched_solve(ctx, work, hps, cap, spl, timing, shape=None):
         t0 = time.perf_counter()
         b.run(steps_per_launch=spl)
         wall = time.perf_counter() - t0
-        ms, nl = b.timing(0 if timing else -1)
+        ms, nl, ab = b.timing(0 if timing else -1, with_bytes=True)
         res = b.poll()
         st = res["status"][slots]
         steps = res["steps"][slots]
@@ -88,7 +92,8 @@ def batched_solve(ctx, work, hps, cap, spl, timing, shape=None):
         modes = np.array([b.info(s)[2] for s in slots])
     finally:
         b.close()
-    return dict(wall=wall, status=st, steps=steps, tts=tts, verified=verified, ms=ms, launches=nl, modes=modes)
+    return dict(wall=wall, status=st, steps=steps, tts=tts, verified=verified, ms=ms, launches=nl, alg=ab,
+                modes=modes)
 
 
 def plain_fp64(ctx, work, idx, cap, budget_s):
@@ -198,12 +203,22 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--profile-pass", action="store_true",
                     help="only the copy-kernel calibration and one batched solve (for rocprofv3 runs)")
+    ap.add_argument("--profile-single", action="store_true",
+                    help="only the copy-kernel calibration and 300 FP32 steps of the n=100k instance")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; the batched instances would shard by instance "
               "across GPUs with no per-step communication (see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    if args.profile_single:
+        ctx = dnx.Context(0)
+        copy_bytes = 256 << 20
+        ms = ctx.device_copy(copy_bytes, repeats=4)
+        r = single_instance(ctx, 100000, 800000, dnx.FP32, 300)
+        print(json.dumps(dict(profile_single=True, copy_bytes=copy_bytes, copy_ms=ms, **r)), flush=True)
+        ctx.close()
+        return
     t0 = time.perf_counter()
     work = make_workload(args.instances, args.seed)
     hps = [dnx.host_problem(w["n"], w["cp"], w["lits"]) for w in work]
@@ -211,19 +226,6 @@ def main():
         f"sum m={sum(w['m'] for w in work)} ({time.perf_counter() - t0:.1f}s to build)")
 
     ctx = dnx.Context(0)
-    if args.profile_pass:
-        copy_bytes = 256 << 20
-        ms = ctx.device_copy(copy_bytes, repeats=4)
-        log(f"copy kernel: {copy_bytes} B in {ms:.3f} ms ({2 * copy_bytes / ms / 1e6:.0f} GB/s read+write)")
-        r = batched_solve(ctx, work, hps, args.cap, args.steps_per_launch, timing=True)
-        ns_all = np.array([w["n"] for w in work])
-        ms_all = np.array([w["m"] for w in work])
-        alg = float(np.sum((r["steps"] + 1) * (28 * ms_all + 8 * ns_all)))
-        print(json.dumps(dict(profile_pass=True, copy_bytes=copy_bytes, copy_ms=ms, batch_ms=r["wall"] * 1e3,
-                              alg_bytes=alg, device_ms=r["ms"].tolist(), launches=r["launches"].tolist(),
-                              verified=int(r["verified"].sum()))), flush=True)
-        ctx.close()
-        return
     shape, tune_rows = None, None
     if not args.no_tune:
         t0 = time.perf_counter()
@@ -233,6 +235,16 @@ def main():
         tb.close()
         log(f"autotuned in {time.perf_counter() - t0:.1f}s: mode0 {shape[0]}, mode1 {shape[1]}, "
             f"wide threshold {shape[2]} clauses")
+    if args.profile_pass:
+        copy_bytes = 256 << 20
+        ms = ctx.device_copy(copy_bytes, repeats=4)
+        log(f"copy kernel: {copy_bytes} B in {ms:.3f} ms ({2 * copy_bytes / ms / 1e6:.0f} GB/s read+write)")
+        r = batched_solve(ctx, work, hps, args.cap, args.steps_per_launch, timing=True, shape=shape)
+        print(json.dumps(dict(profile_pass=True, copy_bytes=copy_bytes, copy_ms=ms, batch_ms=r["wall"] * 1e3,
+                              device_ms=r["ms"].tolist(), launches=r["launches"].tolist(), alg_bytes=r["alg"].tolist(),
+                              verified=int(r["verified"].sum()))), flush=True)
+        ctx.close()
+        return
     runs = []
     for it in range(args.warmup + args.steps):
         r = batched_solve(ctx, work, hps, args.cap, args.steps_per_launch, timing=it >= args.warmup, shape=shape)
@@ -247,26 +259,29 @@ def main():
     value = float(np.mean([r["verified"].sum() / r["wall"] for r in runs]))
     tts = last["tts"][last["verified"]]
 
-    # dominant kernel: the execution shape with the most device time
-    # Roofline of the whole device run: compulsory bytes of every accepted
-    # step (and the final evaluation) of every instance over the summed
-    # device time of all step kernels.  The per-kernel split is reported
-    # beside it; the fused kernel (mode 1) dominates.
+    # Dominant kernel: the execution bucket with the most device time (HIP
+    # events around every launch on the stream it runs on).  Its algorithmic
+    # bytes are the compulsory bytes (bytes_alg) of the instance steps its
+    # launches took, counted by the engine from the status snapshots around
+    # each launch; achieved = those bytes per launch / average launch time.
     ms_sum = np.sum([r["ms"] for r in runs], axis=0)
     nl_sum = np.sum([r["launches"] for r in runs], axis=0)
-    ns_all = np.array([w["n"] for w in work])
-    ms_all = np.array([w["m"] for w in work])
-    alg = float(sum(np.sum((r["steps"] + 1) * (28 * ms_all + 8 * ns_all)) for r in runs))
-    kern_s = float(ms_sum.sum()) / 1e3
-    achieved = alg / kern_s / 1e9 if kern_s > 0 else 0.0
-    names = ["k_batch<mode 0>", "k_batch<mode 1>", "k_batch<mode 2>", "k_wide_clause+k_wide_var"]
+    ab_sum = np.sum([r["alg"] for r in runs], axis=0)
+    names = ["k_batch<mode 0>", "k_batch<mode 1>", "k_batch<mode 2>",
+             "k_wide_clause+k_wide_var (one graph replay of 30 steps)"]
     dom = int(np.argmax(ms_sum))
+    avg_us = float(ms_sum[dom] / nl_sum[dom] * 1e3) if nl_sum[dom] else 0.0
+    per_launch = float(ab_sum[dom] / nl_sum[dom]) if nl_sum[dom] else 0.0
+    achieved = per_launch / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
+    all_s = float(ms_sum.sum()) / 1e3
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBPS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBPS, 4), traffic=None,
-                    kernel="all batched step kernels (dominant: %s)" % names[dom],
+                    frac=round(achieved / HBM_PEAK_GBPS, 4), traffic=TRAFFIC.get(dom),
+                    kernel=names[dom], alg_bytes_per_launch=round(per_launch), avg_launch_us=round(avg_us, 2),
                     device_ms={names[i]: round(float(ms_sum[i]), 2) for i in range(4)},
                     launches={names[i]: int(nl_sum[i]) for i in range(4)},
-                    alg_bytes_total=round(alg))
+                    achieved_GBps={names[i]: round(float(ab_sum[i] / ms_sum[i] / 1e6), 1) if ms_sum[i] else None
+                                   for i in range(4)},
+                    all_kernels_GBps=round(float(ab_sum.sum()) / all_s / 1e9, 1) if all_s else None)
 
     rng = np.random.default_rng(7)
     sample = rng.permutation(len(work))[:args.fp64_sample]

@@ -716,6 +716,87 @@ int dnx_init_reference(const dnx_problem *p, int seed, double *x0)
     return DNX_OK;
 }
 
+int dnx_init_reference_thread(const dnx_problem *p, int seed, int tid, double *x0)
+{
+    if (!p || !x0 || tid < 0) return DNX_E_ARG;
+    init_reference_thread(p->hp, seed, tid, x0);
+    return DNX_OK;
+}
+
+// Portfolio (the reference's -w, dynex.cc:2731-2801): `threads` copies of
+// the problem with the reference's per-thread initial states and alpha/i
+// (dynex.cc:2657) advance in rounds of equal step counts; the first round in
+// which any copy verifies ends the run, and the winner is the verified copy
+// with the fewest steps (lowest thread index on a tie).  The reference takes
+// whichever thread happens to finish first in wall-clock time; this choice
+// is deterministic.  FP32 runs the copies as slots of one batch; FP64 (which
+// the batched engine does not run) as one state per copy, in rounds of one
+// graph chunk each.
+int dnx_portfolio_solve(dnx_problem *p, int prec, const dnx_params *prm, int seed, int threads, int64_t step_cap,
+                        double walltime_s, int8_t *assign_out, int64_t *steps_out, double *t_out, int *status,
+                        int *winner)
+{
+    if (!p || !prm || threads <= 0 || step_cap < 0 || !status || (prec != DNX_FP64 && prec != DNX_FP32))
+        return DNX_E_ARG;
+    if (!p->ctx) return host_only(p);
+    (void)hipSetDevice(p->ctx->device);
+    if (prec == DNX_FP32)
+        return portfolio_batched(p, prm, seed, threads, step_cap, walltime_s, assign_out, steps_out, t_out, status,
+                                 winner);
+    const HostProblem &h = p->hp;
+    const size_t N = (size_t)h.n + 2 * (size_t)h.m;
+    std::vector<dnx_state *> st((size_t)threads, nullptr);
+    struct Free {
+        std::vector<dnx_state *> &v;
+        ~Free()
+        {
+            for (auto *s : v)
+                if (s) dnx_state_free(s);
+        }
+    } guard{st};
+    std::vector<double> x(N ? N : 1);
+    for (int i = 0; i < threads; i++) {
+        init_reference_thread(h, seed, i, x.data());
+        dnx_params pi = *prm;
+        if (i > 0) pi.alpha = prm->alpha / i;  // dynex.cc:2657
+        int rc = dnx_state_create(p, prec, &pi, x.data(), &st[(size_t)i]);
+        if (rc) return rc;
+        if ((rc = set_mode(st[(size_t)i], 1, step_cap))) return rc;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    hipStream_t s = p->ctx->stream;
+    std::vector<Ctl> c((size_t)threads);
+    *status = DNX_UNKNOWN;
+    if (winner) *winner = -1;
+    for (;;) {
+        for (auto *sp : st) {
+            int rc = state_enqueue_steps(sp, kChunk);
+            if (rc) return rc;
+        }
+        hipError_t e = hipSuccess;
+        for (int i = 0; i < threads && e == hipSuccess; i++)
+            e = hipMemcpyAsync(&c[(size_t)i], st[(size_t)i]->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_fail(e, "portfolio poll");
+        int best = -1, running = 0;
+        for (int i = 0; i < threads; i++) {
+            const int sv = ctl_status(c[(size_t)i]);
+            if (sv == DNX_RUNNING) running++;
+            if (sv == DNX_SAT && (best < 0 || c[(size_t)i].steps < c[(size_t)best].steps)) best = i;
+        }
+        if (best >= 0) {
+            int stt = DNX_UNKNOWN;
+            int rc = dnx_state_solve(st[(size_t)best], 0, 0.0, assign_out, steps_out, t_out, &stt);
+            if (rc) return rc;
+            *status = stt;
+            if (winner) *winner = best;
+            return DNX_OK;
+        }
+        const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (!running || (walltime_s > 0 && el > walltime_s)) return DNX_OK;
+    }
+}
+
 int dnx_generate_planted(uint64_t seed, int n, int m, double p1, double p2, int32_t *lits_out, int8_t *planted_out)
 {
     if (n < 3 || m < 0 || !lits_out || p1 < 0 || p2 < 0 || p1 + p2 > 1.0) return DNX_E_ARG;

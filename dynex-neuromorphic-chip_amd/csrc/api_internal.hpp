@@ -63,4 +63,8 @@ struct dnx_state {
 
 namespace dnx {
 DevProb dev_prob(const dnx_problem *p);
+// dnx_portfolio_solve in DNX_FP32: the copies run as slots of one batch.
+int portfolio_batched(dnx_problem *p, const dnx_params *prm, int seed, int threads, int64_t step_cap,
+                      double walltime_s, int8_t *assign_out, int64_t *steps_out, double *t_out, int *status,
+                      int *winner);
 }

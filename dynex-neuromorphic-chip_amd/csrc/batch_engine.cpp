@@ -18,6 +18,7 @@
 #include <cstring>
 #include <deque>
 #include <functional>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -89,6 +90,11 @@ struct SlotHost {
     std::vector<int32_t> cp, lits;  // kept for the host-side clause check
 };
 
+// Compulsory bytes of one FP32 step of an instance: literals read once
+// (4 per literal), Xs and Xl read and written (16 per clause), the (V, dV)
+// pair read and written (8 per variable).
+static double step_bytes(const SlotHost &sl) { return 4.0 * (double)sl.nnz + 16.0 * sl.m + 8.0 * sl.n; }
+
 struct Shape {
     int bs;
     int lds_cap;  // bytes of LDS the shape may use per workgroup
@@ -129,6 +135,9 @@ struct dnx_batch {
     bool t_pending[BM_COUNT + 1] = {false, false, false, false};
     double t_ms[BM_COUNT + 1] = {0, 0, 0, 0};
     long long t_launches[BM_COUNT + 1] = {0, 0, 0, 0};
+    // compulsory bytes of the steps taken by the timed launches of
+    // dnx_batch_run, per bucket (step_bytes per instance step)
+    double t_alg[BM_COUNT + 1] = {0, 0, 0, 0};
     // wide mode: entered when the running instances hold at most
     // wide_clauses clauses in total; wide_chunk steps between status syncs
     long long wide_clauses = 1500000;
@@ -136,7 +145,10 @@ struct dnx_batch {
     // serving: requests with at least this many clauses run in the wide lane
     long long serve_big = 60000;
     // serving: clause budget of the wide lane's active set per chunk
-    long long serve_active_clauses = 2000000;
+    long long serve_active_clauses = 6000000;
+    // portfolio: dnx_batch_run returns at the first status sync at which
+    // any slot is satisfied
+    bool group_stop = false;
     DevBuf b_wctl, b_wlist, b_bmap, b_vmap;
     ~dnx_batch()
     {
@@ -598,6 +610,16 @@ int dnx_batch_poll(dnx_batch *b, int *status, int64_t *steps, double *t, uint64_
     return DNX_OK;
 }
 
+// True when group_stop is set and a slot in the status snapshot h_dyn
+// (buffer 0) is satisfied.
+static bool group_done(const dnx_batch *b)
+{
+    if (!b->group_stop) return false;
+    for (int s = 0; s < b->nslots; s++)
+        if (b->slots[(size_t)s].used && b->h_dyn[s].status == DNX_SAT) return true;
+    return false;
+}
+
 // Runs every running slot in wide mode until none runs or the wall-clock
 // limit passes.  The status table in h_dyn (buffer 0) is current on return.
 static int batch_wide_run(dnx_batch *b, double walltime_s, std::chrono::steady_clock::time_point t0)
@@ -663,7 +685,9 @@ static int batch_wide_run(dnx_batch *b, double walltime_s, std::chrono::steady_c
         }
     } guard{gexec};
     const int r = 0;
+    std::vector<long long> before(list.size());
     for (;;) {
+        for (size_t i = 0; i < list.size(); i++) before[i] = b->h_dyn[list[i]].steps;
         const bool timed = b->timing && !b->t_pending[BM_COUNT];
         if (timed && (e = hipEventRecord(b->t0[BM_COUNT], st)) != hipSuccess) return hip_fail(e, "timing");
         if ((e = hipGraphLaunch(gexec, st)) != hipSuccess) return hip_fail(e, "wide graph launch");
@@ -675,6 +699,10 @@ static int batch_wide_run(dnx_batch *b, double walltime_s, std::chrono::steady_c
         if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "wide launch");
         if ((rc = batch_snapshot(b, 0)) || (rc = batch_wait(b, 0))) return rc;
         batch_collect_timing(b);
+        if (b->timing)
+            for (size_t i = 0; i < list.size(); i++)
+                b->t_alg[BM_COUNT] +=
+                    (double)(b->h_dyn[list[i]].steps - before[i]) * step_bytes(b->slots[(size_t)list[i]]);
         int running = 0;
         long long live = 0;
         for (int s : list)
@@ -682,7 +710,7 @@ static int batch_wide_run(dnx_batch *b, double walltime_s, std::chrono::steady_c
                 running++;
                 live += b->slots[(size_t)s].m;
             }
-        if (!running) break;
+        if (!running || group_done(b)) break;
         // finished instances still cost their blocks; rebuild the maps once
         // most of the clause blocks belong to finished instances
         if (2 * (live / WIDE_BLOCK + running) < nbc) break;
@@ -707,7 +735,7 @@ int dnx_batch_run(dnx_batch *b, int steps_per_launch, double walltime_s, int *re
         const BDyn *known = b->h_dyn;
         running = 0;
         for (int s = 0; s < b->nslots; s++) running += b->slots[(size_t)s].used && known[s].status == DNX_RUNNING;
-        if (!running) break;
+        if (!running || group_done(b)) break;
         if (walltime_s > 0 &&
             std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > walltime_s)
             break;
@@ -718,10 +746,18 @@ int dnx_batch_run(dnx_batch *b, int steps_per_launch, double walltime_s, int *re
             if ((rc = batch_wide_run(b, walltime_s, t0))) return rc;
             continue;
         }
+        std::vector<std::pair<int, long long>> before;
+        if (b->timing)
+            for (int s = 0; s < b->nslots; s++)
+                if (b->slots[(size_t)s].used && known[s].status == DNX_RUNNING) before.push_back({s, known[s].steps});
         if ((rc = batch_enqueue(b, steps_per_launch, known))) return rc;
         if ((rc = batch_snapshot(b, 0))) return rc;
         if ((rc = batch_wait(b, 0))) return rc;
         batch_collect_timing(b);
+        for (auto &q : before) {
+            const SlotHost &sl = b->slots[(size_t)q.first];
+            b->t_alg[sl.mode] += (double)(b->h_dyn[q.first].steps - q.second) * step_bytes(sl);
+        }
     }
     if (remaining) *remaining = running;
     return DNX_OK;
@@ -850,7 +886,7 @@ int dnx_batch_set_serve_big(dnx_batch *b, int64_t clauses, int64_t active_clause
     return DNX_OK;
 }
 
-int dnx_batch_timing(dnx_batch *b, int enable, double *ms, int64_t *launches)
+int dnx_batch_timing(dnx_batch *b, int enable, double *ms, int64_t *launches, double *alg_bytes)
 {
     if (!b) return DNX_E_ARG;
     (void)hipSetDevice(b->ctx->device);
@@ -860,12 +896,14 @@ int dnx_batch_timing(dnx_batch *b, int enable, double *ms, int64_t *launches)
     for (int md = 0; md <= BM_COUNT; md++) {
         if (ms) ms[md] = b->t_ms[md];
         if (launches) launches[md] = b->t_launches[md];
+        if (alg_bytes) alg_bytes[md] = b->t_alg[md];
     }
     if (enable >= 0) {
         b->timing = enable != 0;
         for (int md = 0; md <= BM_COUNT; md++) {
             b->t_ms[md] = 0.0;
             b->t_launches[md] = 0;
+            b->t_alg[md] = 0.0;
         }
     }
     return DNX_OK;
@@ -1368,10 +1406,10 @@ static int tune_run(dnx_ctx *ctx, const std::vector<TuneInst> &inst, const std::
     }
     double ms[BM_COUNT + 1];
     int64_t nl[BM_COUNT + 1];
-    if ((rc = dnx_batch_timing(t, 1, nullptr, nullptr))) return rc;
+    if ((rc = dnx_batch_timing(t, 1, nullptr, nullptr, nullptr))) return rc;
     // one launch runs the fixed steps (steps is a multiple of the wide chunk)
     if ((rc = dnx_batch_run(t, steps, 0.0, nullptr))) return rc;
-    if ((rc = dnx_batch_timing(t, 0, ms, nl))) return rc;
+    if ((rc = dnx_batch_timing(t, 0, ms, nl, nullptr))) return rc;
     double tot = 0.0;
     for (int md = 0; md <= BM_COUNT; md++) tot += ms[md];
     *ms_out = tot;
@@ -1494,3 +1532,65 @@ int dnx_batch_get_shape(dnx_batch *b, int mode, int *block, int *lds_cap, int64_
 }
 
 }  // extern "C"
+
+namespace dnx {
+
+// FP32 portfolio: the `threads` copies are slots of one batch, advanced
+// together (wide mode at every size, so a large instance still spreads over
+// the whole device); the run ends at the first status sync at which a copy
+// is satisfied, and the winner is the satisfied copy with the fewest steps
+// (lowest thread index on a tie).
+int portfolio_batched(dnx_problem *p, const dnx_params *prm, int seed, int threads, int64_t step_cap,
+                      double walltime_s, int8_t *assign_out, int64_t *steps_out, double *t_out, int *status,
+                      int *winner)
+{
+    const HostProblem &h = p->hp;
+    const Layout L = layout_of(h.n, h.m, h.nnz(), !h.uniform3, h.has_unit, BM_GLOBAL);
+    dnx_batch *b = nullptr;
+    int rc = dnx_batch_open(p->ctx, DNX_FP32, threads, (int64_t)((L.total + 4096) * (size_t)threads + (1 << 20)), &b);
+    if (rc) return rc;
+    struct Close {
+        dnx_batch *b;
+        ~Close() { dnx_batch_close(b); }
+    } guard{b};
+    b->group_stop = true;
+    b->wide_clauses = std::numeric_limits<long long>::max();
+    std::vector<int> slot((size_t)threads);
+    std::vector<double> x((size_t)h.n + 2 * (size_t)h.m + 1);
+    for (int i = 0; i < threads; i++) {
+        init_reference_thread(h, seed, i, x.data());
+        dnx_params pi = *prm;
+        if (i > 0) pi.alpha = prm->alpha / i;  // dynex.cc:2657
+        if ((rc = dnx_batch_admit_host(b, h, &pi, x.data(), step_cap, 1, &slot[(size_t)i]))) return rc;
+    }
+    int remaining = 0;
+    if ((rc = dnx_batch_run(b, 32, walltime_s, &remaining))) return rc;
+    std::vector<int> st((size_t)threads);
+    std::vector<int64_t> steps((size_t)threads);
+    std::vector<double> t((size_t)threads);
+    const BDyn *d = b->h_dyn;
+    int best = -1;
+    for (int i = 0; i < threads; i++) {
+        const BDyn &q = d[slot[(size_t)i]];
+        if (q.status == DNX_SAT && (best < 0 || q.steps < d[slot[(size_t)best]].steps)) best = i;
+    }
+    *status = DNX_UNKNOWN;
+    if (winner) *winner = best;
+    const int show = best >= 0 ? best : 0;
+    if (steps_out) *steps_out = d[slot[(size_t)show]].steps;
+    if (t_out) *t_out = d[slot[(size_t)show]].t;
+    if (best < 0) return DNX_OK;
+    int unsat = 0;
+    std::vector<int8_t> a((size_t)(h.n ? h.n : 1));
+    if ((rc = dnx_batch_assignment(b, slot[(size_t)best], a.data(), &unsat))) return rc;
+    if (unsat) {
+        set_error("portfolio: the winning copy's assignment does not verify");
+        *status = DNX_FAILED;
+        return DNX_OK;
+    }
+    if (assign_out) std::copy(a.begin(), a.begin() + h.n, assign_out);
+    *status = DNX_SAT;
+    return DNX_OK;
+}
+
+}  // namespace dnx

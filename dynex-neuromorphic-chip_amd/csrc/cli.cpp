@@ -3,13 +3,14 @@
 //
 //   dynex -i file.cnf [-l seed] [-c alpha] [-b beta] [-n gamma] [-h delta]
 //         [-j epsilon] [-k zeta] [-m xl_max] [-z maxsteps] [-u walltime]
-//         [-q 0|1] [-P 64|32] [-G device]
+//         [-q 0|1] [-w threads] [-P 64|32] [-G device]
 //
-// Flags take "-x value" or "-x=value" (dynex.cc:2208-2233).  -P (precision)
-// and -G (device) are additions; the reference's tuning, portfolio,
-// integrator-mode and partable flags (-t -g -r -v -s -w -o -e -p -x -y -d
-// -a -f) are accepted and ignored: only the default adaptive Euler mode is on
-// this path.  The step cap -z defaults to 100000.
+// Flags take "-x value" or "-x=value" (dynex.cc:2208-2233).  -w runs the
+// reference's thread portfolio on the device (dnx_portfolio_solve).  -P
+// (precision) and -G (device) are additions; the reference's tuning,
+// integrator-mode and partable flags (-t -g -r -v -s -o -e -p -x -y -d -a -f)
+// are accepted and ignored: only the default adaptive Euler mode is on this
+// path.  The step cap -z defaults to 100000.
 #include <getopt.h>
 
 #include <chrono>
@@ -32,14 +33,14 @@ static void usage(const char *prog)
 {
     printf("c Syntax: %s -i file.cnf [-l seed] [-c alpha] [-b beta] [-n gamma] [-h delta]\n"
            "c            [-j epsilon] [-k zeta] [-m xl_max] [-z maxsteps] [-u walltime]\n"
-           "c            [-q 0|1] [-P 64|32] [-G device]\n",
+           "c            [-q 0|1] [-w threads] [-P 64|32] [-G device]\n",
            prog);
 }
 
 int main(int argc, char **argv)
 {
     std::string input;
-    int seed = 0, xlm = 10000, quiet = 0, prec_bits = 64, device = 0;
+    int seed = 0, xlm = 10000, quiet = 0, prec_bits = 64, device = 0, threads = 1;
     double alpha = 2.5, beta = 20.0, gamma = 25.0 / 100.0, delta = 5.0 / 100.0, eps = 1.0 / 10.0,
            zeta = 1.0 / 10.0, walltime = 0.0, maxsteps = 100000;
     int c;
@@ -60,7 +61,8 @@ int main(int argc, char **argv)
             case 'q': quiet = atoi(a); break;
             case 'P': prec_bits = atoi(a); break;
             case 'G': device = atoi(a); break;
-            case 'r': case 'v': case 'w': case 'p': case 'e': case 'f': case 'g':
+            case 'w': threads = atoi(a); break;
+            case 'r': case 'v': case 'p': case 'e': case 'f': case 'g':
             case 's': case 'a': case 'x': case 'y': case 'd': case 'o': case 't':
                 break;
             default: usage(argv[0]); return EXIT_FAILURE;
@@ -113,8 +115,15 @@ int main(int argc, char **argv)
     double t = 0.0;
     int status = DNX_UNKNOWN;
     auto t0 = std::chrono::steady_clock::now();
-    rc = dnx_solve(prob, prec, &prm, x0.data(), (int64_t)maxsteps, walltime, assign.data(), &steps, &t,
-                   &status);
+    int winner = 0;
+    if (threads > 1) {
+        rc = dnx_portfolio_solve(prob, prec, &prm, seed, threads, (int64_t)maxsteps, walltime, assign.data(), &steps,
+                                 &t, &status, &winner);
+        if (winner < 0) winner = 0;
+    } else {
+        rc = dnx_solve(prob, prec, &prm, x0.data(), (int64_t)maxsteps, walltime, assign.data(), &steps, &t,
+                       &status);
+    }
     double spent = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (rc != DNX_OK) {
         printf("c ERROR: %s\n", dnx_last_error());
@@ -126,8 +135,8 @@ int main(int argc, char **argv)
     if (status == DNX_SAT) {
         global = 0;
         if (!quiet) {
-            printf("c [0] T=%.2f SOLUTION FOUND (%lld)\n", t, (long long)steps);  // dynex.cc:666
-            printf("c [0] SAT (VERIFIED)\n");                                    // dynex.cc:690
+            printf("c [%d] T=%.2f SOLUTION FOUND (%lld)\n", winner, t, (long long)steps);  // dynex.cc:666
+            printf("c [%d] SAT (VERIFIED)\n", winner);                                    // dynex.cc:690
         }
         std::string sol = input + ".solution.txt";  // dynex.cc:2477-2480
         FILE *fs = fopen(sol.c_str(), "w");
@@ -136,9 +145,9 @@ int main(int argc, char **argv)
             fclose(fs);
         }
     }
-    if (!quiet) printf("c [0] TIME SPENT: %.5fs\n", spent);  // dynex.cc:1499
+    if (!quiet) printf("c [%d] TIME SPENT: %.5fs\n", winner, spent);  // dynex.cc:1499
     if (status == DNX_SAT) {
-        if (!quiet) printf("\ns [0] SATISFIABLE");  // dynex.cc:1509-1514
+        if (!quiet) printf("\ns [%d] SATISFIABLE", winner);  // dynex.cc:1509-1514
         for (int i = 0; i < n; i++) {
             if (i % 20 == 0)
                 if (!quiet) printf("\nv ");
@@ -146,7 +155,7 @@ int main(int argc, char **argv)
         }
         if (!quiet) printf("0\n");
     } else {
-        if (!quiet) printf("\ns [0] UNKNOWN\n");  // dynex.cc:1520
+        if (!quiet) printf("\ns [%d] UNKNOWN\n", winner);  // dynex.cc:1520
     }
     printf("%d", global);  // dynex.cc:2864
     fflush(stdout);

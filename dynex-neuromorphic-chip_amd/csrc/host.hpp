@@ -42,6 +42,8 @@ int write_dimacs(const char *path, int n, int m, const int32_t *cp,
 
 // Reference thread-0 initial state (dynex.cc:2643-2700).
 void init_reference(const HostProblem &p, int seed, double *x0);
+// Initial state of thread `tid` of a multi-thread run (same lines).
+void init_reference_thread(const HostProblem &p, int seed, int tid, double *x0);
 
 // splitmix64 stream: the portable generator for synthetic workloads.
 struct SplitMix64 {

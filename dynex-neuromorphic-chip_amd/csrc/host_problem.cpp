@@ -156,6 +156,39 @@ int write_dimacs(const char *path, int n, int m, const int32_t *cp,
     return fclose(f) == 0 ? DNX_OK : DNX_E_IO;
 }
 
+// Thread `tid` of a `-w` run (dynex.cc:2643-2700): threads 0, 1, 2 of seed 0
+// set every entry to 1, 0, -1; every other thread draws its V from one
+// mt19937(seed) stream shared in thread order, with Xs = 0 and Xl = 1.
+void init_reference_thread(const HostProblem &p, int seed, int tid, double *x)
+{
+    const size_t n = (size_t)p.n, m = (size_t)p.m, N = n + 2 * m;
+    std::mt19937 gen((std::mt19937::result_type)seed);
+    std::uniform_int_distribution<int> rand_v(-1, 1);
+    for (int i = 0; i <= tid; i++) {
+        if (i <= 2 && seed == 0) {
+            if (i == tid) {
+                const double c = i == 0 ? 1.0 : (i == 1 ? 0.0 : -1.0);
+                for (size_t j = 0; j < N; j++) x[j] = c;
+            }
+        } else {
+            for (size_t j = 0; j < n; j++) {
+                const double v = (double)rand_v(gen);
+                if (i == tid) x[j] = v;
+            }
+            if (i == tid) {
+                for (size_t j = n; j < n + m; j++) x[j] = 0.0;
+                for (size_t j = n + m; j < N; j++) x[j] = 1.0;
+            }
+        }
+    }
+    if (seed == -1)
+        for (size_t j = 0; j < n; j++) x[j] = 0.0;
+    if (seed == -2)
+        for (size_t j = 0; j < n; j++) x[j] = -1.0;
+    for (size_t j = 0; j < n; j++)
+        if (p.unit[j] != 0) x[j] = (double)p.unit[j];
+}
+
 void init_reference(const HostProblem &p, int seed, double *x)
 {
     const size_t n = (size_t)p.n, m = (size_t)p.m, N = n + 2 * m;

@@ -68,6 +68,9 @@ def lib():
     L.dnx_solve.argtypes = [vp, ip, C.POINTER(Params), F64P, C.c_int64, dp, vp, C.POINTER(C.c_int64),
                             C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.dnx_init_reference.argtypes = [vp, ip, F64P]
+    L.dnx_init_reference_thread.argtypes = [vp, ip, ip, F64P]
+    L.dnx_portfolio_solve.argtypes = [vp, ip, C.POINTER(Params), ip, ip, C.c_int64, dp, vp, C.POINTER(C.c_int64),
+                                      C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.dnx_generate_planted.argtypes = [C.c_uint64, ip, ip, dp, dp, I32P, vp]
     L.dnx_init_uniform.argtypes = [C.c_uint64, ip, ip, dp, dp, F64P]
     L.dnx_write_dimacs.argtypes = [C.c_char_p, ip, ip, I32P, I32P, C.c_char_p]
@@ -86,7 +89,7 @@ def lib():
     L.dnx_batch_retire.argtypes = [vp, ip]
     L.dnx_batch_info.argtypes = [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.dnx_batch_memory.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
-    L.dnx_batch_timing.argtypes = [vp, ip, vp, vp]
+    L.dnx_batch_timing.argtypes = [vp, ip, vp, vp, vp]
     L.dnx_batch_set_wide.argtypes = [vp, C.c_int64, ip]
     L.dnx_batch_set_serve_big.argtypes = [vp, C.c_int64, C.c_int64]
     L.dnx_batch_tune.argtypes = [vp, ip, ip, dp, dp, ip, ip, C.c_uint64, vp, ip, C.POINTER(C.c_int)]
@@ -240,6 +243,19 @@ class Problem:
         check(lib().dnx_init_reference(self.h, seed, x))
         return x
 
+    def init_reference_thread(self, seed, tid):
+        x = np.zeros(self.n + 2 * self.m, np.float64)
+        check(lib().dnx_init_reference_thread(self.h, seed, tid, x))
+        return x
+
+    def portfolio_solve(self, prm, seed=0, threads=4, prec=FP64, cap=100000, walltime=0.0):
+        a = np.zeros(max(self.n, 1), np.int8)
+        steps, t, st, w = C.c_int64(), C.c_double(), C.c_int(), C.c_int()
+        check(lib().dnx_portfolio_solve(self.h, prec, C.byref(prm), seed, threads, cap, walltime,
+                                        a.ctypes.data_as(C.c_void_p), C.byref(steps), C.byref(t), C.byref(st),
+                                        C.byref(w)))
+        return dict(status=st.value, steps=steps.value, t=t.value, assign=a[:self.n], winner=w.value)
+
     def check_assignment(self, assign):
         u = C.c_int()
         check(lib().dnx_check_assignment(self.h, np.ascontiguousarray(assign, np.int8), C.byref(u)))
@@ -375,18 +391,21 @@ class Batch:
         check(lib().dnx_batch_get_shape(self.h, int(mode), C.byref(bs), C.byref(cap), C.byref(wide)))
         return bs.value, cap.value, wide.value
 
-    def set_serve_big(self, clauses, active_clauses=2000000):
+    def set_serve_big(self, clauses, active_clauses=6000000):
         check(lib().dnx_batch_set_serve_big(self.h, int(clauses), int(active_clauses)))
 
     def set_wide(self, max_clauses, steps_per_sync=30):
         check(lib().dnx_batch_set_wide(self.h, int(max_clauses), int(steps_per_sync)))
 
-    def timing(self, enable=-1):
-        """Summed device ms and launch counts per fused-kernel mode (0-2) and wide mode (3)."""
+    def timing(self, enable=-1, with_bytes=False):
+        """Summed device ms and launch counts per fused-kernel mode (0-2) and wide mode (3); with_bytes adds
+        the compulsory bytes of the steps those launches took."""
         ms = np.zeros(4, np.float64)
         nl = np.zeros(4, np.int64)
-        check(lib().dnx_batch_timing(self.h, int(enable), ms.ctypes.data_as(C.c_void_p), nl.ctypes.data_as(C.c_void_p)))
-        return ms, nl
+        ab = np.zeros(4, np.float64)
+        check(lib().dnx_batch_timing(self.h, int(enable), ms.ctypes.data_as(C.c_void_p), nl.ctypes.data_as(C.c_void_p),
+                                     ab.ctypes.data_as(C.c_void_p)))
+        return (ms, nl, ab) if with_bytes else (ms, nl)
 
     def memory(self):
         a, f = C.c_int64(), C.c_int64()

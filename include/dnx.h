@@ -148,6 +148,17 @@ int dnx_solve(dnx_problem *p, int prec, const dnx_params *prm, const double *x0,
               int64_t step_cap, double walltime_s, int8_t *assign_out,
               int64_t *steps_out, double *t_out, int *status);
 
+/* The reference's thread portfolio (-w, dynex.cc:2731-2801): `threads`
+ * copies with the reference's per-thread initial states
+ * (dnx_init_reference_thread) and alpha/i for thread i > 0 (dynex.cc:2657)
+ * advance in rounds of equal step counts.  The first round in which a copy
+ * verifies ends the run; the winner is the verified copy with the fewest
+ * steps (lowest index on a tie) -- deterministic, where the reference takes
+ * whichever thread finishes first in wall-clock time.  *winner = its index. */
+int dnx_portfolio_solve(dnx_problem *p, int prec, const dnx_params *prm, int seed, int threads,
+                        int64_t step_cap, double walltime_s, int8_t *assign_out,
+                        int64_t *steps_out, double *t_out, int *status, int *winner);
+
 /* ---- batched engine -------------------------------------------------- */
 /* Many independent instances advanced together on one device (no
  * counterpart in the reference, which runs one instance per thread; its
@@ -213,7 +224,7 @@ int dnx_batch_set_wide(dnx_batch *b, int64_t max_clauses, int steps_per_sync);
 /* Serving: requests with at least `clauses` clauses run in the wide lane
  * (default 60,000; needs wide mode enabled).  Each wide chunk advances the
  * members with the fewest steps so far, up to active_clauses clauses in
- * total (default 2,000,000): least-attained-service order, so new requests
+ * total (default 6,000,000): least-attained-service order, so new requests
  * start at once and, under overload, the longest-running ones wait. */
 int dnx_batch_set_serve_big(dnx_batch *b, int64_t clauses, int64_t active_clauses);
 
@@ -234,9 +245,12 @@ int dnx_batch_get_shape(dnx_batch *b, int mode, int *block, int *lds_cap, int64_
 
 /* Waits for queued work, then reports the summed device time (ms) and the
  * count of timed launches since the last reset, in four buckets: modes 0-2
- * of the fused kernel, then wide mode (arrays of 4).  enable = 1/0 turns
- * launch timing on/off and resets the sums (-1: report only). */
-int dnx_batch_timing(dnx_batch *b, int enable, double *ms, int64_t *launches);
+ * of the fused kernel, then wide mode (arrays of 4).  alg_bytes (may be
+ * NULL) receives the compulsory bytes of the steps those launches took
+ * inside dnx_batch_run: per instance step, 4 per literal + 16 per clause +
+ * 8 per variable.  enable = 1/0 turns launch timing on/off and resets the
+ * sums (-1: report only). */
+int dnx_batch_timing(dnx_batch *b, int enable, double *ms, int64_t *launches, double *alg_bytes);
 int dnx_batch_memory(dnx_batch *b, int64_t *arena_bytes, int64_t *free_bytes);
 
 /* ---- serving ---------------------------------------------------------- */
@@ -282,6 +296,12 @@ int dnx_device_copy(dnx_ctx *ctx, int64_t bytes, int repeats, double *ms);
  * from std::uniform_int_distribution<int>(-1,1) over std::mt19937(seed) with
  * Xs = 0 and Xl = 1; seed -1 / -2 set V to 0 / -1; unit clauses pin V. */
 int dnx_init_reference(const dnx_problem *p, int seed, double *x0);
+
+/* Initial state of thread `tid` of a multi-thread run (dynex.cc:2643-2700):
+ * with seed 0, threads 0/1/2 set every entry to 1/0/-1; every other thread
+ * draws V from one mt19937(seed) stream shared in thread order (Xs = 0,
+ * Xl = 1).  Thread 0 is dnx_init_reference. */
+int dnx_init_reference_thread(const dnx_problem *p, int seed, int tid, double *x0);
 
 /* Planted random k=3 instance (no counterpart in the reference, which reads
  * files only): sigma uniform +-1, three distinct variables per clause, the

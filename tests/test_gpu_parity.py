@@ -249,6 +249,47 @@ def test_step_cap_reports_unknown(ctx):
     gp.free()
 
 
+# ---------------------------------------------------------------------------
+# portfolio (-w, dynex.cc:2643-2700 initial states, 2657 alpha/i, 2731-2801)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("seed", [0, 3])
+def test_portfolio_winner_is_first_single_solve(ctx, cnf_dir, prec, seed):
+    """Every copy advances in lockstep, so the winner is the copy whose own
+    solve needs the fewest steps (lowest index on a tie), with that solve's
+    step count, time and assignment."""
+    gp = ctx.problem_from_dimacs(cnf_path(cnf_dir, 200))
+    pr = dnx.FP64 if prec == 64 else dnx.FP32
+    threads, cap = 5, 30000
+    single = []
+    for i in range(threads):
+        prm = dnx.params(gp.m)
+        if i > 0:
+            prm.alpha = prm.alpha / i
+        single.append(gp.solve(prm, gp.init_reference_thread(seed, i), pr, cap=cap))
+    r = gp.portfolio_solve(dnx.params(gp.m), seed=seed, threads=threads, prec=pr, cap=cap)
+    sat = [i for i in range(threads) if single[i]["status"] == dnx.SAT]
+    assert sat, [s["status"] for s in single]
+    best = min(sat, key=lambda i: (single[i]["steps"], i))
+    assert r["status"] == dnx.SAT and r["winner"] == best
+    assert r["steps"] == single[best]["steps"] and r["t"] == single[best]["t"]
+    assert np.array_equal(r["assign"], single[best]["assign"])
+    assert gp.check_assignment(r["assign"]) == 0
+    gp.free()
+
+
+def test_cli_portfolio(cnf_dir, tmp_path):
+    src = cnf_path(cnf_dir, 100)
+    path = str(tmp_path / os.path.basename(src))
+    with open(src) as f, open(path, "w") as g:
+        g.write(f.read())
+    exe = os.path.join(dnx.HERE, "lib", "dynex")
+    r = subprocess.run([exe, "-i", path, "-w", "4"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "SATISFIABLE" in r.stdout and "SAT (VERIFIED)" in r.stdout
+    assert r.stdout.rstrip().endswith("0")
+
+
 def test_cli_solves_fixture(cnf_dir, tmp_path):
     src = cnf_path(cnf_dir, 200)
     path = str(tmp_path / os.path.basename(src))

@@ -100,6 +100,36 @@ def test_init_reference_matches_oracle(seed):
     assert np.array_equal(p.init_reference(seed), O.init_ref(q, seed))
 
 
+@pytest.mark.parametrize("seed", [0, 5, 1234, -1, -2])
+def test_init_reference_thread_blocks(seed):
+    """dynex.cc:2643-2700: one mt19937(seed) stream serves all threads in
+    thread order; with seed 0 threads 0-2 are constant (1, 0, -1 over every
+    entry) and draw nothing; every other thread takes the next n draws of
+    uniform_int(-1, 1) (pinned to libstdc++ in test_oracle) as V, Xs = 0,
+    Xl = 1; seeds -1/-2 override V; unit clauses pin V last."""
+    n, m = 6, 3
+    cp = np.array([0, 3, 6, 7], np.int32)
+    lits = np.array([1, 2, 3, -4, 5, 6, -2], np.int32)  # var 2 pinned to -1
+    p = host_problem(n, cp, lits)
+    threads = 9
+    draws = O.uniform_int_stream(seed, -1, 1, n * threads).astype(float)
+    block = 0
+    for tid in range(threads):
+        x = p.init_reference_thread(seed, tid)
+        if seed == 0 and tid <= 2:
+            want = np.full(n + 2 * m, (1.0, 0.0, -1.0)[tid])
+        else:
+            want = np.concatenate([draws[block * n:(block + 1) * n], np.zeros(m), np.ones(m)])
+            block += 1
+        if seed == -1:
+            want[:n] = 0.0
+        if seed == -2:
+            want[:n] = -1.0
+        want[1] = -1.0
+        assert np.array_equal(x, want), tid
+    assert np.array_equal(p.init_reference_thread(seed, 0), p.init_reference(seed))
+
+
 def test_device_entry_points_reject_host_only_problem():
     p = host_problem(3, [0, 3], [1, 2, 3])
     x = np.zeros(5)
