@@ -423,6 +423,248 @@ void launch_wide_step(const BInst *insts, Ctl *wctl, const int *bmap, int nbc, c
     if (nbv > 0) k_wide_var<<<nbv, WBS, 0, st>>>(insts, wctl, (const int2 *)vmap, nbv, r);
 }
 
+// ---------------------------------------------------------------------------
+// Gang mode.  For the tail of a batch -- a few instances, each needing
+// thousands of steps -- a per-step kernel pair pays two launch gaps and a
+// chain of dependent device-memory round trips per step.  Here each running
+// instance is advanced by a gang of G workgroups (G <= GANG_MAX) for K steps
+// inside one cooperative launch (every workgroup resident at once).  A
+// workgroup owns a contiguous range of clauses, whose literals and memories
+// stay in LDS for the whole launch, and a range of variables.  A step is:
+// clause pass (contributions to e[]), gang barrier carrying the OR of the
+// "unsolved" verdicts, variable pass (fold, (V, dV) pairs, ladder level),
+// gang barrier carrying the maximum level.  The arithmetic is the
+// single-instance FP32 arithmetic with the same clause-order fold.
+//
+// Barrier: workgroup `rank` of a gang publishes (epoch << 16 | value) in its
+// own 128-byte line of slot (epoch & 1) after an agent-scope release fence;
+// one wave polls the gang's G lines (one lane each) until every epoch has
+// arrived, then takes the OR / max of the values and fences (acquire).  Two
+// slots suffice: no workgroup can publish epoch e+2 before every workgroup
+// has published e+1, which is after it finished reading epoch e.  Epochs
+// grow across launches (the host passes each launch a fresh base), so the
+// lines never need clearing.  Every wait is bounded: a wait that exceeds
+// GANG_SPIN polls sets *err and the launch ends on every workgroup.
+// ---------------------------------------------------------------------------
+constexpr int GBS = 256;
+constexpr int GANG_SPIN = 1 << 21;
+
+__device__ __forceinline__ unsigned long long *gang_line(unsigned long long *words, int gang, int s, int rank)
+{
+    return words + (((size_t)gang * 2 + s) * GANG_MAX + rank) * GANG_LINE;
+}
+
+// All threads of the workgroup call this.  Returns false when the wait
+// timed out (or another workgroup's did); `res` is the gang-wide OR (op 0)
+// or max (op 1) of `value`.
+__device__ __forceinline__ bool gang_barrier(unsigned long long *words, int gang, int G, int rank,
+                                             unsigned long long epoch, int value, int op, int &res, int *err)
+{
+    __shared__ int s_res, s_ok;
+    __syncthreads();
+    const int s = (int)(epoch & 1);
+    if (threadIdx.x == 0) {
+        __threadfence();
+        __hip_atomic_store(gang_line(words, gang, s, rank), (epoch << 16) | (unsigned)(value & 0xffff),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        bool ok = false;
+        int r = 0;
+        for (int it = 0; it < GANG_SPIN; it++) {
+            const unsigned long long w =
+                lane < G ? __hip_atomic_load(gang_line(words, gang, s, lane), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                         : (epoch << 16);
+            if (__all((w >> 16) >= epoch)) {
+                const int v = lane < G ? (int)(w & 0xffff) : 0;
+                r = op ? wave_max(v) : (__ballot(v != 0) != 0ull ? 1 : 0);
+                ok = true;
+                break;
+            }
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (lane == 0) {
+            s_ok = ok;
+            s_res = r;
+            if (!ok) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __threadfence();
+    }
+    __syncthreads();
+    res = s_res;
+    return s_ok != 0;
+}
+
+__global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, BDyn *__restrict__ dyn,
+                                             const int4 *__restrict__ map, int nblk,
+                                             unsigned long long *__restrict__ words, unsigned long long base, int K,
+                                             int *__restrict__ err)
+{
+    __shared__ int32_t s_lits[3 * GANG_CPW];
+    __shared__ float s_xs[GANG_CPW], s_xl[GANG_CPW];
+    __shared__ float stage[GBS / 64][WCAPW];
+    __shared__ int sh[GBS / 64];
+    const int b = blockIdx.x;
+    if (b >= nblk) return;
+    const int4 gm = map[b];
+    const int slot = __builtin_amdgcn_readfirstlane(gm.x), rank = __builtin_amdgcn_readfirstlane(gm.y);
+    const int G = __builtin_amdgcn_readfirstlane(gm.z), gang = __builtin_amdgcn_readfirstlane(gm.w);
+    if (slot < 0) return;
+    const BDyn d = dyn[slot];
+    if (d.status != DNX_RUNNING) return;
+    const BInst I = insts[slot];
+    const int n = I.n, m = I.m, tid = threadIdx.x;
+    const bool general = (I.flags & BF_GENERAL) != 0;
+    PT<float> p;
+    p.alpha = I.alpha;
+    p.beta = I.beta;
+    p.gamma = I.gamma;
+    p.delta = I.delta;
+    p.eps = I.eps;
+    p.zeta = I.zeta;
+    p.xlmax = I.xlmax;
+    // this workgroup's clauses [c0, c0 + cn) and variables [vb, vb + Vb)
+    const int Cb = (m + G - 1) / G;
+    const int c0 = min(m, rank * Cb);
+    const int cn = min(m, c0 + Cb) - c0;
+    const int Vb = ((n + G - 1) / G + GBS - 1) / GBS * GBS;
+    const int vb = rank * Vb;
+    if (!general)
+        for (int i = tid; i < 3 * cn; i += GBS) s_lits[i] = I.lits[3 * c0 + i];
+    for (int i = tid; i < cn; i += GBS) {
+        s_xs[i] = I.xs[c0 + i];
+        s_xl[i] = I.xl[c0 + i];
+    }
+    __syncthreads();
+    bool pend = d.plevel >= 0;
+    int plev = d.plevel;
+    long long steps = d.steps;
+    double t = d.t;
+    int done = 0;
+    unsigned long long epoch = base;
+    for (int k = 0; k < K; k++) {
+        const float h = pend ? level_h<float>(plev) : 0.0f;
+        if (pend) {
+            steps += 1;
+            t += level_dt(plev);
+        }
+        // ---- clause pass ----------------------------------------------------
+        int unsolved = 0;
+        for (int i = tid; i < cn; i += GBS) {
+            const int c = c0 + i;
+            float xs = s_xs[i], xl = s_xl[i];
+            bool sat;
+            float C;
+            if (!general) {
+                const int32_t l0 = s_lits[3 * i], l1 = s_lits[3 * i + 1], l2 = s_lits[3 * i + 2];
+                const BF2 a0 = I.vd[abs(l0) - 1], a1 = I.vd[abs(l1) - 1], a2 = I.vd[abs(l2) - 1];
+                float r0 = a0.v, r1 = a1.v, r2 = a2.v;
+                if (pend) {
+                    memory_update(p, clause3_C_vals<float>(l0, l1, l2, r0, r1, r2), h, xs, xl);
+                    s_xs[i] = xs;
+                    s_xl[i] = xl;
+                    r0 = clampV(a0.v + h * a0.d);
+                    r1 = clampV(a1.v + h * a1.d);
+                    r2 = clampV(a2.v + h * a2.d);
+                }
+                float *ep = I.e + 3 * c;
+                auto emit = [&](int j, float val) { ep[j] = val; };
+                C = clause3_vals<float>(p, l0, l1, l2, r0, r1, r2, clamp01(xs), clampXl(xl, p.xlmax), emit, sat);
+            } else {
+                const int bb = I.cp[c];
+                const int kk = I.cp[c + 1] - bb;
+                const int32_t *cl = I.lits + bb;
+                auto prevV = [&](int v) { return I.vd[v].v; };
+                auto curV = [&](int v) {
+                    BF2 a = I.vd[v];
+                    return pend ? clampV(a.v + h * a.d) : a.v;
+                };
+                if (pend) {
+                    memory_update(p, clauseK_C<float>(cl, kk, prevV), h, xs, xl);
+                    s_xs[i] = xs;
+                    s_xl[i] = xl;
+                }
+                float *ep = I.e + bb;
+                auto emit = [&](int j, float val) { ep[j] = val; };
+                C = clauseK<float>(p, cl, kk, clamp01(xs), clampXl(xl, p.xlmax), curV, emit, sat);
+            }
+            unsolved |= !(C < 0.5f) || !sat;
+        }
+        int any_unsolved = 0;
+        if (!gang_barrier(words, gang, G, rank, ++epoch, __syncthreads_or(unsolved), 0, any_unsolved, err)) return;
+        if (d.stop_on_solve && !any_unsolved)
+            done = DNX_SAT;
+        else if (steps >= d.cap)
+            done = DNX_UNKNOWN;
+        // ---- variable pass --------------------------------------------------
+        int fmax = 0;
+        const int w = tid >> 6;
+        for (int v0 = vb + w * 64; v0 < vb + Vb; v0 += GBS) {
+            const float dVf = wave_fold<float, false, WCAPW>(I.e, I.eidx, I.row_ptr, n, v0, !done, stage[w], nullptr);
+            const int v = v0 + (tid & 63);
+            if (v < n) {
+                BF2 a = I.vd[v];
+                const float V = pend ? clampV(a.v + h * a.d) : a.v;
+                float dV = 0.0f;
+                if (!done) {
+                    dV = dVf;
+                    if ((I.flags & BF_UNIT) && I.unit[v] != 0) dV = 0.0f;  // dynex.cc:306
+                    fmax = max(fmax, fail_level(V, dV));
+                }
+                BF2 o;
+                o.v = V;
+                o.d = dV;
+                I.vd[v] = o;
+            }
+        }
+        if (done) break;
+        const int bmx = block_max_int<GBS>(fmax, sh);
+        if (tid == 0) sh[0] = bmx;
+        __syncthreads();
+        const int wg_level = sh[0];
+        if (!gang_barrier(words, gang, G, rank, ++epoch, wg_level, 1, plev, err)) return;
+        pend = true;
+    }
+    __syncthreads();
+    for (int i = tid; i < cn; i += GBS) {
+        I.xs[c0 + i] = s_xs[i];
+        I.xl[c0 + i] = s_xl[i];
+    }
+    if (rank == 0 && tid == 0) {
+        BDyn o = d;
+        o.steps = steps;
+        o.t = t;
+        if (done) {
+            o.status = done;
+            o.plevel = -1;
+            o.t_done = wall_clock64();
+        } else {
+            o.plevel = pend ? plev : -1;
+        }
+        dyn[slot] = o;
+    }
+}
+
+int gang_occupancy()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gang, GBS, 0) != hipSuccess) return 0;
+    return nb;
+}
+
+int launch_gang(const BInst *insts, BDyn *dyn, const int *map, int nblk, unsigned long long *words,
+                unsigned long long base, int K, int *err, void *stream)
+{
+    if (nblk <= 0) return 0;
+    const int4 *mp = (const int4 *)map;
+    void *args[] = {(void *)&insts, (void *)&dyn, (void *)&mp, (void *)&nblk, (void *)&words, (void *)&base,
+                    (void *)&K, (void *)&err};
+    return (int)hipLaunchCooperativeKernel((const void *)k_gang, dim3(nblk), dim3(GBS), args, 0,
+                                           (hipStream_t)stream);
+}
+
 __global__ void k_batch_zero(int *p, int count)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;

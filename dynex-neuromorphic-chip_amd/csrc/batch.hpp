@@ -69,4 +69,14 @@ void launch_wide_step(const BInst *insts, Ctl *wctl, const int *bmap, int nbc, c
 // resident workgroups per CU for a shape (0 if it cannot launch)
 int batch_occupancy(int mode, int bs, int lds_bytes);
 
+// gang mode (see batch.hip): map holds (slot, rank, G, gang) per workgroup;
+// words holds 2 * GANG_MAX lines of GANG_LINE u64 per gang
+constexpr int GANG_MAX = 32;     // workgroups per gang
+constexpr int GANG_CPW = 2048;   // clauses per workgroup at most (LDS)
+constexpr int GANG_LINE = 16;    // u64 per barrier line (128 bytes)
+int gang_occupancy();
+// cooperative launch of K steps; returns the hipError_t of the launch
+int launch_gang(const BInst *insts, BDyn *dyn, const int *map, int nblk, unsigned long long *words,
+                unsigned long long base, int K, int *err, void *stream);
+
 }  // namespace dnx

@@ -95,6 +95,9 @@ struct SlotHost {
 // pair read and written (8 per variable).
 static double step_bytes(const SlotHost &sl) { return 4.0 * (double)sl.nnz + 16.0 * sl.m + 8.0 * sl.n; }
 
+// timing buckets: the three fused-kernel modes, wide mode, gang mode
+constexpr int BK_WIDE = BM_COUNT, BK_GANG = BM_COUNT + 1, NBUCKET = BM_COUNT + 2;
+
 struct Shape {
     int bs;
     int lds_cap;  // bytes of LDS the shape may use per workgroup
@@ -130,14 +133,14 @@ struct dnx_batch {
     // optional per-shape launch timing (start/stop events around each launch)
     bool timing = false;
     // timing buckets: the three fused-kernel modes, then wide mode
-    hipEvent_t t0[BM_COUNT + 1] = {nullptr, nullptr, nullptr, nullptr};
-    hipEvent_t t1[BM_COUNT + 1] = {nullptr, nullptr, nullptr, nullptr};
-    bool t_pending[BM_COUNT + 1] = {false, false, false, false};
-    double t_ms[BM_COUNT + 1] = {0, 0, 0, 0};
-    long long t_launches[BM_COUNT + 1] = {0, 0, 0, 0};
+    hipEvent_t t0[NBUCKET] = {};
+    hipEvent_t t1[NBUCKET] = {};
+    bool t_pending[NBUCKET] = {};
+    double t_ms[NBUCKET] = {};
+    long long t_launches[NBUCKET] = {};
     // compulsory bytes of the steps taken by the timed launches of
     // dnx_batch_run, per bucket (step_bytes per instance step)
-    double t_alg[BM_COUNT + 1] = {0, 0, 0, 0};
+    double t_alg[NBUCKET] = {};
     // wide mode: entered when the running instances hold at most
     // wide_clauses clauses in total; wide_chunk steps between status syncs
     long long wide_clauses = 1500000;
@@ -150,6 +153,14 @@ struct dnx_batch {
     // any slot is satisfied
     bool group_stop = false;
     DevBuf b_wctl, b_wlist, b_bmap, b_vmap;
+    // gang mode: taken for the tail when the running instances need at
+    // most gang_blocks workgroups in total (0 disables it)
+    int gang_blocks = 128;
+    int gang_cpw = 1024;      // target clauses per gang workgroup
+    int gang_steps = 128;     // steps per cooperative launch
+    int gang_cap = 0;         // co-resident gang workgroups (0: not known yet)
+    unsigned long long gang_epoch = 0;
+    DevBuf b_gmap, b_gwords, b_gerr;
     ~dnx_batch()
     {
         if (arena_base) (void)hipFree(arena_base);
@@ -162,7 +173,7 @@ struct dnx_batch {
             if (mstream[i]) (void)hipStreamDestroy(mstream[i]);
             if (join[i]) (void)hipEventDestroy(join[i]);
         }
-        for (int i = 0; i <= BM_COUNT; i++) {
+        for (int i = 0; i < NBUCKET; i++) {
             if (t0[i]) (void)hipEventDestroy(t0[i]);
             if (t1[i]) (void)hipEventDestroy(t1[i]);
         }
@@ -216,7 +227,7 @@ static Layout layout_of(int n, int m, int64_t nnz, bool general, bool has_unit, 
 // has waited for the work).
 static void batch_collect_timing(dnx_batch *b)
 {
-    for (int md = 0; md <= BM_COUNT; md++) {
+    for (int md = 0; md < NBUCKET; md++) {
         if (!b->t_pending[md]) continue;
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, b->t0[md], b->t1[md]) == hipSuccess) {
@@ -375,7 +386,7 @@ int dnx_batch_open(dnx_ctx *ctx, int prec, int slots, int64_t arena_bytes, dnx_b
             delete b;
             return hip_fail(e, "stream");
         }
-    for (int i = 0; i <= BM_COUNT; i++)
+    for (int i = 0; i < NBUCKET; i++)
         if ((e = hipEventCreate(&b->t0[i])) != hipSuccess || (e = hipEventCreate(&b->t1[i])) != hipSuccess) {
             delete b;
             return hip_fail(e, "event");
@@ -688,12 +699,12 @@ static int batch_wide_run(dnx_batch *b, double walltime_s, std::chrono::steady_c
     std::vector<long long> before(list.size());
     for (;;) {
         for (size_t i = 0; i < list.size(); i++) before[i] = b->h_dyn[list[i]].steps;
-        const bool timed = b->timing && !b->t_pending[BM_COUNT];
-        if (timed && (e = hipEventRecord(b->t0[BM_COUNT], st)) != hipSuccess) return hip_fail(e, "timing");
+        const bool timed = b->timing && !b->t_pending[BK_WIDE];
+        if (timed && (e = hipEventRecord(b->t0[BK_WIDE], st)) != hipSuccess) return hip_fail(e, "timing");
         if ((e = hipGraphLaunch(gexec, st)) != hipSuccess) return hip_fail(e, "wide graph launch");
         if (timed) {
-            if ((e = hipEventRecord(b->t1[BM_COUNT], st)) != hipSuccess) return hip_fail(e, "timing");
-            b->t_pending[BM_COUNT] = true;
+            if ((e = hipEventRecord(b->t1[BK_WIDE], st)) != hipSuccess) return hip_fail(e, "timing");
+            b->t_pending[BK_WIDE] = true;
         }
         launch_wide_sync(dyn, wctl, (const int *)b->b_wlist.ptr, count, (r + 2) % 3, st);
         if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "wide launch");
@@ -701,7 +712,7 @@ static int batch_wide_run(dnx_batch *b, double walltime_s, std::chrono::steady_c
         batch_collect_timing(b);
         if (b->timing)
             for (size_t i = 0; i < list.size(); i++)
-                b->t_alg[BM_COUNT] +=
+                b->t_alg[BK_WIDE] +=
                     (double)(b->h_dyn[list[i]].steps - before[i]) * step_bytes(b->slots[(size_t)list[i]]);
         int running = 0;
         long long live = 0;
@@ -714,6 +725,97 @@ static int batch_wide_run(dnx_batch *b, double walltime_s, std::chrono::steady_c
         // finished instances still cost their blocks; rebuild the maps once
         // most of the clause blocks belong to finished instances
         if (2 * (live / WIDE_BLOCK + running) < nbc) break;
+        if (walltime_s > 0 &&
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > walltime_s)
+            break;
+    }
+    return DNX_OK;
+}
+
+// Gang mode for the tail (see batch.hip).  *used stays false when the
+// running set does not suit it -- an instance too large for one gang, more
+// workgroups than gang_blocks or than can be resident at once -- and the
+// caller then runs wide mode.  Returns after the running set has lost half
+// its workgroups (to rebuild the map), when nothing runs, or at the
+// wall-clock limit; the status table in h_dyn (buffer 0) is then current.
+static int batch_gang_run(dnx_batch *b, double walltime_s, std::chrono::steady_clock::time_point t0, bool *used)
+{
+    *used = false;
+    if (b->gang_blocks <= 0) return DNX_OK;
+    if (!b->gang_cap) b->gang_cap = std::max(0, gang_occupancy()) * b->cus;
+    std::vector<int> list, map;
+    int gangs = 0;
+    for (int s = 0; s < b->nslots; s++) {
+        if (!b->slots[(size_t)s].used || b->h_dyn[s].status != DNX_RUNNING) continue;
+        const int m = b->slots[(size_t)s].m;
+        const int G = std::max(1, std::min(GANG_MAX, (m + b->gang_cpw - 1) / b->gang_cpw));
+        if ((m + G - 1) / G > GANG_CPW) return DNX_OK;
+        for (int r = 0; r < G; r++) {
+            map.push_back(s);
+            map.push_back(r);
+            map.push_back(G);
+            map.push_back(gangs);
+        }
+        gangs++;
+        list.push_back(s);
+        if ((int)map.size() / 4 > std::min(b->gang_blocks, b->gang_cap)) return DNX_OK;
+    }
+    if (list.empty()) return DNX_OK;
+    *used = true;
+    const int nblk = (int)map.size() / 4;
+    hipStream_t st = b->ctx->stream;
+    int rc;
+    hipError_t e;
+    const size_t wbytes = (size_t)gangs * 2 * GANG_MAX * GANG_LINE * sizeof(unsigned long long);
+    if (b->b_gmap.size < sizeof(int) * map.size() && (rc = b->b_gmap.alloc(sizeof(int) * map.size()))) return rc;
+    if (b->b_gwords.size < wbytes) {
+        // fresh lines are zero; epochs only grow, so lines never need clearing again
+        if ((rc = b->b_gwords.alloc(wbytes))) return rc;
+        if ((e = hipMemset(b->b_gwords.ptr, 0, wbytes)) != hipSuccess) return hip_fail(e, "gang words");
+    }
+    if (!b->b_gerr.ptr) {
+        if ((rc = b->b_gerr.alloc(sizeof(int)))) return rc;
+        if ((e = hipMemset(b->b_gerr.ptr, 0, sizeof(int))) != hipSuccess) return hip_fail(e, "gang flag");
+    }
+    if ((e = hipMemcpy(b->b_gmap.ptr, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "gang map upload");
+    const int K = std::max(1, b->gang_steps);
+    std::vector<long long> before(list.size());
+    for (;;) {
+        for (size_t i = 0; i < list.size(); i++) before[i] = b->h_dyn[list[i]].steps;
+        const bool timed = b->timing && !b->t_pending[BK_GANG];
+        if (timed && (e = hipEventRecord(b->t0[BK_GANG], st)) != hipSuccess) return hip_fail(e, "timing");
+        const int le = launch_gang((const BInst *)b->b_inst.ptr, (BDyn *)b->b_dyn.ptr, (const int *)b->b_gmap.ptr, nblk,
+                                   (unsigned long long *)b->b_gwords.ptr, b->gang_epoch, K, (int *)b->b_gerr.ptr, st);
+        b->gang_epoch += 2 * (unsigned long long)K + 2;
+        if (le != 0) return hip_fail((hipError_t)le, "gang launch");
+        if (timed) {
+            if ((e = hipEventRecord(b->t1[BK_GANG], st)) != hipSuccess) return hip_fail(e, "timing");
+            b->t_pending[BK_GANG] = true;
+        }
+        if ((rc = batch_snapshot(b, 0)) || (rc = batch_wait(b, 0))) return rc;
+        batch_collect_timing(b);
+        int err = 0;
+        if ((e = hipMemcpy(&err, b->b_gerr.ptr, sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess)
+            return hip_fail(e, "gang flag read");
+        if (err) {
+            set_error("gang mode: a workgroup barrier wait timed out; the gang's instances are in an undefined state");
+            return DNX_E_STATE;
+        }
+        if (b->timing)
+            for (size_t i = 0; i < list.size(); i++)
+                b->t_alg[BK_GANG] +=
+                    (double)(b->h_dyn[list[i]].steps - before[i]) * step_bytes(b->slots[(size_t)list[i]]);
+        int live = 0;
+        bool any = false;
+        for (size_t i = 0; i < list.size(); i++)
+            if (b->h_dyn[list[i]].status == DNX_RUNNING) {
+                any = true;
+                const int m = b->slots[(size_t)list[i]].m;
+                live += std::max(1, std::min(GANG_MAX, (m + b->gang_cpw - 1) / b->gang_cpw));
+            }
+        if (!any || group_done(b)) break;
+        if (2 * live < nblk) break;
         if (walltime_s > 0 &&
             std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > walltime_s)
             break;
@@ -743,7 +845,9 @@ int dnx_batch_run(dnx_batch *b, int steps_per_launch, double walltime_s, int *re
         for (int s = 0; s < b->nslots; s++)
             if (b->slots[(size_t)s].used && known[s].status == DNX_RUNNING) clauses += b->slots[(size_t)s].m;
         if (clauses <= b->wide_clauses) {
-            if ((rc = batch_wide_run(b, walltime_s, t0))) return rc;
+            bool gang = false;
+            if ((rc = batch_gang_run(b, walltime_s, t0, &gang))) return rc;
+            if (!gang && (rc = batch_wide_run(b, walltime_s, t0))) return rc;
             continue;
         }
         std::vector<std::pair<int, long long>> before;
@@ -878,6 +982,17 @@ int dnx_batch_set_wide(dnx_batch *b, int64_t max_clauses, int steps_per_sync)
     return DNX_OK;
 }
 
+int dnx_batch_set_gang(dnx_batch *b, int max_blocks, int clauses_per_workgroup, int steps_per_launch)
+{
+    if (!b || max_blocks < 0 || clauses_per_workgroup <= 0 || clauses_per_workgroup > GANG_CPW ||
+        steps_per_launch <= 0)
+        return DNX_E_ARG;
+    b->gang_blocks = max_blocks;
+    b->gang_cpw = clauses_per_workgroup;
+    b->gang_steps = steps_per_launch;
+    return DNX_OK;
+}
+
 int dnx_batch_set_serve_big(dnx_batch *b, int64_t clauses, int64_t active_clauses)
 {
     if (!b || clauses < 0 || active_clauses <= 0) return DNX_E_ARG;
@@ -893,14 +1008,14 @@ int dnx_batch_timing(dnx_batch *b, int enable, double *ms, int64_t *launches, do
     int rc = batch_sync(b);
     if (rc) return rc;
     batch_collect_timing(b);
-    for (int md = 0; md <= BM_COUNT; md++) {
+    for (int md = 0; md < NBUCKET; md++) {
         if (ms) ms[md] = b->t_ms[md];
         if (launches) launches[md] = b->t_launches[md];
         if (alg_bytes) alg_bytes[md] = b->t_alg[md];
     }
     if (enable >= 0) {
         b->timing = enable != 0;
-        for (int md = 0; md <= BM_COUNT; md++) {
+        for (int md = 0; md < NBUCKET; md++) {
             b->t_ms[md] = 0.0;
             b->t_launches[md] = 0;
             b->t_alg[md] = 0.0;
@@ -1404,14 +1519,14 @@ static int tune_run(dnx_ctx *ctx, const std::vector<TuneInst> &inst, const std::
         if ((rc = dnx_batch_admit_host(t, inst[(size_t)i].hp, &prm, inst[(size_t)i].x0.data(), steps, 0, &slot)))
             return rc;
     }
-    double ms[BM_COUNT + 1];
-    int64_t nl[BM_COUNT + 1];
+    double ms[NBUCKET];
+    int64_t nl[NBUCKET];
     if ((rc = dnx_batch_timing(t, 1, nullptr, nullptr, nullptr))) return rc;
     // one launch runs the fixed steps (steps is a multiple of the wide chunk)
     if ((rc = dnx_batch_run(t, steps, 0.0, nullptr))) return rc;
     if ((rc = dnx_batch_timing(t, 0, ms, nl, nullptr))) return rc;
     double tot = 0.0;
-    for (int md = 0; md <= BM_COUNT; md++) tot += ms[md];
+    for (int md = 0; md < NBUCKET; md++) tot += ms[md];
     *ms_out = tot;
     return DNX_OK;
 }

@@ -92,6 +92,7 @@ def lib():
     L.dnx_batch_timing.argtypes = [vp, ip, vp, vp, vp]
     L.dnx_batch_set_wide.argtypes = [vp, C.c_int64, ip]
     L.dnx_batch_set_serve_big.argtypes = [vp, C.c_int64, C.c_int64]
+    L.dnx_batch_set_gang.argtypes = [vp, ip, ip, ip]
     L.dnx_batch_tune.argtypes = [vp, ip, ip, dp, dp, ip, ip, C.c_uint64, vp, ip, C.POINTER(C.c_int)]
     L.dnx_batch_get_shape.argtypes = [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)]
     L.dnx_state_time.argtypes = [vp, ip, C.POINTER(C.c_double)]
@@ -391,6 +392,9 @@ class Batch:
         check(lib().dnx_batch_get_shape(self.h, int(mode), C.byref(bs), C.byref(cap), C.byref(wide)))
         return bs.value, cap.value, wide.value
 
+    def set_gang(self, max_blocks=128, clauses_per_workgroup=1024, steps_per_launch=128):
+        check(lib().dnx_batch_set_gang(self.h, int(max_blocks), int(clauses_per_workgroup), int(steps_per_launch)))
+
     def set_serve_big(self, clauses, active_clauses=6000000):
         check(lib().dnx_batch_set_serve_big(self.h, int(clauses), int(active_clauses)))
 
@@ -398,11 +402,11 @@ class Batch:
         check(lib().dnx_batch_set_wide(self.h, int(max_clauses), int(steps_per_sync)))
 
     def timing(self, enable=-1, with_bytes=False):
-        """Summed device ms and launch counts per fused-kernel mode (0-2) and wide mode (3); with_bytes adds
-        the compulsory bytes of the steps those launches took."""
-        ms = np.zeros(4, np.float64)
-        nl = np.zeros(4, np.int64)
-        ab = np.zeros(4, np.float64)
+        """Summed device ms and launch counts per fused-kernel mode (0-2), wide mode (3) and gang mode (4);
+        with_bytes adds the compulsory bytes of the steps those launches took."""
+        ms = np.zeros(5, np.float64)
+        nl = np.zeros(5, np.int64)
+        ab = np.zeros(5, np.float64)
         check(lib().dnx_batch_timing(self.h, int(enable), ms.ctypes.data_as(C.c_void_p), nl.ctypes.data_as(C.c_void_p),
                                      ab.ctypes.data_as(C.c_void_p)))
         return (ms, nl, ab) if with_bytes else (ms, nl)

@@ -243,9 +243,17 @@ int dnx_batch_tune(dnx_batch *b, int n_min, int n_max, double r_min, double r_ma
 /* The current shape of a mode and the wide-mode threshold. */
 int dnx_batch_get_shape(dnx_batch *b, int mode, int *block, int *lds_cap, int64_t *wide_clauses);
 
+/* Gang mode: the tail of a run (the instances left when wide mode would be
+ * taken) is advanced by gangs of up to 32 cooperating workgroups per
+ * instance, steps_per_launch steps per cooperative launch, about
+ * clauses_per_workgroup clauses (at most 2048) per workgroup, as long as the
+ * running instances need at most max_blocks workgroups in total (default
+ * 128; 0 disables it).  Results are bit-identical to the other modes. */
+int dnx_batch_set_gang(dnx_batch *b, int max_blocks, int clauses_per_workgroup, int steps_per_launch);
+
 /* Waits for queued work, then reports the summed device time (ms) and the
- * count of timed launches since the last reset, in four buckets: modes 0-2
- * of the fused kernel, then wide mode (arrays of 4).  alg_bytes (may be
+ * count of timed launches since the last reset, in five buckets: modes 0-2
+ * of the fused kernel, wide mode, gang mode (arrays of 5).  alg_bytes (may be
  * NULL) receives the compulsory bytes of the steps those launches took
  * inside dnx_batch_run: per instance step, 4 per literal + 16 per clause +
  * 8 per variable.  enable = 1/0 turns launch timing on/off and resets the

@@ -48,9 +48,15 @@ def oracle_solve(n, cp, lits, x0, cap):
     return steps, t, status, x
 
 
-def run_batch(ctx, inst, cap, spl, shapes=None, stop=True, wide=0):
+def run_batch(ctx, inst, cap, spl, shapes=None, stop=True, wide=0, gang=None):
+    """gang = (max_blocks, clauses_per_workgroup, steps_per_launch) enables
+    gang mode for the tail (the instances left once wide mode applies)."""
     b = dnx.Batch(ctx, slots=len(inst) + 2, arena_bytes=1 << 30)
-    b.set_wide(wide, 30 if wide else 30)
+    b.set_wide(wide, 30)
+    if gang:
+        b.set_gang(*gang)
+    else:
+        b.set_gang(0)
     if shapes:
         for md, (bs, lds) in shapes.items():
             b.set_shape(md, bs, lds)
@@ -118,6 +124,36 @@ def test_wide_mode_gives_identical_bits(ctx, inst_set, oracle_results, wide):
         assert np.array_equal(r["x"], xo)
         if status == 1:
             assert r["unsat"] == 0
+
+
+@pytest.mark.parametrize("wide,gang", [
+    (1 << 40, (512, 256, 37)),     # gangs of up to 32 workgroups, relaunch every 37 steps
+    (1 << 40, (128, 2048, 128)),   # few large workgroups
+    (60000, (128, 1024, 64)),      # fused kernel, then wide mode, then gangs
+])
+def test_gang_mode_gives_identical_bits(ctx, inst_set, oracle_results, wide, gang):
+    out = run_batch(ctx, inst_set, 1500, spl=40, wide=wide, gang=gang)
+    for r, (steps, t, status, xo) in zip(out, oracle_results):
+        assert r["status"] == (dnx.SAT if status == 1 else dnx.UNKNOWN)
+        assert r["steps"] == steps and r["t"] == t
+        assert np.array_equal(r["x"], xo)
+        if status == 1:
+            assert r["unsat"] == 0
+
+
+def test_gang_fixed_step_trajectory(ctx):
+    """Fixed steps (no stop on solve) with k=1, 2, 4 clauses and unit pins,
+    ending inside a launch and across launch boundaries."""
+    inst = make_set(14, 6, nmin=100, nmax=3000, with_general=True)
+    out = run_batch(ctx, inst, 45, spl=25, stop=False, wide=1 << 40, gang=(512, 300, 16))
+    for r, (n, cp, lits, x0) in zip(out, inst):
+        op = O.Problem(n, len(cp) - 1, cp, lits)
+        x = x0.astype(np.float32)
+        if op.has_unit:
+            x[:n] = np.where(op.unit != 0, op.unit, x[:n])
+        O.trajectory(op, O.params(op.m), x, 45, prec=32)
+        assert r["status"] == dnx.UNKNOWN and r["steps"] == 45
+        assert np.array_equal(r["x"], x)
 
 
 def test_wide_fixed_step_trajectory(ctx):
