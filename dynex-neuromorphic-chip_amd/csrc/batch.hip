@@ -54,7 +54,13 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
         p.xlmax = I.xlmax;
 
         BF2 *pv = (MODE == BM_GLOBAL) ? I.vd : (BF2 *)smem;
-        float *se = (float *)(smem + ((size_t)n * sizeof(BF2) + 15) / 16 * 16);
+        // mode 0 keeps the contributions of one chunk of consecutive clauses
+        // at a time in LDS (one chunk when they all fit); with several chunks
+        // each variable's partial fold waits in sacc between chunks
+        const int nch = MODE == BM_LDS_ALL ? I.nchunk : 1;
+        const size_t pvb = ((size_t)n * sizeof(BF2) + 15) / 16 * 16;
+        float *sacc = (float *)(smem + pvb);
+        float *se = (float *)(smem + pvb + (nch > 1 ? ((size_t)n * 4 + 15) / 16 * 16 : 0));
         if (MODE != BM_GLOBAL) {
             for (int v = tid; v < n; v += BS) pv[v] = I.vd[v];
             __syncthreads();
@@ -77,75 +83,104 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                 BF2 a = pv[v];
                 return pend ? clampV(a.v + h * a.d) : a.v;
             };
-            if (!general) {
-                // 3-SAT: the next clause's literals and memories are loaded
-                // while the current one is evaluated
-                int c = tid;
-                int32_t n0 = 0, n1 = 0, n2 = 0;
-                float nxs = 0.0f, nxl = 0.0f;
-                if (c < m) {
-                    n0 = I.lits[3 * c];
-                    n1 = I.lits[3 * c + 1];
-                    n2 = I.lits[3 * c + 2];
-                    nxs = I.xs[c];
-                    nxl = I.xl[c];
+            for (int ch = 0; ch < nch; ch++) {
+                const int c_lo = MODE == BM_LDS_ALL ? I.cbound[ch] : 0;
+                const int c_hi = MODE == BM_LDS_ALL ? I.cbound[ch + 1] : m;
+                if (!general) {
+                    // 3-SAT: each thread takes its clauses four at a time, with
+                    // all four clauses' literals, memories (and LDS slots) loaded
+                    // before any is evaluated, so enough loads are in flight to
+                    // cover device-memory latency; one (V, dV) pair read per
+                    // literal serves both the memory update and the clause
+                    constexpr int U = 4;
+                    for (int cb = c_lo + tid; cb < c_hi; cb += U * BS) {
+                        int32_t L[U][3], P[U][3];
+                        float XS[U], XL[U];
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                            const int c = cb + u * BS;
+                            if (c < c_hi) {
+                                L[u][0] = I.lits[3 * c];
+                                L[u][1] = I.lits[3 * c + 1];
+                                L[u][2] = I.lits[3 * c + 2];
+                                XS[u] = I.xs[c];
+                                XL[u] = I.xl[c];
+                                if (MODE == BM_LDS_ALL) {
+                                    P[u][0] = I.lpos[3 * c];
+                                    P[u][1] = I.lpos[3 * c + 1];
+                                    P[u][2] = I.lpos[3 * c + 2];
+                                }
+                            }
+                        }
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                            const int c = cb + u * BS;
+                            if (c >= c_hi) break;
+                            const int32_t l0 = L[u][0], l1 = L[u][1], l2 = L[u][2];
+                            float xs = XS[u], xl = XL[u];
+                            const BF2 a0 = pv[abs(l0) - 1], a1 = pv[abs(l1) - 1], a2 = pv[abs(l2) - 1];
+                            float r0 = a0.v, r1 = a1.v, r2 = a2.v;
+                            if (pend) {
+                                memory_update(p, clause3_C_vals<float>(l0, l1, l2, r0, r1, r2), h, xs, xl);
+                                I.xs[c] = xs;
+                                I.xl[c] = xl;
+                                r0 = clampV(a0.v + h * a0.d);
+                                r1 = clampV(a1.v + h * a1.d);
+                                r2 = clampV(a2.v + h * a2.d);
+                            }
+                            const float cxs = clamp01(xs), cxl = clampXl(xl, p.xlmax);
+                            bool sat;
+                            float C;
+                            if (MODE == BM_LDS_ALL) {
+                                auto emit = [&](int j, float val) { se[P[u][j]] = val; };
+                                C = clause3_vals<float>(p, l0, l1, l2, r0, r1, r2, cxs, cxl, emit, sat);
+                            } else {
+                                float *ep = I.e + 3 * c;
+                                auto emit = [&](int j, float val) { ep[j] = val; };
+                                C = clause3_vals<float>(p, l0, l1, l2, r0, r1, r2, cxs, cxl, emit, sat);
+                            }
+                            unsolved |= !(C < 0.5f) || !sat;
+                        }
+                    }
+                } else {
+                    for (int c = c_lo + tid; c < c_hi; c += BS) {
+                        const int b = I.cp[c];
+                        const int kk = I.cp[c + 1] - b;
+                        const int32_t *cl = I.lits + b;
+                        float xs = I.xs[c], xl = I.xl[c];
+                        if (pend) {
+                            float Cp = clauseK_C<float>(cl, kk, prevV);
+                            memory_update(p, Cp, h, xs, xl);
+                            I.xs[c] = xs;
+                            I.xl[c] = xl;
+                        }
+                        const float cxs = clamp01(xs), cxl = clampXl(xl, p.xlmax);
+                        bool sat;
+                        float C;
+                        if (MODE == BM_LDS_ALL) {
+                            const int32_t *lp = I.lpos + b;
+                            auto emit = [&](int j, float val) { se[lp[j]] = val; };
+                            C = clauseK<float>(p, cl, kk, cxs, cxl, curV, emit, sat);
+                        } else {
+                            float *ep = I.e + b;
+                            auto emit = [&](int j, float val) { ep[j] = val; };
+                            C = clauseK<float>(p, cl, kk, cxs, cxl, curV, emit, sat);
+                        }
+                        unsolved |= !(C < 0.5f) || !sat;
+                    }
                 }
-                for (; c < m; c += BS) {
-                    const int32_t l0 = n0, l1 = n1, l2 = n2;
-                    float xs = nxs, xl = nxl;
-                    const int cn = c + BS;
-                    if (cn < m) {
-                        n0 = I.lits[3 * cn];
-                        n1 = I.lits[3 * cn + 1];
-                        n2 = I.lits[3 * cn + 2];
-                        nxs = I.xs[cn];
-                        nxl = I.xl[cn];
+                if (ch + 1 < nch) {
+                    // fold this chunk's contributions (clause order within each
+                    // variable's slots), then free se for the next chunk
+                    __syncthreads();
+                    const int32_t *cbase = I.cbase + (size_t)ch * (n + 1);
+                    for (int v = tid; v < n; v += BS) {
+                        float s = ch ? sacc[v] : 0.0f;
+                        const int rb = cbase[v], re = cbase[v + 1];
+                        for (int q = rb; q < re; q++) s = s + se[q];
+                        sacc[v] = s;
                     }
-                    if (pend) {
-                        float Cp = clause3v_C<float>(l0, l1, l2, prevV);
-                        memory_update(p, Cp, h, xs, xl);
-                        I.xs[c] = xs;
-                        I.xl[c] = xl;
-                    }
-                    const float cxs = clamp01(xs), cxl = clampXl(xl, p.xlmax);
-                    bool sat;
-                    float C;
-                    if (MODE == BM_LDS_ALL) {
-                        const int32_t *lp = I.lpos + 3 * c;
-                        auto emit = [&](int j, float val) { se[lp[j]] = val; };
-                        C = clause3v<float>(p, l0, l1, l2, cxs, cxl, curV, emit, sat);
-                    } else {
-                        float *ep = I.e + 3 * c;
-                        auto emit = [&](int j, float val) { ep[j] = val; };
-                        C = clause3v<float>(p, l0, l1, l2, cxs, cxl, curV, emit, sat);
-                    }
-                    unsolved |= !(C < 0.5f) || !sat;
-                }
-            } else {
-                for (int c = tid; c < m; c += BS) {
-                    const int b = I.cp[c];
-                    const int kk = I.cp[c + 1] - b;
-                    const int32_t *cl = I.lits + b;
-                    float xs = I.xs[c], xl = I.xl[c];
-                    if (pend) {
-                        float Cp = clauseK_C<float>(cl, kk, prevV);
-                        memory_update(p, Cp, h, xs, xl);
-                        I.xs[c] = xs;
-                        I.xl[c] = xl;
-                    }
-                    const float cxs = clamp01(xs), cxl = clampXl(xl, p.xlmax);
-                    bool sat;
-                    float C;
-                    if (MODE == BM_LDS_ALL) {
-                        const int32_t *lp = I.lpos + b;
-                        auto emit = [&](int j, float val) { se[lp[j]] = val; };
-                        C = clauseK<float>(p, cl, kk, cxs, cxl, curV, emit, sat);
-                    } else {
-                        float *ep = I.e + b;
-                        auto emit = [&](int j, float val) { ep[j] = val; };
-                        C = clauseK<float>(p, cl, kk, cxs, cxl, curV, emit, sat);
-                    }
-                    unsolved |= !(C < 0.5f) || !sat;
+                    __syncthreads();
                 }
             }
             const bool any_unsolved = __syncthreads_or(unsolved);
@@ -160,10 +195,13 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                 const float V = pend ? clampV(a.v + h * a.d) : a.v;
                 float dV = 0.0f;
                 if (!done) {
-                    const int rb = I.row_ptr[v], re = I.row_ptr[v + 1];
                     if (MODE == BM_LDS_ALL) {
+                        const int32_t *cbase = I.cbase + (size_t)(nch - 1) * (n + 1);
+                        const int rb = cbase[v], re = cbase[v + 1];
+                        if (nch > 1) dV = sacc[v];
                         for (int q = rb; q < re; q++) dV = dV + se[q];
                     } else {
+                        const int rb = I.row_ptr[v], re = I.row_ptr[v + 1];
                         dV = fold_gather<float>(I.e, I.eidx, rb, re);
                     }
                     if ((I.flags & BF_UNIT) && I.unit[v] != 0) dV = 0.0f;  // dynex.cc:306

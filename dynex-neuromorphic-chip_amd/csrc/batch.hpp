@@ -25,13 +25,19 @@ struct BInst {
     int flags;                // BF_GENERAL: clause_ptr valid; BF_UNIT: pins
     const int32_t *lits;      // signed 1-based literals, clause order
     const int32_t *cp;        // m+1 offsets (general clause lengths only)
-    const int32_t *lpos;      // variable-major slot of each literal (LDS e)
+    const int32_t *lpos;      // mode 0: LDS slot of each literal within its chunk
     const int32_t *eidx;      // literal of each variable-major slot (global e)
     const int32_t *row_ptr;   // n+1
     const int8_t *unit;       // n pins
     float *xs, *xl;           // m each
     BF2 *vd;                  // n resident [V, dV] pairs
-    float *e;                 // nnz contributions (modes 1, 2)
+    float *e;                 // nnz contributions (modes 1, 2, wide, gang)
+    // mode 0: nchunk chunks of consecutive clauses [cbound[k], cbound[k+1]);
+    // chunk k's LDS slots of variable v are [cbase[k(n+1)+v], cbase[k(n+1)+v+1])
+    // (one chunk: cbase == row_ptr)
+    const int32_t *cbound;
+    const int32_t *cbase;
+    int nchunk, pad0;
     float alpha, beta, gamma, delta, eps, zeta, xlmax, pad;
 };
 enum { BF_GENERAL = 1, BF_UNIT = 2 };

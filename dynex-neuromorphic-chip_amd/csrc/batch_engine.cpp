@@ -86,6 +86,7 @@ struct SlotHost {
     int n = 0, m = 0;
     int64_t nnz = 0;
     int mode = 0;
+    size_t lds = 0;  // dynamic LDS bytes the fused kernel needs for it
     size_t off = 0, size = 0;
     std::vector<int32_t> cp, lits;  // kept for the host-side clause check
 };
@@ -186,19 +187,70 @@ namespace dnx {
 
 int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *prm, const double *x0,
                          int64_t step_cap, int stop_on_solve, int *slot_out);
-static int choose_mode(const dnx_batch *b, int n, int64_t nnz)
+// Mode 0 holds the (V, dV) pairs and the contributions in LDS.  When the
+// contributions do not fit beside the pairs, the clauses are cut into at
+// most KC_MAX chunks of consecutive clauses whose contributions fit; a step
+// then evaluates and folds one chunk at a time, keeping each variable's
+// partial fold in LDS as well.
+constexpr int KC_MAX = 16;
+struct Mode0Plan {
+    int nchunk = 0;
+    size_t lds = 0;                 // dynamic LDS bytes the instance needs
+    std::vector<int32_t> bound;     // nchunk + 1 clause indices
+};
+
+static bool plan_mode0(const dnx_batch *b, const HostProblem &h, Mode0Plan &P)
 {
-    size_t pv = ((size_t)n * sizeof(BF2) + 15) / 16 * 16;
-    if (pv + (size_t)nnz * 4 <= (size_t)b->shape[BM_LDS_ALL].lds_cap) return BM_LDS_ALL;
+    const size_t cap = (size_t)b->shape[BM_LDS_ALL].lds_cap;
+    const size_t pv = ((size_t)h.n * sizeof(BF2) + 15) / 16 * 16;
+    const int64_t nnz = h.nnz();
+    P.bound.clear();
+    if (pv + (size_t)nnz * 4 <= cap) {
+        P.nchunk = 1;
+        P.lds = pv + (size_t)nnz * 4;
+        P.bound = {0, h.m};
+        return true;
+    }
+    const size_t acc = ((size_t)h.n * 4 + 15) / 16 * 16;
+    if (pv + acc >= cap) return false;
+    const int64_t slots = (int64_t)((cap - pv - acc) / 4);
+    if (slots < DNX_MAX_LITS || slots * KC_MAX < nnz) return false;
+    P.bound.push_back(0);
+    int64_t used = 0, most = 0;
+    for (int c = 0; c < h.m; c++) {
+        const int k = h.cp[(size_t)c + 1] - h.cp[(size_t)c];
+        if (used + k > slots) {
+            P.bound.push_back(c);
+            most = std::max(most, used);
+            used = 0;
+        }
+        used += k;
+    }
+    P.bound.push_back(h.m);
+    most = std::max(most, used);
+    P.nchunk = (int)P.bound.size() - 1;
+    if (P.nchunk > KC_MAX) return false;
+    P.lds = pv + acc + (size_t)most * 4;
+    return true;
+}
+
+static int choose_mode(const dnx_batch *b, const HostProblem &h, Mode0Plan &P)
+{
+    if (plan_mode0(b, h, P)) return BM_LDS_ALL;
+    P.nchunk = 1;
+    P.bound = {0, h.m};
+    const size_t pv = ((size_t)h.n * sizeof(BF2) + 15) / 16 * 16;
+    P.lds = pv;
     if (pv <= (size_t)b->shape[BM_LDS_V].lds_cap) return BM_LDS_V;
+    P.lds = 0;
     return BM_GLOBAL;
 }
 
 // Byte layout of one instance inside its arena range.
 struct Layout {
-    size_t lits, cp, lpos, eidx, row, unit, xs, xl, vd, e, total;
+    size_t lits, cp, lpos, eidx, row, unit, xs, xl, vd, e, cbound, cbase, total;
 };
-static Layout layout_of(int n, int m, int64_t nnz, bool general, bool has_unit, int mode)
+static Layout layout_of(int n, int m, int64_t nnz, bool general, bool has_unit, int nchunk)
 {
     Layout L{};
     size_t o = 0;
@@ -219,6 +271,8 @@ static Layout layout_of(int n, int m, int64_t nnz, bool general, bool has_unit, 
     L.xl = take((size_t)m * 4);
     L.vd = take((size_t)n * sizeof(BF2));
     L.e = take((size_t)nnz * 4);
+    L.cbound = take(((size_t)nchunk + 1) * 4);
+    L.cbase = nchunk > 1 ? take((size_t)nchunk * ((size_t)n + 1) * 4) : 0;
     L.total = o ? o : 256;
     return L;
 }
@@ -297,11 +351,7 @@ static int batch_enqueue_slots(dnx_batch *b, int ksteps, const std::vector<int> 
         int lds = 0;
         if (md != BM_GLOBAL) {
             // the largest LDS need among this shape's instances
-            for (int i = 0; i < cnt[md]; i++) {
-                const SlotHost &sl = b->slots[(size_t)hw[(size_t)md * S + i]];
-                size_t need = ((size_t)sl.n * sizeof(BF2) + 15) / 16 * 16 + (md == BM_LDS_ALL ? (size_t)sl.nnz * 4 : 0);
-                lds = std::max(lds, (int)need);
-            }
+            for (int i = 0; i < cnt[md]; i++) lds = std::max(lds, (int)b->slots[(size_t)hw[(size_t)md * S + i]].lds);
         }
         // When few instances remain, each gets a larger workgroup: the aim is
         // about four waves per SIMD across the chip, so lone long-running
@@ -403,7 +453,9 @@ int dnx_batch_open(dnx_ctx *ctx, int prec, int slots, int64_t arena_bytes, dnx_b
     b->cus = prop.multiProcessorCount;
     int lds_max = 0;
     (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
-    b->lds_max = lds_max > 0 ? lds_max : 65536;
+    // the fused kernel also holds a few hundred bytes of static LDS (work
+    // ticket, level words, reduction scratch) beside the dynamic part
+    b->lds_max = (lds_max > 0 ? lds_max : 65536) - 1024;
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device) == hipSuccess && khz > 0)
         b->clock_hz = (double)khz * 1e3;
@@ -474,8 +526,10 @@ int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *p
     const int n = h.n, m = h.m;
     const int64_t nnz = h.nnz();
     const bool general = !h.uniform3;
-    const int mode = choose_mode(b, n, nnz);
-    const Layout L = layout_of(n, m, nnz, general, h.has_unit, mode);
+    Mode0Plan plan;
+    const int mode = choose_mode(b, h, plan);
+    const int nchunk = plan.nchunk;
+    const Layout L = layout_of(n, m, nnz, general, h.has_unit, nchunk);
     size_t off;
     if (!b->arena.alloc(L.total, off)) {
         set_error("instance needs " + std::to_string(L.total) + " bytes; arena has " +
@@ -490,7 +544,25 @@ int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *p
     };
     put(L.lits, h.lits.data(), (size_t)nnz * 4);
     if (general) put(L.cp, h.cp.data(), ((size_t)m + 1) * 4);
-    put(L.lpos, h.lpos.data(), (size_t)nnz * 4);
+    if (nchunk == 1) {
+        put(L.lpos, h.lpos.data(), (size_t)nnz * 4);
+    } else {
+        // chunk-local LDS slots: within chunk k, variable-major, each
+        // variable's slots in clause order
+        int32_t *lp = (int32_t *)(stage.data() + L.lpos);
+        int32_t *cb = (int32_t *)(stage.data() + L.cbase);
+        std::vector<int32_t> run((size_t)n + 1);
+        for (int k = 0; k < nchunk; k++) {
+            int32_t *base = cb + (size_t)k * ((size_t)n + 1);
+            std::fill(base, base + n + 1, 0);
+            const int q0 = h.cp[(size_t)plan.bound[(size_t)k]], q1 = h.cp[(size_t)plan.bound[(size_t)k + 1]];
+            for (int q = q0; q < q1; q++) base[std::abs(h.lits[(size_t)q]) - 1 + 1]++;
+            for (int v = 0; v < n; v++) base[v + 1] += base[v];
+            std::copy(base, base + n, run.begin());
+            for (int q = q0; q < q1; q++) lp[q] = run[(size_t)std::abs(h.lits[(size_t)q]) - 1]++;
+        }
+    }
+    put(L.cbound, plan.bound.data(), ((size_t)nchunk + 1) * 4);
     put(L.eidx, h.eidx.data(), (size_t)nnz * 4);
     put(L.row, h.row_ptr.data(), ((size_t)n + 1) * 4);
     if (h.has_unit) put(L.unit, h.unit.data(), (size_t)n);
@@ -525,6 +597,9 @@ int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *p
     I.xl = (float *)(base + L.xl);
     I.vd = (BF2 *)(base + L.vd);
     I.e = (float *)(base + L.e);
+    I.cbound = (const int32_t *)(base + L.cbound);
+    I.cbase = nchunk > 1 ? (const int32_t *)(base + L.cbase) : I.row_ptr;
+    I.nchunk = nchunk;
     I.alpha = (float)prm->alpha;
     I.beta = (float)prm->beta;
     I.gamma = (float)prm->gamma;
@@ -561,6 +636,7 @@ int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *p
     sl.m = m;
     sl.nnz = nnz;
     sl.mode = mode;
+    sl.lds = plan.lds;
     sl.off = off;
     sl.size = L.total;
     sl.cp = h.cp;
@@ -1500,7 +1576,7 @@ static int tune_run(dnx_ctx *ctx, const std::vector<TuneInst> &inst, const std::
     size_t bytes = 1 << 20;
     for (int i : pick) {
         const HostProblem &h = inst[(size_t)i].hp;
-        bytes += layout_of(h.n, h.m, h.nnz(), !h.uniform3, h.has_unit, BM_GLOBAL).total;
+        bytes += layout_of(h.n, h.m, h.nnz(), !h.uniform3, h.has_unit, KC_MAX).total;
     }
     dnx_batch *t = nullptr;
     int rc = dnx_batch_open(ctx, DNX_FP32, (int)pick.size(), (int64_t)bytes, &t);
@@ -1581,7 +1657,7 @@ int dnx_batch_tune(dnx_batch *b, int n_min, int n_max, double r_min, double r_ma
     };
     // 1. fused-kernel shapes
     const int bss[3] = {256, 512, 1024};
-    const int caps[4] = {0, 48 * 1024, 96 * 1024, 160 * 1024};
+    const int caps[4] = {0, 48 * 1024, 96 * 1024, b->lds_max};
     double best = -1.0;
     int b0 = b->shape[BM_LDS_ALL].bs, c0 = b->shape[BM_LDS_ALL].lds_cap, b1 = b->shape[BM_LDS_V].bs;
     for (int cap : caps) {
@@ -1660,7 +1736,7 @@ int portfolio_batched(dnx_problem *p, const dnx_params *prm, int seed, int threa
                       int *winner)
 {
     const HostProblem &h = p->hp;
-    const Layout L = layout_of(h.n, h.m, h.nnz(), !h.uniform3, h.has_unit, BM_GLOBAL);
+    const Layout L = layout_of(h.n, h.m, h.nnz(), !h.uniform3, h.has_unit, KC_MAX);
     dnx_batch *b = nullptr;
     int rc = dnx_batch_open(p->ctx, DNX_FP32, threads, (int64_t)((L.total + 4096) * (size_t)threads + (1 << 20)), &b);
     if (rc) return rc;

@@ -156,6 +156,23 @@ def test_gang_fixed_step_trajectory(ctx):
         assert np.array_equal(r["x"], x)
 
 
+def test_lds_edge_instance(ctx):
+    """An instance whose all-in-LDS form needs 163,700 bytes -- under the
+    device's 160 KiB, but not with the fused kernel's static LDS on top --
+    must be cut into two LDS chunks (the launch stays within the device
+    limit) and compute the oracle's bits."""
+    n, m = 1000, 12975
+    lits = dnx.generate_planted(4242, n, m)
+    cp = planted_cp(m)
+    x0 = dnx.init_uniform(4242, n, m)
+    assert 8 * n + 12 * m == 163700
+    out = run_batch(ctx, [(n, cp, lits, x0)], 300, spl=64, shapes={0: (1024, 160 * 1024)})[0]
+    steps, t, status, xo = oracle_solve(n, cp, lits, x0, 300)
+    assert out["mode"] == 0
+    assert out["steps"] == steps and out["t"] == t
+    assert np.array_equal(out["x"], xo)
+
+
 def test_wide_fixed_step_trajectory(ctx):
     inst = make_set(12, 5, nmin=100, nmax=3000, with_general=True)
     out = run_batch(ctx, inst, 45, spl=25, stop=False, wide=1 << 40)
