@@ -62,10 +62,11 @@ def percentile_ms(vals, q):
 
 
 def apply_shape(b, shape):
+    """Autotuned fused-kernel shapes; the tail goes to wide mode once 2% of
+    the run's clauses are left (the library default)."""
     if shape:
         for md in (0, 1):
             b.set_shape(md, shape[md][0], shape[md][1])
-        b.set_wide(shape[2], 30)
 
 
 def batched_solve(ctx, work, hps, cap, spl, timing, shape=None):
@@ -191,7 +192,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--cap", type=int, default=100000)
-    ap.add_argument("--steps-per-launch", type=int, default=512)
+    ap.add_argument("--steps-per-launch", type=int, default=2048)
     ap.add_argument("--fp64-sample", type=int, default=64)
     ap.add_argument("--fp64-seconds", type=float, default=25.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -231,10 +232,9 @@ def main():
         t0 = time.perf_counter()
         tb = dnx.Batch(ctx, slots=1024, arena_bytes=8 << 30)
         tune_rows = tb.tune(n_min=100, n_max=5000, r_min=4.3, r_max=8.0, sample=512, steps=60, seed=11)
-        shape = (tb.get_shape(0)[:2], tb.get_shape(1)[:2], tb.get_shape(0)[2])
+        shape = (tb.get_shape(0)[:2], tb.get_shape(1)[:2])
         tb.close()
-        log(f"autotuned in {time.perf_counter() - t0:.1f}s: mode0 {shape[0]}, mode1 {shape[1]}, "
-            f"wide threshold {shape[2]} clauses")
+        log(f"autotuned in {time.perf_counter() - t0:.1f}s: mode0 {shape[0]}, mode1 {shape[1]}")
     if args.profile_pass:
         copy_bytes = 256 << 20
         ms = ctx.device_copy(copy_bytes, repeats=4)
@@ -345,8 +345,7 @@ def main():
         "speedup_vs_plain_fp64": round(value / p64["ips"], 1) if p64["ips"] else None,
         "solve_rate_fp32_vs_fp64_on_sample": [fp32_rate, fp64_rate],
         "served": srv,
-        "autotuned_shape": None if shape is None else dict(mode0=list(shape[0]), mode1=list(shape[1]),
-                                                           wide_clauses=int(shape[2])),
+        "autotuned_shape": None if shape is None else dict(mode0=list(shape[0]), mode1=list(shape[1])),
         "single_instance_n100k": single,
         "roofline": roofline,
         "cpu_baseline": cpu_line,

@@ -120,7 +120,7 @@ struct dnx_batch {
     BDyn *h_dyn = nullptr;    // pinned: [2][nslots] status snapshots
     BDyn *h_stage = nullptr;  // pinned: [nslots] source of status uploads
     int *h_work = nullptr;  // pinned: [2][BM_COUNT * nslots]
-    Shape shape[BM_COUNT] = {{256, 48 * 1024}, {256, 40 * 1024}, {1024, 0}};
+    Shape shape[BM_COUNT] = {{1024, 160 * 1024}, {256, 40 * 1024}, {1024, 0}};
     int cus = 0, lds_max = 0;
     double clock_hz = 1e8;
     int buf = 0;            // alternate pinned buffer
@@ -143,8 +143,10 @@ struct dnx_batch {
     // dnx_batch_run, per bucket (step_bytes per instance step)
     double t_alg[NBUCKET] = {};
     // wide mode: entered when the running instances hold at most
-    // wide_clauses clauses in total; wide_chunk steps between status syncs
-    long long wide_clauses = 1500000;
+    // max(wide_clauses, wide_frac x the clauses running when dnx_batch_run
+    // starts) clauses in total; wide_chunk steps between status syncs
+    long long wide_clauses = 0;
+    double wide_frac = 0.02;
     int wide_chunk = 30;
     // serving: requests with at least this many clauses run in the wide lane
     long long serve_big = 60000;
@@ -156,7 +158,7 @@ struct dnx_batch {
     DevBuf b_wctl, b_wlist, b_bmap, b_vmap;
     // gang mode: taken for the tail when the running instances need at
     // most gang_blocks workgroups in total (0 disables it)
-    int gang_blocks = 128;
+    int gang_blocks = 0;
     int gang_cpw = 1024;      // target clauses per gang workgroup
     int gang_steps = 128;     // steps per cooperative launch
     int gang_cap = 0;         // co-resident gang workgroups (0: not known yet)
@@ -909,6 +911,10 @@ int dnx_batch_run(dnx_batch *b, int steps_per_launch, double walltime_s, int *re
     if (rc) return rc;
     b->buf = 0;
     int running = 0;
+    long long start_clauses = 0;
+    for (int s = 0; s < b->nslots; s++)
+        if (b->slots[(size_t)s].used && b->h_dyn[s].status == DNX_RUNNING) start_clauses += b->slots[(size_t)s].m;
+    const long long threshold = std::max(b->wide_clauses, (long long)(b->wide_frac * (double)start_clauses));
     for (;;) {
         const BDyn *known = b->h_dyn;
         running = 0;
@@ -920,7 +926,7 @@ int dnx_batch_run(dnx_batch *b, int steps_per_launch, double walltime_s, int *re
         long long clauses = 0;
         for (int s = 0; s < b->nslots; s++)
             if (b->slots[(size_t)s].used && known[s].status == DNX_RUNNING) clauses += b->slots[(size_t)s].m;
-        if (clauses <= b->wide_clauses) {
+        if (clauses <= threshold) {
             bool gang = false;
             if ((rc = batch_gang_run(b, walltime_s, t0, &gang))) return rc;
             if (!gang && (rc = batch_wide_run(b, walltime_s, t0))) return rc;
@@ -1054,6 +1060,7 @@ int dnx_batch_set_wide(dnx_batch *b, int64_t max_clauses, int steps_per_sync)
 {
     if (!b || max_clauses < 0 || steps_per_sync <= 0) return DNX_E_ARG;
     b->wide_clauses = max_clauses;
+    b->wide_frac = 0.0;
     b->wide_chunk = steps_per_sync;
     return DNX_OK;
 }
@@ -1066,6 +1073,15 @@ int dnx_batch_set_gang(dnx_batch *b, int max_blocks, int clauses_per_workgroup, 
     b->gang_blocks = max_blocks;
     b->gang_cpw = clauses_per_workgroup;
     b->gang_steps = steps_per_launch;
+    return DNX_OK;
+}
+
+int dnx_batch_set_wide_fraction(dnx_batch *b, double fraction, int steps_per_sync)
+{
+    if (!b || !(fraction >= 0.0 && fraction <= 1.0) || steps_per_sync <= 0) return DNX_E_ARG;
+    b->wide_clauses = 0;
+    b->wide_frac = fraction;
+    b->wide_chunk = steps_per_sync;
     return DNX_OK;
 }
 
@@ -1151,7 +1167,8 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
     int rc = dnx_batch_stamp(b, &tick0, &rate);
     if (rc) return rc;
     const double host0 = 0.5 * (hb + now_s());
-    const long long big = b->wide_clauses > 0 ? std::max<long long>(b->serve_big, 1) : LLONG_MAX;
+    const long long big =
+        b->wide_clauses > 0 || b->wide_frac > 0 ? std::max<long long>(b->serve_big, 1) : LLONG_MAX;
     hipStream_t st = b->ctx->stream;
     hipError_t e;
     // lane bookkeeping
@@ -1689,26 +1706,9 @@ int dnx_batch_tune(dnx_batch *b, int n_min, int n_max, double r_min, double r_ma
             b1 = bs;
         }
     }
-    // 2. wide-mode threshold on the largest instances
-    std::vector<int> bym = all;
-    std::sort(bym.begin(), bym.end(), [&](int x, int y) { return inst[(size_t)x].hp.m > inst[(size_t)y].hp.m; });
-    long long threshold = 0;
-    for (int k : {1, 4, 16, 64}) {
-        if (k > sample) break;
-        std::vector<int> pick(bym.begin(), bym.begin() + k);
-        long long clauses = 0;
-        for (int i : pick) clauses += inst[(size_t)i].hp.m;
-        double ms_n = 0.0, ms_w = 0.0;
-        int rc = tune_run(b->ctx, inst, pick, b0, c0, b1, 0, steps, &ms_n);
-        if (!rc) rc = tune_run(b->ctx, inst, pick, b0, c0, b1, 1LL << 50, steps, &ms_w);
-        if (rc) return rc;
-        rep(2, k, (double)clauses, ms_n, ms_w);
-        if (ms_w < ms_n) threshold = std::max(threshold, 2 * clauses);
-    }
     b->shape[BM_LDS_ALL].bs = b0;
     b->shape[BM_LDS_ALL].lds_cap = c0;
     b->shape[BM_LDS_V].bs = b1;
-    b->wide_clauses = threshold;
     if (report_count) *report_count = nrep;
     return DNX_OK;
 }

@@ -93,6 +93,7 @@ def lib():
     L.dnx_batch_set_wide.argtypes = [vp, C.c_int64, ip]
     L.dnx_batch_set_serve_big.argtypes = [vp, C.c_int64, C.c_int64]
     L.dnx_batch_set_gang.argtypes = [vp, ip, ip, ip]
+    L.dnx_batch_set_wide_fraction.argtypes = [vp, dp, ip]
     L.dnx_batch_tune.argtypes = [vp, ip, ip, dp, dp, ip, ip, C.c_uint64, vp, ip, C.POINTER(C.c_int)]
     L.dnx_batch_get_shape.argtypes = [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)]
     L.dnx_state_time.argtypes = [vp, ip, C.POINTER(C.c_double)]
@@ -400,6 +401,9 @@ class Batch:
 
     def set_wide(self, max_clauses, steps_per_sync=30):
         check(lib().dnx_batch_set_wide(self.h, int(max_clauses), int(steps_per_sync)))
+
+    def set_wide_fraction(self, fraction=0.02, steps_per_sync=30):
+        check(lib().dnx_batch_set_wide_fraction(self.h, float(fraction), int(steps_per_sync)))
 
     def timing(self, enable=-1, with_bytes=False):
         """Summed device ms and launch counts per fused-kernel mode (0-2), wide mode (3) and gang mode (4);

@@ -221,6 +221,11 @@ int dnx_batch_info(dnx_batch *b, int slot, int *n, int *m, int *mode);
  * 0 disables it.  Results are bit-identical either way. */
 int dnx_batch_set_wide(dnx_batch *b, int64_t max_clauses, int steps_per_sync);
 
+/* Wide mode with a threshold relative to the run: entered once the running
+ * instances hold at most `fraction` of the clauses that were running when
+ * dnx_batch_run began (the default: 0.02, 30 steps per sync). */
+int dnx_batch_set_wide_fraction(dnx_batch *b, double fraction, int steps_per_sync);
+
 /* Serving: requests with at least `clauses` clauses run in the wide lane
  * (default 60,000; needs wide mode enabled).  Each wide chunk advances the
  * members with the fewest steps so far, up to active_clauses clauses in
@@ -231,12 +236,10 @@ int dnx_batch_set_serve_big(dnx_batch *b, int64_t clauses, int64_t active_clause
 /* Autotuner: times candidate execution shapes on `sample` synthetic planted
  * instances (n log-uniform in [n_min, n_max], clause ratio uniform in
  * [r_min, r_max]) for `steps` fixed steps each -- workgroup size of modes 0
- * and 1, the LDS budget admitting an instance to mode 0, and the wide-mode
- * threshold (narrow against wide on the 1, 4, 16, 64 largest instances) --
- * and applies the fastest to the empty batch b.  report (may be NULL)
- * receives report_len doubles: one 5-tuple per measurement
- * (kind, block|k, lds_cap|clauses, ms, GB/s|wide ms); *report_count is the
- * number of doubles written. */
+ * and 1 and the LDS budget of mode 0 -- and applies the fastest to the
+ * empty batch b.  report (may be NULL) receives report_len doubles: one
+ * 5-tuple per measurement (kind, block, lds_cap, ms, GB/s); *report_count is
+ * the number of doubles written. */
 int dnx_batch_tune(dnx_batch *b, int n_min, int n_max, double r_min, double r_max, int sample,
                    int steps, uint64_t seed, double *report, int report_len, int *report_count);
 

@@ -98,7 +98,9 @@ def test_batch_matches_oracle(ctx, inst_set, oracle_results):
         assert np.array_equal(r["x"], xo)
         if status == 1:
             assert r["unsat"] == 0
-    assert modes >= {0, 1}
+    # with the default 48 KiB mode-0 budget the larger instances hold their
+    # contributions in LDS chunk by chunk; mode 1 is covered by the shapes below
+    assert 0 in modes
 
 
 @pytest.mark.parametrize("shapes", [
