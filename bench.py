@@ -268,7 +268,7 @@ def main():
     nl_sum = np.sum([r["launches"] for r in runs], axis=0)
     ab_sum = np.sum([r["alg"] for r in runs], axis=0)
     names = ["k_batch<mode 0>", "k_batch<mode 1>", "k_batch<mode 2>",
-             "k_wide_clause+k_wide_var (one graph replay of 30 steps)", "k_gang"]
+             "k_wide_clause+k_wide_var (one chunk of 30 steps)", "k_gang"]
     dom = int(np.argmax(ms_sum))
     avg_us = float(ms_sum[dom] / nl_sum[dom] * 1e3) if nl_sum[dom] else 0.0
     per_launch = float(ab_sum[dom] / nl_sum[dom]) if nl_sum[dom] else 0.0

@@ -750,36 +750,22 @@ static int batch_wide_run(dnx_batch *b, double walltime_s, std::chrono::steady_c
     Ctl *wctl = (Ctl *)b->b_wctl.ptr;
     BDyn *dyn = (BDyn *)b->b_dyn.ptr;
     launch_wide_enter(dyn, wctl, (const int *)b->b_wlist.ptr, count, 0, st);
-    // one chunk of wide_chunk steps (a multiple of 3, so the rotation
-    // returns to 0) is captured once and replayed
+    // chunks of wide_chunk steps (a multiple of 3, so the rotation returns
+    // to 0), launched directly: the host enqueues a chunk far faster than
+    // the device runs it.  (The maps change at every entry, so a captured
+    // graph would be rebuilt each time; and under rocprofv3 kernel tracing
+    // hipGraphLaunch of this graph ended the process with SIGSEGV inside the
+    // profiler.)
     const int chunk = (b->wide_chunk + 2) / 3 * 3;
-    hipGraphExec_t gexec = nullptr;
-    {
-        hipGraph_t g;
-        if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess)
-            return hip_fail(e, "wide capture");
-        for (int k = 0; k < chunk; k++)
-            launch_wide_step((const BInst *)b->b_inst.ptr, wctl, (const int *)b->b_bmap.ptr, nbc,
-                             (const int *)b->b_vmap.ptr, nbv, k % 3, st);
-        if ((e = hipStreamEndCapture(st, &g)) != hipSuccess) return hip_fail(e, "wide capture end");
-        e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) return hip_fail(e, "wide graph");
-    }
-    struct GraphGuard {
-        hipGraphExec_t &g;
-        ~GraphGuard()
-        {
-            if (g) (void)hipGraphExecDestroy(g);
-        }
-    } guard{gexec};
     const int r = 0;
     std::vector<long long> before(list.size());
     for (;;) {
         for (size_t i = 0; i < list.size(); i++) before[i] = b->h_dyn[list[i]].steps;
         const bool timed = b->timing && !b->t_pending[BK_WIDE];
         if (timed && (e = hipEventRecord(b->t0[BK_WIDE], st)) != hipSuccess) return hip_fail(e, "timing");
-        if ((e = hipGraphLaunch(gexec, st)) != hipSuccess) return hip_fail(e, "wide graph launch");
+        for (int k = 0; k < chunk; k++)
+            launch_wide_step((const BInst *)b->b_inst.ptr, wctl, (const int *)b->b_bmap.ptr, nbc,
+                             (const int *)b->b_vmap.ptr, nbv, k % 3, st);
         if (timed) {
             if ((e = hipEventRecord(b->t1[BK_WIDE], st)) != hipSuccess) return hip_fail(e, "timing");
             b->t_pending[BK_WIDE] = true;
