@@ -25,10 +25,13 @@ sys.path.insert(0, os.path.join(ROOT, "dynex-neuromorphic-chip_amd"))
 import dnx  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0
-# HBM bytes per launch of each bucket's kernels from rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE, scaled by the counters' reading on the copy
-# kernel (None where not measured).
-TRAFFIC = {}
+# HBM bytes per launch of the dominant kernel of the default workload, from
+# two rocprofv3 --pmc runs of `bench.py --profile-pass` (FETCH_SIZE, then
+# WRITE_SIZE, per dispatch of the batched solve), each scaled by what the
+# counter reads on the copy kernel (256 MiB each way: FETCH_SIZE x 2.000,
+# WRITE_SIZE x 1.000).  Keyed by bucket; k_batch<mode 0> with 2048 steps per
+# launch: 144.35 GB read + 38.56 GB written.
+TRAFFIC = {0: 182.91e9}
 METRIC = "solved instances/s + p50/p99 time-to-solution (batched FP32 vs plain FP64, 1 GPU)"
 
 
@@ -275,7 +278,9 @@ def main():
     achieved = per_launch / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
     all_s = float(ms_sum.sum()) / 1e3
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBPS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBPS, 4), traffic=TRAFFIC.get(dom),
+                    frac=round(achieved / HBM_PEAK_GBPS, 4),
+                    traffic=TRAFFIC.get(dom) if (args.instances, args.steps_per_launch, args.seed) == (4096, 2048, 4096)
+                    else None,
                     kernel=names[dom], alg_bytes_per_launch=round(per_launch), avg_launch_us=round(avg_us, 2),
                     device_ms={names[i]: round(float(ms_sum[i]), 2) for i in range(5)},
                     launches={names[i]: int(nl_sum[i]) for i in range(5)},
