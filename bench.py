@@ -103,22 +103,23 @@ def batched_solve(ctx, work, hps, cap, spl, timing, shape=None):
 
 
 def plain_fp64(ctx, work, idx, cap, budget_s):
-    done, solved, t_used, steps = 0, 0, 0.0, []
+    done, solved, t_used, steps, oks = 0, 0, 0.0, [], []
     for i in idx:
         w = work[i]
         gp = ctx.problem_from_lits(w["n"], w["cp"], w["lits"])
         t0 = time.perf_counter()
         r = gp.solve(dnx.params(w["m"]), w["x0"], dnx.FP64, cap=cap)
         t_used += time.perf_counter() - t0
-        if r["status"] == dnx.SAT and gp.check_assignment(r["assign"]) == 0:
-            solved += 1
+        ok = r["status"] == dnx.SAT and gp.check_assignment(r["assign"]) == 0
+        solved += ok
+        oks.append(bool(ok))
         gp.free()
         done += 1
         steps.append(r["steps"])
         if t_used > budget_s:
             break
     return dict(instances=done, solved=solved, seconds=t_used, ips=solved / t_used if t_used else 0.0,
-                idx=list(idx[:done]), steps=steps)
+                idx=list(idx[:done]), steps=steps, ok=oks)
 
 
 def single_instance(ctx, n, m, prec, nsteps):
@@ -172,9 +173,16 @@ def served_sample(count, seed):
     return out
 
 
-def cpu_baseline(work, idx, cap, budget_s):
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
+    return O
+
+
+def cpu_baseline(work, idx, cap, budget_s):
+    """The FP64 oracle (the reference's algorithm in C, one thread) on instances of the workload until
+    budget_s seconds of solving have passed."""
+    O = _oracle()
     t_used, done, solved = 0.0, 0, 0
     for i in idx:
         w = work[i]
@@ -188,6 +196,76 @@ def cpu_baseline(work, idx, cap, budget_s):
         if t_used > budget_s:
             break
     return dict(value=solved / t_used if t_used else 0.0, instances=done, solved=solved, seconds=t_used)
+
+
+def _cpu_worker(args):
+    insts, cap, budget_s = args
+    return cpu_baseline(insts, range(len(insts)), cap, budget_s)
+
+
+def cpu_baseline_cores(work, idx, cap, budget_s, procs):
+    """The same oracle on `procs` processes at once (one instance per process at a time, distinct
+    instances), for budget_s seconds each: instances/s of the host."""
+    import multiprocessing as mp
+    chunks = [[] for _ in range(procs)]
+    for k, i in enumerate(idx):
+        w = work[i]
+        chunks[k % procs].append(dict(n=w["n"], m=w["m"], cp=w["cp"], lits=w["lits"], x0=w["x0"]))
+    t0 = time.perf_counter()
+    with mp.get_context("spawn").Pool(procs) as pool:
+        res = pool.map(_cpu_worker, [(c, cap, budget_s) for c in chunks])
+    wall = time.perf_counter() - t0
+    solved = sum(r["solved"] for r in res)
+    busy = max(r["seconds"] for r in res)
+    return dict(value=solved / busy if busy else 0.0, instances=sum(r["instances"] for r in res), solved=solved,
+                seconds=busy, wall=wall)
+
+
+def cpu_reference_points(cap):
+    """Oracle per-step time on the n=100,000 single instance and time-to-solution on n=100, m=800."""
+    O = _oracle()
+    lits = dnx.generate_planted(90001, 100000, 800000)
+    cp = np.arange(0, 3 * 800000 + 1, 3, dtype=np.int32)
+    op = O.Problem(100000, 800000, cp, lits)
+    x = dnx.init_uniform(90001, 100000, 800000)
+    t0 = time.perf_counter()
+    O.trajectory(op, O.params(800000), x, 3)
+    c3 = (time.perf_counter() - t0) / 3
+    tts = []
+    for s in range(5):
+        lits = dnx.generate_planted(500 + s, 100, 800)
+        cp = np.arange(0, 3 * 800 + 1, 3, dtype=np.int32)
+        op = O.Problem(100, 800, cp, lits)
+        x = dnx.init_uniform(500 + s, 100, 800)
+        t0 = time.perf_counter()
+        O.solve(op, O.params(800), x, cap)
+        tts.append(time.perf_counter() - t0)
+    return dict(n100k_step_ms=round(c3 * 1e3, 1), n100_m800_tts_ms_median=round(float(np.median(tts)) * 1e3, 2))
+
+
+def precision_report(ctx, steps=1000):
+    """FP32 against FP64 on the n=10,000, m=80,000 instance from one initial state: the largest |V32 - V64|
+    and the share of variables whose sign differs, after 10, 100 and `steps` fixed steps."""
+    lits = dnx.generate_planted(22, 10000, 80000)
+    cp = np.arange(0, 3 * 80000 + 1, 3, dtype=np.int32)
+    gp = ctx.problem_from_lits(10000, cp, lits)
+    x0 = dnx.init_uniform(22, 10000, 80000)
+    s32 = gp.state(dnx.params(80000), x0, dnx.FP32)
+    s64 = gp.state(dnx.params(80000), x0, dnx.FP64)
+    out, done = [], 0
+    try:
+        for k in (10, 100, steps):
+            s32.step(k - done)
+            s64.step(k - done)
+            done = k
+            a, b = s32.read()["x"][:10000], s64.read()["x"][:10000]
+            out.append(dict(step=k, max_abs_dV=float(np.max(np.abs(a - b))),
+                            sign_differs=float(np.mean(np.sign(a) != np.sign(b)))))
+    finally:
+        s32.free()
+        s64.free()
+        gp.free()
+    return out
 
 
 def main():
@@ -311,6 +389,14 @@ def main():
             if p64s["ips"] and srv["instances_per_s"]:
                 srv["speedup_vs_plain_fp64"] = round(srv["instances_per_s"] / p64s["ips"], 1)
 
+    precision = dict(c2_fp32_vs_fp64=precision_report(ctx))
+    # solve rate at fixed step budgets on the plain-FP64 sample: FP32 batched against FP64
+    st32 = last["steps"][p64["idx"]]
+    ok32 = last["verified"][p64["idx"]]
+    st64, ok64 = np.array(p64["steps"]), np.array(p64["ok"], bool)
+    precision["solve_rate_at_budget"] = {
+        str(bgt): [float(np.mean(ok32 & (st32 <= bgt))), float(np.mean(ok64 & (st64 <= bgt)))]
+        for bgt in (100, 1000, 10000, args.cap)}
     single = None
     if not args.skip_single:
         log("single instance n=100k ...")
@@ -319,10 +405,16 @@ def main():
     ctx.close()
 
     log("CPU oracle baseline ...")
-    cpu = cpu_baseline(work, sample, args.cap, args.cpu_seconds)
-    cpu_line = dict(value=round(cpu["value"], 3), unit="instances/s", cores=1, kind="port",
+    procs = max(1, min(16, os.cpu_count() or 1))
+    cpu_sample = np.random.default_rng(8).permutation(len(work))
+    cpu = cpu_baseline_cores(work, cpu_sample, args.cap, args.cpu_seconds, procs)
+    cpu1 = cpu_baseline(work, sample, args.cap, min(args.cpu_seconds, 10.0))
+    cpu_line = dict(value=round(cpu["value"], 3), unit="instances/s", cores=procs, kind="port",
                     sample=f"{cpu['instances']} instances of the same workload (random sample), FP64 oracle, "
-                           f"1 thread, {cpu['seconds']:.1f}s")
+                           f"{procs} processes x {cpu['seconds']:.1f}s",
+                    one_thread=dict(value=round(cpu1["value"], 3), instances=cpu1["instances"],
+                                    seconds=round(cpu1["seconds"], 1)),
+                    **cpu_reference_points(args.cap))
 
     out = {
         "metric": METRIC,
@@ -356,6 +448,7 @@ def main():
                            instances_per_s=round(p64["ips"], 2)),
         "speedup_vs_plain_fp64": round(value / p64["ips"], 1) if p64["ips"] else None,
         "solve_rate_fp32_vs_fp64_on_sample": [fp32_rate, fp64_rate],
+        "precision": precision,
         "served": srv,
         "autotuned_shape": None if shape is None else dict(mode0=list(shape[0]), mode1=list(shape[1])),
         "single_instance_n100k": single,
