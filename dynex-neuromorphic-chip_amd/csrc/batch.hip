@@ -58,6 +58,7 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
         // at a time in LDS (one chunk when they all fit); with several chunks
         // each variable's partial fold waits in sacc between chunks
         const int nch = MODE == BM_LDS_ALL ? I.nchunk : 1;
+        constexpr int UV = 4;  // variables whose slot ranges one thread loads at once
         const size_t pvb = ((size_t)n * sizeof(BF2) + 15) / 16 * 16;
         float *sacc = (float *)(smem + pvb);
         float *se = (float *)(smem + pvb + (nch > 1 ? ((size_t)n * 4 + 15) / 16 * 16 : 0));
@@ -174,11 +175,24 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                     // variable's slots), then free se for the next chunk
                     __syncthreads();
                     const int32_t *cbase = I.cbase + (size_t)ch * (n + 1);
-                    for (int v = tid; v < n; v += BS) {
-                        float s = ch ? sacc[v] : 0.0f;
-                        const int rb = cbase[v], re = cbase[v + 1];
-                        for (int q = rb; q < re; q++) s = s + se[q];
-                        sacc[v] = s;
+                    // each thread's next UV variables' slot ranges are loaded
+                    // together, then folded
+                    for (int vb = tid; vb < n; vb += UV * BS) {
+                        int RB[UV], RE[UV];
+#pragma unroll
+                        for (int u = 0; u < UV; u++) {
+                            const int v = vb + u * BS;
+                            RB[u] = v < n ? cbase[v] : 0;
+                            RE[u] = v < n ? cbase[v + 1] : 0;
+                        }
+#pragma unroll
+                        for (int u = 0; u < UV; u++) {
+                            const int v = vb + u * BS;
+                            if (v >= n) break;
+                            float s = ch ? sacc[v] : 0.0f;
+                            for (int q = RB[u]; q < RE[u]; q++) s = s + se[q];
+                            sacc[v] = s;
+                        }
                     }
                     __syncthreads();
                 }
@@ -190,27 +204,38 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                 done = DNX_UNKNOWN;
             // ---- variable pass ------------------------------------------
             int fmax = 0;
-            for (int v = tid; v < n; v += BS) {
-                BF2 a = pv[v];
-                const float V = pend ? clampV(a.v + h * a.d) : a.v;
-                float dV = 0.0f;
-                if (!done) {
-                    if (MODE == BM_LDS_ALL) {
-                        const int32_t *cbase = I.cbase + (size_t)(nch - 1) * (n + 1);
-                        const int rb = cbase[v], re = cbase[v + 1];
-                        if (nch > 1) dV = sacc[v];
-                        for (int q = rb; q < re; q++) dV = dV + se[q];
-                    } else {
-                        const int rb = I.row_ptr[v], re = I.row_ptr[v + 1];
-                        dV = fold_gather<float>(I.e, I.eidx, rb, re);
-                    }
-                    if ((I.flags & BF_UNIT) && I.unit[v] != 0) dV = 0.0f;  // dynex.cc:306
-                    fmax = max(fmax, fail_level(V, dV));
+            const int32_t *rowp = MODE == BM_LDS_ALL ? I.cbase + (size_t)(nch - 1) * (n + 1) : I.row_ptr;
+            for (int vb = tid; vb < n; vb += UV * BS) {
+                int RB[UV], RE[UV], PIN[UV];
+#pragma unroll
+                for (int u = 0; u < UV; u++) {
+                    const int v = vb + u * BS;
+                    RB[u] = v < n ? rowp[v] : 0;
+                    RE[u] = v < n ? rowp[v + 1] : 0;
+                    PIN[u] = v < n && (I.flags & BF_UNIT) ? I.unit[v] : 0;
                 }
-                BF2 o;
-                o.v = V;
-                o.d = dV;
-                pv[v] = o;
+#pragma unroll
+                for (int u = 0; u < UV; u++) {
+                    const int v = vb + u * BS;
+                    if (v >= n) break;
+                    BF2 a = pv[v];
+                    const float V = pend ? clampV(a.v + h * a.d) : a.v;
+                    float dV = 0.0f;
+                    if (!done) {
+                        if (MODE == BM_LDS_ALL) {
+                            if (nch > 1) dV = sacc[v];
+                            for (int q = RB[u]; q < RE[u]; q++) dV = dV + se[q];
+                        } else {
+                            dV = fold_gather<float>(I.e, I.eidx, RB[u], RE[u]);
+                        }
+                        if (PIN[u] != 0) dV = 0.0f;  // dynex.cc:306
+                        fmax = max(fmax, fail_level(V, dV));
+                    }
+                    BF2 o;
+                    o.v = V;
+                    o.d = dV;
+                    pv[v] = o;
+                }
             }
             if (done) break;
             const int wm = wave_max(fmax);
