@@ -158,6 +158,46 @@ def test_gang_fixed_step_trajectory(ctx):
         assert np.array_equal(r["x"], x)
 
 
+def test_full_mixed_workload(ctx):
+    """The benchmark's workload at full size (4096 planted instances, n
+    log-uniform in [100, 5000], ratio U[4.3, 8]) with the default policy:
+    every instance ends SAT with an assignment that passes the exact clause
+    check, and sampled instances compute the same steps and time as the
+    single-instance FP32 path and the float oracle."""
+    rng = np.random.default_rng(4096)
+    count = 4096
+    ns = np.exp(rng.uniform(np.log(100), np.log(5000), count)).round().astype(int)
+    rs = rng.uniform(4.3, 8.0, count)
+    inst = []
+    for i in range(count):
+        n = int(ns[i])
+        m = int(round(rs[i] * n))
+        s = 4096 * 100003 + i
+        inst.append((n, planted_cp(m), dnx.generate_planted(s, n, m), dnx.init_uniform(s, n, m)))
+    b = dnx.Batch(ctx, slots=count)
+    slots = []
+    for n, cp, lits, x0 in inst:
+        hp = dnx.host_problem(n, cp, lits)
+        slots.append(b.admit(hp, dnx.params(len(cp) - 1), x0, cap=100000))
+        hp.free()
+    b.run(steps_per_launch=2048)
+    res = b.poll()
+    assert np.all(res["status"][slots] == dnx.SAT)
+    for (n, cp, lits, x0), s in zip(inst, slots):
+        assert b.assignment(s, n)[1] == 0
+    steps = res["steps"][slots]
+    b.close()
+    pick = [i for i in np.random.default_rng(1).permutation(count) if steps[i] < 1500][:6]
+    for i in pick:
+        n, cp, lits, x0 = inst[i]
+        gp = ctx.problem_from_lits(n, cp, lits)
+        r = gp.solve(dnx.params(len(cp) - 1), x0, dnx.FP32, cap=100000)
+        gp.free()
+        assert r["steps"] == steps[i]
+        o_steps, o_t, o_status, _ = oracle_solve(n, cp, lits, x0, 100000)
+        assert o_status == 1 and o_steps == steps[i]
+
+
 def test_lds_edge_instance(ctx):
     """An instance whose all-in-LDS form needs 163,700 bytes -- under the
     device's 160 KiB, but not with the fused kernel's static LDS on top --

@@ -185,6 +185,17 @@ def test_trajectory_c2_fp64_1000_steps(ctx):
     gp.free()
 
 
+@pytest.mark.parametrize("prec", [64, 32])
+def test_trajectory_c3_full_size(ctx, prec):
+    """n=100,000, m=800,000 (the largest single-instance size): the first
+    steps bit-identical to the oracle."""
+    cp, lits = gen(100000, 800000, 23)
+    gp, op = both(ctx, 100000, cp, lits)
+    x0 = dnx.init_uniform(23, 100000, 800000)
+    run_traj(gp, op, x0, [1, 4], prec)
+    gp.free()
+
+
 def test_trajectory_c2_fp32_vs_fp64(ctx):
     """FP32 is bit-identical to the float oracle; its distance from the FP64
     trajectory is reported (not a parity bound)."""
