@@ -19,6 +19,74 @@
 
 namespace dnx {
 
+// The descriptor's pointers arrive as generic pointers (they are loaded from
+// device memory, not from the kernel arguments), so the compiler would access
+// them with FLAT instructions, which count against the LDS counter as well
+// and make every LDS wait also wait for device-memory traffic.  They all
+// point into the device arena: the kernels read the descriptor through this
+// view, whose pointers are typed as global memory.
+#define GLB __attribute__((address_space(1)))
+struct GInst {
+    int n, m;
+    int mode;
+    int flags;
+    GLB const int32_t *lits;
+    GLB const int32_t *cp;
+    GLB const int32_t *lpos;
+    GLB const int32_t *eidx;
+    GLB const int32_t *row_ptr;
+    GLB const int8_t *unit;
+    GLB float *xs, *xl;
+    GLB unsigned long long *vd;  // (V, dV) pairs as 8-byte words
+    GLB float *e;
+    GLB const int32_t *cbound;
+    GLB const int32_t *cbase;
+    int nchunk;
+    float alpha, beta, gamma, delta, eps, zeta, xlmax;
+};
+__device__ __forceinline__ GInst global_view(const BInst &B)
+{
+    GInst I;
+    I.n = B.n;
+    I.m = B.m;
+    I.mode = B.mode;
+    I.flags = B.flags;
+    I.lits = (GLB const int32_t *)B.lits;
+    I.cp = (GLB const int32_t *)B.cp;
+    I.lpos = (GLB const int32_t *)B.lpos;
+    I.eidx = (GLB const int32_t *)B.eidx;
+    I.row_ptr = (GLB const int32_t *)B.row_ptr;
+    I.unit = (GLB const int8_t *)B.unit;
+    I.xs = (GLB float *)B.xs;
+    I.xl = (GLB float *)B.xl;
+    I.vd = (GLB unsigned long long *)B.vd;
+    I.e = (GLB float *)B.e;
+    I.cbound = (GLB const int32_t *)B.cbound;
+    I.cbase = (GLB const int32_t *)B.cbase;
+    I.nchunk = B.nchunk;
+    I.alpha = B.alpha;
+    I.beta = B.beta;
+    I.gamma = B.gamma;
+    I.delta = B.delta;
+    I.eps = B.eps;
+    I.zeta = B.zeta;
+    I.xlmax = B.xlmax;
+    return I;
+}
+// one (V, dV) pair of the view: V in the low word, dV in the high word
+__device__ __forceinline__ BF2 vdr(const GInst &I, int i)
+{
+    const unsigned long long u = I.vd[i];
+    BF2 r;
+    r.v = __uint_as_float((unsigned)u);
+    r.d = __uint_as_float((unsigned)(u >> 32));
+    return r;
+}
+__device__ __forceinline__ void vdw(const GInst &I, int i, BF2 o)
+{
+    I.vd[i] = (unsigned long long)__float_as_uint(o.v) | ((unsigned long long)__float_as_uint(o.d) << 32);
+}
+
 template <int MODE, int BS>
 __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, BDyn *__restrict__ dyn,
                                               const int *__restrict__ work, int nwork, int *__restrict__ head,
@@ -41,7 +109,7 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
         const int slot = __builtin_amdgcn_readfirstlane(work[it]);
         BDyn d = dyn[slot];
         if (d.status != DNX_RUNNING) continue;
-        const BInst I = insts[slot];
+        const GInst I = global_view(insts[slot]);
         const int n = I.n, m = I.m;
         const bool general = (I.flags & BF_GENERAL) != 0;
         PT<float> p;
@@ -53,17 +121,33 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
         p.zeta = I.zeta;
         p.xlmax = I.xlmax;
 
-        BF2 *pv = (MODE == BM_GLOBAL) ? I.vd : (BF2 *)smem;
+        // the (V, dV) pairs: LDS in modes 0 and 1, device memory in mode 2
+        BF2 *lpv = (BF2 *)smem;
+        auto pvr = [&](int i) -> BF2 {
+            if constexpr (MODE == BM_GLOBAL) {
+                return vdr(I, i);
+            } else {
+                const float2 t = ((const float2 *)lpv)[i];  // one 8-byte LDS read
+                BF2 r;
+                r.v = t.x;
+                r.d = t.y;
+                return r;
+            }
+        };
+        auto pvw = [&](int i, BF2 val) {
+            if constexpr (MODE == BM_GLOBAL) vdw(I, i, val);
+            else lpv[i] = val;
+        };
         // mode 0 keeps the contributions of one chunk of consecutive clauses
         // at a time in LDS (one chunk when they all fit); with several chunks
         // each variable's partial fold waits in sacc between chunks
         const int nch = MODE == BM_LDS_ALL ? I.nchunk : 1;
-        constexpr int UV = 4;  // variables whose slot ranges one thread loads at once
+        constexpr int UV = 8;  // variables whose slot ranges one thread loads at once
         const size_t pvb = ((size_t)n * sizeof(BF2) + 15) / 16 * 16;
         float *sacc = (float *)(smem + pvb);
         float *se = (float *)(smem + pvb + (nch > 1 ? ((size_t)n * 4 + 15) / 16 * 16 : 0));
         if (MODE != BM_GLOBAL) {
-            for (int v = tid; v < n; v += BS) pv[v] = I.vd[v];
+            for (int v = tid; v < n; v += BS) lpv[v] = vdr(I, v);
             __syncthreads();
         }
         bool pend = d.plevel >= 0;
@@ -79,24 +163,25 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
             }
             // ---- clause pass --------------------------------------------
             int unsolved = 0;
-            auto prevV = [&](int v) { return pv[v].v; };
+            auto prevV = [&](int v) { return pvr(v).v; };
             auto curV = [&](int v) {
-                BF2 a = pv[v];
+                BF2 a = pvr(v);
                 return pend ? clampV(a.v + h * a.d) : a.v;
             };
             for (int ch = 0; ch < nch; ch++) {
                 const int c_lo = MODE == BM_LDS_ALL ? I.cbound[ch] : 0;
                 const int c_hi = MODE == BM_LDS_ALL ? I.cbound[ch + 1] : m;
                 if (!general) {
-                    // 3-SAT: each thread takes its clauses four at a time, with
-                    // all four clauses' literals, memories (and LDS slots) loaded
-                    // before any is evaluated, so enough loads are in flight to
-                    // cover device-memory latency; one (V, dV) pair read per
-                    // literal serves both the memory update and the clause
+                    // 3-SAT: each thread takes its clauses four at a time.  The
+                    // next four clauses' literals, memories (and LDS slots) are
+                    // loaded while the current four are evaluated, so the
+                    // device-memory latency overlaps the arithmetic; one
+                    // (V, dV) pair read per literal serves both the memory
+                    // update and the clause.
                     constexpr int U = 4;
-                    for (int cb = c_lo + tid; cb < c_hi; cb += U * BS) {
-                        int32_t L[U][3], P[U][3];
-                        float XS[U], XL[U];
+                    int32_t L[U][3], P[U][3];
+                    float XS[U], XL[U];
+                    auto fetch = [&](int cb) {
 #pragma unroll
                         for (int u = 0; u < U; u++) {
                             const int c = cb + u * BS;
@@ -113,13 +198,30 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                                 }
                             }
                         }
+                    };
+                    fetch(c_lo + tid);
+                    for (int cb = c_lo + tid; cb < c_hi; cb += U * BS) {
+                        int32_t CL[U][3], CP[U][3];
+                        float CXS[U], CXL[U];
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                            CL[u][0] = L[u][0];
+                            CL[u][1] = L[u][1];
+                            CL[u][2] = L[u][2];
+                            CP[u][0] = P[u][0];
+                            CP[u][1] = P[u][1];
+                            CP[u][2] = P[u][2];
+                            CXS[u] = XS[u];
+                            CXL[u] = XL[u];
+                        }
+                        if (cb + U * BS < c_hi) fetch(cb + U * BS);
 #pragma unroll
                         for (int u = 0; u < U; u++) {
                             const int c = cb + u * BS;
                             if (c >= c_hi) break;
-                            const int32_t l0 = L[u][0], l1 = L[u][1], l2 = L[u][2];
-                            float xs = XS[u], xl = XL[u];
-                            const BF2 a0 = pv[abs(l0) - 1], a1 = pv[abs(l1) - 1], a2 = pv[abs(l2) - 1];
+                            const int32_t l0 = CL[u][0], l1 = CL[u][1], l2 = CL[u][2];
+                            float xs = CXS[u], xl = CXL[u];
+                            const BF2 a0 = pvr(abs(l0) - 1), a1 = pvr(abs(l1) - 1), a2 = pvr(abs(l2) - 1);
                             float r0 = a0.v, r1 = a1.v, r2 = a2.v;
                             if (pend) {
                                 memory_update(p, clause3_C_vals<float>(l0, l1, l2, r0, r1, r2), h, xs, xl);
@@ -133,10 +235,10 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                             bool sat;
                             float C;
                             if (MODE == BM_LDS_ALL) {
-                                auto emit = [&](int j, float val) { se[P[u][j]] = val; };
+                                auto emit = [&](int j, float val) { se[CP[u][j]] = val; };
                                 C = clause3_vals<float>(p, l0, l1, l2, r0, r1, r2, cxs, cxl, emit, sat);
                             } else {
-                                float *ep = I.e + 3 * c;
+                                auto ep = I.e + 3 * c;
                                 auto emit = [&](int j, float val) { ep[j] = val; };
                                 C = clause3_vals<float>(p, l0, l1, l2, r0, r1, r2, cxs, cxl, emit, sat);
                             }
@@ -147,7 +249,7 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                     for (int c = c_lo + tid; c < c_hi; c += BS) {
                         const int b = I.cp[c];
                         const int kk = I.cp[c + 1] - b;
-                        const int32_t *cl = I.lits + b;
+                        auto cl = I.lits + b;
                         float xs = I.xs[c], xl = I.xl[c];
                         if (pend) {
                             float Cp = clauseK_C<float>(cl, kk, prevV);
@@ -159,11 +261,11 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                         bool sat;
                         float C;
                         if (MODE == BM_LDS_ALL) {
-                            const int32_t *lp = I.lpos + b;
+                            auto lp = I.lpos + b;
                             auto emit = [&](int j, float val) { se[lp[j]] = val; };
                             C = clauseK<float>(p, cl, kk, cxs, cxl, curV, emit, sat);
                         } else {
-                            float *ep = I.e + b;
+                            auto ep = I.e + b;
                             auto emit = [&](int j, float val) { ep[j] = val; };
                             C = clauseK<float>(p, cl, kk, cxs, cxl, curV, emit, sat);
                         }
@@ -174,7 +276,7 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                     // fold this chunk's contributions (clause order within each
                     // variable's slots), then free se for the next chunk
                     __syncthreads();
-                    const int32_t *cbase = I.cbase + (size_t)ch * (n + 1);
+                    auto cbase = I.cbase + (size_t)ch * (n + 1);
                     // each thread's next UV variables' slot ranges are loaded
                     // together, then folded
                     for (int vb = tid; vb < n; vb += UV * BS) {
@@ -204,7 +306,7 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                 done = DNX_UNKNOWN;
             // ---- variable pass ------------------------------------------
             int fmax = 0;
-            const int32_t *rowp = MODE == BM_LDS_ALL ? I.cbase + (size_t)(nch - 1) * (n + 1) : I.row_ptr;
+            auto rowp = MODE == BM_LDS_ALL ? I.cbase + (size_t)(nch - 1) * (n + 1) : I.row_ptr;
             for (int vb = tid; vb < n; vb += UV * BS) {
                 int RB[UV], RE[UV], PIN[UV];
 #pragma unroll
@@ -218,7 +320,7 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                 for (int u = 0; u < UV; u++) {
                     const int v = vb + u * BS;
                     if (v >= n) break;
-                    BF2 a = pv[v];
+                    BF2 a = pvr(v);
                     const float V = pend ? clampV(a.v + h * a.d) : a.v;
                     float dV = 0.0f;
                     if (!done) {
@@ -234,7 +336,7 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                     BF2 o;
                     o.v = V;
                     o.d = dV;
-                    pv[v] = o;
+                    pvw(v, o);
                 }
             }
             if (done) break;
@@ -247,7 +349,7 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
         }
         __syncthreads();
         if (MODE != BM_GLOBAL)
-            for (int v = tid; v < n; v += BS) I.vd[v] = pv[v];
+            for (int v = tid; v < n; v += BS) vdw(I, v, lpv[v]);
         if (tid == 0) {
             BDyn o = d;
             o.steps = steps;
@@ -351,7 +453,7 @@ __global__ __launch_bounds__(WBS) void k_wide_clause(const BInst *__restrict__ i
         if (lb == 0 && threadIdx.x == 0) ctl->status = done;
         return;
     }
-    const BInst I = insts[slot];
+    const GInst I = global_view(insts[slot]);
     const bool pend = ctl->stepped[ps] != 0;
     const int plev = dev_level(ctl, ps);
     const float h = pend ? level_h<float>(plev) : 0.0f;
@@ -371,7 +473,7 @@ __global__ __launch_bounds__(WBS) void k_wide_clause(const BInst *__restrict__ i
     int unsolved = 0;
     if (!(I.flags & BF_GENERAL) && c < I.m) {
         const int32_t l0 = I.lits[3 * c], l1 = I.lits[3 * c + 1], l2 = I.lits[3 * c + 2];
-        const BF2 a0 = I.vd[abs(l0) - 1], a1 = I.vd[abs(l1) - 1], a2 = I.vd[abs(l2) - 1];
+        const BF2 a0 = vdr(I, abs(l0) - 1), a1 = vdr(I, abs(l1) - 1), a2 = vdr(I, abs(l2) - 1);
         float xs = I.xs[c], xl = I.xl[c];
         float r0 = a0.v, r1 = a1.v, r2 = a2.v;
         if (pend) {
@@ -382,7 +484,7 @@ __global__ __launch_bounds__(WBS) void k_wide_clause(const BInst *__restrict__ i
             r1 = clampV(a1.v + h * a1.d);
             r2 = clampV(a2.v + h * a2.d);
         }
-        float *ep = I.e + 3 * c;
+        auto ep = I.e + 3 * c;
         auto emit = [&](int j, float val) { ep[j] = val; };
         bool sat;
         float C = clause3_vals<float>(p, l0, l1, l2, r0, r1, r2, clamp01(xs), clampXl(xl, p.xlmax), emit, sat);
@@ -390,11 +492,11 @@ __global__ __launch_bounds__(WBS) void k_wide_clause(const BInst *__restrict__ i
     } else if ((I.flags & BF_GENERAL) && c < I.m) {
         const int bb = I.cp[c];
         const int k = I.cp[c + 1] - bb;
-        const int32_t *cl = I.lits + bb;
+        auto cl = I.lits + bb;
         float xs = I.xs[c], xl = I.xl[c];
-        auto prevV = [&](int v) { return I.vd[v].v; };
+        auto prevV = [&](int v) { return vdr(I, v).v; };
         auto curV = [&](int v) {
-            BF2 a = I.vd[v];
+            BF2 a = vdr(I, v);
             return pend ? clampV(a.v + h * a.d) : a.v;
         };
         if (pend) {
@@ -402,7 +504,7 @@ __global__ __launch_bounds__(WBS) void k_wide_clause(const BInst *__restrict__ i
             I.xs[c] = xs;
             I.xl[c] = xl;
         }
-        float *ep = I.e + bb;
+        auto ep = I.e + bb;
         auto emit = [&](int j, float val) { ep[j] = val; };
         bool sat;
         float C = clauseK<float>(p, cl, k, clamp01(xs), clampXl(xl, p.xlmax), curV, emit, sat);
@@ -424,7 +526,7 @@ __global__ __launch_bounds__(WBS) void k_wide_var(const BInst *__restrict__ inst
     if (ctl->status != DNX_RUNNING) return;
     const int ps = (r + 2) % 3;
     if (ctl->done[ps]) return;
-    const BInst I = insts[slot];
+    const GInst I = global_view(insts[slot]);
     const bool pend = ctl->stepped[ps] != 0;
     const float h = pend ? level_h<float>(dev_level(ctl, ps)) : 0.0f;
     const bool solved = ctl->stop_on_solve && !any_flag(ctl->flags[r][FG_UNSOLVED]);
@@ -435,7 +537,7 @@ __global__ __launch_bounds__(WBS) void k_wide_var(const BInst *__restrict__ inst
     const float dVf = wave_fold<float, false, WCAPW>(I.e, I.eidx, I.row_ptr, I.n, v0, !done, stage[w], nullptr);
     int f = 0;
     if (v < I.n) {
-        BF2 a = I.vd[v];
+        BF2 a = vdr(I, v);
         const float V = pend ? clampV(a.v + h * a.d) : a.v;
         float dV = 0.0f;
         if (!done) {
@@ -446,7 +548,7 @@ __global__ __launch_bounds__(WBS) void k_wide_var(const BInst *__restrict__ inst
         BF2 o;
         o.v = V;
         o.d = dV;
-        I.vd[v] = o;
+        vdw(I, v, o);
     }
     const int ns = (r + 1) % 3;
     if (done) {
@@ -577,7 +679,7 @@ __global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, B
     if (slot < 0) return;
     const BDyn d = dyn[slot];
     if (d.status != DNX_RUNNING) return;
-    const BInst I = insts[slot];
+    const GInst I = global_view(insts[slot]);
     const int n = I.n, m = I.m, tid = threadIdx.x;
     const bool general = (I.flags & BF_GENERAL) != 0;
     PT<float> p;
@@ -622,7 +724,7 @@ __global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, B
             float C;
             if (!general) {
                 const int32_t l0 = s_lits[3 * i], l1 = s_lits[3 * i + 1], l2 = s_lits[3 * i + 2];
-                const BF2 a0 = I.vd[abs(l0) - 1], a1 = I.vd[abs(l1) - 1], a2 = I.vd[abs(l2) - 1];
+                const BF2 a0 = vdr(I, abs(l0) - 1), a1 = vdr(I, abs(l1) - 1), a2 = vdr(I, abs(l2) - 1);
                 float r0 = a0.v, r1 = a1.v, r2 = a2.v;
                 if (pend) {
                     memory_update(p, clause3_C_vals<float>(l0, l1, l2, r0, r1, r2), h, xs, xl);
@@ -632,16 +734,16 @@ __global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, B
                     r1 = clampV(a1.v + h * a1.d);
                     r2 = clampV(a2.v + h * a2.d);
                 }
-                float *ep = I.e + 3 * c;
+                auto ep = I.e + 3 * c;
                 auto emit = [&](int j, float val) { ep[j] = val; };
                 C = clause3_vals<float>(p, l0, l1, l2, r0, r1, r2, clamp01(xs), clampXl(xl, p.xlmax), emit, sat);
             } else {
                 const int bb = I.cp[c];
                 const int kk = I.cp[c + 1] - bb;
-                const int32_t *cl = I.lits + bb;
-                auto prevV = [&](int v) { return I.vd[v].v; };
+                auto cl = I.lits + bb;
+                auto prevV = [&](int v) { return vdr(I, v).v; };
                 auto curV = [&](int v) {
-                    BF2 a = I.vd[v];
+                    BF2 a = vdr(I, v);
                     return pend ? clampV(a.v + h * a.d) : a.v;
                 };
                 if (pend) {
@@ -649,7 +751,7 @@ __global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, B
                     s_xs[i] = xs;
                     s_xl[i] = xl;
                 }
-                float *ep = I.e + bb;
+                auto ep = I.e + bb;
                 auto emit = [&](int j, float val) { ep[j] = val; };
                 C = clauseK<float>(p, cl, kk, clamp01(xs), clampXl(xl, p.xlmax), curV, emit, sat);
             }
@@ -668,7 +770,7 @@ __global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, B
             const float dVf = wave_fold<float, false, WCAPW>(I.e, I.eidx, I.row_ptr, n, v0, !done, stage[w], nullptr);
             const int v = v0 + (tid & 63);
             if (v < n) {
-                BF2 a = I.vd[v];
+                BF2 a = vdr(I, v);
                 const float V = pend ? clampV(a.v + h * a.d) : a.v;
                 float dV = 0.0f;
                 if (!done) {
@@ -679,7 +781,7 @@ __global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, B
                 BF2 o;
                 o.v = V;
                 o.d = dV;
-                I.vd[v] = o;
+                vdw(I, v, o);
             }
         }
         if (done) break;
@@ -743,14 +845,14 @@ __global__ void k_batch_stamp(unsigned long long *p)
 __global__ __launch_bounds__(256) void k_batch_materialize(const BInst *__restrict__ ip, const BDyn *__restrict__ dp,
                                                            float *__restrict__ xo)
 {
-    const BInst I = *ip;
+    const GInst I = global_view(*ip);
     const int i = blockIdx.x * 256 + threadIdx.x;
     const int n = I.n, m = I.m;
     const int plev = dp->status == DNX_RUNNING ? dp->plevel : -1;
     const bool pend = plev >= 0;
     const float h = pend ? level_h<float>(plev) : 0.0f;
     if (i < n) {
-        BF2 a = I.vd[i];
+        BF2 a = vdr(I, i);
         xo[i] = pend ? clampV(a.v + h * a.d) : a.v;
     }
     if (i < m) {
@@ -766,7 +868,7 @@ __global__ __launch_bounds__(256) void k_batch_materialize(const BInst *__restri
             p.xlmax = I.xlmax;
             const bool general = (I.flags & BF_GENERAL) != 0;
             const int b = general ? I.cp[i] : 3 * i;
-            auto getV = [&](int v) { return I.vd[v].v; };
+            auto getV = [&](int v) { return vdr(I, v).v; };
             float Cp = general ? clauseK_C<float>(I.lits + b, I.cp[i + 1] - b, getV)
                                : clause3_C<float>(I.lits + b, getV);
             memory_update(p, Cp, h, xs, xl);

@@ -14,7 +14,7 @@ namespace dnx {
 //   2  both in device memory (instances too large for LDS)
 enum BatchMode { BM_LDS_ALL = 0, BM_LDS_V = 1, BM_GLOBAL = 2, BM_COUNT = 3 };
 
-struct BF2 {
+struct alignas(8) BF2 {
     float v, d;
 };
 

@@ -41,12 +41,21 @@ static inline PT<T> to_pt(const DevParams &p)
     return r;
 }
 
+// Clamps and min.  The reference's compare-and-select (boost clamp,
+// dynex.cc:313-324) equals the hardware min/max on every value the
+// integrator can hold: the state is finite, and the one case where they part
+// (a signed zero against a zero bound) cannot occur -- V's bounds are +-1, and
+// the clause terms i = 1 - q V are never -0 (1 - 1 = +0).  clamp01 keeps the
+// compare form, since Xs could in principle start at -0.
+__device__ __forceinline__ float dmin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ float dmax(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ double dmin(double a, double b) { return fmin(a, b); }
+__device__ __forceinline__ double dmax(double a, double b) { return fmax(a, b); }
+
 template <typename T>
 __device__ __forceinline__ T clampV(T v)
 {
-    if (v < T(-1.0)) v = T(-1.0);
-    if (v > T(1.0)) v = T(1.0);
-    return v;
+    return dmin(dmax(v, T(-1.0)), T(1.0));
 }
 template <typename T>
 __device__ __forceinline__ T clamp01(T v)
@@ -58,14 +67,12 @@ __device__ __forceinline__ T clamp01(T v)
 template <typename T>
 __device__ __forceinline__ T clampXl(T v, T xlmax)
 {
-    if (v < T(1.0)) v = T(1.0);
-    if (v > xlmax) v = xlmax;
-    return v;
+    return dmin(dmax(v, T(1.0)), xlmax);
 }
 template <typename T>
 __device__ __forceinline__ T tmin(T a, T b)
 {
-    return b < a ? b : a;
+    return dmin(a, b);
 }
 
 // step size of ladder level L (the fallback level 4 keeps the 1/64 state)
@@ -169,8 +176,8 @@ __device__ __forceinline__ T clause3v(const PT<T> &p, int32_t l0, int32_t l1, in
                            sat);
 }
 
-template <typename T, typename GetV, typename Emit>
-__device__ __forceinline__ T clause3(const PT<T> &p, const int32_t *cl, T Xs, T Xl, GetV getV, Emit emit, bool &sat)
+template <typename T, typename GetV, typename Emit, typename CL>
+__device__ __forceinline__ T clause3(const PT<T> &p, CL cl, T Xs, T Xl, GetV getV, Emit emit, bool &sat)
 {
     return clause3v<T>(p, cl[0], cl[1], cl[2], Xs, Xl, getV, emit, sat);
 }
@@ -181,8 +188,8 @@ __device__ __forceinline__ T clause3v_C(int32_t l0, int32_t l1, int32_t l2, GetV
 {
     return clause3_C_vals<T>(l0, l1, l2, getV(abs(l0) - 1), getV(abs(l1) - 1), getV(abs(l2) - 1));
 }
-template <typename T, typename GetV>
-__device__ __forceinline__ T clause3_C(const int32_t *cl, GetV getV)
+template <typename T, typename GetV, typename CL>
+__device__ __forceinline__ T clause3_C(CL cl, GetV getV)
 {
     return clause3v_C<T>(cl[0], cl[1], cl[2], getV);
 }
@@ -191,9 +198,8 @@ __device__ __forceinline__ T clause3_C(const int32_t *cl, GetV getV)
 // empty clause follows the k-SAT branch with min = INT_MAX).  Two passes over
 // the literals keep everything in registers: the first finds the smallest
 // i_l (and its first index) and the smallest of the rest; the second emits.
-template <typename T, typename GetV, typename Emit>
-__device__ __forceinline__ T clauseK(const PT<T> &p, const int32_t *cl, int k, T Xs, T Xl,
-                                     GetV getV, Emit emit, bool &sat)
+template <typename T, typename GetV, typename Emit, typename CL>
+__device__ __forceinline__ T clauseK(const PT<T> &p, CL cl, int k, T Xs, T Xl, GetV getV, Emit emit, bool &sat)
 {
     sat = false;
     if (k == 1) {
@@ -228,8 +234,8 @@ __device__ __forceinline__ T clauseK(const PT<T> &p, const int32_t *cl, int k, T
     return C;
 }
 
-template <typename T, typename GetV>
-__device__ __forceinline__ T clauseK_C(const int32_t *cl, int k, GetV getV)
+template <typename T, typename GetV, typename CL>
+__device__ __forceinline__ T clauseK_C(CL cl, int k, GetV getV)
 {
     if (k == 1) return T(0.0);
     T min1 = T(INT_MAX);
@@ -246,8 +252,8 @@ __device__ __forceinline__ T clauseK_C(const int32_t *cl, int k, GetV getV)
 // slots [rb, re) of a variable name literal positions eidx[q] whose values
 // sit in e[].  Eight independent gathers are kept in flight; the additions
 // stay in slot order.
-template <typename T>
-__device__ __forceinline__ T fold_gather(const T *__restrict__ e, const int32_t *__restrict__ eidx, int rb, int re)
+template <typename T, typename EP, typename IP>
+__device__ __forceinline__ T fold_gather(EP e, IP eidx, int rb, int re)
 {
     T acc = T(0.0);
     int q = rb;
@@ -346,9 +352,8 @@ __device__ __forceinline__ void clear_slot(Ctl *c, int s)
 // holds the literal position eidx[q] whose contribution e[] carries (the
 // clause pass writes e in literal order, coalesced).
 // ---------------------------------------------------------------------------
-template <typename T, bool SORT, int BS, int CAP>
-__device__ __forceinline__ T fold_var(const T *__restrict__ e, const int32_t *__restrict__ eidx, int b, int deg,
-                                      T *buf)
+template <typename T, bool SORT, int BS, int CAP, typename EP, typename IP>
+__device__ __forceinline__ T fold_var(EP e, IP eidx, int b, int deg, T *buf)
 {
     T acc = T(0.0);
     if (!SORT) return fold_gather<T>(e, eidx, b, b + deg);
@@ -411,9 +416,8 @@ __device__ __forceinline__ T fold_var(const T *__restrict__ e, const int32_t *__
 // first in FP64 -- exactly as a lane-serial fold would.  A wave whose
 // segment exceeds the staging area folds from device memory instead.
 // Must be reached by every thread of the block (it synchronises).
-template <typename T, bool SORT, int CAPW>
-__device__ __forceinline__ T wave_fold(const T *__restrict__ e, const int32_t *__restrict__ eidx,
-                                       const int32_t *__restrict__ row_ptr, int n, int v0, bool active, T *stage,
+template <typename T, bool SORT, int CAPW, typename EP, typename IP>
+__device__ __forceinline__ T wave_fold(EP e, IP eidx, IP row_ptr, int n, int v0, bool active, T *stage,
                                        T *sortbuf_unused)
 {
     (void)sortbuf_unused;
