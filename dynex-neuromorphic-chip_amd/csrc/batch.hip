@@ -41,6 +41,7 @@ struct GInst {
     GLB float *e;
     GLB const int32_t *cbound;
     GLB const int32_t *cbase;
+    GLB float *cst;
     int nchunk;
     float alpha, beta, gamma, delta, eps, zeta, xlmax;
 };
@@ -63,6 +64,7 @@ __device__ __forceinline__ GInst global_view(const BInst &B)
     I.e = (GLB float *)B.e;
     I.cbound = (GLB const int32_t *)B.cbound;
     I.cbase = (GLB const int32_t *)B.cbase;
+    I.cst = (GLB float *)B.cst;
     I.nchunk = B.nchunk;
     I.alpha = B.alpha;
     I.beta = B.beta;
@@ -134,6 +136,10 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                 return r;
             }
         };
+        auto pvv = [&](int i) -> float {
+            if constexpr (MODE == BM_GLOBAL) return vdr(I, i).v;
+            else return lpv[i].v;
+        };
         auto pvw = [&](int i, BF2 val) {
             if constexpr (MODE == BM_GLOBAL) vdw(I, i, val);
             else lpv[i] = val;
@@ -151,6 +157,11 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
             __syncthreads();
         }
         bool pend = d.plevel >= 0;
+        // mat: V of the current state is already in place (every step of a
+        // launch after its first), and cst holds each clause's C of the
+        // previous state, so the clause pass needs neither h*dV nor a second
+        // evaluation of C for the memory update
+        bool mat = false;
         int plev = d.plevel;
         long long steps = d.steps;
         double t = d.t;
@@ -165,6 +176,7 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
             int unsolved = 0;
             auto prevV = [&](int v) { return pvr(v).v; };
             auto curV = [&](int v) {
+                if (mat) return pvv(v);
                 BF2 a = pvr(v);
                 return pend ? clampV(a.v + h * a.d) : a.v;
             };
@@ -172,16 +184,14 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                 const int c_lo = MODE == BM_LDS_ALL ? I.cbound[ch] : 0;
                 const int c_hi = MODE == BM_LDS_ALL ? I.cbound[ch + 1] : m;
                 if (!general) {
-                    // 3-SAT: each thread takes its clauses four at a time.  The
-                    // next four clauses' literals, memories (and LDS slots) are
-                    // loaded while the current four are evaluated, so the
-                    // device-memory latency overlaps the arithmetic; one
-                    // (V, dV) pair read per literal serves both the memory
-                    // update and the clause.
+                    // 3-SAT: each thread takes its clauses four at a time, with
+                    // all four clauses' literals, memories, stored C (and LDS
+                    // slots) loaded before any is evaluated, so that enough
+                    // loads are in flight to cover device-memory latency.
                     constexpr int U = 4;
-                    int32_t L[U][3], P[U][3];
-                    float XS[U], XL[U];
-                    auto fetch = [&](int cb) {
+                    for (int cb = c_lo + tid; cb < c_hi; cb += U * BS) {
+                        int32_t L[U][3], P[U][3];
+                        float XS[U], XL[U], CX[U];
 #pragma unroll
                         for (int u = 0; u < U; u++) {
                             const int c = cb + u * BS;
@@ -191,6 +201,7 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                                 L[u][2] = I.lits[3 * c + 2];
                                 XS[u] = I.xs[c];
                                 XL[u] = I.xl[c];
+                                if (mat) CX[u] = I.cst[c];
                                 if (MODE == BM_LDS_ALL) {
                                     P[u][0] = I.lpos[3 * c];
                                     P[u][1] = I.lpos[3 * c + 1];
@@ -198,50 +209,50 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                                 }
                             }
                         }
-                    };
-                    fetch(c_lo + tid);
-                    for (int cb = c_lo + tid; cb < c_hi; cb += U * BS) {
-                        int32_t CL[U][3], CP[U][3];
-                        float CXS[U], CXL[U];
-#pragma unroll
-                        for (int u = 0; u < U; u++) {
-                            CL[u][0] = L[u][0];
-                            CL[u][1] = L[u][1];
-                            CL[u][2] = L[u][2];
-                            CP[u][0] = P[u][0];
-                            CP[u][1] = P[u][1];
-                            CP[u][2] = P[u][2];
-                            CXS[u] = XS[u];
-                            CXL[u] = XL[u];
-                        }
-                        if (cb + U * BS < c_hi) fetch(cb + U * BS);
 #pragma unroll
                         for (int u = 0; u < U; u++) {
                             const int c = cb + u * BS;
                             if (c >= c_hi) break;
-                            const int32_t l0 = CL[u][0], l1 = CL[u][1], l2 = CL[u][2];
-                            float xs = CXS[u], xl = CXL[u];
-                            const BF2 a0 = pvr(abs(l0) - 1), a1 = pvr(abs(l1) - 1), a2 = pvr(abs(l2) - 1);
-                            float r0 = a0.v, r1 = a1.v, r2 = a2.v;
-                            if (pend) {
-                                memory_update(p, clause3_C_vals<float>(l0, l1, l2, r0, r1, r2), h, xs, xl);
+                            const int32_t l0 = L[u][0], l1 = L[u][1], l2 = L[u][2];
+                            float xs = XS[u], xl = XL[u];
+                            float r0, r1, r2;
+                            if (mat) {
+                                // V of this state is in place; the previous
+                                // state's C was stored by the previous pass
+                                r0 = pvv(abs(l0) - 1);
+                                r1 = pvv(abs(l1) - 1);
+                                r2 = pvv(abs(l2) - 1);
+                                memory_update(p, CX[u], h, xs, xl);
                                 I.xs[c] = xs;
                                 I.xl[c] = xl;
-                                r0 = clampV(a0.v + h * a0.d);
-                                r1 = clampV(a1.v + h * a1.d);
-                                r2 = clampV(a2.v + h * a2.d);
+                            } else {
+                                // one (V, dV) pair read per literal serves both
+                                // the memory update and the clause
+                                const BF2 a0 = pvr(abs(l0) - 1), a1 = pvr(abs(l1) - 1), a2 = pvr(abs(l2) - 1);
+                                r0 = a0.v;
+                                r1 = a1.v;
+                                r2 = a2.v;
+                                if (pend) {
+                                    memory_update(p, clause3_C_vals<float>(l0, l1, l2, r0, r1, r2), h, xs, xl);
+                                    I.xs[c] = xs;
+                                    I.xl[c] = xl;
+                                    r0 = clampV(a0.v + h * a0.d);
+                                    r1 = clampV(a1.v + h * a1.d);
+                                    r2 = clampV(a2.v + h * a2.d);
+                                }
                             }
                             const float cxs = clamp01(xs), cxl = clampXl(xl, p.xlmax);
                             bool sat;
                             float C;
                             if (MODE == BM_LDS_ALL) {
-                                auto emit = [&](int j, float val) { se[CP[u][j]] = val; };
+                                auto emit = [&](int j, float val) { se[P[u][j]] = val; };
                                 C = clause3_vals<float>(p, l0, l1, l2, r0, r1, r2, cxs, cxl, emit, sat);
                             } else {
                                 auto ep = I.e + 3 * c;
                                 auto emit = [&](int j, float val) { ep[j] = val; };
                                 C = clause3_vals<float>(p, l0, l1, l2, r0, r1, r2, cxs, cxl, emit, sat);
                             }
+                            I.cst[c] = C;
                             unsolved |= !(C < 0.5f) || !sat;
                         }
                     }
@@ -252,7 +263,7 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                         auto cl = I.lits + b;
                         float xs = I.xs[c], xl = I.xl[c];
                         if (pend) {
-                            float Cp = clauseK_C<float>(cl, kk, prevV);
+                            float Cp = mat ? I.cst[c] : clauseK_C<float>(cl, kk, prevV);
                             memory_update(p, Cp, h, xs, xl);
                             I.xs[c] = xs;
                             I.xl[c] = xl;
@@ -269,6 +280,7 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                             auto emit = [&](int j, float val) { ep[j] = val; };
                             C = clauseK<float>(p, cl, kk, cxs, cxl, curV, emit, sat);
                         }
+                        I.cst[c] = C;
                         unsolved |= !(C < 0.5f) || !sat;
                     }
                 }
@@ -321,7 +333,7 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                     const int v = vb + u * BS;
                     if (v >= n) break;
                     BF2 a = pvr(v);
-                    const float V = pend ? clampV(a.v + h * a.d) : a.v;
+                    const float V = pend && !mat ? clampV(a.v + h * a.d) : a.v;
                     float dV = 0.0f;
                     if (!done) {
                         if (MODE == BM_LDS_ALL) {
@@ -346,6 +358,19 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
             plev = s_lev[k & 1];
             pend = true;
             if (tid == 0) s_lev[(k + 1) & 1] = 0;
+            if (k + 1 < K) {
+                // another step follows in this launch: put V of the next
+                // state in place (the last step of a launch leaves the pending
+                // form, V and dV with the level, which every path resumes from)
+                const float hn = level_h<float>(plev);
+                for (int v = tid; v < n; v += BS) {
+                    BF2 a = pvr(v);
+                    a.v = clampV(a.v + hn * a.d);
+                    pvw(v, a);
+                }
+                __syncthreads();
+                mat = true;
+            }
         }
         __syncthreads();
         if (MODE != BM_GLOBAL)

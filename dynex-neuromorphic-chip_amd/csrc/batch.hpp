@@ -37,6 +37,7 @@ struct BInst {
     // (one chunk: cbase == row_ptr)
     const int32_t *cbound;
     const int32_t *cbase;
+    float *cst;               // fused kernel: each clause's C of the last state it evaluated
     int nchunk, pad0;
     float alpha, beta, gamma, delta, eps, zeta, xlmax, pad;
 };

@@ -250,7 +250,7 @@ static int choose_mode(const dnx_batch *b, const HostProblem &h, Mode0Plan &P)
 
 // Byte layout of one instance inside its arena range.
 struct Layout {
-    size_t lits, cp, lpos, eidx, row, unit, xs, xl, vd, e, cbound, cbase, total;
+    size_t lits, cp, lpos, eidx, row, unit, xs, xl, vd, e, cbound, cbase, cst, total;
 };
 static Layout layout_of(int n, int m, int64_t nnz, bool general, bool has_unit, int nchunk)
 {
@@ -275,6 +275,7 @@ static Layout layout_of(int n, int m, int64_t nnz, bool general, bool has_unit, 
     L.e = take((size_t)nnz * 4);
     L.cbound = take(((size_t)nchunk + 1) * 4);
     L.cbase = nchunk > 1 ? take((size_t)nchunk * ((size_t)n + 1) * 4) : 0;
+    L.cst = take((size_t)m * 4);
     L.total = o ? o : 256;
     return L;
 }
@@ -601,6 +602,7 @@ int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *p
     I.e = (float *)(base + L.e);
     I.cbound = (const int32_t *)(base + L.cbound);
     I.cbase = nchunk > 1 ? (const int32_t *)(base + L.cbase) : I.row_ptr;
+    I.cst = (float *)(base + L.cst);
     I.nchunk = nchunk;
     I.alpha = (float)prm->alpha;
     I.beta = (float)prm->beta;

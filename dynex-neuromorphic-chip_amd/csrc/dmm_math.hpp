@@ -125,9 +125,13 @@ __device__ __forceinline__ T contrib(const PT<T> &p, T Xs, T Xl, T G, T R)
     return a + b;
 }
 
-__device__ __forceinline__ bool lit_true(int32_t lit, double x)
+// The exact sign check of one literal (dynex.cc:674-687) on the raw value,
+// in the state's own precision (a float's sign compares the same as its
+// double's).
+template <typename T>
+__device__ __forceinline__ bool lit_true(int32_t lit, T x)
 {
-    return (x > 0.0 && lit > 0) || (x < 0.0 && lit < 0);
+    return (x > T(0.0) && lit > 0) || (x < T(0.0) && lit < 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -140,7 +144,7 @@ template <typename T, typename Emit>
 __device__ __forceinline__ T clause3_vals(const PT<T> &p, int32_t l0, int32_t l1, int32_t l2, T r0, T r1, T r2, T Xs,
                                           T Xl, Emit emit, bool &sat)
 {
-    sat = lit_true(l0, (double)r0) || lit_true(l1, (double)r1) || lit_true(l2, (double)r2);
+    sat = lit_true(l0, r0) || lit_true(l1, r1) || lit_true(l2, r2);
     T q0 = l0 > 0 ? T(1) : T(-1), q1 = l1 > 0 ? T(1) : T(-1), q2 = l2 > 0 ? T(1) : T(-1);
     T V0 = clampV(r0), V1 = clampV(r1), V2 = clampV(r2);
     T i0 = T(1.0) - q0 * V0, i1 = T(1.0) - q1 * V1, i2 = T(1.0) - q2 * V2;
@@ -203,7 +207,7 @@ __device__ __forceinline__ T clauseK(const PT<T> &p, CL cl, int k, T Xs, T Xl, G
 {
     sat = false;
     if (k == 1) {
-        sat = lit_true(cl[0], (double)getV(abs(cl[0]) - 1));
+        sat = lit_true(cl[0], getV(abs(cl[0]) - 1));
         emit(0, T(0.0));
         return T(0.0);
     }
@@ -211,7 +215,7 @@ __device__ __forceinline__ T clauseK(const PT<T> &p, CL cl, int k, T Xs, T Xl, G
     int idx1 = -1;
     for (int j = 0; j < k; j++) {
         T r = getV(abs(cl[j]) - 1);
-        sat = sat || lit_true(cl[j], (double)r);
+        sat = sat || lit_true(cl[j], r);
         T I = T(1.0) - (cl[j] > 0 ? T(1) : T(-1)) * clampV(r);
         if (I < min1) {
             if (idx1 >= 0) min2 = tmin(min2, min1);
