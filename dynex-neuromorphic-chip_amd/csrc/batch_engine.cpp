@@ -86,6 +86,7 @@ struct SlotHost {
     int n = 0, m = 0;
     int64_t nnz = 0;
     int mode = 0;
+    int nchunk = 1;  // mode 0: LDS chunks of its contributions
     size_t lds = 0;  // dynamic LDS bytes the fused kernel needs for it
     size_t off = 0, size = 0;
     std::vector<int32_t> cp, lits;  // kept for the host-side clause check
@@ -128,6 +129,7 @@ struct dnx_batch {
     bool ev_pending[2] = {false, false};
     hipStream_t mstream[BM_COUNT] = {nullptr, nullptr, nullptr};
     hipEvent_t fork = nullptr, join[BM_COUNT] = {nullptr, nullptr, nullptr};
+    hipEvent_t side_done = nullptr;  // end of a tail launch of the fused kernel on its own stream
     // admission uploads run on their own stream and complete before admit
     // returns, so they never wait behind queued kernels
     hipStream_t ustream = nullptr;
@@ -155,6 +157,9 @@ struct dnx_batch {
     // portfolio: dnx_batch_run returns at the first status sync at which
     // any slot is satisfied
     bool group_stop = false;
+    // tail: single-chunk mode-0 instances run in the fused kernel beside
+    // wide mode (batch_split_tail)
+    bool tail_split = true;
     DevBuf b_wctl, b_wlist, b_bmap, b_vmap;
     // gang mode: taken for the tail when the running instances need at
     // most gang_blocks workgroups in total (0 disables it)
@@ -181,6 +186,7 @@ struct dnx_batch {
             if (t1[i]) (void)hipEventDestroy(t1[i]);
         }
         if (fork) (void)hipEventDestroy(fork);
+        if (side_done) (void)hipEventDestroy(side_done);
         if (ustream) (void)hipStreamDestroy(ustream);
     }
 };
@@ -429,6 +435,7 @@ int dnx_batch_open(dnx_ctx *ctx, int prec, int slots, int64_t arena_bytes, dnx_b
             return hip_fail(e, "event");
         }
     if ((e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&b->side_done, hipEventDisableTiming)) != hipSuccess ||
         (e = hipStreamCreateWithFlags(&b->ustream, hipStreamNonBlocking)) != hipSuccess) {
         delete b;
         return hip_fail(e, "event");
@@ -640,6 +647,7 @@ int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *p
     sl.m = m;
     sl.nnz = nnz;
     sl.mode = mode;
+    sl.nchunk = nchunk;
     sl.lds = plan.lds;
     sl.off = off;
     sl.size = L.total;
@@ -713,12 +721,14 @@ static bool group_done(const dnx_batch *b)
 
 // Runs every running slot in wide mode until none runs or the wall-clock
 // limit passes.  The status table in h_dyn (buffer 0) is current on return.
-static int batch_wide_run(dnx_batch *b, double walltime_s, std::chrono::steady_clock::time_point t0)
+static int batch_wide_run(dnx_batch *b, double walltime_s, std::chrono::steady_clock::time_point t0,
+                          const std::vector<char> *skip = nullptr)
 {
     hipStream_t st = b->ctx->stream;
     std::vector<int> list;
     for (int s = 0; s < b->nslots; s++)
-        if (b->slots[(size_t)s].used && b->h_dyn[s].status == DNX_RUNNING) list.push_back(s);
+        if (b->slots[(size_t)s].used && b->h_dyn[s].status == DNX_RUNNING && !(skip && (*skip)[(size_t)s]))
+            list.push_back(s);
     if (list.empty()) return DNX_OK;
     std::sort(list.begin(), list.end(), [&](int x, int y) {
         return b->slots[(size_t)x].m != b->slots[(size_t)y].m ? b->slots[(size_t)x].m > b->slots[(size_t)y].m
@@ -889,6 +899,94 @@ static int batch_gang_run(dnx_batch *b, double walltime_s, std::chrono::steady_c
     return DNX_OK;
 }
 
+// The tail split: single-chunk mode-0 instances (all their state in LDS) step
+// faster in one workgroup than spread over the chip, the others faster in
+// wide mode.  This launches the fused kernel for the first kind on its own
+// stream, running each of them until it ends; wide mode then advances the
+// rest on the context stream while it runs (the fused kernel holds one CU per
+// instance and leaves the others to the wide kernels).  side_done marks its
+// end; until then the status snapshots hold no valid entry for its slots.
+static int batch_side_narrow(dnx_batch *b, const std::vector<int> &slots, int ksteps)
+{
+    const int S = b->nslots;
+    // work-list region 1 is free: the fused kernel is not otherwise launched
+    // while the tail runs
+    int *hw = b->h_work + (size_t)BM_COUNT * S;
+    int *dw = (int *)b->b_work.ptr + (size_t)BM_COUNT * S;
+    int *dh = (int *)b->b_head.ptr + BM_COUNT;
+    std::vector<int> order(slots);
+    std::sort(order.begin(), order.end(), [&](int x, int y) {
+        return b->slots[(size_t)x].m != b->slots[(size_t)y].m ? b->slots[(size_t)x].m > b->slots[(size_t)y].m
+                                                                : x < y;
+    });
+    int lds = 0;
+    for (size_t i = 0; i < order.size(); i++) {
+        hw[i] = order[i];
+        lds = std::max(lds, (int)b->slots[(size_t)order[i]].lds);
+    }
+    const int cnt = (int)order.size();
+    hipStream_t ms = b->mstream[BM_LDS_ALL];
+    hipError_t e;
+    if ((e = hipMemcpyAsync(dw, hw, sizeof(int) * cnt, hipMemcpyHostToDevice, ms)) != hipSuccess)
+        return hip_fail(e, "tail work list upload");
+    launch_batch_zero(dh, 1, ms);
+    const int bs = batch_max_threads();
+    int per_cu = batch_occupancy(BM_LDS_ALL, bs, lds);
+    if (per_cu < 1) per_cu = 1;
+    // a quarter of the CUs stays free for the wide kernels
+    const int grid = std::min(cnt, std::max(1, b->cus * per_cu - b->cus / 4));
+    const bool timed = b->timing && !b->t_pending[BM_LDS_ALL];
+    if (timed && (e = hipEventRecord(b->t0[BM_LDS_ALL], ms)) != hipSuccess) return hip_fail(e, "timing");
+    launch_batch(BM_LDS_ALL, bs, lds, grid, (const BInst *)b->b_inst.ptr, (BDyn *)b->b_dyn.ptr, dw, cnt, dh, ksteps,
+                 ms);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "tail launch");
+    if (timed) {
+        if ((e = hipEventRecord(b->t1[BM_LDS_ALL], ms)) != hipSuccess) return hip_fail(e, "timing");
+        b->t_pending[BM_LDS_ALL] = true;
+    }
+    if ((e = hipEventRecord(b->side_done, ms)) != hipSuccess) return hip_fail(e, "tail event");
+    return DNX_OK;
+}
+
+// The tail with the split above: returns when every running slot has ended
+// (the status table in h_dyn, buffer 0, is then current).  *used is false
+// when there is no single-chunk mode-0 instance to split off.
+static int batch_split_tail(dnx_batch *b, std::chrono::steady_clock::time_point t0, bool *used)
+{
+    *used = false;
+    std::vector<int> nset;
+    std::vector<char> skip((size_t)b->nslots, 0);
+    for (int s = 0; s < b->nslots; s++) {
+        const SlotHost &sl = b->slots[(size_t)s];
+        if (sl.used && b->h_dyn[s].status == DNX_RUNNING && sl.mode == BM_LDS_ALL && sl.nchunk == 1) {
+            nset.push_back(s);
+            skip[(size_t)s] = 1;
+        }
+    }
+    if (nset.empty()) return DNX_OK;
+    *used = true;
+    std::vector<long long> before(nset.size());
+    for (size_t i = 0; i < nset.size(); i++) before[i] = b->h_dyn[nset[i]].steps;
+    int rc = batch_side_narrow(b, nset, INT_MAX);
+    if (rc) return rc;
+    for (;;) {
+        bool wide_running = false;
+        for (int s = 0; s < b->nslots; s++)
+            if (!skip[(size_t)s] && b->slots[(size_t)s].used && b->h_dyn[s].status == DNX_RUNNING)
+                wide_running = true;
+        if (!wide_running) break;
+        if ((rc = batch_wide_run(b, 0.0, t0, &skip))) return rc;
+    }
+    hipError_t e;
+    if ((e = hipStreamWaitEvent(b->ctx->stream, b->side_done, 0)) != hipSuccess) return hip_fail(e, "tail join");
+    if ((rc = batch_snapshot(b, 0)) || (rc = batch_wait(b, 0))) return rc;
+    batch_collect_timing(b);
+    if (b->timing)
+        for (size_t i = 0; i < nset.size(); i++)
+            b->t_alg[BM_LDS_ALL] += (double)(b->h_dyn[nset[i]].steps - before[i]) * step_bytes(b->slots[(size_t)nset[i]]);
+    return DNX_OK;
+}
+
 int dnx_batch_run(dnx_batch *b, int steps_per_launch, double walltime_s, int *remaining)
 {
     if (!b || steps_per_launch <= 0) return DNX_E_ARG;
@@ -915,9 +1013,12 @@ int dnx_batch_run(dnx_batch *b, int steps_per_launch, double walltime_s, int *re
         for (int s = 0; s < b->nslots; s++)
             if (b->slots[(size_t)s].used && known[s].status == DNX_RUNNING) clauses += b->slots[(size_t)s].m;
         if (clauses <= threshold) {
-            bool gang = false;
-            if ((rc = batch_gang_run(b, walltime_s, t0, &gang))) return rc;
-            if (!gang && (rc = batch_wide_run(b, walltime_s, t0))) return rc;
+            bool used = false;
+            if ((rc = batch_gang_run(b, walltime_s, t0, &used))) return rc;
+            if (!used && b->tail_split && !b->group_stop && walltime_s <= 0 &&
+                (rc = batch_split_tail(b, t0, &used)))
+                return rc;
+            if (!used && (rc = batch_wide_run(b, walltime_s, t0))) return rc;
             continue;
         }
         std::vector<std::pair<int, long long>> before;
@@ -1061,6 +1162,13 @@ int dnx_batch_set_gang(dnx_batch *b, int max_blocks, int clauses_per_workgroup, 
     b->gang_blocks = max_blocks;
     b->gang_cpw = clauses_per_workgroup;
     b->gang_steps = steps_per_launch;
+    return DNX_OK;
+}
+
+int dnx_batch_set_tail_split(dnx_batch *b, int enable)
+{
+    if (!b) return DNX_E_ARG;
+    b->tail_split = enable != 0;
     return DNX_OK;
 }
 

@@ -94,6 +94,7 @@ def lib():
     L.dnx_batch_set_serve_big.argtypes = [vp, C.c_int64, C.c_int64]
     L.dnx_batch_set_gang.argtypes = [vp, ip, ip, ip]
     L.dnx_batch_set_wide_fraction.argtypes = [vp, dp, ip]
+    L.dnx_batch_set_tail_split.argtypes = [vp, ip]
     L.dnx_batch_tune.argtypes = [vp, ip, ip, dp, dp, ip, ip, C.c_uint64, vp, ip, C.POINTER(C.c_int)]
     L.dnx_batch_get_shape.argtypes = [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)]
     L.dnx_state_time.argtypes = [vp, ip, C.POINTER(C.c_double)]
@@ -401,6 +402,9 @@ class Batch:
 
     def set_wide(self, max_clauses, steps_per_sync=30):
         check(lib().dnx_batch_set_wide(self.h, int(max_clauses), int(steps_per_sync)))
+
+    def set_tail_split(self, enable=True):
+        check(lib().dnx_batch_set_tail_split(self.h, int(bool(enable))))
 
     def set_wide_fraction(self, fraction=0.02, steps_per_sync=30):
         check(lib().dnx_batch_set_wide_fraction(self.h, float(fraction), int(steps_per_sync)))

@@ -221,6 +221,13 @@ int dnx_batch_info(dnx_batch *b, int slot, int *n, int *m, int *mode);
  * 0 disables it.  Results are bit-identical either way. */
 int dnx_batch_set_wide(dnx_batch *b, int64_t max_clauses, int steps_per_sync);
 
+/* The tail split (default on): when wide mode would be entered, the running
+ * instances whose whole state fits in LDS in one piece keep running in the
+ * fused kernel -- on a stream of their own, until each ends -- while wide
+ * mode advances the others beside them.  Off: wide mode takes them all.
+ * Results are bit-identical either way. */
+int dnx_batch_set_tail_split(dnx_batch *b, int enable);
+
 /* Wide mode with a threshold relative to the run: entered once the running
  * instances hold at most `fraction` of the clauses that were running when
  * dnx_batch_run began (the default: 0.02, 30 steps per sync). */

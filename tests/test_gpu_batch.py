@@ -48,11 +48,14 @@ def oracle_solve(n, cp, lits, x0, cap):
     return steps, t, status, x
 
 
-def run_batch(ctx, inst, cap, spl, shapes=None, stop=True, wide=0, gang=None):
+def run_batch(ctx, inst, cap, spl, shapes=None, stop=True, wide=0, gang=None, split=False):
     """gang = (max_blocks, clauses_per_workgroup, steps_per_launch) enables
-    gang mode for the tail (the instances left once wide mode applies)."""
+    gang mode for the tail (the instances left once wide mode applies);
+    split keeps the single-chunk LDS instances of the tail in the fused
+    kernel beside wide mode."""
     b = dnx.Batch(ctx, slots=len(inst) + 2, arena_bytes=1 << 30)
     b.set_wide(wide, 30)
+    b.set_tail_split(split)
     if gang:
         b.set_gang(*gang)
     else:
@@ -141,6 +144,31 @@ def test_gang_mode_gives_identical_bits(ctx, inst_set, oracle_results, wide, gan
         assert np.array_equal(r["x"], xo)
         if status == 1:
             assert r["unsat"] == 0
+
+
+@pytest.mark.parametrize("wide", [1 << 40, 60000])
+def test_tail_split_gives_identical_bits(ctx, inst_set, oracle_results, wide):
+    """The tail split from the start, and from 60,000 running clauses on."""
+    out = run_batch(ctx, inst_set, 1500, spl=40, wide=wide, split=True)
+    for r, (steps, t, status, xo) in zip(out, oracle_results):
+        assert r["status"] == (dnx.SAT if status == 1 else dnx.UNKNOWN)
+        assert r["steps"] == steps and r["t"] == t
+        assert np.array_equal(r["x"], xo)
+        if status == 1:
+            assert r["unsat"] == 0
+
+
+def test_tail_split_fixed_steps(ctx):
+    inst = make_set(15, 6, nmin=100, nmax=3000, with_general=True)
+    out = run_batch(ctx, inst, 45, spl=25, stop=False, wide=1 << 40, split=True)
+    for r, (n, cp, lits, x0) in zip(out, inst):
+        op = O.Problem(n, len(cp) - 1, cp, lits)
+        x = x0.astype(np.float32)
+        if op.has_unit:
+            x[:n] = np.where(op.unit != 0, op.unit, x[:n])
+        O.trajectory(op, O.params(op.m), x, 45, prec=32)
+        assert r["status"] == dnx.UNKNOWN and r["steps"] == 45
+        assert np.array_equal(r["x"], x)
 
 
 def test_gang_fixed_step_trajectory(ctx):
