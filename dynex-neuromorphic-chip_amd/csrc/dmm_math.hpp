@@ -260,18 +260,18 @@ template <typename T, typename EP, typename IP>
 __device__ __forceinline__ T fold_gather(EP e, IP eidx, int rb, int re)
 {
     T acc = T(0.0);
-    int q = rb;
-    for (; q + 8 <= re; q += 8) {
+    for (int q = rb; q < re; q += 8) {
+        // eight gathers in flight, the last round predicated
         int i[8];
         T x[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) i[u] = eidx[q + u];
+        for (int u = 0; u < 8; u++) i[u] = q + u < re ? eidx[q + u] : -1;
 #pragma unroll
-        for (int u = 0; u < 8; u++) x[u] = e[i[u]];
+        for (int u = 0; u < 8; u++) x[u] = i[u] >= 0 ? e[i[u]] : T(0.0);
 #pragma unroll
-        for (int u = 0; u < 8; u++) acc = acc + x[u];
+        for (int u = 0; u < 8; u++)
+            if (q + u < re) acc = acc + x[u];
     }
-    for (; q < re; q++) acc = acc + e[eidx[q]];
     return acc;
 }
 
@@ -367,15 +367,17 @@ __device__ __forceinline__ T fold_var(EP e, IP eidx, int b, int deg, T *buf)
     const int t = threadIdx.x;
     if (CAP > 0 && deg <= CAP) {
         // stage the gathered values (eight loads in flight), then sort
-        int j = 0;
-        for (; j + 8 <= deg; j += 8) {
+        for (int j = 0; j < deg; j += 8) {
             int i[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) i[u] = eidx[b + j + u];
+            for (int u = 0; u < 8; u++) i[u] = j + u < deg ? eidx[b + j + u] : -1;
+            T x[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) buf[(j + u) * BS + t] = e[i[u]];
+            for (int u = 0; u < 8; u++) x[u] = i[u] >= 0 ? e[i[u]] : T(0.0);
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (j + u < deg) buf[(j + u) * BS + t] = x[u];
         }
-        for (; j < deg; j++) buf[j * BS + t] = e[eidx[b + j]];
         for (int j = 1; j < deg; j++) {
             T x = buf[j * BS + t];
             int i = j;
@@ -432,15 +434,19 @@ __device__ __forceinline__ T wave_fold(EP e, IP eidx, IP row_ptr, int n, int v0,
     const int total = v0 < n ? row_ptr[vend] - sb : 0;
     const bool fits = total <= CAPW;
     if (active && fits) {
-        int q = lane;
-        for (; q + 7 * 64 < total; q += 8 * 64) {
+        // eight gathers per lane in flight, the last round predicated rather
+        // than finished one gather at a time
+        for (int q = lane; q < total; q += 8 * 64) {
             int i[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) i[u] = eidx[sb + q + u * 64];
+            for (int u = 0; u < 8; u++) i[u] = q + u * 64 < total ? eidx[sb + q + u * 64] : -1;
+            T x[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) stage[q + u * 64] = e[i[u]];
+            for (int u = 0; u < 8; u++) x[u] = i[u] >= 0 ? e[i[u]] : T(0.0);
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (q + u * 64 < total) stage[q + u * 64] = x[u];
         }
-        for (; q < total; q += 64) stage[q] = e[eidx[sb + q]];
     }
     __syncthreads();
     T acc = T(0.0);
