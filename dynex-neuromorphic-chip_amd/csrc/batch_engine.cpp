@@ -97,8 +97,9 @@ struct SlotHost {
 // pair read and written (8 per variable).
 static double step_bytes(const SlotHost &sl) { return 4.0 * (double)sl.nnz + 16.0 * sl.m + 8.0 * sl.n; }
 
-// timing buckets: the three fused-kernel modes, wide mode, gang mode
-constexpr int BK_WIDE = BM_COUNT, BK_GANG = BM_COUNT + 1, NBUCKET = BM_COUNT + 2;
+// timing buckets: the three fused-kernel modes, wide mode, gang mode, and the
+// fused kernel's tail launch beside wide mode
+constexpr int BK_WIDE = BM_COUNT, BK_GANG = BM_COUNT + 1, BK_TAIL = BM_COUNT + 2, NBUCKET = BM_COUNT + 3;
 
 struct Shape {
     int bs;
@@ -292,6 +293,9 @@ static void batch_collect_timing(dnx_batch *b)
 {
     for (int md = 0; md < NBUCKET; md++) {
         if (!b->t_pending[md]) continue;
+        // a launch on another stream (the tail's fused-kernel launch) may
+        // still run: its timing stays pending until it has ended
+        if (hipEventQuery(b->t1[md]) == hipErrorNotReady) continue;
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, b->t0[md], b->t1[md]) == hipSuccess) {
             b->t_ms[md] += ms;
@@ -935,14 +939,14 @@ static int batch_side_narrow(dnx_batch *b, const std::vector<int> &slots, int ks
     if (per_cu < 1) per_cu = 1;
     // a quarter of the CUs stays free for the wide kernels
     const int grid = std::min(cnt, std::max(1, b->cus * per_cu - b->cus / 4));
-    const bool timed = b->timing && !b->t_pending[BM_LDS_ALL];
-    if (timed && (e = hipEventRecord(b->t0[BM_LDS_ALL], ms)) != hipSuccess) return hip_fail(e, "timing");
+    const bool timed = b->timing && !b->t_pending[BK_TAIL];
+    if (timed && (e = hipEventRecord(b->t0[BK_TAIL], ms)) != hipSuccess) return hip_fail(e, "timing");
     launch_batch(BM_LDS_ALL, bs, lds, grid, (const BInst *)b->b_inst.ptr, (BDyn *)b->b_dyn.ptr, dw, cnt, dh, ksteps,
                  ms);
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "tail launch");
     if (timed) {
-        if ((e = hipEventRecord(b->t1[BM_LDS_ALL], ms)) != hipSuccess) return hip_fail(e, "timing");
-        b->t_pending[BM_LDS_ALL] = true;
+        if ((e = hipEventRecord(b->t1[BK_TAIL], ms)) != hipSuccess) return hip_fail(e, "timing");
+        b->t_pending[BK_TAIL] = true;
     }
     if ((e = hipEventRecord(b->side_done, ms)) != hipSuccess) return hip_fail(e, "tail event");
     return DNX_OK;
@@ -983,7 +987,7 @@ static int batch_split_tail(dnx_batch *b, std::chrono::steady_clock::time_point 
     batch_collect_timing(b);
     if (b->timing)
         for (size_t i = 0; i < nset.size(); i++)
-            b->t_alg[BM_LDS_ALL] += (double)(b->h_dyn[nset[i]].steps - before[i]) * step_bytes(b->slots[(size_t)nset[i]]);
+            b->t_alg[BK_TAIL] += (double)(b->h_dyn[nset[i]].steps - before[i]) * step_bytes(b->slots[(size_t)nset[i]]);
     return DNX_OK;
 }
 

@@ -410,11 +410,12 @@ class Batch:
         check(lib().dnx_batch_set_wide_fraction(self.h, float(fraction), int(steps_per_sync)))
 
     def timing(self, enable=-1, with_bytes=False):
-        """Summed device ms and launch counts per fused-kernel mode (0-2), wide mode (3) and gang mode (4);
-        with_bytes adds the compulsory bytes of the steps those launches took."""
-        ms = np.zeros(5, np.float64)
-        nl = np.zeros(5, np.int64)
-        ab = np.zeros(5, np.float64)
+        """Summed device ms and launch counts per fused-kernel mode (0-2), wide mode (3), gang mode (4) and
+        the fused kernel's tail launch beside wide mode (5); with_bytes adds the compulsory bytes of the
+        steps those launches took."""
+        ms = np.zeros(6, np.float64)
+        nl = np.zeros(6, np.int64)
+        ab = np.zeros(6, np.float64)
         check(lib().dnx_batch_timing(self.h, int(enable), ms.ctypes.data_as(C.c_void_p), nl.ctypes.data_as(C.c_void_p),
                                      ab.ctypes.data_as(C.c_void_p)))
         return (ms, nl, ab) if with_bytes else (ms, nl)

@@ -262,8 +262,9 @@ int dnx_batch_get_shape(dnx_batch *b, int mode, int *block, int *lds_cap, int64_
 int dnx_batch_set_gang(dnx_batch *b, int max_blocks, int clauses_per_workgroup, int steps_per_launch);
 
 /* Waits for queued work, then reports the summed device time (ms) and the
- * count of timed launches since the last reset, in five buckets: modes 0-2
- * of the fused kernel, wide mode, gang mode (arrays of 5).  alg_bytes (may be
+ * count of timed launches since the last reset, in six buckets: modes 0-2
+ * of the fused kernel, wide mode, gang mode, and the fused kernel's tail
+ * launch beside wide mode (arrays of 6).  alg_bytes (may be
  * NULL) receives the compulsory bytes of the steps those launches took
  * inside dnx_batch_run: per instance step, 4 per literal + 16 per clause +
  * 8 per variable.  enable = 1/0 turns launch timing on/off and resets the
