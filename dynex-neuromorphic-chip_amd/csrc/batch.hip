@@ -33,6 +33,7 @@ struct GInst {
     GLB const int32_t *lits;
     GLB const int32_t *cp;
     GLB const int32_t *lpos;
+    GLB const uint32_t *pk;
     GLB const int32_t *eidx;
     GLB const int32_t *row_ptr;
     GLB const int8_t *unit;
@@ -55,6 +56,7 @@ __device__ __forceinline__ GInst global_view(const BInst &B)
     I.lits = (GLB const int32_t *)B.lits;
     I.cp = (GLB const int32_t *)B.cp;
     I.lpos = (GLB const int32_t *)B.lpos;
+    I.pk = (GLB const uint32_t *)B.pk;
     I.eidx = (GLB const int32_t *)B.eidx;
     I.row_ptr = (GLB const int32_t *)B.row_ptr;
     I.unit = (GLB const int8_t *)B.unit;
@@ -173,6 +175,9 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                 t += level_dt(plev);
             }
             // ---- clause pass --------------------------------------------
+            // steady: V of this state in place, and the memories of the state
+            // being updated clamped (it is not the initial state)
+            const bool steady = mat && steps >= 2;
             int unsolved = 0;
             auto prevV = [&](int v) { return pvr(v).v; };
             auto curV = [&](int v) {
@@ -183,7 +188,64 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
             for (int ch = 0; ch < nch; ch++) {
                 const int c_lo = MODE == BM_LDS_ALL ? I.cbound[ch] : 0;
                 const int c_hi = MODE == BM_LDS_ALL ? I.cbound[ch + 1] : m;
-                if (!general) {
+                if (!general && steady) {
+                    // 3-SAT in the steady state: V in place and clamped, the
+                    // memories clamped by the previous update, C of the
+                    // previous state stored (the steady-state forms return
+                    // the same bits as the general path below)
+                    // Mode 0 reads one word per literal (sign, variable and
+                    // LDS slot packed), the other modes the signed literals.
+                    constexpr int U = 4;
+                    for (int cb = c_lo + tid; cb < c_hi; cb += U * BS) {
+                        int32_t L[U][3];
+                        float XS[U], XL[U], CX[U];
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                            const int c = cb + u * BS;
+                            if (c < c_hi) {
+                                if constexpr (MODE == BM_LDS_ALL) {
+                                    L[u][0] = (int32_t)I.pk[3 * c];
+                                    L[u][1] = (int32_t)I.pk[3 * c + 1];
+                                    L[u][2] = (int32_t)I.pk[3 * c + 2];
+                                } else {
+                                    L[u][0] = I.lits[3 * c];
+                                    L[u][1] = I.lits[3 * c + 1];
+                                    L[u][2] = I.lits[3 * c + 2];
+                                }
+                                XS[u] = I.xs[c];
+                                XL[u] = I.xl[c];
+                                CX[u] = I.cst[c];
+                            }
+                        }
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                            const int c = cb + u * BS;
+                            if (c >= c_hi) break;
+                            // bit 31 of a literal word is the literal's sign in both forms
+                            const int32_t l0 = L[u][0], l1 = L[u][1], l2 = L[u][2];
+                            auto var = [](int32_t w) -> int {
+                                if constexpr (MODE == BM_LDS_ALL) return (int)(((uint32_t)w >> 16) & 0x7fffu);
+                                else return abs(w) - 1;
+                            };
+                            float xs = XS[u], xl = XL[u];
+                            const float V0 = pvv(var(l0)), V1 = pvv(var(l1)), V2 = pvv(var(l2));
+                            memory_update_steady(p, CX[u], h, xs, xl);
+                            I.xs[c] = xs;
+                            I.xl[c] = xl;
+                            float C;
+                            if constexpr (MODE == BM_LDS_ALL) {
+                                auto emit = [&](int j, float val) { se[(uint32_t)L[u][j] & 0xffffu] = val; };
+                                C = clause3_steady(p, l0, l1, l2, V0, V1, V2, xs, xl, emit);
+                            } else {
+                                auto ep = I.e + 3 * c;
+                                auto emit = [&](int j, float val) { ep[j] = val; };
+                                C = clause3_steady(p, l0, l1, l2, V0, V1, V2, xs, xl, emit);
+                            }
+                            I.cst[c] = C;
+                            unsolved |= !(C < 0.5f);
+                        }
+                    }
+                } else if (!general) {
                     // 3-SAT: each thread takes its clauses four at a time, with
                     // all four clauses' literals, memories, stored C (and LDS
                     // slots) loaded before any is evaluated, so that enough

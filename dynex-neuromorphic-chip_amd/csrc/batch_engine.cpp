@@ -214,6 +214,8 @@ static bool plan_mode0(const dnx_batch *b, const HostProblem &h, Mode0Plan &P)
     const size_t pv = ((size_t)h.n * sizeof(BF2) + 15) / 16 * 16;
     const int64_t nnz = h.nnz();
     P.bound.clear();
+    // packed literal words hold a 15-bit variable and a 16-bit LDS slot
+    if (h.n > (1 << 15) || cap / 4 > (1 << 16)) return false;
     if (pv + (size_t)nnz * 4 <= cap) {
         P.nchunk = 1;
         P.lds = pv + (size_t)nnz * 4;
@@ -257,7 +259,7 @@ static int choose_mode(const dnx_batch *b, const HostProblem &h, Mode0Plan &P)
 
 // Byte layout of one instance inside its arena range.
 struct Layout {
-    size_t lits, cp, lpos, eidx, row, unit, xs, xl, vd, e, cbound, cbase, cst, total;
+    size_t lits, cp, lpos, pk, eidx, row, unit, xs, xl, vd, e, cbound, cbase, cst, total;
 };
 static Layout layout_of(int n, int m, int64_t nnz, bool general, bool has_unit, int nchunk)
 {
@@ -273,6 +275,7 @@ static Layout layout_of(int n, int m, int64_t nnz, bool general, bool has_unit, 
     // both transposes and the device contribution array exist for every
     // instance, so any instance can also run in wide mode
     L.lpos = take((size_t)nnz * 4);
+    L.pk = general ? 0 : take((size_t)nnz * 4);  // filled for mode-0 3-SAT instances
     L.eidx = take((size_t)nnz * 4);
     L.row = take(((size_t)n + 1) * 4);
     L.unit = has_unit ? take((size_t)n) : 0;
@@ -576,6 +579,17 @@ int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *p
             for (int q = q0; q < q1; q++) lp[q] = run[(size_t)std::abs(h.lits[(size_t)q]) - 1]++;
         }
     }
+    if (mode == BM_LDS_ALL && !general) {
+        // one word per literal for the fused kernel's steady state: sign in
+        // bit 31, variable in bits 16-30, LDS slot in bits 0-15 (plan_mode0
+        // admits only instances whose variables and slots fit)
+        const int32_t *lp = (const int32_t *)(stage.data() + L.lpos);
+        uint32_t *pk = (uint32_t *)(stage.data() + L.pk);
+        for (int64_t q = 0; q < nnz; q++) {
+            const int32_t l = h.lits[(size_t)q];
+            pk[q] = (l < 0 ? 0x80000000u : 0u) | ((uint32_t)(std::abs(l) - 1) << 16) | (uint32_t)lp[q];
+        }
+    }
     put(L.cbound, plan.bound.data(), ((size_t)nchunk + 1) * 4);
     put(L.eidx, h.eidx.data(), (size_t)nnz * 4);
     put(L.row, h.row_ptr.data(), ((size_t)n + 1) * 4);
@@ -604,6 +618,7 @@ int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *p
     I.lits = (const int32_t *)(base + L.lits);
     I.cp = general ? (const int32_t *)(base + L.cp) : nullptr;
     I.lpos = (const int32_t *)(base + L.lpos);
+    I.pk = mode == BM_LDS_ALL && !general ? (const uint32_t *)(base + L.pk) : nullptr;
     I.eidx = (const int32_t *)(base + L.eidx);
     I.row_ptr = (const int32_t *)(base + L.row);
     I.unit = h.has_unit ? (const int8_t *)(base + L.unit) : nullptr;

@@ -252,6 +252,58 @@ __device__ __forceinline__ T clauseK_C(CL cl, int k, GetV getV)
     return C;
 }
 
+// ---------------------------------------------------------------------------
+// Steady-state forms (FP32).  Valid when every V read lies in [-1, 1] (V is
+// already clamped) and Xs, Xl come out of a memory update (already clamped).
+// On such inputs they return the bits of memory_update<float> and
+// clause3_vals<float>, with fewer instructions:
+//  * q = +-1, so q V is exact and 1 - q V == fma(-q, V, 1) (one rounding of
+//    the same exact value);
+//  * i = 1 - q V lies in {0} U [2^-24, 2] (1 - x is exact for x in [1/2, 1]
+//    and rounds to a multiple of 2^-24 below), so C = min(i) / 2 lies in
+//    [0, 1] and its clamp01 is the identity; halving an i, a min of them or
+//    q - V is exact, so C == i_j / 2 iff min(i) == i_j, q m / 2 == (q/2) m and
+//    (q - V) / 2 == fma(-1/2, V, q/2), signed zeros included;
+//  * C < 1/2 means some i_j < 1, i.e. q_j V_j > 0, so that literal's sign
+//    check holds: a clause with C < 1/2 is satisfied by the signs.
+// Only bit 31 of l0..l2 is read (the literal's sign), so they may be signed
+// literals or packed literal words.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float signed_const(int32_t lit, uint32_t magnitude_bits)
+{
+    return __builtin_bit_cast(float, ((uint32_t)lit & 0x80000000u) | magnitude_bits);
+}
+
+__device__ __forceinline__ void memory_update_steady(const PT<float> &p, float Cprev, float h, float &xs, float &xl)
+{
+    float dxs = p.beta * (xs + p.eps);
+    dxs = dxs * (Cprev - p.gamma);
+    float dxl = p.alpha * (Cprev - p.delta);
+    xs = clamp01(xs + h * dxs);
+    xl = clampXl(xl + h * dxl, p.xlmax);
+}
+
+template <typename Emit>
+__device__ __forceinline__ float clause3_steady(const PT<float> &p, int32_t l0, int32_t l1, int32_t l2, float V0,
+                                                float V1, float V2, float Xs, float Xl, Emit emit)
+{
+    const float q0 = signed_const(l0, 0x3f800000u), q1 = signed_const(l1, 0x3f800000u),
+                q2 = signed_const(l2, 0x3f800000u);
+    const float h0 = signed_const(l0, 0x3f000000u), h1 = signed_const(l1, 0x3f000000u),
+                h2 = signed_const(l2, 0x3f000000u);
+    const float i0 = __builtin_fmaf(-q0, V0, 1.0f), i1 = __builtin_fmaf(-q1, V1, 1.0f),
+                i2 = __builtin_fmaf(-q2, V2, 1.0f);
+    const float m12 = dmin(i1, i2), m02 = dmin(i0, i2), m01 = dmin(i0, i1);
+    const float mn = dmin(i0, m12);
+    const float R0 = mn == i0 ? __builtin_fmaf(-0.5f, V0, h0) : 0.0f;
+    const float R1 = mn == i1 ? __builtin_fmaf(-0.5f, V1, h1) : 0.0f;
+    const float R2 = mn == i2 ? __builtin_fmaf(-0.5f, V2, h2) : 0.0f;
+    emit(0, contrib(p, Xs, Xl, h0 * m12, R0));
+    emit(1, contrib(p, Xs, Xl, h1 * m02, R1));
+    emit(2, contrib(p, Xs, Xl, h2 * m01, R2));
+    return mn * 0.5f;
+}
+
 // Clause-order fold of the contributions gathered through the transpose:
 // slots [rb, re) of a variable name literal positions eidx[q] whose values
 // sit in e[].  Eight independent gathers are kept in flight; the additions
