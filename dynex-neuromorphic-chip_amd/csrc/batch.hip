@@ -34,6 +34,7 @@ struct GInst {
     GLB const int32_t *cp;
     GLB const int32_t *lpos;
     GLB const uint32_t *pk;
+    GLB const int32_t *gpos;
     GLB const int32_t *eidx;
     GLB const int32_t *row_ptr;
     GLB const int8_t *unit;
@@ -57,6 +58,7 @@ __device__ __forceinline__ GInst global_view(const BInst &B)
     I.cp = (GLB const int32_t *)B.cp;
     I.lpos = (GLB const int32_t *)B.lpos;
     I.pk = (GLB const uint32_t *)B.pk;
+    I.gpos = (GLB const int32_t *)B.gpos;
     I.eidx = (GLB const int32_t *)B.eidx;
     I.row_ptr = (GLB const int32_t *)B.row_ptr;
     I.unit = (GLB const int8_t *)B.unit;
@@ -560,6 +562,7 @@ __global__ __launch_bounds__(WBS) void k_wide_clause(const BInst *__restrict__ i
     int unsolved = 0;
     if (!(I.flags & BF_GENERAL) && c < I.m) {
         const int32_t l0 = I.lits[3 * c], l1 = I.lits[3 * c + 1], l2 = I.lits[3 * c + 2];
+        const int32_t g0 = I.gpos[3 * c], g1 = I.gpos[3 * c + 1], g2 = I.gpos[3 * c + 2];
         const BF2 a0 = vdr(I, abs(l0) - 1), a1 = vdr(I, abs(l1) - 1), a2 = vdr(I, abs(l2) - 1);
         float xs = I.xs[c], xl = I.xl[c];
         float r0 = a0.v, r1 = a1.v, r2 = a2.v;
@@ -571,8 +574,9 @@ __global__ __launch_bounds__(WBS) void k_wide_clause(const BInst *__restrict__ i
             r1 = clampV(a1.v + h * a1.d);
             r2 = clampV(a2.v + h * a2.d);
         }
-        auto ep = I.e + 3 * c;
-        auto emit = [&](int j, float val) { ep[j] = val; };
+        // contributions go to their variable-major slots, so the variable
+        // kernel reads each variable's slots as one contiguous range
+        auto emit = [&](int j, float val) { I.e[j == 0 ? g0 : j == 1 ? g1 : g2] = val; };
         bool sat;
         float C = clause3_vals<float>(p, l0, l1, l2, r0, r1, r2, clamp01(xs), clampXl(xl, p.xlmax), emit, sat);
         unsolved = !(C < 0.5f) || !sat;
@@ -591,8 +595,8 @@ __global__ __launch_bounds__(WBS) void k_wide_clause(const BInst *__restrict__ i
             I.xs[c] = xs;
             I.xl[c] = xl;
         }
-        auto ep = I.e + bb;
-        auto emit = [&](int j, float val) { ep[j] = val; };
+        auto gp = I.gpos + bb;
+        auto emit = [&](int j, float val) { I.e[gp[j]] = val; };
         bool sat;
         float C = clauseK<float>(p, cl, k, clamp01(xs), clampXl(xl, p.xlmax), curV, emit, sat);
         unsolved = !(C < 0.5f) || !sat;
@@ -610,26 +614,33 @@ __global__ __launch_bounds__(WBS) void k_wide_var(const BInst *__restrict__ inst
     const int2 bm = vmap[b];
     const int slot = __builtin_amdgcn_readfirstlane(bm.x), lb = __builtin_amdgcn_readfirstlane(bm.y);
     Ctl *__restrict__ ctl = &wctl[slot];
+    // the descriptor is read beside the control words (it is always valid
+    // memory); its arrays only once the instance is known to run
+    const GInst I = global_view(insts[slot]);
     if (ctl->status != DNX_RUNNING) return;
     const int ps = (r + 2) % 3;
     if (ctl->done[ps]) return;
-    const GInst I = global_view(insts[slot]);
+    const int w = threadIdx.x >> 6;
+    const int v0 = (lb * (WBS / 64) + w) * 64;
+    const int v = v0 + (threadIdx.x & 63);
+    const bool vok = v < I.n;
+    // everything of the lane's variable is requested at once
+    const int rb = vok ? I.row_ptr[v] : 0, re = vok ? I.row_ptr[v + 1] : 0;
+    const BF2 a = vok ? vdr(I, v) : BF2{0.0f, 0.0f};
+    const bool pinned = vok && (I.flags & BF_UNIT) && I.unit[v] != 0;
     const bool pend = ctl->stepped[ps] != 0;
     const float h = pend ? level_h<float>(dev_level(ctl, ps)) : 0.0f;
     const bool solved = ctl->stop_on_solve && !any_flag(ctl->flags[r][FG_UNSOLVED]);
     const int done = solved ? DNX_SAT : (ctl->steps >= ctl->cap ? DNX_UNKNOWN : 0);
-    const int w = threadIdx.x >> 6;
-    const int v0 = (lb * (WBS / 64) + w) * 64;
-    const int v = v0 + (threadIdx.x & 63);
-    const float dVf = wave_fold<float, false, WCAPW>(I.e, I.eidx, I.row_ptr, I.n, v0, !done, stage[w], nullptr);
+    // contributions are variable-major in wide mode (k_wide_clause)
+    const float dVf = wave_fold_vm<WCAPW>(I.e, rb, re, v0 < I.n ? min(63, I.n - 1 - v0) : 0, stage[w]);
     int f = 0;
-    if (v < I.n) {
-        BF2 a = vdr(I, v);
+    if (vok) {
         const float V = pend ? clampV(a.v + h * a.d) : a.v;
         float dV = 0.0f;
         if (!done) {
             dV = dVf;
-            if ((I.flags & BF_UNIT) && I.unit[v] != 0) dV = 0.0f;  // dynex.cc:306
+            if (pinned) dV = 0.0f;  // dynex.cc:306
             f = fail_level(V, dV);
         }
         BF2 o;

@@ -27,6 +27,7 @@ struct BInst {
     const int32_t *cp;        // m+1 offsets (general clause lengths only)
     const int32_t *lpos;      // mode 0: LDS slot of each literal within its chunk
     const uint32_t *pk;       // mode 0, 3-SAT: literal words, sign << 31 | var << 16 | LDS slot
+    const int32_t *gpos;      // variable-major (transpose) slot of each literal
     const int32_t *eidx;      // literal of each variable-major slot (global e)
     const int32_t *row_ptr;   // n+1
     const int8_t *unit;       // n pins

@@ -259,7 +259,7 @@ static int choose_mode(const dnx_batch *b, const HostProblem &h, Mode0Plan &P)
 
 // Byte layout of one instance inside its arena range.
 struct Layout {
-    size_t lits, cp, lpos, pk, eidx, row, unit, xs, xl, vd, e, cbound, cbase, cst, total;
+    size_t lits, cp, lpos, pk, gpos, eidx, row, unit, xs, xl, vd, e, cbound, cbase, cst, total;
 };
 static Layout layout_of(int n, int m, int64_t nnz, bool general, bool has_unit, int nchunk)
 {
@@ -276,6 +276,9 @@ static Layout layout_of(int n, int m, int64_t nnz, bool general, bool has_unit, 
     // instance, so any instance can also run in wide mode
     L.lpos = take((size_t)nnz * 4);
     L.pk = general ? 0 : take((size_t)nnz * 4);  // filled for mode-0 3-SAT instances
+    // transpose slots of the literals; with one chunk (or outside mode 0)
+    // lpos holds them already
+    L.gpos = nchunk > 1 ? take((size_t)nnz * 4) : 0;
     L.eidx = take((size_t)nnz * 4);
     L.row = take(((size_t)n + 1) * 4);
     L.unit = has_unit ? take((size_t)n) : 0;
@@ -590,6 +593,7 @@ int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *p
             pk[q] = (l < 0 ? 0x80000000u : 0u) | ((uint32_t)(std::abs(l) - 1) << 16) | (uint32_t)lp[q];
         }
     }
+    if (nchunk > 1) put(L.gpos, h.lpos.data(), (size_t)nnz * 4);
     put(L.cbound, plan.bound.data(), ((size_t)nchunk + 1) * 4);
     put(L.eidx, h.eidx.data(), (size_t)nnz * 4);
     put(L.row, h.row_ptr.data(), ((size_t)n + 1) * 4);
@@ -619,6 +623,7 @@ int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *p
     I.cp = general ? (const int32_t *)(base + L.cp) : nullptr;
     I.lpos = (const int32_t *)(base + L.lpos);
     I.pk = mode == BM_LDS_ALL && !general ? (const uint32_t *)(base + L.pk) : nullptr;
+    I.gpos = nchunk > 1 ? (const int32_t *)(base + L.gpos) : I.lpos;
     I.eidx = (const int32_t *)(base + L.eidx);
     I.row_ptr = (const int32_t *)(base + L.row);
     I.unit = h.has_unit ? (const int8_t *)(base + L.unit) : nullptr;
