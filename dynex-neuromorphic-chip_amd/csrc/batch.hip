@@ -101,6 +101,7 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_item;
     __shared__ int s_lev[2];
+    __shared__ int s_cb[KC_MAX + 1];  // mode 0: chunk bounds of the current instance
     const int tid = threadIdx.x;
     for (;;) {
         __syncthreads();  // previous item finished with LDS and s_item
@@ -152,10 +153,12 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
         // at a time in LDS (one chunk when they all fit); with several chunks
         // each variable's partial fold waits in sacc between chunks
         const int nch = MODE == BM_LDS_ALL ? I.nchunk : 1;
-        constexpr int UV = 8;  // variables whose slot ranges one thread loads at once
+        constexpr int UV = 4;  // variables whose slot ranges one thread loads at once
         const size_t pvb = ((size_t)n * sizeof(BF2) + 15) / 16 * 16;
         float *sacc = (float *)(smem + pvb);
         float *se = (float *)(smem + pvb + (nch > 1 ? ((size_t)n * 4 + 15) / 16 * 16 : 0));
+        if (MODE == BM_LDS_ALL)
+            for (int i = tid; i <= nch; i += BS) s_cb[i] = I.cbound[i];
         if (MODE != BM_GLOBAL) {
             for (int v = tid; v < n; v += BS) lpv[v] = vdr(I, v);
             __syncthreads();
@@ -188,8 +191,8 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                 return pend ? clampV(a.v + h * a.d) : a.v;
             };
             for (int ch = 0; ch < nch; ch++) {
-                const int c_lo = MODE == BM_LDS_ALL ? I.cbound[ch] : 0;
-                const int c_hi = MODE == BM_LDS_ALL ? I.cbound[ch + 1] : m;
+                const int c_lo = MODE == BM_LDS_ALL ? s_cb[ch] : 0;
+                const int c_hi = MODE == BM_LDS_ALL ? s_cb[ch + 1] : m;
                 if (!general && steady) {
                     // 3-SAT in the steady state: V in place and clamped, the
                     // memories clamped by the previous update, C of the
@@ -350,31 +353,51 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                 }
                 if (ch + 1 < nch) {
                     // fold this chunk's contributions (clause order within each
-                    // variable's slots), then free se for the next chunk
-                    __syncthreads();
+                    // variable's slots), then free se for the next chunk.  Each
+                    // thread's next UV variables' slot ranges are loaded
+                    // together, the first ones before the barrier so that the
+                    // loads travel while the other waves finish the chunk.
                     auto cbase = I.cbase + (size_t)ch * (n + 1);
-                    // each thread's next UV variables' slot ranges are loaded
-                    // together, then folded
-                    for (int vb = tid; vb < n; vb += UV * BS) {
-                        int RB[UV], RE[UV];
+                    int RB[UV], RE[UV];
+                    auto rows = [&](int vb) {
 #pragma unroll
                         for (int u = 0; u < UV; u++) {
                             const int v = vb + u * BS;
                             RB[u] = v < n ? cbase[v] : 0;
                             RE[u] = v < n ? cbase[v + 1] : 0;
                         }
+                    };
+                    rows(tid);
+                    __syncthreads();
+                    for (int vb = tid; vb < n; vb += UV * BS) {
+                        if (vb != tid) rows(vb);
 #pragma unroll
                         for (int u = 0; u < UV; u++) {
                             const int v = vb + u * BS;
                             if (v >= n) break;
-                            float s = ch ? sacc[v] : 0.0f;
-                            for (int q = RB[u]; q < RE[u]; q++) s = s + se[q];
-                            sacc[v] = s;
+                            sacc[v] = lds_fold(se, RB[u], RE[u], ch ? sacc[v] : 0.0f);
                         }
                     }
                     __syncthreads();
                 }
             }
+            // the variable pass's first slot ranges are requested before the
+            // barrier, so they arrive while the other waves finish
+            auto rowp = MODE == BM_LDS_ALL ? I.cbase + (size_t)(nch - 1) * (n + 1) : I.row_ptr;
+            // (UP variables per thread at a time: these stay live across the
+            // barrier)
+            constexpr int UP = 4;
+            int RB[UP], RE[UP], PIN[UP];
+            auto rows = [&](int vb) {
+#pragma unroll
+                for (int u = 0; u < UP; u++) {
+                    const int v = vb + u * BS;
+                    RB[u] = v < n ? rowp[v] : 0;
+                    RE[u] = v < n ? rowp[v + 1] : 0;
+                    PIN[u] = v < n && (I.flags & BF_UNIT) ? I.unit[v] : 0;
+                }
+            };
+            rows(tid);
             const bool any_unsolved = __syncthreads_or(unsolved);
             if (d.stop_on_solve && !any_unsolved)
                 done = DNX_SAT;
@@ -382,18 +405,10 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                 done = DNX_UNKNOWN;
             // ---- variable pass ------------------------------------------
             int fmax = 0;
-            auto rowp = MODE == BM_LDS_ALL ? I.cbase + (size_t)(nch - 1) * (n + 1) : I.row_ptr;
-            for (int vb = tid; vb < n; vb += UV * BS) {
-                int RB[UV], RE[UV], PIN[UV];
+            for (int vb = tid; vb < n; vb += UP * BS) {
+                if (vb != tid) rows(vb);
 #pragma unroll
-                for (int u = 0; u < UV; u++) {
-                    const int v = vb + u * BS;
-                    RB[u] = v < n ? rowp[v] : 0;
-                    RE[u] = v < n ? rowp[v + 1] : 0;
-                    PIN[u] = v < n && (I.flags & BF_UNIT) ? I.unit[v] : 0;
-                }
-#pragma unroll
-                for (int u = 0; u < UV; u++) {
+                for (int u = 0; u < UP; u++) {
                     const int v = vb + u * BS;
                     if (v >= n) break;
                     BF2 a = pvr(v);
@@ -401,8 +416,7 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                     float dV = 0.0f;
                     if (!done) {
                         if (MODE == BM_LDS_ALL) {
-                            if (nch > 1) dV = sacc[v];
-                            for (int q = RB[u]; q < RE[u]; q++) dV = dV + se[q];
+                            dV = lds_fold(se, RB[u], RE[u], nch > 1 ? sacc[v] : 0.0f);
                         } else {
                             dV = fold_gather<float>(I.e, I.eidx, RB[u], RE[u]);
                         }

@@ -13,6 +13,7 @@ namespace dnx {
 //   1  V/dV pairs live in LDS, contributions in device memory
 //   2  both in device memory (instances too large for LDS)
 enum BatchMode { BM_LDS_ALL = 0, BM_LDS_V = 1, BM_GLOBAL = 2, BM_COUNT = 3 };
+constexpr int KC_MAX = 16;  // mode 0: chunks of consecutive clauses per instance at most
 
 struct alignas(8) BF2 {
     float v, d;

@@ -201,7 +201,6 @@ int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *p
 // most KC_MAX chunks of consecutive clauses whose contributions fit; a step
 // then evaluates and folds one chunk at a time, keeping each variable's
 // partial fold in LDS as well.
-constexpr int KC_MAX = 16;
 struct Mode0Plan {
     int nchunk = 0;
     size_t lds = 0;                 // dynamic LDS bytes the instance needs

@@ -304,6 +304,22 @@ __device__ __forceinline__ float clause3_steady(const PT<float> &p, int32_t l0, 
     return mn * 0.5f;
 }
 
+// s plus LDS slots [rb, re) in slot order: the reads go out eight at a time,
+// the additions stay one after another (a slot past re is neither read nor
+// added).
+__device__ __forceinline__ float lds_fold(const float *se, int rb, int re, float s)
+{
+    for (int q = rb; q < re; q += 8) {
+        float x[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) x[u] = q + u < re ? se[q + u] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (q + u < re) s = s + x[u];
+    }
+    return s;
+}
+
 // Clause-order fold of the contributions gathered through the transpose:
 // slots [rb, re) of a variable name literal positions eidx[q] whose values
 // sit in e[].  Eight independent gathers are kept in flight; the additions
