@@ -153,7 +153,7 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
         // at a time in LDS (one chunk when they all fit); with several chunks
         // each variable's partial fold waits in sacc between chunks
         const int nch = MODE == BM_LDS_ALL ? I.nchunk : 1;
-        constexpr int UV = 4;  // variables whose slot ranges one thread loads at once
+        constexpr int UV = 2;  // chunk folds: variables whose slot ranges one thread loads at once
         const size_t pvb = ((size_t)n * sizeof(BF2) + 15) / 16 * 16;
         float *sacc = (float *)(smem + pvb);
         float *se = (float *)(smem + pvb + (nch > 1 ? ((size_t)n * 4 + 15) / 16 * 16 : 0));
@@ -162,6 +162,22 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
         if (MODE != BM_GLOBAL) {
             for (int v = tid; v < n; v += BS) lpv[v] = vdr(I, v);
             __syncthreads();
+        }
+        // the variable pass's slot ranges (the last chunk's with several
+        // chunks) and unit pins of each thread's first two variables never
+        // change: they are read once per instance, and instances of up to
+        // 2 BS variables keep them in registers
+        auto rowp = MODE == BM_LDS_ALL ? I.cbase + (size_t)(nch - 1) * (n + 1) : I.row_ptr;
+        int RB0 = 0, RE0 = 0, RB1 = 0, RE1 = 0, PIN01 = 0;
+        if (tid < n) {
+            RB0 = rowp[tid];
+            RE0 = rowp[tid + 1];
+            PIN01 |= (I.flags & BF_UNIT) && I.unit[tid] != 0;
+        }
+        if (tid + BS < n) {
+            RB1 = rowp[tid + BS];
+            RE1 = rowp[tid + BS + 1];
+            PIN01 |= ((I.flags & BF_UNIT) && I.unit[tid + BS] != 0) << 1;
         }
         bool pend = d.plevel >= 0;
         // mat: V of the current state is already in place (every step of a
@@ -381,53 +397,42 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                     __syncthreads();
                 }
             }
-            // the variable pass's first slot ranges are requested before the
-            // barrier, so they arrive while the other waves finish
-            auto rowp = MODE == BM_LDS_ALL ? I.cbase + (size_t)(nch - 1) * (n + 1) : I.row_ptr;
-            // (UP variables per thread at a time: these stay live across the
-            // barrier)
-            constexpr int UP = 4;
-            int RB[UP], RE[UP], PIN[UP];
-            auto rows = [&](int vb) {
-#pragma unroll
-                for (int u = 0; u < UP; u++) {
-                    const int v = vb + u * BS;
-                    RB[u] = v < n ? rowp[v] : 0;
-                    RE[u] = v < n ? rowp[v + 1] : 0;
-                    PIN[u] = v < n && (I.flags & BF_UNIT) ? I.unit[v] : 0;
-                }
-            };
-            rows(tid);
             const bool any_unsolved = __syncthreads_or(unsolved);
             if (d.stop_on_solve && !any_unsolved)
                 done = DNX_SAT;
             else if (steps >= d.cap)
                 done = DNX_UNKNOWN;
             // ---- variable pass ------------------------------------------
+            // (one variable at a time per thread: batching the slot-range
+            // loads of several variables measured slower)
             int fmax = 0;
-            for (int vb = tid; vb < n; vb += UP * BS) {
-                if (vb != tid) rows(vb);
-#pragma unroll
-                for (int u = 0; u < UP; u++) {
-                    const int v = vb + u * BS;
-                    if (v >= n) break;
-                    BF2 a = pvr(v);
-                    const float V = pend && !mat ? clampV(a.v + h * a.d) : a.v;
-                    float dV = 0.0f;
-                    if (!done) {
-                        if (MODE == BM_LDS_ALL) {
-                            dV = lds_fold(se, RB[u], RE[u], nch > 1 ? sacc[v] : 0.0f);
-                        } else {
-                            dV = fold_gather<float>(I.e, I.eidx, RB[u], RE[u]);
-                        }
-                        if (PIN[u] != 0) dV = 0.0f;  // dynex.cc:306
-                        fmax = max(fmax, fail_level(V, dV));
+            auto var_one = [&](int v, int rb, int re, bool pinned) {
+                BF2 a = pvr(v);
+                const float V = pend && !mat ? clampV(a.v + h * a.d) : a.v;
+                float dV = 0.0f;
+                if (!done) {
+                    if (MODE == BM_LDS_ALL) {
+                        dV = lds_fold(se, rb, re, nch > 1 ? sacc[v] : 0.0f);
+                    } else {
+                        dV = fold_gather<float>(I.e, I.eidx, rb, re);
                     }
-                    BF2 o;
-                    o.v = V;
-                    o.d = dV;
-                    pvw(v, o);
+                    if (pinned) dV = 0.0f;  // dynex.cc:306
+                    fmax = max(fmax, fail_level(V, dV));
                 }
+                BF2 o;
+                o.v = V;
+                o.d = dV;
+                pvw(v, o);
+            };
+            if (n <= 2 * BS) {
+                // slot ranges and pins from registers: in mode 0 this pass
+                // then touches no device memory and does not wait for the
+                // clause pass's stores to complete
+                if (tid < n) var_one(tid, RB0, RE0, PIN01 & 1);
+                if (tid + BS < n) var_one(tid + BS, RB1, RE1, (PIN01 >> 1) & 1);
+            } else {
+                for (int v = tid; v < n; v += BS)
+                    var_one(v, rowp[v], rowp[v + 1], (I.flags & BF_UNIT) && I.unit[v] != 0);
             }
             if (done) break;
             const int wm = wave_max(fmax);
