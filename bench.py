@@ -65,7 +65,7 @@ def percentile_ms(vals, q):
 
 
 def apply_shape(b, shape):
-    """Autotuned fused-kernel shapes; the tail goes to wide mode once 2% of
+    """Autotuned fused-kernel shapes; the tail goes to wide mode once 1% of
     the run's clauses are left (the library default)."""
     if shape:
         for md in (0, 1):

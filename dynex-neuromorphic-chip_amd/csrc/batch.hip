@@ -96,17 +96,19 @@ __device__ __forceinline__ void vdw(const GInst &I, int i, BF2 o)
 template <int MODE, int BS>
 __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, BDyn *__restrict__ dyn,
                                               const int *__restrict__ work, int nwork, int *__restrict__ head,
-                                              int K)
+                                              int K, BFlow *__restrict__ flow)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_item;
     __shared__ int s_lev[2];
     __shared__ int s_cb[KC_MAX + 1];  // mode 0: chunk bounds of the current instance
+    __shared__ int s_stop;             // flow: the stop flag as read in this step
     const int tid = threadIdx.x;
     for (;;) {
         __syncthreads();  // previous item finished with LDS and s_item
         if (tid == 0) {
-            s_item = atomicAdd(head, 1);
+            const bool stopped = flow && __hip_atomic_load(&flow->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_item = stopped ? nwork : atomicAdd(head, 1);
             s_lev[0] = 0;
             s_lev[1] = 0;
         }
@@ -190,6 +192,9 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
         double t = d.t;
         int done = 0;
         for (int k = 0; k < K; k++) {
+            // flow: the stop flag is requested now and judged at the end of
+            // the step, so its load travels during the step
+            const int fl = flow && tid == 0 ? __hip_atomic_load(&flow->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
             const float h = pend ? level_h<float>(plev) : 0.0f;
             if (pend) {
                 steps += 1;
@@ -437,10 +442,12 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
             if (done) break;
             const int wm = wave_max(fmax);
             if ((tid & 63) == 0 && wm > 0) atomicMax(&s_lev[k & 1], wm);
+            if (tid == 0) s_stop = fl;
             __syncthreads();
             plev = s_lev[k & 1];
             pend = true;
             if (tid == 0) s_lev[(k + 1) & 1] = 0;
+            if (flow && s_stop) break;
             if (k + 1 < K) {
                 // another step follows in this launch: put V of the next
                 // state in place (the last step of a launch leaves the pending
@@ -470,6 +477,12 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                 o.plevel = plev;
             }
             dyn[slot] = o;
+            if (flow && done) {
+                const long long left =
+                    __hip_atomic_fetch_add(&flow->live, -(long long)m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - m;
+                if (left <= flow->threshold)
+                    __hip_atomic_store(&flow->stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
     }
 }
@@ -997,14 +1010,14 @@ __global__ __launch_bounds__(256) void k_batch_materialize(const BInst *__restri
 
 template <int MODE, int BS>
 static void launch_one(int lds, int grid, const BInst *insts, BDyn *dyn, const int *work, int nwork, int *head,
-                       int K, hipStream_t st)
+                       int K, BFlow *flow, hipStream_t st)
 {
     static int lds_set = 0;
     if (lds > 65536 && lds > lds_set) {
         (void)hipFuncSetAttribute((const void *)k_batch<MODE, BS>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
         lds_set = lds;
     }
-    k_batch<MODE, BS><<<grid, BS, lds, st>>>(insts, dyn, work, nwork, head, K);
+    k_batch<MODE, BS><<<grid, BS, lds, st>>>(insts, dyn, work, nwork, head, K, flow);
 }
 
 template <int MODE, int BS>
@@ -1019,13 +1032,13 @@ static int occ_one(int lds)
 
 template <int MODE>
 static void launch_mode(int bs, int lds, int grid, const BInst *insts, BDyn *dyn, const int *work, int nwork,
-                        int *head, int K, hipStream_t st)
+                        int *head, int K, BFlow *flow, hipStream_t st)
 {
     switch (bs) {
-        case 128: launch_one<MODE, 128>(lds, grid, insts, dyn, work, nwork, head, K, st); break;
-        case 256: launch_one<MODE, 256>(lds, grid, insts, dyn, work, nwork, head, K, st); break;
-        case 512: launch_one<MODE, 512>(lds, grid, insts, dyn, work, nwork, head, K, st); break;
-        default: launch_one<MODE, 1024>(lds, grid, insts, dyn, work, nwork, head, K, st); break;
+        case 128: launch_one<MODE, 128>(lds, grid, insts, dyn, work, nwork, head, K, flow, st); break;
+        case 256: launch_one<MODE, 256>(lds, grid, insts, dyn, work, nwork, head, K, flow, st); break;
+        case 512: launch_one<MODE, 512>(lds, grid, insts, dyn, work, nwork, head, K, flow, st); break;
+        default: launch_one<MODE, 1024>(lds, grid, insts, dyn, work, nwork, head, K, flow, st); break;
     }
 }
 
@@ -1050,16 +1063,16 @@ int batch_occupancy(int mode, int bs, int lds)
 }
 
 void launch_batch(int mode, int bs, int lds_bytes, int grid, const BInst *insts, BDyn *dyn, const int *work,
-                  int nwork, int *head, int ksteps, void *stream)
+                  int nwork, int *head, int ksteps, BFlow *flow, void *stream)
 {
     hipStream_t st = (hipStream_t)stream;
     if (nwork <= 0 || grid <= 0) return;
     if (mode == BM_LDS_ALL)
-        launch_mode<BM_LDS_ALL>(bs, lds_bytes, grid, insts, dyn, work, nwork, head, ksteps, st);
+        launch_mode<BM_LDS_ALL>(bs, lds_bytes, grid, insts, dyn, work, nwork, head, ksteps, flow, st);
     else if (mode == BM_LDS_V)
-        launch_mode<BM_LDS_V>(bs, lds_bytes, grid, insts, dyn, work, nwork, head, ksteps, st);
+        launch_mode<BM_LDS_V>(bs, lds_bytes, grid, insts, dyn, work, nwork, head, ksteps, flow, st);
     else
-        launch_mode<BM_GLOBAL>(bs, 0, grid, insts, dyn, work, nwork, head, ksteps, st);
+        launch_mode<BM_GLOBAL>(bs, 0, grid, insts, dyn, work, nwork, head, ksteps, flow, st);
 }
 
 void launch_batch_zero(int *p, int count, void *stream)

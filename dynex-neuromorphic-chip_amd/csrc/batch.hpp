@@ -61,9 +61,23 @@ struct BDyn {
     unsigned long long t_done;  // device wall clock when the instance ended
 };
 
-// one launch of the fused kernel over `nwork` slots listed in `work`
+// Flow control of a fused launch whose workgroups keep their instances
+// until they end (instead of K steps each): an instance that ends takes its
+// clauses off `live`, and the one that brings live to `threshold` or below
+// raises `stop`; every workgroup then leaves after its current step, with
+// its instance's state written back, and takes no further instance.
+struct BFlow {
+    long long live;
+    long long threshold;
+    int stop;
+    int pad[11];
+};
+
+// one launch of the fused kernel over `nwork` slots listed in `work`: K steps
+// per instance, or (flow != nullptr) each instance until it ends or the flow
+// stops
 void launch_batch(int mode, int bs, int lds_bytes, int grid, const BInst *insts, BDyn *dyn,
-                  const int *work, int nwork, int *head, int ksteps, void *stream);
+                  const int *work, int nwork, int *head, int ksteps, BFlow *flow, void *stream);
 void launch_batch_zero(int *p, int count, void *stream);
 void launch_batch_stamp(unsigned long long *p, void *stream);
 // materialised state of one slot in the reference layout (float)

@@ -149,7 +149,7 @@ struct dnx_batch {
     // max(wide_clauses, wide_frac x the clauses running when dnx_batch_run
     // starts) clauses in total; wide_chunk steps between status syncs
     long long wide_clauses = 0;
-    double wide_frac = 0.02;
+    double wide_frac = 0.01;
     int wide_chunk = 30;
     // serving: requests with at least this many clauses run in the wide lane
     long long serve_big = 60000;
@@ -161,6 +161,11 @@ struct dnx_batch {
     // tail: single-chunk mode-0 instances run in the fused kernel beside
     // wide mode (batch_split_tail)
     bool tail_split = true;
+    // dnx_batch_run: the fused launch keeps each instance on its workgroup
+    // until it ends or the running clauses fall to the wide-mode threshold
+    // (BFlow), instead of relaunching every steps_per_launch steps
+    bool flow = true;
+    DevBuf b_flow;
     DevBuf b_wctl, b_wlist, b_bmap, b_vmap;
     // gang mode: taken for the tail when the running instances need at
     // most gang_blocks workgroups in total (0 disables it)
@@ -319,20 +324,21 @@ static int batch_sync(dnx_batch *b)
 // Enqueue one advance of every running slot by up to ksteps steps.  The work
 // lists come from `known` (the last polled status); slots that ended since
 // are skipped by the kernel itself.
-static int batch_enqueue_slots(dnx_batch *b, int ksteps, const std::vector<int> &slots);
+static int batch_enqueue_slots(dnx_batch *b, int ksteps, const std::vector<int> &slots, BFlow *flow = nullptr);
 
-static int batch_enqueue(dnx_batch *b, int ksteps, const BDyn *known)
+static int batch_enqueue(dnx_batch *b, int ksteps, const BDyn *known, BFlow *flow = nullptr)
 {
     std::vector<int> run;
     for (int s = 0; s < b->nslots; s++)
         if (b->slots[(size_t)s].used && known[s].status == DNX_RUNNING) run.push_back(s);
-    return batch_enqueue_slots(b, ksteps, run);
+    return batch_enqueue_slots(b, ksteps, run, flow);
 }
 
 // Enqueue one advance of the listed slots (fused kernel, one launch per
 // execution shape, each on its own stream, joined back into the context
-// stream).
-static int batch_enqueue_slots(dnx_batch *b, int ksteps, const std::vector<int> &slots)
+// stream).  With a flow (mode 0 only), ksteps is ignored and the launch runs
+// until the flow stops.
+static int batch_enqueue_slots(dnx_batch *b, int ksteps, const std::vector<int> &slots, BFlow *flow)
 {
     hipStream_t st = b->ctx->stream;
     const int S = b->nslots;
@@ -383,8 +389,9 @@ static int batch_enqueue_slots(dnx_batch *b, int ksteps, const std::vector<int> 
         int grid = std::min(cnt[md], b->cus * per_cu);
         const bool timed = b->timing && !b->t_pending[md];
         if (timed && (e = hipEventRecord(b->t0[md], ms)) != hipSuccess) return hip_fail(e, "timing");
+        BFlow *f = md == BM_LDS_ALL ? flow : nullptr;
         launch_batch(md, bs, lds, grid, (const BInst *)b->b_inst.ptr, (BDyn *)b->b_dyn.ptr, dw + (size_t)md * S,
-                     cnt[md], dh + md, ksteps, ms);
+                     cnt[md], dh + md, f ? INT_MAX : ksteps, f, ms);
         if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "batch launch");
         if (timed) {
             if ((e = hipEventRecord(b->t1[md], ms)) != hipSuccess) return hip_fail(e, "timing");
@@ -961,7 +968,7 @@ static int batch_side_narrow(dnx_batch *b, const std::vector<int> &slots, int ks
     const bool timed = b->timing && !b->t_pending[BK_TAIL];
     if (timed && (e = hipEventRecord(b->t0[BK_TAIL], ms)) != hipSuccess) return hip_fail(e, "timing");
     launch_batch(BM_LDS_ALL, bs, lds, grid, (const BInst *)b->b_inst.ptr, (BDyn *)b->b_dyn.ptr, dw, cnt, dh, ksteps,
-                 ms);
+                 nullptr, ms);
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "tail launch");
     if (timed) {
         if ((e = hipEventRecord(b->t1[BK_TAIL], ms)) != hipSuccess) return hip_fail(e, "timing");
@@ -1048,7 +1055,30 @@ int dnx_batch_run(dnx_batch *b, int steps_per_launch, double walltime_s, int *re
         if (b->timing)
             for (int s = 0; s < b->nslots; s++)
                 if (b->slots[(size_t)s].used && known[s].status == DNX_RUNNING) before.push_back({s, known[s].steps});
-        if ((rc = batch_enqueue(b, steps_per_launch, known))) return rc;
+        // flow launch: when every running instance is a mode-0 instance and
+        // the run has no wall-clock limit or group stop (those need the host
+        // to look at the statuses every steps_per_launch steps)
+        BFlow *flow = nullptr;
+        if (b->flow && walltime_s <= 0 && !b->group_stop) {
+            bool all0 = true;
+            for (int s = 0; s < b->nslots && all0; s++)
+                if (b->slots[(size_t)s].used && known[s].status == DNX_RUNNING && b->slots[(size_t)s].mode != BM_LDS_ALL)
+                    all0 = false;
+            if (all0) {
+                if (!b->b_flow.ptr && (rc = b->b_flow.alloc(sizeof(BFlow)))) return rc;
+                BFlow f{};
+                f.live = clauses;
+                f.threshold = threshold;
+                f.stop = 0;
+                hipError_t e;
+                if ((e = hipMemcpyAsync(b->b_flow.ptr, &f, sizeof f, hipMemcpyHostToDevice, b->ctx->stream)) !=
+                        hipSuccess ||
+                    (e = hipStreamSynchronize(b->ctx->stream)) != hipSuccess)
+                    return hip_fail(e, "flow upload");
+                flow = (BFlow *)b->b_flow.ptr;
+            }
+        }
+        if ((rc = batch_enqueue(b, steps_per_launch, known, flow))) return rc;
         if ((rc = batch_snapshot(b, 0))) return rc;
         if ((rc = batch_wait(b, 0))) return rc;
         batch_collect_timing(b);
@@ -1192,6 +1222,13 @@ int dnx_batch_set_tail_split(dnx_batch *b, int enable)
 {
     if (!b) return DNX_E_ARG;
     b->tail_split = enable != 0;
+    return DNX_OK;
+}
+
+int dnx_batch_set_flow(dnx_batch *b, int enable)
+{
+    if (!b) return DNX_E_ARG;
+    b->flow = enable != 0;
     return DNX_OK;
 }
 

@@ -95,6 +95,7 @@ def lib():
     L.dnx_batch_set_gang.argtypes = [vp, ip, ip, ip]
     L.dnx_batch_set_wide_fraction.argtypes = [vp, dp, ip]
     L.dnx_batch_set_tail_split.argtypes = [vp, ip]
+    L.dnx_batch_set_flow.argtypes = [vp, ip]
     L.dnx_batch_tune.argtypes = [vp, ip, ip, dp, dp, ip, ip, C.c_uint64, vp, ip, C.POINTER(C.c_int)]
     L.dnx_batch_get_shape.argtypes = [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)]
     L.dnx_state_time.argtypes = [vp, ip, C.POINTER(C.c_double)]
@@ -406,7 +407,10 @@ class Batch:
     def set_tail_split(self, enable=True):
         check(lib().dnx_batch_set_tail_split(self.h, int(bool(enable))))
 
-    def set_wide_fraction(self, fraction=0.02, steps_per_sync=30):
+    def set_flow(self, enable=True):
+        check(lib().dnx_batch_set_flow(self.h, int(bool(enable))))
+
+    def set_wide_fraction(self, fraction=0.01, steps_per_sync=30):
         check(lib().dnx_batch_set_wide_fraction(self.h, float(fraction), int(steps_per_sync)))
 
     def timing(self, enable=-1, with_bytes=False):

@@ -228,9 +228,19 @@ int dnx_batch_set_wide(dnx_batch *b, int64_t max_clauses, int steps_per_sync);
  * Results are bit-identical either way. */
 int dnx_batch_set_tail_split(dnx_batch *b, int enable);
 
+/* Flow launches (default on): while every running instance is held in LDS
+ * (mode 0) and dnx_batch_run has neither a wall-clock limit nor a group
+ * stop, one fused launch keeps each instance on its workgroup until it ends
+ * -- largest instances first, the next instance as soon as a workgroup
+ * frees -- and ends once the running instances hold no more clauses than
+ * the wide-mode threshold; steps_per_launch then only applies to the other
+ * launches.  Off: every fused launch advances each instance at most
+ * steps_per_launch steps.  Results are bit-identical either way. */
+int dnx_batch_set_flow(dnx_batch *b, int enable);
+
 /* Wide mode with a threshold relative to the run: entered once the running
  * instances hold at most `fraction` of the clauses that were running when
- * dnx_batch_run began (the default: 0.02, 30 steps per sync). */
+ * dnx_batch_run began (the default: 0.01, 30 steps per sync). */
 int dnx_batch_set_wide_fraction(dnx_batch *b, double fraction, int steps_per_sync);
 
 /* Serving: requests with at least `clauses` clauses run in the wide lane

@@ -48,14 +48,16 @@ def oracle_solve(n, cp, lits, x0, cap):
     return steps, t, status, x
 
 
-def run_batch(ctx, inst, cap, spl, shapes=None, stop=True, wide=0, gang=None, split=False):
+def run_batch(ctx, inst, cap, spl, shapes=None, stop=True, wide=0, gang=None, split=False, flow=True):
     """gang = (max_blocks, clauses_per_workgroup, steps_per_launch) enables
     gang mode for the tail (the instances left once wide mode applies);
     split keeps the single-chunk LDS instances of the tail in the fused
-    kernel beside wide mode."""
+    kernel beside wide mode; flow runs the fused launch until the wide-mode
+    threshold instead of spl steps per launch."""
     b = dnx.Batch(ctx, slots=len(inst) + 2, arena_bytes=1 << 30)
     b.set_wide(wide, 30)
     b.set_tail_split(split)
+    b.set_flow(flow)
     if gang:
         b.set_gang(*gang)
     else:
@@ -104,6 +106,18 @@ def test_batch_matches_oracle(ctx, inst_set, oracle_results):
     # with the default 48 KiB mode-0 budget the larger instances hold their
     # contributions in LDS chunk by chunk; mode 1 is covered by the shapes below
     assert 0 in modes
+
+
+@pytest.mark.parametrize("flow,wide", [(False, 0), (True, 0), (True, 60000), (True, 1 << 40)])
+def test_flow_gives_identical_bits(ctx, inst_set, oracle_results, flow, wide):
+    """Flow launches (each instance kept on its workgroup until it ends or
+    the running clauses fall to the wide-mode threshold) against launches of
+    a fixed number of steps: the same bits, with and without wide mode."""
+    out = run_batch(ctx, inst_set, 1500, spl=29, flow=flow, wide=wide)
+    for r, (steps, t, status, xo) in zip(out, oracle_results):
+        assert r["status"] == (dnx.SAT if status == 1 else dnx.UNKNOWN)
+        assert r["steps"] == steps and r["t"] == t
+        assert np.array_equal(r["x"], xo)
 
 
 @pytest.mark.parametrize("shapes", [
