@@ -214,13 +214,53 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
             for (int ch = 0; ch < nch; ch++) {
                 const int c_lo = MODE == BM_LDS_ALL ? s_cb[ch] : 0;
                 const int c_hi = MODE == BM_LDS_ALL ? s_cb[ch + 1] : m;
-                if (!general && steady) {
-                    // 3-SAT in the steady state: V in place and clamped, the
-                    // memories clamped by the previous update, C of the
-                    // previous state stored (the steady-state forms return
-                    // the same bits as the general path below)
-                    // Mode 0 reads one word per literal (sign, variable and
-                    // LDS slot packed), the other modes the signed literals.
+                if (MODE == BM_LDS_ALL && !general && steady) {
+                    // 3-SAT in the steady state, mode 0: V in place and
+                    // clamped, the memories clamped by the previous update,
+                    // C of the previous state stored (the steady-state forms
+                    // return the same bits as the general path below).  One
+                    // word per literal packs its sign, variable and LDS slot.
+                    // (Buffer-resource addressing of the device arrays
+                    // measured slower than these global accesses.)
+                    float cmax = 0.0f;  // C >= 1/2 anywhere means unsolved
+                    constexpr int U = 4;
+                    for (int cb = c_lo + tid; cb < c_hi; cb += U * BS) {
+                        uint32_t W[U][3];
+                        float XS[U], XL[U], CX[U];
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                            const int c = cb + u * BS;
+                            if (c < c_hi) {
+                                W[u][0] = I.pk[3 * c];
+                                W[u][1] = I.pk[3 * c + 1];
+                                W[u][2] = I.pk[3 * c + 2];
+                                XS[u] = I.xs[c];
+                                XL[u] = I.xl[c];
+                                CX[u] = I.cst[c];
+                            }
+                        }
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                            const int c = cb + u * BS;
+                            if (c >= c_hi) break;
+                            const int32_t l0 = (int32_t)W[u][0], l1 = (int32_t)W[u][1], l2 = (int32_t)W[u][2];
+                            float xs = XS[u], xl = XL[u];
+                            const float V0 = pvv(__builtin_amdgcn_ubfe(W[u][0], 16, 15));
+                            const float V1 = pvv(__builtin_amdgcn_ubfe(W[u][1], 16, 15));
+                            const float V2 = pvv(__builtin_amdgcn_ubfe(W[u][2], 16, 15));
+                            memory_update_steady(p, CX[u], h, xs, xl);
+                            I.xs[c] = xs;
+                            I.xl[c] = xl;
+                            auto emit = [&](int j, float val) { se[W[u][j] & 0xffffu] = val; };
+                            const float C = clause3_steady(p, l0, l1, l2, V0, V1, V2, xs, xl, emit);
+                            I.cst[c] = C;
+                            cmax = dmax(cmax, C);
+                        }
+                    }
+                    unsolved |= !(cmax < 0.5f);
+                } else if (!general && steady) {
+                    // the same for modes 1 and 2 (signed literals,
+                    // contributions in device memory)
                     constexpr int U = 4;
                     for (int cb = c_lo + tid; cb < c_hi; cb += U * BS) {
                         int32_t L[U][3];
@@ -272,11 +312,11 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                         }
                     }
                 } else if (!general) {
-                    // 3-SAT: each thread takes its clauses four at a time, with
-                    // all four clauses' literals, memories, stored C (and LDS
-                    // slots) loaded before any is evaluated, so that enough
-                    // loads are in flight to cover device-memory latency.
-                    constexpr int U = 4;
+                    // 3-SAT outside the steady state (a launch's first steps):
+                    // each thread takes its clauses two at a time, with both
+                    // clauses' literals, memories, stored C (and LDS slots)
+                    // loaded before either is evaluated.
+                    constexpr int U = 2;
                     for (int cb = c_lo + tid; cb < c_hi; cb += U * BS) {
                         int32_t L[U][3], P[U][3];
                         float XS[U], XL[U], CX[U];
