@@ -150,7 +150,10 @@ struct dnx_batch {
     // starts) clauses in total; wide_chunk steps between status syncs
     long long wide_clauses = 0;
     double wide_frac = 0.01;
-    int wide_chunk = 30;
+    int wide_chunk = 300;
+    // serving: steps per chunk of the wide lane (status syncs and active-set
+    // choices happen between chunks)
+    int serve_chunk = 30;
     // serving: requests with at least this many clauses run in the wide lane
     long long serve_big = 60000;
     // serving: clause budget of the wide lane's active set per chunk
@@ -1555,7 +1558,7 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
                     (void)hipGraphExecDestroy(wgraph);
                     wgraph = nullptr;
                 }
-                const int chunk = (b->wide_chunk + 2) / 3 * 3;
+                const int chunk = (b->serve_chunk + 2) / 3 * 3;
                 hipGraph_t g;
                 if ((e = hipStreamBeginCapture(wst, hipStreamCaptureModeThreadLocal)) != hipSuccess)
                     return hip_fail(e, "wide capture");

@@ -401,7 +401,7 @@ class Batch:
     def set_serve_big(self, clauses, active_clauses=6000000):
         check(lib().dnx_batch_set_serve_big(self.h, int(clauses), int(active_clauses)))
 
-    def set_wide(self, max_clauses, steps_per_sync=30):
+    def set_wide(self, max_clauses, steps_per_sync=300):
         check(lib().dnx_batch_set_wide(self.h, int(max_clauses), int(steps_per_sync)))
 
     def set_tail_split(self, enable=True):
@@ -410,7 +410,7 @@ class Batch:
     def set_flow(self, enable=True):
         check(lib().dnx_batch_set_flow(self.h, int(bool(enable))))
 
-    def set_wide_fraction(self, fraction=0.01, steps_per_sync=30):
+    def set_wide_fraction(self, fraction=0.01, steps_per_sync=300):
         check(lib().dnx_batch_set_wide_fraction(self.h, float(fraction), int(steps_per_sync)))
 
     def timing(self, enable=-1, with_bytes=False):

@@ -240,14 +240,15 @@ int dnx_batch_set_flow(dnx_batch *b, int enable);
 
 /* Wide mode with a threshold relative to the run: entered once the running
  * instances hold at most `fraction` of the clauses that were running when
- * dnx_batch_run began (the default: 0.01, 30 steps per sync). */
+ * dnx_batch_run began (the default: 0.01, 300 steps per sync). */
 int dnx_batch_set_wide_fraction(dnx_batch *b, double fraction, int steps_per_sync);
 
 /* Serving: requests with at least `clauses` clauses run in the wide lane
- * (default 60,000; needs wide mode enabled).  Each wide chunk advances the
- * members with the fewest steps so far, up to active_clauses clauses in
- * total (default 6,000,000): least-attained-service order, so new requests
- * start at once and, under overload, the longest-running ones wait. */
+ * (default 60,000; needs wide mode enabled).  Each wide chunk (30 steps,
+ * then a status sync) advances the members with the fewest steps so far, up
+ * to active_clauses clauses in total (default 6,000,000):
+ * least-attained-service order, so new requests start at once and, under
+ * overload, the longest-running ones wait. */
 int dnx_batch_set_serve_big(dnx_batch *b, int64_t clauses, int64_t active_clauses);
 
 /* Autotuner: times candidate execution shapes on `sample` synthetic planted
