@@ -29,9 +29,9 @@ HBM_PEAK_GBPS = 8000.0
 # two rocprofv3 --pmc runs of `bench.py --profile-pass` (FETCH_SIZE, then
 # WRITE_SIZE, per dispatch of the batched solve), each scaled by what the
 # counter reads on the copy kernel (256 MiB each way: FETCH_SIZE x 2.000,
-# WRITE_SIZE x 1.000).  Keyed by bucket; k_batch<mode 0> with 2048 steps per
-# launch: 168.98 GB read + 58.79 GB written.
-TRAFFIC = {0: 227.77e9}
+# WRITE_SIZE x 1.000).  Keyed by bucket; k_batch<mode 0>, the flow launch:
+# 477.40 GB read + 280.67 GB written.
+TRAFFIC = {0: 758.07e9}
 METRIC = "solved instances/s + p50/p99 time-to-solution (batched FP32 vs plain FP64, 1 GPU)"
 
 
@@ -351,7 +351,7 @@ def main():
     nl_sum = np.sum([r["launches"] for r in runs], axis=0)
     ab_sum = np.sum([r["alg"] for r in runs], axis=0)
     names = ["k_batch<mode 0>", "k_batch<mode 1>", "k_batch<mode 2>",
-             "k_wide_clause+k_wide_var (one chunk of 30 steps)", "k_gang",
+             "k_wide_clause+k_wide_var (one chunk of steps between status syncs)", "k_gang",
              "k_batch<mode 0> tail launch beside wide mode"]
     dom = int(np.argmax(ms_sum))
     avg_us = float(ms_sum[dom] / nl_sum[dom] * 1e3) if nl_sum[dom] else 0.0
@@ -360,8 +360,7 @@ def main():
     all_s = float(ms_sum.sum()) / 1e3
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBPS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBPS, 4),
-                    traffic=TRAFFIC.get(dom) if (args.instances, args.steps_per_launch, args.seed) == (4096, 2048, 4096)
-                    else None,
+                    traffic=TRAFFIC.get(dom) if (args.instances, args.seed) == (4096, 4096) else None,
                     kernel=names[dom], alg_bytes_per_launch=round(per_launch), avg_launch_us=round(avg_us, 2),
                     device_ms={names[i]: round(float(ms_sum[i]), 2) for i in range(6)},
                     launches={names[i]: int(nl_sum[i]) for i in range(6)},
@@ -434,7 +433,7 @@ def main():
             "workload": "mixed planted 3-SAT, 4096 instances: n~LogU[100,5000], m=round(U[4.3,8]*n), "
                         "P(t)=.46/.46/.08, v~U[-1,1), Xs=0.5, Xl=1, step cap 100k, batched FP32",
             "instances": len(work),
-            "steps_per_launch": args.steps_per_launch,
+            "fused_launch": "flow: each instance on its workgroup until it ends or 1% of the clauses run",
         },
         "solved": int((last["status"] == dnx.SAT).sum()),
         # admission (host tables -> device arena over PCIe, one instance at a time) is outside the
