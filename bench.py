@@ -275,7 +275,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--cap", type=int, default=100000)
-    ap.add_argument("--steps-per-launch", type=int, default=2048)
+    # fused launches of fixed length: the served stream's narrow lane (a new
+    # request waits for the running launch to end) and any fallback from
+    # flow launches in the batched solve
+    ap.add_argument("--steps-per-launch", type=int, default=256)
     ap.add_argument("--fp64-sample", type=int, default=64)
     ap.add_argument("--fp64-seconds", type=float, default=25.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)

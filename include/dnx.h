@@ -288,7 +288,9 @@ int dnx_batch_memory(dnx_batch *b, int64_t *arena_bytes, int64_t *free_bytes);
  * Request i becomes visible arrival_s[i] seconds after the call starts
  * (non-decreasing); it is admitted as soon as a slot and an arena range are
  * free (first come, first served), advanced alongside everything resident in
- * launches of steps_per_launch steps, and retired the moment its state
+ * launches of steps_per_launch steps (a new request waits for the running
+ * launch to end: 128-256 keeps that wait to a few ms), and retired the
+ * moment its state
  * verifies or reaches step_cap, freeing its memory for the queue.  Per
  * request: status (DNX_SAT only after the exact clause check; DNX_FAILED for
  * a request larger than the arena), steps, time-to-solution from arrival to
