@@ -221,6 +221,47 @@ def cpu_baseline_cores(work, idx, cap, budget_s, procs):
                 seconds=busy, wall=wall)
 
 
+def _cpu_served_worker(args):
+    insts, cap, budget_s, us_per_clause_step = args
+    O = _oracle()
+    t_used, done, solved = 0.0, 0, 0
+    for w in insts:
+        left = budget_s - t_used
+        if left <= 0:
+            break
+        # a step cap that ends the solve near the end of the budget (at half
+        # the oracle's measured time per clause-step, so no instance that
+        # would finish inside the budget is stopped early)
+        c = int(min(cap, left / (0.5 * us_per_clause_step * 1e-6 * w["m"])))
+        if c < 1:
+            break
+        op = O.Problem(w["n"], w["m"], w["cp"], w["lits"])
+        x = w["x0"].copy()
+        t0 = time.perf_counter()
+        steps, t, status = O.solve(op, O.params(w["m"]), x, c)
+        t_used += time.perf_counter() - t0
+        done += 1
+        solved += status == 1
+    return dict(instances=done, solved=solved, seconds=t_used)
+
+
+def cpu_served_cores(count, seed, cap, budget_s, procs, us_per_clause_step):
+    """The FP64 oracle on instances of the served stream's distribution (n log-uniform in [100, 100000]),
+    `procs` processes at once for about budget_s seconds each: instances solved per second of the host."""
+    import multiprocessing as mp
+    work = served_sample(count, seed)
+    chunks = [work[k::procs] for k in range(procs)]
+    with mp.get_context("spawn").Pool(procs) as pool:
+        res = pool.map(_cpu_served_worker, [(c, cap, budget_s, us_per_clause_step) for c in chunks])
+    busy = max(r["seconds"] for r in res)
+    solved = sum(r["solved"] for r in res)
+    return dict(value=round(solved / busy, 3) if busy else 0.0, instances=sum(r["instances"] for r in res),
+                solved=solved, seconds=round(busy, 1), cores=procs,
+                sample=f"{count} instances of the served distribution (n~LogU[100,100000], r~U[4.3,8]), "
+                       f"FP64 oracle, {procs} processes x ~{budget_s:.0f}s; an instance not solved inside "
+                       f"the budget counts as unsolved")
+
+
 def cpu_reference_points(cap):
     """Oracle per-step time on the n=100,000 single instance and time-to-solution on n=100, m=800."""
     O = _oracle()
@@ -412,12 +453,17 @@ def main():
     cpu_sample = np.random.default_rng(8).permutation(len(work))
     cpu = cpu_baseline_cores(work, cpu_sample, args.cap, args.cpu_seconds, procs)
     cpu1 = cpu_baseline(work, sample, args.cap, min(args.cpu_seconds, 10.0))
+    ref_points = cpu_reference_points(args.cap)
     cpu_line = dict(value=round(cpu["value"], 3), unit="instances/s", cores=procs, kind="port",
                     sample=f"{cpu['instances']} instances of the same workload (random sample), FP64 oracle, "
                            f"{procs} processes x {cpu['seconds']:.1f}s",
                     one_thread=dict(value=round(cpu1["value"], 3), instances=cpu1["instances"],
                                     seconds=round(cpu1["seconds"], 1)),
-                    **cpu_reference_points(args.cap))
+                    **ref_points)
+    if args.served_seconds > 0:
+        log("CPU oracle on a sample of the served distribution ...")
+        cpu_line["served_sample"] = cpu_served_cores(procs * 24, 991, args.cap, args.cpu_seconds, procs,
+                                                     ref_points["n100k_step_ms"] * 1e3 / 800000)
 
     out = {
         "metric": METRIC,
