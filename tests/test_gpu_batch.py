@@ -11,6 +11,13 @@ from conftest import planted_cp
 pytestmark = pytest.mark.gpu
 
 
+def same_bits(a, b):
+    """Equal float32 arrays bit for bit (signed zeros included)."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
 @pytest.fixture(scope="module")
 def ctx():
     c = dnx.Context(0)
@@ -100,7 +107,7 @@ def test_batch_matches_oracle(ctx, inst_set, oracle_results):
         modes.add(r["mode"])
         assert r["status"] == (dnx.SAT if status == 1 else dnx.UNKNOWN)
         assert r["steps"] == steps and r["t"] == t
-        assert np.array_equal(r["x"], xo)
+        assert same_bits(r["x"], xo)
         if status == 1:
             assert r["unsat"] == 0
     # with the default 48 KiB mode-0 budget the larger instances hold their
@@ -117,7 +124,7 @@ def test_flow_gives_identical_bits(ctx, inst_set, oracle_results, flow, wide):
     for r, (steps, t, status, xo) in zip(out, oracle_results):
         assert r["status"] == (dnx.SAT if status == 1 else dnx.UNKNOWN)
         assert r["steps"] == steps and r["t"] == t
-        assert np.array_equal(r["x"], xo)
+        assert same_bits(r["x"], xo)
 
 
 @pytest.mark.parametrize("shapes", [
@@ -129,7 +136,7 @@ def test_shapes_give_identical_bits(ctx, inst_set, oracle_results, shapes):
     out = run_batch(ctx, inst_set, 1500, spl=64, shapes=shapes)
     for r, (steps, t, status, xo) in zip(out, oracle_results):
         assert r["steps"] == steps and r["t"] == t
-        assert np.array_equal(r["x"], xo)
+        assert same_bits(r["x"], xo)
 
 
 @pytest.mark.parametrize("wide", [1 << 40, 60000])
@@ -140,7 +147,7 @@ def test_wide_mode_gives_identical_bits(ctx, inst_set, oracle_results, wide):
     for r, (steps, t, status, xo) in zip(out, oracle_results):
         assert r["status"] == (dnx.SAT if status == 1 else dnx.UNKNOWN)
         assert r["steps"] == steps and r["t"] == t
-        assert np.array_equal(r["x"], xo)
+        assert same_bits(r["x"], xo)
         if status == 1:
             assert r["unsat"] == 0
 
@@ -155,7 +162,7 @@ def test_gang_mode_gives_identical_bits(ctx, inst_set, oracle_results, wide, gan
     for r, (steps, t, status, xo) in zip(out, oracle_results):
         assert r["status"] == (dnx.SAT if status == 1 else dnx.UNKNOWN)
         assert r["steps"] == steps and r["t"] == t
-        assert np.array_equal(r["x"], xo)
+        assert same_bits(r["x"], xo)
         if status == 1:
             assert r["unsat"] == 0
 
@@ -167,7 +174,7 @@ def test_tail_split_gives_identical_bits(ctx, inst_set, oracle_results, wide):
     for r, (steps, t, status, xo) in zip(out, oracle_results):
         assert r["status"] == (dnx.SAT if status == 1 else dnx.UNKNOWN)
         assert r["steps"] == steps and r["t"] == t
-        assert np.array_equal(r["x"], xo)
+        assert same_bits(r["x"], xo)
         if status == 1:
             assert r["unsat"] == 0
 
@@ -182,7 +189,7 @@ def test_tail_split_fixed_steps(ctx):
             x[:n] = np.where(op.unit != 0, op.unit, x[:n])
         O.trajectory(op, O.params(op.m), x, 45, prec=32)
         assert r["status"] == dnx.UNKNOWN and r["steps"] == 45
-        assert np.array_equal(r["x"], x)
+        assert same_bits(r["x"], x)
 
 
 def test_gang_fixed_step_trajectory(ctx):
@@ -197,7 +204,7 @@ def test_gang_fixed_step_trajectory(ctx):
             x[:n] = np.where(op.unit != 0, op.unit, x[:n])
         O.trajectory(op, O.params(op.m), x, 45, prec=32)
         assert r["status"] == dnx.UNKNOWN and r["steps"] == 45
-        assert np.array_equal(r["x"], x)
+        assert same_bits(r["x"], x)
 
 
 def test_full_mixed_workload(ctx):
@@ -254,7 +261,7 @@ def test_lds_edge_instance(ctx):
     steps, t, status, xo = oracle_solve(n, cp, lits, x0, 300)
     assert out["mode"] == 0
     assert out["steps"] == steps and out["t"] == t
-    assert np.array_equal(out["x"], xo)
+    assert same_bits(out["x"], xo)
 
 
 def test_wide_fixed_step_trajectory(ctx):
@@ -267,7 +274,7 @@ def test_wide_fixed_step_trajectory(ctx):
             x[:n] = np.where(op.unit != 0, op.unit, x[:n])
         O.trajectory(op, O.params(op.m), x, 45, prec=32)
         assert r["status"] == dnx.UNKNOWN and r["steps"] == 45
-        assert np.array_equal(r["x"], x)
+        assert same_bits(r["x"], x)
 
 
 def test_alone_equals_in_the_mix(ctx, inst_set):
@@ -275,7 +282,7 @@ def test_alone_equals_in_the_mix(ctx, inst_set):
     for i in (0, 5, 17, 35):
         alone = run_batch(ctx, [inst_set[i]], 1500, spl=9)[0]
         assert alone["steps"] == mix[i]["steps"] and alone["t"] == mix[i]["t"]
-        assert np.array_equal(alone["x"], mix[i]["x"])
+        assert same_bits(alone["x"], mix[i]["x"])
 
 
 def test_fixed_step_trajectory(ctx):
@@ -288,7 +295,7 @@ def test_fixed_step_trajectory(ctx):
             x[:n] = np.where(op.unit != 0, op.unit, x[:n])
         O.trajectory(op, O.params(op.m), x, 60, prec=32)
         assert r["status"] == dnx.UNKNOWN and r["steps"] == 60
-        assert np.array_equal(r["x"], x)
+        assert same_bits(r["x"], x)
 
 
 def test_single_instance_path_agrees(ctx, inst_set):
@@ -392,5 +399,5 @@ def test_autotuner_applies_a_shape(ctx, inst_set, oracle_results):
     res = b.poll()
     for s, (steps, t, status, xo), (n, cp, lits, x0) in zip(slots, oracle_results[:10], inst_set[:10]):
         assert res["steps"][s] == steps and res["t"][s] == t
-        assert np.array_equal(b.read_state(s, n, len(cp) - 1)["x"].astype(np.float32), xo)
+        assert same_bits(b.read_state(s, n, len(cp) - 1)["x"].astype(np.float32), xo)
     b.close()
