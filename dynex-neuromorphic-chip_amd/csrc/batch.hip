@@ -705,7 +705,8 @@ __global__ __launch_bounds__(WBS) void k_wide_var(const BInst *__restrict__ inst
     const bool solved = ctl->stop_on_solve && !any_flag(ctl->flags[r][FG_UNSOLVED]);
     const int done = solved ? DNX_SAT : (ctl->steps >= ctl->cap ? DNX_UNKNOWN : 0);
     // contributions are variable-major in wide mode (k_wide_clause)
-    const float dVf = wave_fold_vm<WCAPW>(I.e, rb, re, v0 < I.n ? min(63, I.n - 1 - v0) : 0, stage[w]);
+    const float dVf =
+        wave_fold_vm<WCAPW>([&](int q) { return I.e[q]; }, rb, re, v0 < I.n ? min(63, I.n - 1 - v0) : 0, stage[w]);
     int f = 0;
     if (vok) {
         const float V = pend ? clampV(a.v + h * a.d) : a.v;
@@ -764,29 +765,56 @@ void launch_wide_step(const BInst *insts, Ctl *wctl, const int *bmap, int nbc, c
 // chain of dependent device-memory round trips per step.  Here each running
 // instance is advanced by a gang of G workgroups (G <= GANG_MAX) for K steps
 // inside one cooperative launch (every workgroup resident at once).  A
-// workgroup owns a contiguous range of clauses, whose literals and memories
-// stay in LDS for the whole launch, and a range of variables.  A step is:
-// clause pass (contributions to e[]), gang barrier carrying the OR of the
-// "unsolved" verdicts, variable pass (fold, (V, dV) pairs, ladder level),
-// gang barrier carrying the maximum level.  The arithmetic is the
-// single-instance FP32 arithmetic with the same clause-order fold.
+// workgroup owns a contiguous range of clauses, whose literals, transpose
+// slots and memories stay in LDS for the whole launch, and a range of
+// variables.  A step is: clause pass (contributions to their variable-major
+// slots of e[]), gang barrier carrying the OR of the "unsolved" verdicts,
+// variable pass (contiguous fold, (V, dV) pairs, ladder level), gang barrier
+// carrying the maximum level.  The arithmetic is the single-instance FP32
+// arithmetic with the same clause-order fold.
 //
-// Barrier: workgroup `rank` of a gang publishes (epoch << 16 | value) in its
-// own 128-byte line of slot (epoch & 1) after an agent-scope release fence;
-// one wave polls the gang's G lines (one lane each) until every epoch has
-// arrived, then takes the OR / max of the values and fences (acquire).  Two
-// slots suffice: no workgroup can publish epoch e+2 before every workgroup
-// has published e+1, which is after it finished reading epoch e.  Epochs
-// grow across launches (the host passes each launch a fresh base), so the
-// lines never need clearing.  Every wait is bounded: a wait that exceeds
-// GANG_SPIN polls sets *err and the launch ends on every workgroup.
+// All workgroups of a gang run on one XCD (the host places gang g's
+// workgroups at block indices congruent to its XCD modulo 8, the order in
+// which the dispatcher hands workgroups to XCDs; every workgroup checks its
+// XCD id before the first step), so the XCD's L2 is the one point where
+// they see each other's data.  A barrier is then: every wave waits until
+// its stores have reached L2 (vmcnt 0), workgroup `rank` publishes
+// (epoch << 16 | value) in its own 128-byte line of slot (epoch & 1), one
+// wave polls the gang's G lines (one lane each) until every epoch has
+// arrived and takes the OR / max of the values.  Loads of data that other
+// workgroups of the gang write during the launch (V/dV pairs,
+// contributions) bypass the CU's L1 (agent-scope relaxed loads).  No
+// agent-scope fence is needed: those write back and invalidate the whole
+// L2, which measured 3-5x slower per barrier.  Two line slots suffice: no
+// workgroup can publish epoch e+2 before every workgroup has published e+1,
+// which is after it finished reading epoch e.  Epochs grow across launches
+// (the host passes each launch a fresh base), so the lines never need
+// clearing.  Every wait is bounded: a wait that exceeds GANG_SPIN polls sets
+// *err = 1 and the launch ends on every workgroup.
 // ---------------------------------------------------------------------------
-constexpr int GBS = 256;
+constexpr int GBS = 512;
 constexpr int GANG_SPIN = 1 << 21;
+
+__device__ __forceinline__ int xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20); }
 
 __device__ __forceinline__ unsigned long long *gang_line(unsigned long long *words, int gang, int s, int rank)
 {
     return words + (((size_t)gang * 2 + s) * GANG_MAX + rank) * GANG_LINE;
+}
+
+// (V, dV) pair and contribution loads from L2 (bypassing the CU's L1)
+__device__ __forceinline__ BF2 vdr_l2(const GInst &I, int i)
+{
+    const unsigned long long u =
+        __hip_atomic_load((unsigned long long *)(I.vd + i), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    BF2 r;
+    r.v = __uint_as_float((unsigned)u);
+    r.d = __uint_as_float((unsigned)(u >> 32));
+    return r;
+}
+__device__ __forceinline__ float er_l2(const GInst &I, int q)
+{
+    return __hip_atomic_load((float *)(I.e + q), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // All threads of the workgroup call this.  Returns false when the wait
@@ -796,13 +824,14 @@ __device__ __forceinline__ bool gang_barrier(unsigned long long *words, int gang
                                              unsigned long long epoch, int value, int op, int &res, int *err)
 {
     __shared__ int s_res, s_ok;
+    // this wave's stores of the step have reached the XCD's L2
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     __syncthreads();
     const int s = (int)(epoch & 1);
-    if (threadIdx.x == 0) {
-        __threadfence();
+    if (threadIdx.x == 0)
         __hip_atomic_store(gang_line(words, gang, s, rank), (epoch << 16) | (unsigned)(value & 0xffff),
                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
         bool ok = false;
@@ -825,24 +854,45 @@ __device__ __forceinline__ bool gang_barrier(unsigned long long *words, int gang
             s_res = r;
             if (!ok) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        __threadfence();
     }
     __syncthreads();
     res = s_res;
     return s_ok != 0;
 }
 
+// map[b] = (slot, rank, G, gang) for block b (slot -1: idle block).  Before
+// the first step every block checks that it runs on XCD b % 8; one that does
+// not sets *err = 2, and since every block passes a launch-wide arrival count
+// (*arrive, compared against arrive_base + nblk) before touching any state,
+// a misplaced launch ends with every instance unchanged.
 __global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, BDyn *__restrict__ dyn,
                                              const int4 *__restrict__ map, int nblk,
                                              unsigned long long *__restrict__ words, unsigned long long base, int K,
-                                             int *__restrict__ err)
+                                             int *__restrict__ err, int *__restrict__ arrive, int arrive_base)
 {
     __shared__ int32_t s_lits[3 * GANG_CPW];
+    __shared__ int32_t s_gpos[3 * GANG_CPW];
     __shared__ float s_xs[GANG_CPW], s_xl[GANG_CPW];
     __shared__ float stage[GBS / 64][WCAPW];
     __shared__ int sh[GBS / 64];
+    __shared__ int s_go;
     const int b = blockIdx.x;
     if (b >= nblk) return;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        if (xcc_id() != (b & 7)) __hip_atomic_store(err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        __hip_atomic_fetch_add(arrive, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int it = 0;
+        while (__hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - arrive_base < nblk &&
+               ++it < GANG_SPIN)
+            __builtin_amdgcn_s_sleep(1);
+        __threadfence();
+        if (it >= GANG_SPIN) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_go = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+    }
+    __syncthreads();
+    if (!s_go) return;
     const int4 gm = map[b];
     const int slot = __builtin_amdgcn_readfirstlane(gm.x), rank = __builtin_amdgcn_readfirstlane(gm.y);
     const int G = __builtin_amdgcn_readfirstlane(gm.z), gang = __builtin_amdgcn_readfirstlane(gm.w);
@@ -850,7 +900,7 @@ __global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, B
     const BDyn d = dyn[slot];
     if (d.status != DNX_RUNNING) return;
     const GInst I = global_view(insts[slot]);
-    const int n = I.n, m = I.m, tid = threadIdx.x;
+    const int n = I.n, m = I.m;
     const bool general = (I.flags & BF_GENERAL) != 0;
     PT<float> p;
     p.alpha = I.alpha;
@@ -867,12 +917,24 @@ __global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, B
     const int Vb = ((n + G - 1) / G + GBS - 1) / GBS * GBS;
     const int vb = rank * Vb;
     if (!general)
-        for (int i = tid; i < 3 * cn; i += GBS) s_lits[i] = I.lits[3 * c0 + i];
+        for (int i = tid; i < 3 * cn; i += GBS) {
+            s_lits[i] = I.lits[3 * c0 + i];
+            s_gpos[i] = I.gpos[3 * c0 + i];
+        }
     for (int i = tid; i < cn; i += GBS) {
         s_xs[i] = I.xs[c0 + i];
         s_xl[i] = I.xl[c0 + i];
     }
     __syncthreads();
+    // slot range of each thread's first variable (read once per launch)
+    int RB0 = 0, RE0 = 0;
+    {
+        const int v = vb + tid;
+        if (v < n && v < vb + Vb) {
+            RB0 = I.row_ptr[v];
+            RE0 = I.row_ptr[v + 1];
+        }
+    }
     bool pend = d.plevel >= 0;
     int plev = d.plevel;
     long long steps = d.steps;
@@ -887,14 +949,50 @@ __global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, B
         }
         // ---- clause pass ----------------------------------------------------
         int unsolved = 0;
-        for (int i = tid; i < cn; i += GBS) {
+        if (!general) {
+            // 3-SAT: each thread takes its clauses two at a time, all six
+            // (V, dV) pairs requested before either clause is evaluated
+            for (int ib = tid; ib < cn; ib += 2 * GBS) {
+                BF2 A[2][3];
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const int i = ib + u * GBS;
+                    if (i < cn)
+#pragma unroll
+                        for (int j = 0; j < 3; j++) A[u][j] = vdr_l2(I, abs(s_lits[3 * i + j]) - 1);
+                }
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const int i = ib + u * GBS;
+                    if (i >= cn) break;
+                    const int32_t l0 = s_lits[3 * i], l1 = s_lits[3 * i + 1], l2 = s_lits[3 * i + 2];
+                    const BF2 a0 = A[u][0], a1 = A[u][1], a2 = A[u][2];
+                    float xs = s_xs[i], xl = s_xl[i];
+                    float r0 = a0.v, r1 = a1.v, r2 = a2.v;
+                    if (pend) {
+                        memory_update(p, clause3_C_vals<float>(l0, l1, l2, r0, r1, r2), h, xs, xl);
+                        s_xs[i] = xs;
+                        s_xl[i] = xl;
+                        r0 = clampV(a0.v + h * a0.d);
+                        r1 = clampV(a1.v + h * a1.d);
+                        r2 = clampV(a2.v + h * a2.d);
+                    }
+                    auto emit = [&](int j, float val) { I.e[s_gpos[3 * i + j]] = val; };
+                    bool sat;
+                    const float C = clause3_vals<float>(p, l0, l1, l2, r0, r1, r2, clamp01(xs), clampXl(xl, p.xlmax),
+                                                        emit, sat);
+                    unsolved |= !(C < 0.5f) || !sat;
+                }
+            }
+        }
+        for (int i = tid; general && i < cn; i += GBS) {
             const int c = c0 + i;
             float xs = s_xs[i], xl = s_xl[i];
             bool sat;
             float C;
             if (!general) {
                 const int32_t l0 = s_lits[3 * i], l1 = s_lits[3 * i + 1], l2 = s_lits[3 * i + 2];
-                const BF2 a0 = vdr(I, abs(l0) - 1), a1 = vdr(I, abs(l1) - 1), a2 = vdr(I, abs(l2) - 1);
+                const BF2 a0 = vdr_l2(I, abs(l0) - 1), a1 = vdr_l2(I, abs(l1) - 1), a2 = vdr_l2(I, abs(l2) - 1);
                 float r0 = a0.v, r1 = a1.v, r2 = a2.v;
                 if (pend) {
                     memory_update(p, clause3_C_vals<float>(l0, l1, l2, r0, r1, r2), h, xs, xl);
@@ -904,16 +1002,15 @@ __global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, B
                     r1 = clampV(a1.v + h * a1.d);
                     r2 = clampV(a2.v + h * a2.d);
                 }
-                auto ep = I.e + 3 * c;
-                auto emit = [&](int j, float val) { ep[j] = val; };
+                auto emit = [&](int j, float val) { I.e[s_gpos[3 * i + j]] = val; };
                 C = clause3_vals<float>(p, l0, l1, l2, r0, r1, r2, clamp01(xs), clampXl(xl, p.xlmax), emit, sat);
             } else {
                 const int bb = I.cp[c];
                 const int kk = I.cp[c + 1] - bb;
                 auto cl = I.lits + bb;
-                auto prevV = [&](int v) { return vdr(I, v).v; };
+                auto prevV = [&](int v) { return vdr_l2(I, v).v; };
                 auto curV = [&](int v) {
-                    BF2 a = vdr(I, v);
+                    BF2 a = vdr_l2(I, v);
                     return pend ? clampV(a.v + h * a.d) : a.v;
                 };
                 if (pend) {
@@ -921,8 +1018,8 @@ __global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, B
                     s_xs[i] = xs;
                     s_xl[i] = xl;
                 }
-                auto ep = I.e + bb;
-                auto emit = [&](int j, float val) { ep[j] = val; };
+                auto gp = I.gpos + bb;
+                auto emit = [&](int j, float val) { I.e[gp[j]] = val; };
                 C = clauseK<float>(p, cl, kk, clamp01(xs), clampXl(xl, p.xlmax), curV, emit, sat);
             }
             unsolved |= !(C < 0.5f) || !sat;
@@ -937,10 +1034,14 @@ __global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, B
         int fmax = 0;
         const int w = tid >> 6;
         for (int v0 = vb + w * 64; v0 < vb + Vb; v0 += GBS) {
-            const float dVf = wave_fold<float, false, WCAPW>(I.e, I.eidx, I.row_ptr, n, v0, !done, stage[w], nullptr);
             const int v = v0 + (tid & 63);
-            if (v < n) {
-                BF2 a = vdr(I, v);
+            const bool vok = v < n;
+            const bool first = v0 == vb + w * 64;
+            const int rb = first ? RB0 : vok ? I.row_ptr[v] : 0, re = first ? RE0 : vok ? I.row_ptr[v + 1] : 0;
+            const BF2 a = vok ? vdr_l2(I, v) : BF2{0.0f, 0.0f};  // requested before the fold
+            const float dVf = wave_fold_vm<WCAPW>([&](int q) { return er_l2(I, q); }, rb, re,
+                                                  v0 < n ? min(63, n - 1 - v0) : 0, stage[w]);
+            if (vok) {
                 const float V = pend ? clampV(a.v + h * a.d) : a.v;
                 float dV = 0.0f;
                 if (!done) {
@@ -990,12 +1091,12 @@ int gang_occupancy()
 }
 
 int launch_gang(const BInst *insts, BDyn *dyn, const int *map, int nblk, unsigned long long *words,
-                unsigned long long base, int K, int *err, void *stream)
+                unsigned long long base, int K, int *err, int *arrive, int arrive_base, void *stream)
 {
     if (nblk <= 0) return 0;
     const int4 *mp = (const int4 *)map;
     void *args[] = {(void *)&insts, (void *)&dyn, (void *)&mp, (void *)&nblk, (void *)&words, (void *)&base,
-                    (void *)&K, (void *)&err};
+                    (void *)&K, (void *)&err, (void *)&arrive, (void *)&arrive_base};
     return (int)hipLaunchCooperativeKernel((const void *)k_gang, dim3(nblk), dim3(GBS), args, 0,
                                            (hipStream_t)stream);
 }

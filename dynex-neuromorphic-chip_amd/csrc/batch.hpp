@@ -93,14 +93,19 @@ void launch_wide_step(const BInst *insts, Ctl *wctl, const int *bmap, int nbc, c
 // resident workgroups per CU for a shape (0 if it cannot launch)
 int batch_occupancy(int mode, int bs, int lds_bytes);
 
-// gang mode (see batch.hip): map holds (slot, rank, G, gang) per workgroup;
+// gang mode (see batch.hip): map holds (slot, rank, G, gang) per workgroup,
+// every gang's workgroups at block indices congruent to one XCD modulo 8;
 // words holds 2 * GANG_MAX lines of GANG_LINE u64 per gang
 constexpr int GANG_MAX = 32;     // workgroups per gang
 constexpr int GANG_CPW = 2048;   // clauses per workgroup at most (LDS)
 constexpr int GANG_LINE = 16;    // u64 per barrier line (128 bytes)
 int gang_occupancy();
 // cooperative launch of K steps; returns the hipError_t of the launch
+constexpr int GANG_XCD = 8;      // XCDs the dispatcher deals workgroups to, in turn
+// *err: 1 a barrier wait timed out, 2 a workgroup ran on another XCD than its
+// gang (the launch then changed nothing); *arrive counts arrivals across
+// launches (this launch waits for arrive_base + nblk)
 int launch_gang(const BInst *insts, BDyn *dyn, const int *map, int nblk, unsigned long long *words,
-                unsigned long long base, int K, int *err, void *stream);
+                unsigned long long base, int K, int *err, int *arrive, int arrive_base, void *stream);
 
 }  // namespace dnx

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <climits>
@@ -172,11 +173,12 @@ struct dnx_batch {
     DevBuf b_wctl, b_wlist, b_bmap, b_vmap;
     // gang mode: taken for the tail when the running instances need at
     // most gang_blocks workgroups in total (0 disables it)
-    int gang_blocks = 0;
-    int gang_cpw = 1024;      // target clauses per gang workgroup
-    int gang_steps = 128;     // steps per cooperative launch
+    int gang_blocks = 256;
+    int gang_cpw = 384;       // target clauses per gang workgroup
+    int gang_steps = 512;     // steps per cooperative launch
     int gang_cap = 0;         // co-resident gang workgroups (0: not known yet)
     unsigned long long gang_epoch = 0;
+    int gang_arrive = 0;      // arrivals counted on the device so far
     DevBuf b_gmap, b_gwords, b_gerr;
     ~dnx_batch()
     {
@@ -852,24 +854,52 @@ static int batch_gang_run(dnx_batch *b, double walltime_s, std::chrono::steady_c
     *used = false;
     if (b->gang_blocks <= 0) return DNX_OK;
     if (!b->gang_cap) b->gang_cap = std::max(0, gang_occupancy()) * b->cus;
-    std::vector<int> list, map;
-    int gangs = 0;
+    // one gang per running instance, each gang on one XCD: its workgroups at
+    // block indices x, x + 8, x + 16, ... (the dispatcher deals workgroups to
+    // the XCDs in turn); gangs go largest first to the least-loaded XCD
+    std::vector<int> list;
+    std::vector<std::pair<int, int>> gs;  // (G, slot)
+    int total = 0;
     for (int s = 0; s < b->nslots; s++) {
         if (!b->slots[(size_t)s].used || b->h_dyn[s].status != DNX_RUNNING) continue;
         const int m = b->slots[(size_t)s].m;
         const int G = std::max(1, std::min(GANG_MAX, (m + b->gang_cpw - 1) / b->gang_cpw));
         if ((m + G - 1) / G > GANG_CPW) return DNX_OK;
-        for (int r = 0; r < G; r++) {
-            map.push_back(s);
-            map.push_back(r);
-            map.push_back(G);
-            map.push_back(gangs);
-        }
-        gangs++;
-        list.push_back(s);
-        if ((int)map.size() / 4 > std::min(b->gang_blocks, b->gang_cap)) return DNX_OK;
+        gs.push_back({G, s});
+        total += G;
+        if (total > b->gang_blocks) return DNX_OK;
     }
-    if (list.empty()) return DNX_OK;
+    if (gs.empty()) return DNX_OK;
+    std::sort(gs.begin(), gs.end(), [](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+        return x.first != y.first ? x.first > y.first : x.second < y.second;
+    });
+    const int per_xcd = b->gang_cap / GANG_XCD;
+    std::vector<std::vector<std::array<int, 4>>> on(GANG_XCD);
+    int gangs = 0;
+    for (auto &g : gs) {
+        int x = 0;
+        for (int y = 1; y < GANG_XCD; y++)
+            if (on[(size_t)y].size() < on[(size_t)x].size()) x = y;
+        if ((int)on[(size_t)x].size() + g.first > per_xcd) return DNX_OK;
+        for (int r = 0; r < g.first; r++) on[(size_t)x].push_back({g.second, r, g.first, gangs});
+        gangs++;
+        list.push_back(g.second);
+    }
+    size_t W = 0;
+    for (auto &v : on) W = std::max(W, v.size());
+    std::vector<int> map((size_t)GANG_XCD * W * 4);
+    for (size_t j = 0; j < W; j++)
+        for (int x = 0; x < GANG_XCD; x++) {
+            const size_t at = (j * GANG_XCD + (size_t)x) * 4;
+            if (j < on[(size_t)x].size())
+                for (int q = 0; q < 4; q++) map[at + (size_t)q] = on[(size_t)x][j][(size_t)q];
+            else {
+                map[at] = -1;
+                map[at + 1] = 0;
+                map[at + 2] = 1;
+                map[at + 3] = 0;
+            }
+        }
     *used = true;
     const int nblk = (int)map.size() / 4;
     hipStream_t st = b->ctx->stream;
@@ -883,8 +913,10 @@ static int batch_gang_run(dnx_batch *b, double walltime_s, std::chrono::steady_c
         if ((e = hipMemset(b->b_gwords.ptr, 0, wbytes)) != hipSuccess) return hip_fail(e, "gang words");
     }
     if (!b->b_gerr.ptr) {
-        if ((rc = b->b_gerr.alloc(sizeof(int)))) return rc;
-        if ((e = hipMemset(b->b_gerr.ptr, 0, sizeof(int))) != hipSuccess) return hip_fail(e, "gang flag");
+        // [0] error flag, [1] arrival count
+        if ((rc = b->b_gerr.alloc(2 * sizeof(int)))) return rc;
+        if ((e = hipMemset(b->b_gerr.ptr, 0, 2 * sizeof(int))) != hipSuccess) return hip_fail(e, "gang flag");
+        b->gang_arrive = 0;
     }
     if ((e = hipMemcpy(b->b_gmap.ptr, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice)) != hipSuccess)
         return hip_fail(e, "gang map upload");
@@ -895,8 +927,10 @@ static int batch_gang_run(dnx_batch *b, double walltime_s, std::chrono::steady_c
         const bool timed = b->timing && !b->t_pending[BK_GANG];
         if (timed && (e = hipEventRecord(b->t0[BK_GANG], st)) != hipSuccess) return hip_fail(e, "timing");
         const int le = launch_gang((const BInst *)b->b_inst.ptr, (BDyn *)b->b_dyn.ptr, (const int *)b->b_gmap.ptr, nblk,
-                                   (unsigned long long *)b->b_gwords.ptr, b->gang_epoch, K, (int *)b->b_gerr.ptr, st);
+                                   (unsigned long long *)b->b_gwords.ptr, b->gang_epoch, K, (int *)b->b_gerr.ptr,
+                                   (int *)b->b_gerr.ptr + 1, b->gang_arrive, st);
         b->gang_epoch += 2 * (unsigned long long)K + 2;
+        b->gang_arrive += nblk;
         if (le != 0) return hip_fail((hipError_t)le, "gang launch");
         if (timed) {
             if ((e = hipEventRecord(b->t1[BK_GANG], st)) != hipSuccess) return hip_fail(e, "timing");
@@ -907,6 +941,13 @@ static int batch_gang_run(dnx_batch *b, double walltime_s, std::chrono::steady_c
         int err = 0;
         if ((e = hipMemcpy(&err, b->b_gerr.ptr, sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess)
             return hip_fail(e, "gang flag read");
+        if (err == 2) {
+            // a workgroup ran on another XCD than its gang: the launch changed
+            // nothing; gang mode is switched off for this batch
+            b->gang_blocks = 0;
+            if ((e = hipMemset(b->b_gerr.ptr, 0, sizeof(int))) != hipSuccess) return hip_fail(e, "gang flag");
+            break;
+        }
         if (err) {
             set_error("gang mode: a workgroup barrier wait timed out; the gang's instances are in an undefined state");
             return DNX_E_STATE;
@@ -1002,12 +1043,25 @@ static int batch_split_tail(dnx_batch *b, std::chrono::steady_clock::time_point 
     for (size_t i = 0; i < nset.size(); i++) before[i] = b->h_dyn[nset[i]].steps;
     int rc = batch_side_narrow(b, nset, INT_MAX);
     if (rc) return rc;
+    bool side_over = false;
     for (;;) {
         bool wide_running = false;
         for (int s = 0; s < b->nslots; s++)
             if (!skip[(size_t)s] && b->slots[(size_t)s].used && b->h_dyn[s].status == DNX_RUNNING)
                 wide_running = true;
         if (!wide_running) break;
+        // once the side launch has ended (nothing else holds CUs, and a
+        // fresh snapshot has its slots' final states), the instances left
+        // may fit gang mode
+        if (!side_over && b->gang_blocks > 0 && hipEventQuery(b->side_done) == hipSuccess) {
+            side_over = true;
+            if ((rc = batch_snapshot(b, 0)) || (rc = batch_wait(b, 0))) return rc;
+        }
+        if (side_over) {
+            bool g = false;
+            if ((rc = batch_gang_run(b, 0.0, t0, &g))) return rc;
+            if (g) continue;
+        }
         if ((rc = batch_wide_run(b, 0.0, t0, &skip))) return rc;
     }
     hipError_t e;

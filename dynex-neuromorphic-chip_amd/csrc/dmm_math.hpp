@@ -546,13 +546,14 @@ __device__ __forceinline__ T wave_fold(EP e, IP eidx, IP row_ptr, int n, int v0,
 }
 
 // Wave-cooperative fold over contributions stored variable-major (slot q of
-// the transpose at e[q]), as wide mode writes them: a wave's 64 variables own
-// one contiguous range, staged in LDS with coalesced loads (no index array to
-// wait for), then each lane folds its own slots in order.  rb/re are the
-// lane's slot range (re = rb = 0 past the last variable), `last` the index of
-// the wave's last valid lane.  Must be reached by every thread of the block.
-template <int CAPW, typename EP>
-__device__ __forceinline__ float wave_fold_vm(EP e, int rb, int re, int last, float *stage)
+// the transpose at ld(q)), as wide and gang mode write them: a wave's 64
+// variables own one contiguous range, staged in LDS with coalesced loads (no
+// index array to wait for), then each lane folds its own slots in order.
+// rb/re are the lane's slot range (re = rb = 0 past the last variable),
+// `last` the index of the wave's last valid lane.  Must be reached by every
+// thread of the block.
+template <int CAPW, typename LD>
+__device__ __forceinline__ float wave_fold_vm(LD ld, int rb, int re, int last, float *stage)
 {
     const int lane = threadIdx.x & 63;
     const int sb = __builtin_amdgcn_readfirstlane(rb);
@@ -562,7 +563,7 @@ __device__ __forceinline__ float wave_fold_vm(EP e, int rb, int re, int last, fl
         for (int q = lane; q < total; q += 16 * 64) {
             float x[16];
 #pragma unroll
-            for (int u = 0; u < 16; u++) x[u] = q + u * 64 < total ? e[sb + q + u * 64] : 0.0f;
+            for (int u = 0; u < 16; u++) x[u] = q + u * 64 < total ? ld(sb + q + u * 64) : 0.0f;
 #pragma unroll
             for (int u = 0; u < 16; u++)
                 if (q + u * 64 < total) stage[q + u * 64] = x[u];
@@ -574,7 +575,7 @@ __device__ __forceinline__ float wave_fold_vm(EP e, int rb, int re, int last, fl
         const float *s = stage + (rb - sb);
         for (int j = 0; j < re - rb; j++) acc = acc + s[j];
     } else {
-        for (int q = rb; q < re; q++) acc = acc + e[q];
+        for (int q = rb; q < re; q++) acc = acc + ld(q);
     }
     return acc;
 }

@@ -395,7 +395,7 @@ class Batch:
         check(lib().dnx_batch_get_shape(self.h, int(mode), C.byref(bs), C.byref(cap), C.byref(wide)))
         return bs.value, cap.value, wide.value
 
-    def set_gang(self, max_blocks=128, clauses_per_workgroup=1024, steps_per_launch=128):
+    def set_gang(self, max_blocks=256, clauses_per_workgroup=384, steps_per_launch=512):
         check(lib().dnx_batch_set_gang(self.h, int(max_blocks), int(clauses_per_workgroup), int(steps_per_launch)))
 
     def set_serve_big(self, clauses, active_clauses=6000000):

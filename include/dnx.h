@@ -265,11 +265,14 @@ int dnx_batch_tune(dnx_batch *b, int n_min, int n_max, double r_min, double r_ma
 int dnx_batch_get_shape(dnx_batch *b, int mode, int *block, int *lds_cap, int64_t *wide_clauses);
 
 /* Gang mode: the tail of a run (the instances left when wide mode would be
- * taken) is advanced by gangs of up to 32 cooperating workgroups per
- * instance, steps_per_launch steps per cooperative launch, about
- * clauses_per_workgroup clauses (at most 2048) per workgroup, as long as the
- * running instances need at most max_blocks workgroups in total (default
- * 128; 0 disables it).  Results are bit-identical to the other modes. */
+ * taken, or -- with the tail split -- the wide-mode instances left once the
+ * fused side launch has ended) is advanced by gangs of up to 32 cooperating
+ * workgroups per instance, all of one gang on one XCD, steps_per_launch
+ * steps per cooperative launch, about clauses_per_workgroup clauses (at most
+ * 2048) per workgroup, as long as the running instances need at most
+ * max_blocks workgroups in total and fit the XCDs (defaults 256, 384, 512;
+ * max_blocks 0 disables it).  Results are bit-identical to the other
+ * modes. */
 int dnx_batch_set_gang(dnx_batch *b, int max_blocks, int clauses_per_workgroup, int steps_per_launch);
 
 /* Waits for queued work, then reports the summed device time (ms) and the
