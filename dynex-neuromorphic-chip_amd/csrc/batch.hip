@@ -926,13 +926,16 @@ __global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, B
         s_xl[i] = I.xl[c0 + i];
     }
     __syncthreads();
-    // slot range of each thread's first variable (read once per launch)
+    // slot range and (V, dV) pair of each thread's first variable: the range
+    // is read once per launch, the pair kept from the thread's own last write
     int RB0 = 0, RE0 = 0;
+    BF2 PA{0.0f, 0.0f};
     {
         const int v = vb + tid;
         if (v < n && v < vb + Vb) {
             RB0 = I.row_ptr[v];
             RE0 = I.row_ptr[v + 1];
+            PA = vdr(I, v);
         }
     }
     bool pend = d.plevel >= 0;
@@ -1038,7 +1041,7 @@ __global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, B
             const bool vok = v < n;
             const bool first = v0 == vb + w * 64;
             const int rb = first ? RB0 : vok ? I.row_ptr[v] : 0, re = first ? RE0 : vok ? I.row_ptr[v + 1] : 0;
-            const BF2 a = vok ? vdr_l2(I, v) : BF2{0.0f, 0.0f};  // requested before the fold
+            const BF2 a = first ? PA : vok ? vdr_l2(I, v) : BF2{0.0f, 0.0f};  // requested before the fold
             const float dVf = wave_fold_vm<WCAPW>([&](int q) { return er_l2(I, q); }, rb, re,
                                                   v0 < n ? min(63, n - 1 - v0) : 0, stage[w]);
             if (vok) {
@@ -1053,6 +1056,7 @@ __global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, B
                 o.v = V;
                 o.d = dV;
                 vdw(I, v, o);
+                if (first) PA = o;
             }
         }
         if (done) break;
