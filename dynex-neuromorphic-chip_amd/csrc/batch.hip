@@ -269,15 +269,9 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                         for (int u = 0; u < U; u++) {
                             const int c = cb + u * BS;
                             if (c < c_hi) {
-                                if constexpr (MODE == BM_LDS_ALL) {
-                                    L[u][0] = (int32_t)I.pk[3 * c];
-                                    L[u][1] = (int32_t)I.pk[3 * c + 1];
-                                    L[u][2] = (int32_t)I.pk[3 * c + 2];
-                                } else {
-                                    L[u][0] = I.lits[3 * c];
-                                    L[u][1] = I.lits[3 * c + 1];
-                                    L[u][2] = I.lits[3 * c + 2];
-                                }
+                                L[u][0] = I.lits[3 * c];
+                                L[u][1] = I.lits[3 * c + 1];
+                                L[u][2] = I.lits[3 * c + 2];
                                 XS[u] = I.xs[c];
                                 XL[u] = I.xl[c];
                                 CX[u] = I.cst[c];
@@ -287,26 +281,15 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                         for (int u = 0; u < U; u++) {
                             const int c = cb + u * BS;
                             if (c >= c_hi) break;
-                            // bit 31 of a literal word is the literal's sign in both forms
                             const int32_t l0 = L[u][0], l1 = L[u][1], l2 = L[u][2];
-                            auto var = [](int32_t w) -> int {
-                                if constexpr (MODE == BM_LDS_ALL) return (int)(((uint32_t)w >> 16) & 0x7fffu);
-                                else return abs(w) - 1;
-                            };
                             float xs = XS[u], xl = XL[u];
-                            const float V0 = pvv(var(l0)), V1 = pvv(var(l1)), V2 = pvv(var(l2));
+                            const float V0 = pvv(abs(l0) - 1), V1 = pvv(abs(l1) - 1), V2 = pvv(abs(l2) - 1);
                             memory_update_steady(p, CX[u], h, xs, xl);
                             I.xs[c] = xs;
                             I.xl[c] = xl;
-                            float C;
-                            if constexpr (MODE == BM_LDS_ALL) {
-                                auto emit = [&](int j, float val) { se[(uint32_t)L[u][j] & 0xffffu] = val; };
-                                C = clause3_steady(p, l0, l1, l2, V0, V1, V2, xs, xl, emit);
-                            } else {
-                                auto ep = I.e + 3 * c;
-                                auto emit = [&](int j, float val) { ep[j] = val; };
-                                C = clause3_steady(p, l0, l1, l2, V0, V1, V2, xs, xl, emit);
-                            }
+                            auto ep = I.e + 3 * c;
+                            auto emit = [&](int j, float val) { ep[j] = val; };
+                            const float C = clause3_steady(p, l0, l1, l2, V0, V1, V2, xs, xl, emit);
                             I.cst[c] = C;
                             unsolved |= !(C < 0.5f);
                         }
@@ -988,43 +971,27 @@ __global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, B
                 }
             }
         }
+        // any clause length (instances with general clauses)
         for (int i = tid; general && i < cn; i += GBS) {
             const int c = c0 + i;
             float xs = s_xs[i], xl = s_xl[i];
-            bool sat;
-            float C;
-            if (!general) {
-                const int32_t l0 = s_lits[3 * i], l1 = s_lits[3 * i + 1], l2 = s_lits[3 * i + 2];
-                const BF2 a0 = vdr_l2(I, abs(l0) - 1), a1 = vdr_l2(I, abs(l1) - 1), a2 = vdr_l2(I, abs(l2) - 1);
-                float r0 = a0.v, r1 = a1.v, r2 = a2.v;
-                if (pend) {
-                    memory_update(p, clause3_C_vals<float>(l0, l1, l2, r0, r1, r2), h, xs, xl);
-                    s_xs[i] = xs;
-                    s_xl[i] = xl;
-                    r0 = clampV(a0.v + h * a0.d);
-                    r1 = clampV(a1.v + h * a1.d);
-                    r2 = clampV(a2.v + h * a2.d);
-                }
-                auto emit = [&](int j, float val) { I.e[s_gpos[3 * i + j]] = val; };
-                C = clause3_vals<float>(p, l0, l1, l2, r0, r1, r2, clamp01(xs), clampXl(xl, p.xlmax), emit, sat);
-            } else {
-                const int bb = I.cp[c];
-                const int kk = I.cp[c + 1] - bb;
-                auto cl = I.lits + bb;
-                auto prevV = [&](int v) { return vdr_l2(I, v).v; };
-                auto curV = [&](int v) {
-                    BF2 a = vdr_l2(I, v);
-                    return pend ? clampV(a.v + h * a.d) : a.v;
-                };
-                if (pend) {
-                    memory_update(p, clauseK_C<float>(cl, kk, prevV), h, xs, xl);
-                    s_xs[i] = xs;
-                    s_xl[i] = xl;
-                }
-                auto gp = I.gpos + bb;
-                auto emit = [&](int j, float val) { I.e[gp[j]] = val; };
-                C = clauseK<float>(p, cl, kk, clamp01(xs), clampXl(xl, p.xlmax), curV, emit, sat);
+            const int bb = I.cp[c];
+            const int kk = I.cp[c + 1] - bb;
+            auto cl = I.lits + bb;
+            auto prevV = [&](int v) { return vdr_l2(I, v).v; };
+            auto curV = [&](int v) {
+                BF2 a = vdr_l2(I, v);
+                return pend ? clampV(a.v + h * a.d) : a.v;
+            };
+            if (pend) {
+                memory_update(p, clauseK_C<float>(cl, kk, prevV), h, xs, xl);
+                s_xs[i] = xs;
+                s_xl[i] = xl;
             }
+            auto gp = I.gpos + bb;
+            auto emit = [&](int j, float val) { I.e[gp[j]] = val; };
+            bool sat;
+            const float C = clauseK<float>(p, cl, kk, clamp01(xs), clampXl(xl, p.xlmax), curV, emit, sat);
             unsolved |= !(C < 0.5f) || !sat;
         }
         int any_unsolved = 0;
