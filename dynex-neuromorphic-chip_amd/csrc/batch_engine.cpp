@@ -844,11 +844,12 @@ static int batch_wide_run(dnx_batch *b, double walltime_s, std::chrono::steady_c
 }
 
 // Gang mode for the tail (see batch.hip).  *used stays false when the
-// running set does not suit it -- an instance too large for one gang, more
-// workgroups than gang_blocks or than can be resident at once -- and the
-// caller then runs wide mode.  Returns after the running set has lost half
-// its workgroups (to rebuild the map), when nothing runs, or at the
-// wall-clock limit; the status table in h_dyn (buffer 0) is then current.
+// running set does not suit it -- an instance too large for one gang, or
+// more workgroups than gang_blocks or than can be resident at once even at
+// the fewest workgroups each instance's clauses allow -- and the caller
+// then runs wide mode.  Returns after the running set has lost half its
+// workgroups (to rebuild the map), when nothing runs, or at the wall-clock
+// limit; the status table in h_dyn (buffer 0) is then current.
 static int batch_gang_run(dnx_batch *b, double walltime_s, std::chrono::steady_clock::time_point t0, bool *used)
 {
     *used = false;
@@ -856,35 +857,62 @@ static int batch_gang_run(dnx_batch *b, double walltime_s, std::chrono::steady_c
     if (!b->gang_cap) b->gang_cap = std::max(0, gang_occupancy()) * b->cus;
     // one gang per running instance, each gang on one XCD: its workgroups at
     // block indices x, x + 8, x + 16, ... (the dispatcher deals workgroups to
-    // the XCDs in turn); gangs go largest first to the least-loaded XCD
-    std::vector<int> list;
-    std::vector<std::pair<int, int>> gs;  // (G, slot)
-    int total = 0;
+    // the XCDs in turn).  An instance asks for m / gang_cpw workgroups; when
+    // the running set asks for more than the budget, every gang is cut to a
+    // common level L (never below what its clauses need in LDS), the highest
+    // level whose gangs fit the budget and pack onto the XCDs, largest first
+    // to the least-loaded XCD.
+    struct Want {
+        int lo, hi, slot;
+    };
+    std::vector<Want> want;
     for (int s = 0; s < b->nslots; s++) {
         if (!b->slots[(size_t)s].used || b->h_dyn[s].status != DNX_RUNNING) continue;
         const int m = b->slots[(size_t)s].m;
-        const int G = std::max(1, std::min(GANG_MAX, (m + b->gang_cpw - 1) / b->gang_cpw));
-        if ((m + G - 1) / G > GANG_CPW) return DNX_OK;
-        gs.push_back({G, s});
-        total += G;
-        if (total > b->gang_blocks) return DNX_OK;
+        const int lo = std::max(1, (m + GANG_CPW - 1) / GANG_CPW);
+        if (lo > GANG_MAX) return DNX_OK;
+        want.push_back({lo, std::max(lo, std::min(GANG_MAX, (m + b->gang_cpw - 1) / b->gang_cpw)), s});
     }
-    if (gs.empty()) return DNX_OK;
-    std::sort(gs.begin(), gs.end(), [](const std::pair<int, int> &x, const std::pair<int, int> &y) {
-        return x.first != y.first ? x.first > y.first : x.second < y.second;
-    });
+    if (want.empty()) return DNX_OK;
+    const int budget = std::min(b->gang_blocks, b->gang_cap);
     const int per_xcd = b->gang_cap / GANG_XCD;
+    std::vector<std::pair<int, int>> gs;  // (G, slot)
     std::vector<std::vector<std::array<int, 4>>> on(GANG_XCD);
-    int gangs = 0;
-    for (auto &g : gs) {
-        int x = 0;
-        for (int y = 1; y < GANG_XCD; y++)
-            if (on[(size_t)y].size() < on[(size_t)x].size()) x = y;
-        if ((int)on[(size_t)x].size() + g.first > per_xcd) return DNX_OK;
-        for (int r = 0; r < g.first; r++) on[(size_t)x].push_back({g.second, r, g.first, gangs});
-        gangs++;
-        list.push_back(g.second);
+    std::vector<int> list, glist;
+    bool fit = false;
+    for (int L = GANG_MAX; L >= 1 && !fit; L--) {
+        gs.clear();
+        int total = 0;
+        for (const Want &w : want) {
+            const int G = std::max(w.lo, std::min(w.hi, L));
+            gs.push_back({G, w.slot});
+            total += G;
+        }
+        if (total > budget) continue;
+        std::sort(gs.begin(), gs.end(), [](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+            return x.first != y.first ? x.first > y.first : x.second < y.second;
+        });
+        for (auto &v : on) v.clear();
+        list.clear();
+        glist.clear();
+        fit = true;
+        int gangs = 0;
+        for (auto &g : gs) {
+            int x = 0;
+            for (int y = 1; y < GANG_XCD; y++)
+                if (on[(size_t)y].size() < on[(size_t)x].size()) x = y;
+            if ((int)on[(size_t)x].size() + g.first > per_xcd) {
+                fit = false;
+                break;
+            }
+            for (int r = 0; r < g.first; r++) on[(size_t)x].push_back({g.second, r, g.first, gangs});
+            gangs++;
+            list.push_back(g.second);
+            glist.push_back(g.first);
+        }
     }
+    if (!fit) return DNX_OK;
+    const int gangs = (int)list.size();
     size_t W = 0;
     for (auto &v : on) W = std::max(W, v.size());
     std::vector<int> map((size_t)GANG_XCD * W * 4);
@@ -961,8 +989,7 @@ static int batch_gang_run(dnx_batch *b, double walltime_s, std::chrono::steady_c
         for (size_t i = 0; i < list.size(); i++)
             if (b->h_dyn[list[i]].status == DNX_RUNNING) {
                 any = true;
-                const int m = b->slots[(size_t)list[i]].m;
-                live += std::max(1, std::min(GANG_MAX, (m + b->gang_cpw - 1) / b->gang_cpw));
+                live += glist[i];
             }
         if (!any || group_done(b)) break;
         if (2 * live < nblk) break;

@@ -268,11 +268,13 @@ int dnx_batch_get_shape(dnx_batch *b, int mode, int *block, int *lds_cap, int64_
  * taken, or -- with the tail split -- the wide-mode instances left once the
  * fused side launch has ended) is advanced by gangs of up to 32 cooperating
  * workgroups per instance, all of one gang on one XCD, steps_per_launch
- * steps per cooperative launch, about clauses_per_workgroup clauses (at most
- * 2048) per workgroup, as long as the running instances need at most
- * max_blocks workgroups in total and fit the XCDs (defaults 256, 384, 512;
- * max_blocks 0 disables it).  Results are bit-identical to the other
- * modes. */
+ * steps per cooperative launch, about clauses_per_workgroup clauses per
+ * workgroup (defaults 256, 384, 512; max_blocks 0 disables it).  When the
+ * running instances ask for more than max_blocks workgroups (or more than
+ * can be resident at once), every gang is cut to a common size, never below
+ * what holds its clauses in LDS (2048 per workgroup); if even that does not
+ * fit, wide mode runs until enough instances have ended.  Results are
+ * bit-identical to the other modes. */
 int dnx_batch_set_gang(dnx_batch *b, int max_blocks, int clauses_per_workgroup, int steps_per_launch);
 
 /* Waits for queued work, then reports the summed device time (ms) and the

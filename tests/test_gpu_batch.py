@@ -156,6 +156,7 @@ def test_wide_mode_gives_identical_bits(ctx, inst_set, oracle_results, wide):
     (1 << 40, (512, 256, 37)),     # gangs of up to 32 workgroups, relaunch every 37 steps
     (1 << 40, (128, 2048, 128)),   # few large workgroups
     (60000, (128, 1024, 64)),      # fused kernel, then wide mode, then gangs
+    (1 << 40, (48, 128, 50)),      # budget below the running set's wish: gangs cut to a common level
 ])
 def test_gang_mode_gives_identical_bits(ctx, inst_set, oracle_results, wide, gang):
     out = run_batch(ctx, inst_set, 1500, spl=40, wide=wide, gang=gang)
