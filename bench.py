@@ -65,8 +65,8 @@ def percentile_ms(vals, q):
 
 
 def apply_shape(b, shape):
-    """Autotuned fused-kernel shapes; the tail goes to wide mode once 1% of
-    the run's clauses are left (the library default)."""
+    """Autotuned fused-kernel shapes; the tail goes to gangs (or wide mode)
+    once 1.25% of the run's clauses are left (the library default)."""
     if shape:
         for md in (0, 1):
             b.set_shape(md, shape[md][0], shape[md][1])
@@ -482,7 +482,7 @@ def main():
             "workload": "mixed planted 3-SAT, 4096 instances: n~LogU[100,5000], m=round(U[4.3,8]*n), "
                         "P(t)=.46/.46/.08, v~U[-1,1), Xs=0.5, Xl=1, step cap 100k, batched FP32",
             "instances": len(work),
-            "fused_launch": "flow: each instance on its workgroup until it ends or 1% of the clauses run",
+            "fused_launch": "flow: each instance on its workgroup until it ends or 1.25% of the clauses run",
         },
         "solved": int((last["status"] == dnx.SAT).sum()),
         # admission (host tables -> device arena over PCIe, one instance at a time) is outside the

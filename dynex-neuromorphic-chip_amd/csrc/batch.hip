@@ -936,19 +936,20 @@ __global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, B
         // ---- clause pass ----------------------------------------------------
         int unsolved = 0;
         if (!general) {
-            // 3-SAT: each thread takes its clauses two at a time, all six
-            // (V, dV) pairs requested before either clause is evaluated
-            for (int ib = tid; ib < cn; ib += 2 * GBS) {
-                BF2 A[2][3];
+            // 3-SAT: the (V, dV) pairs of all of a thread's clauses (at most
+            // four: GANG_CPW = 4 GBS) are requested before any is evaluated
+            static_assert(GANG_CPW <= 4 * GBS, "one pass of four clauses per thread");
+            for (int ib = tid; ib < cn; ib += 4 * GBS) {
+                BF2 A[4][3];
 #pragma unroll
-                for (int u = 0; u < 2; u++) {
+                for (int u = 0; u < 4; u++) {
                     const int i = ib + u * GBS;
                     if (i < cn)
 #pragma unroll
                         for (int j = 0; j < 3; j++) A[u][j] = vdr_l2(I, abs(s_lits[3 * i + j]) - 1);
                 }
 #pragma unroll
-                for (int u = 0; u < 2; u++) {
+                for (int u = 0; u < 4; u++) {
                     const int i = ib + u * GBS;
                     if (i >= cn) break;
                     const int32_t l0 = s_lits[3 * i], l1 = s_lits[3 * i + 1], l2 = s_lits[3 * i + 2];

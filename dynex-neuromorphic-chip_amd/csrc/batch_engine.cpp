@@ -146,11 +146,12 @@ struct dnx_batch {
     // compulsory bytes of the steps taken by the timed launches of
     // dnx_batch_run, per bucket (step_bytes per instance step)
     double t_alg[NBUCKET] = {};
-    // wide mode: entered when the running instances hold at most
-    // max(wide_clauses, wide_frac x the clauses running when dnx_batch_run
-    // starts) clauses in total; wide_chunk steps between status syncs
+    // the tail (gang mode, else wide mode): entered when the running
+    // instances hold at most max(wide_clauses, wide_frac x the clauses
+    // running when dnx_batch_run starts) clauses in total; wide_chunk steps
+    // between status syncs in wide mode
     long long wide_clauses = 0;
-    double wide_frac = 0.01;
+    double wide_frac = 0.0125;
     int wide_chunk = 300;
     // serving: steps per chunk of the wide lane (status syncs and active-set
     // choices happen between chunks)

@@ -410,7 +410,7 @@ class Batch:
     def set_flow(self, enable=True):
         check(lib().dnx_batch_set_flow(self.h, int(bool(enable))))
 
-    def set_wide_fraction(self, fraction=0.01, steps_per_sync=300):
+    def set_wide_fraction(self, fraction=0.0125, steps_per_sync=300):
         check(lib().dnx_batch_set_wide_fraction(self.h, float(fraction), int(steps_per_sync)))
 
     def timing(self, enable=-1, with_bytes=False):
