@@ -223,38 +223,102 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                     // (Buffer-resource addressing of the device arrays
                     // measured slower than these global accesses.)
                     float cmax = 0.0f;  // C >= 1/2 anywhere means unsolved
-                    constexpr int U = 4;
-                    for (int cb = c_lo + tid; cb < c_hi; cb += U * BS) {
-                        uint32_t W[U][3];
-                        float XS[U], XL[U], CX[U];
+                    // Each thread takes four consecutive clauses 4g .. 4g+3 at
+                    // a time: their packed words (48 bytes) and their
+                    // memories and stored C (16 bytes each) move as 16-byte
+                    // accesses at one 32-bit offset from each array's base
+                    // (every array starts 256-byte aligned), which leaves
+                    // almost no address arithmetic; a group cut by the
+                    // chunk's ends takes its clauses one at a time.
+                    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+                    typedef float f32x4 __attribute__((ext_vector_type(4)));
+                    GLB const char *pkb = (GLB const char *)I.pk;
+                    GLB char *xsb = (GLB char *)I.xs, *xlb = (GLB char *)I.xl, *csb = (GLB char *)I.cst;
+                    const int g_end = (c_hi + 3) >> 2;
+                    for (int g = (c_lo >> 2) + tid; g < g_end; g += BS) {
+                        const int c4 = g << 2;
+                        const bool full = c4 >= c_lo && c4 + 4 <= c_hi;
+                        const uint32_t o16 = (uint32_t)g * 16u, o48 = (uint32_t)g * 48u;
+                        uint32_t W[4][3];
+                        float XS[4], XL[4], CX[4];
+                        if (full) {
+                            const u32x4 w0 = *(GLB const u32x4 *)(pkb + o48);
+                            const u32x4 w1 = *(GLB const u32x4 *)(pkb + o48 + 16u);
+                            const u32x4 w2 = *(GLB const u32x4 *)(pkb + o48 + 32u);
+                            const f32x4 a = *(GLB const f32x4 *)(xsb + o16);
+                            const f32x4 b = *(GLB const f32x4 *)(xlb + o16);
+                            const f32x4 q = *(GLB const f32x4 *)(csb + o16);
+                            W[0][0] = w0.x;
+                            W[0][1] = w0.y;
+                            W[0][2] = w0.z;
+                            W[1][0] = w0.w;
+                            W[1][1] = w1.x;
+                            W[1][2] = w1.y;
+                            W[2][0] = w1.z;
+                            W[2][1] = w1.w;
+                            W[2][2] = w2.x;
+                            W[3][0] = w2.y;
+                            W[3][1] = w2.z;
+                            W[3][2] = w2.w;
+                            XS[0] = a.x;
+                            XS[1] = a.y;
+                            XS[2] = a.z;
+                            XS[3] = a.w;
+                            XL[0] = b.x;
+                            XL[1] = b.y;
+                            XL[2] = b.z;
+                            XL[3] = b.w;
+                            CX[0] = q.x;
+                            CX[1] = q.y;
+                            CX[2] = q.z;
+                            CX[3] = q.w;
+                        } else {
 #pragma unroll
-                        for (int u = 0; u < U; u++) {
-                            const int c = cb + u * BS;
-                            if (c < c_hi) {
-                                W[u][0] = I.pk[3 * c];
-                                W[u][1] = I.pk[3 * c + 1];
-                                W[u][2] = I.pk[3 * c + 2];
-                                XS[u] = I.xs[c];
-                                XL[u] = I.xl[c];
-                                CX[u] = I.cst[c];
+                            for (int u = 0; u < 4; u++) {
+                                const int c = c4 + u;
+                                W[u][0] = W[u][1] = W[u][2] = 0u;
+                                XS[u] = XL[u] = CX[u] = 0.0f;
+                                if (c >= c_lo && c < c_hi) {
+                                    W[u][0] = I.pk[3 * c];
+                                    W[u][1] = I.pk[3 * c + 1];
+                                    W[u][2] = I.pk[3 * c + 2];
+                                    XS[u] = I.xs[c];
+                                    XL[u] = I.xl[c];
+                                    CX[u] = I.cst[c];
+                                }
                             }
                         }
 #pragma unroll
-                        for (int u = 0; u < U; u++) {
-                            const int c = cb + u * BS;
-                            if (c >= c_hi) break;
+                        for (int u = 0; u < 4; u++) {
+                            const int c = c4 + u;
+                            if (c < c_lo || c >= c_hi) continue;
                             const int32_t l0 = (int32_t)W[u][0], l1 = (int32_t)W[u][1], l2 = (int32_t)W[u][2];
                             float xs = XS[u], xl = XL[u];
                             const float V0 = pvv(__builtin_amdgcn_ubfe(W[u][0], 16, 15));
                             const float V1 = pvv(__builtin_amdgcn_ubfe(W[u][1], 16, 15));
                             const float V2 = pvv(__builtin_amdgcn_ubfe(W[u][2], 16, 15));
                             memory_update_steady(p, CX[u], h, xs, xl);
-                            I.xs[c] = xs;
-                            I.xl[c] = xl;
+                            XS[u] = xs;
+                            XL[u] = xl;
                             auto emit = [&](int j, float val) { se[W[u][j] & 0xffffu] = val; };
                             const float C = clause3_steady(p, l0, l1, l2, V0, V1, V2, xs, xl, emit);
-                            I.cst[c] = C;
+                            CX[u] = C;
                             cmax = dmax(cmax, C);
+                        }
+                        if (full) {
+                            *(GLB f32x4 *)(xsb + o16) = f32x4{XS[0], XS[1], XS[2], XS[3]};
+                            *(GLB f32x4 *)(xlb + o16) = f32x4{XL[0], XL[1], XL[2], XL[3]};
+                            *(GLB f32x4 *)(csb + o16) = f32x4{CX[0], CX[1], CX[2], CX[3]};
+                        } else {
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                const int c = c4 + u;
+                                if (c >= c_lo && c < c_hi) {
+                                    I.xs[c] = XS[u];
+                                    I.xl[c] = XL[u];
+                                    I.cst[c] = CX[u];
+                                }
+                            }
                         }
                     }
                     unsolved |= !(cmax < 0.5f);
