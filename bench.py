@@ -72,9 +72,10 @@ def apply_shape(b, shape):
             b.set_shape(md, shape[md][0], shape[md][1])
 
 
-def batched_solve(ctx, work, hps, cap, spl, timing, shape=None):
+def batched_solve(ctx, work, hps, cap, spl, timing, shape=None, order=0):
     b = dnx.Batch(ctx, slots=len(work))
     apply_shape(b, shape)
+    b.set_order(order)
     try:
         ta = time.perf_counter()
         slots = [b.admit(hp, dnx.params(w["m"]), w["x0"], cap=cap) for w, hp in zip(work, hps)]
@@ -320,6 +321,11 @@ def main():
     # request waits for the running launch to end) and any fallback from
     # flow launches in the batched solve
     ap.add_argument("--steps-per-launch", type=int, default=256)
+    # work-list order of the batched solve: instances of at least this many
+    # clauses start first, largest first, then the others smallest first
+    # (0: all largest first).  Same makespan, p50 time to solution 78 -> 31 ms,
+    # p99 161 -> 176 ms on this workload.
+    ap.add_argument("--order", type=int, default=14000)
     ap.add_argument("--fp64-sample", type=int, default=64)
     ap.add_argument("--fp64-seconds", type=float, default=25.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -366,7 +372,7 @@ def main():
         copy_bytes = 256 << 20
         ms = ctx.device_copy(copy_bytes, repeats=4)
         log(f"copy kernel: {copy_bytes} B in {ms:.3f} ms ({2 * copy_bytes / ms / 1e6:.0f} GB/s read+write)")
-        r = batched_solve(ctx, work, hps, args.cap, args.steps_per_launch, timing=True, shape=shape)
+        r = batched_solve(ctx, work, hps, args.cap, args.steps_per_launch, timing=True, shape=shape, order=args.order)
         print(json.dumps(dict(profile_pass=True, copy_bytes=copy_bytes, copy_ms=ms, batch_ms=r["wall"] * 1e3,
                               device_ms=r["ms"].tolist(), launches=r["launches"].tolist(), alg_bytes=r["alg"].tolist(),
                               verified=int(r["verified"].sum()))), flush=True)
@@ -374,7 +380,8 @@ def main():
         return
     runs = []
     for it in range(args.warmup + args.steps):
-        r = batched_solve(ctx, work, hps, args.cap, args.steps_per_launch, timing=it >= args.warmup, shape=shape)
+        r = batched_solve(ctx, work, hps, args.cap, args.steps_per_launch, timing=it >= args.warmup, shape=shape,
+                          order=args.order)
         tag = "warmup" if it < args.warmup else "timed"
         log(f"{tag} batch {it}: {r['wall'] * 1e3:.1f} ms, solved {int((r['status'] == dnx.SAT).sum())}, "
             f"verified {int(r['verified'].sum())}")
@@ -483,6 +490,8 @@ def main():
                         "P(t)=.46/.46/.08, v~U[-1,1), Xs=0.5, Xl=1, step cap 100k, batched FP32",
             "instances": len(work),
             "fused_launch": "flow: each instance on its workgroup until it ends or 1.25% of the clauses run",
+            "order": (f"instances of >= {args.order} clauses first, largest first, then the others smallest first"
+                      if args.order > 0 else "largest first"),
         },
         "solved": int((last["status"] == dnx.SAT).sum()),
         # admission (host tables -> device arena over PCIe, one instance at a time) is outside the

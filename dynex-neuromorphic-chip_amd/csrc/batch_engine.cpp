@@ -166,6 +166,10 @@ struct dnx_batch {
     // tail: single-chunk mode-0 instances run in the fused kernel beside
     // wide mode (batch_split_tail)
     bool tail_split = true;
+    // work-list order of the fused kernel: instances of at least order_big
+    // clauses first, largest first, then the others smallest first (0: all
+    // largest first)
+    long long order_big = 0;
     // dnx_batch_run: the fused launch keeps each instance on its workgroup
     // until it ends or the running clauses fall to the wide-mode threshold
     // (BFlow), instead of relaunching every steps_per_launch steps
@@ -353,8 +357,12 @@ static int batch_enqueue_slots(dnx_batch *b, int ksteps, const std::vector<int> 
     std::vector<std::pair<int, int>> order;
     order.reserve(slots.size());
     for (int s : slots) order.push_back({b->slots[(size_t)s].m, s});
-    std::sort(order.begin(), order.end(), [](const std::pair<int, int> &x, const std::pair<int, int> &y) {
-        return x.first != y.first ? x.first > y.first : x.second < y.second;
+    const long long big = b->order_big;
+    std::sort(order.begin(), order.end(), [big](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+        const bool bx = x.first >= big, by = y.first >= big;
+        if (bx != by) return bx;
+        if (x.first != y.first) return bx ? x.first > y.first : x.first < y.first;
+        return x.second < y.second;
     });
     for (auto &o : order) {
         int md = b->slots[(size_t)o.second].mode;
@@ -1300,6 +1308,13 @@ int dnx_batch_set_gang(dnx_batch *b, int max_blocks, int clauses_per_workgroup, 
     b->gang_blocks = max_blocks;
     b->gang_cpw = clauses_per_workgroup;
     b->gang_steps = steps_per_launch;
+    return DNX_OK;
+}
+
+int dnx_batch_set_order(dnx_batch *b, int64_t large_clauses)
+{
+    if (!b || large_clauses < 0) return DNX_E_ARG;
+    b->order_big = large_clauses;
     return DNX_OK;
 }
 

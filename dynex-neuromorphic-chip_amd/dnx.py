@@ -95,6 +95,7 @@ def lib():
     L.dnx_batch_set_gang.argtypes = [vp, ip, ip, ip]
     L.dnx_batch_set_wide_fraction.argtypes = [vp, dp, ip]
     L.dnx_batch_set_tail_split.argtypes = [vp, ip]
+    L.dnx_batch_set_order.argtypes = [vp, C.c_int64]
     L.dnx_batch_set_flow.argtypes = [vp, ip]
     L.dnx_batch_tune.argtypes = [vp, ip, ip, dp, dp, ip, ip, C.c_uint64, vp, ip, C.POINTER(C.c_int)]
     L.dnx_batch_get_shape.argtypes = [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)]
@@ -406,6 +407,9 @@ class Batch:
 
     def set_tail_split(self, enable=True):
         check(lib().dnx_batch_set_tail_split(self.h, int(bool(enable))))
+
+    def set_order(self, large_clauses=0):
+        check(lib().dnx_batch_set_order(self.h, int(large_clauses)))
 
     def set_flow(self, enable=True):
         check(lib().dnx_batch_set_flow(self.h, int(bool(enable))))

@@ -228,6 +228,14 @@ int dnx_batch_set_wide(dnx_batch *b, int64_t max_clauses, int steps_per_sync);
  * Results are bit-identical either way. */
 int dnx_batch_set_tail_split(dnx_batch *b, int enable);
 
+/* Order in which the fused kernel's workgroups take instances: those of at
+ * least large_clauses clauses first, largest first (the longest ones start
+ * at once), then the others smallest first (short instances end early
+ * instead of waiting behind long ones).  0 (the default): all largest
+ * first.  No counterpart in the reference, which solves one instance per
+ * process; results are bit-identical in any order. */
+int dnx_batch_set_order(dnx_batch *b, int64_t large_clauses);
+
 /* Flow launches (default on): while every running instance is held in LDS
  * (mode 0) and dnx_batch_run has neither a wall-clock limit nor a group
  * stop, one fused launch keeps each instance on its workgroup until it ends

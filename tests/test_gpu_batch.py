@@ -55,16 +55,18 @@ def oracle_solve(n, cp, lits, x0, cap):
     return steps, t, status, x
 
 
-def run_batch(ctx, inst, cap, spl, shapes=None, stop=True, wide=0, gang=None, split=False, flow=True):
+def run_batch(ctx, inst, cap, spl, shapes=None, stop=True, wide=0, gang=None, split=False, flow=True, order=0):
     """gang = (max_blocks, clauses_per_workgroup, steps_per_launch) enables
     gang mode for the tail (the instances left once wide mode applies);
     split keeps the single-chunk LDS instances of the tail in the fused
     kernel beside wide mode; flow runs the fused launch until the wide-mode
-    threshold instead of spl steps per launch."""
+    threshold instead of spl steps per launch; order > 0 starts instances of
+    at least that many clauses first and the others smallest first."""
     b = dnx.Batch(ctx, slots=len(inst) + 2, arena_bytes=1 << 30)
     b.set_wide(wide, 30)
     b.set_tail_split(split)
     b.set_flow(flow)
+    b.set_order(order)
     if gang:
         b.set_gang(*gang)
     else:
@@ -160,6 +162,19 @@ def test_wide_mode_gives_identical_bits(ctx, inst_set, oracle_results, wide):
 ])
 def test_gang_mode_gives_identical_bits(ctx, inst_set, oracle_results, wide, gang):
     out = run_batch(ctx, inst_set, 1500, spl=40, wide=wide, gang=gang)
+    for r, (steps, t, status, xo) in zip(out, oracle_results):
+        assert r["status"] == (dnx.SAT if status == 1 else dnx.UNKNOWN)
+        assert r["steps"] == steps and r["t"] == t
+        assert same_bits(r["x"], xo)
+        if status == 1:
+            assert r["unsat"] == 0
+
+
+@pytest.mark.parametrize("flow,order", [(True, 6000), (False, 1 << 40)])
+def test_order_gives_identical_bits(ctx, inst_set, oracle_results, flow, order):
+    """Large instances first then the others smallest first (flow launch),
+    and every instance smallest first (launches of a fixed step count)."""
+    out = run_batch(ctx, inst_set, 1500, spl=33, flow=flow, order=order)
     for r, (steps, t, status, xo) in zip(out, oracle_results):
         assert r["status"] == (dnx.SAT if status == 1 else dnx.UNKNOWN)
         assert r["steps"] == steps and r["t"] == t
