@@ -341,8 +341,8 @@ def main():
                     help="only the copy-kernel calibration and 300 FP32 steps of the n=100k instance")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only; the batched instances would shard by instance "
-              "across GPUs with no per-step communication (see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only (--gpus 1); a batch would shard by instance across GPUs, "
+              "with no per-step communication between them", file=sys.stderr)
         sys.exit(2)
 
     if args.profile_single:

@@ -188,3 +188,16 @@ def test_init_uniform():
 def test_cli_usage_without_input():
     r = subprocess.run([ROOT_BIN], capture_output=True, text=True)
     assert r.returncode != 0 and "Syntax" in r.stdout
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 (or a multi-rank launch)
+    exits with a message before any device work."""
+    import os
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    for args, env in (["--gpus", "2"], {}), ([], {"WORLD_SIZE": "2"}):
+        r = subprocess.run([sys.executable, os.path.join(root, "bench.py")] + args, capture_output=True, text=True,
+                           timeout=120, env={**os.environ, **env})
+        assert r.returncode == 2
+        assert "one GPU" in r.stderr and r.stdout == ""
