@@ -856,9 +856,11 @@ static int batch_wide_run(dnx_batch *b, double walltime_s, std::chrono::steady_c
 // running set does not suit it -- an instance too large for one gang, or
 // more workgroups than gang_blocks or than can be resident at once even at
 // the fewest workgroups each instance's clauses allow -- and the caller
-// then runs wide mode.  Returns after the running set has lost half its
-// workgroups (to rebuild the map), when nothing runs, or at the wall-clock
-// limit; the status table in h_dyn (buffer 0) is then current.
+// then runs wide mode.  Returns (for the caller to rebuild the map) after
+// the running set has lost half its workgroups or, while some gang is cut
+// below its instance's wish, after any instance has ended; also when
+// nothing runs, or at the wall-clock limit.  The status table in h_dyn
+// (buffer 0) is then current.
 static int batch_gang_run(dnx_batch *b, double walltime_s, std::chrono::steady_clock::time_point t0, bool *used)
 {
     *used = false;
@@ -888,6 +890,7 @@ static int batch_gang_run(dnx_batch *b, double walltime_s, std::chrono::steady_c
     std::vector<std::pair<int, int>> gs;  // (G, slot)
     std::vector<std::vector<std::array<int, 4>>> on(GANG_XCD);
     std::vector<int> list, glist;
+    std::vector<char> gcut;  // the gang is smaller than its instance asked for
     bool fit = false;
     for (int L = GANG_MAX; L >= 1 && !fit; L--) {
         gs.clear();
@@ -904,6 +907,7 @@ static int batch_gang_run(dnx_batch *b, double walltime_s, std::chrono::steady_c
         for (auto &v : on) v.clear();
         list.clear();
         glist.clear();
+        gcut.clear();
         fit = true;
         int gangs = 0;
         for (auto &g : gs) {
@@ -918,6 +922,9 @@ static int batch_gang_run(dnx_batch *b, double walltime_s, std::chrono::steady_c
             gangs++;
             list.push_back(g.second);
             glist.push_back(g.first);
+            const int m = b->slots[(size_t)g.second].m;
+            const int hi = std::max((m + GANG_CPW - 1) / GANG_CPW, std::min(GANG_MAX, (m + b->gang_cpw - 1) / b->gang_cpw));
+            gcut.push_back(g.first < hi);
         }
     }
     if (!fit) return DNX_OK;
@@ -994,14 +1001,20 @@ static int batch_gang_run(dnx_batch *b, double walltime_s, std::chrono::steady_c
                 b->t_alg[BK_GANG] +=
                     (double)(b->h_dyn[list[i]].steps - before[i]) * step_bytes(b->slots[(size_t)list[i]]);
         int live = 0;
-        bool any = false;
+        bool any = false, ended = false, grow = false;
         for (size_t i = 0; i < list.size(); i++)
             if (b->h_dyn[list[i]].status == DNX_RUNNING) {
                 any = true;
                 live += glist[i];
+                grow |= gcut[i] != 0;
+            } else {
+                ended = true;
             }
         if (!any || group_done(b)) break;
-        if (2 * live < nblk) break;
+        // rebuild once half the workgroups idle, or as soon as an instance
+        // has ended while a running gang is smaller than it asked for (the
+        // freed workgroups can enlarge it)
+        if (2 * live < nblk || (ended && grow)) break;
         if (walltime_s > 0 &&
             std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > walltime_s)
             break;
