@@ -185,9 +185,14 @@ struct dnx_batch {
     unsigned long long gang_epoch = 0;
     int gang_arrive = 0;      // arrivals counted on the device so far
     DevBuf b_gmap, b_gwords, b_gerr;
+    // pinned staging of one instance's arena range at admission (reused: an
+    // admission waits for its upload before it returns)
+    unsigned char *h_upload = nullptr;
+    size_t h_upload_size = 0;
     ~dnx_batch()
     {
         if (arena_base) (void)hipFree(arena_base);
+        if (h_upload) (void)hipHostFree(h_upload);
         if (h_dyn) (void)hipHostFree(h_dyn);
         if (h_stage) (void)hipHostFree(h_stage);
         if (h_work) (void)hipHostFree(h_work);
@@ -577,10 +582,23 @@ int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *p
         return DNX_E_BUDGET;
     }
     const int slot = b->free_slots.back();
-    // stage the whole range on the host, then one copy
-    std::vector<unsigned char> stage(L.total, 0);
+    // stage the whole range in pinned host memory, then one copy
+    hipError_t e;
+    if (b->h_upload_size < L.total) {
+        if (b->h_upload) (void)hipHostFree(b->h_upload);
+        b->h_upload = nullptr;
+        b->h_upload_size = 0;
+        const size_t want = std::max(L.total, (size_t)8 << 20);
+        if ((e = hipHostMalloc((void **)&b->h_upload, want, hipHostMallocDefault)) != hipSuccess) {
+            b->arena.release(off, L.total);
+            return hip_fail(e, "admit staging");
+        }
+        b->h_upload_size = want;
+    }
+    unsigned char *stage = b->h_upload;
+    memset(stage, 0, L.total);
     auto put = [&](size_t at, const void *src, size_t bytes) {
-        if (bytes) memcpy(stage.data() + at, src, bytes);
+        if (bytes) memcpy(stage + at, src, bytes);
     };
     put(L.lits, h.lits.data(), (size_t)nnz * 4);
     if (general) put(L.cp, h.cp.data(), ((size_t)m + 1) * 4);
@@ -589,8 +607,8 @@ int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *p
     } else {
         // chunk-local LDS slots: within chunk k, variable-major, each
         // variable's slots in clause order
-        int32_t *lp = (int32_t *)(stage.data() + L.lpos);
-        int32_t *cb = (int32_t *)(stage.data() + L.cbase);
+        int32_t *lp = (int32_t *)(stage + L.lpos);
+        int32_t *cb = (int32_t *)(stage + L.cbase);
         std::vector<int32_t> run((size_t)n + 1);
         for (int k = 0; k < nchunk; k++) {
             int32_t *base = cb + (size_t)k * ((size_t)n + 1);
@@ -606,8 +624,8 @@ int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *p
         // one word per literal for the fused kernel's steady state: sign in
         // bit 31, variable in bits 16-30, LDS slot in bits 0-15 (plan_mode0
         // admits only instances whose variables and slots fit)
-        const int32_t *lp = (const int32_t *)(stage.data() + L.lpos);
-        uint32_t *pk = (uint32_t *)(stage.data() + L.pk);
+        const int32_t *lp = (const int32_t *)(stage + L.lpos);
+        uint32_t *pk = (uint32_t *)(stage + L.pk);
         for (int64_t q = 0; q < nnz; q++) {
             const int32_t l = h.lits[(size_t)q];
             pk[q] = (l < 0 ? 0x80000000u : 0u) | ((uint32_t)(std::abs(l) - 1) << 16) | (uint32_t)lp[q];
@@ -618,19 +636,18 @@ int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *p
     put(L.eidx, h.eidx.data(), (size_t)nnz * 4);
     put(L.row, h.row_ptr.data(), ((size_t)n + 1) * 4);
     if (h.has_unit) put(L.unit, h.unit.data(), (size_t)n);
-    float *xs = (float *)(stage.data() + L.xs), *xl = (float *)(stage.data() + L.xl);
+    float *xs = (float *)(stage + L.xs), *xl = (float *)(stage + L.xl);
     for (int c = 0; c < m; c++) {
         xs[c] = (float)x0[(size_t)n + c];
         xl[c] = (float)x0[(size_t)n + m + c];
     }
-    BF2 *vd = (BF2 *)(stage.data() + L.vd);
+    BF2 *vd = (BF2 *)(stage + L.vd);
     for (int v = 0; v < n; v++) {  // unit pins at admission (dynex.cc:2694-2699)
         vd[v].v = h.unit[(size_t)v] != 0 ? (float)h.unit[(size_t)v] : (float)x0[v];
         vd[v].d = 0.0f;
     }
     char *base = (char *)b->arena_base + off;
-    hipError_t e;
-    if ((e = hipMemcpyAsync(base, stage.data(), L.total, hipMemcpyHostToDevice, b->ustream)) != hipSuccess) {
+    if ((e = hipMemcpyAsync(base, stage, L.total, hipMemcpyHostToDevice, b->ustream)) != hipSuccess) {
         b->arena.release(off, L.total);
         return hip_fail(e, "admit upload");
     }
@@ -683,7 +700,7 @@ int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *p
         b->arena.release(off, L.total);
         return hip_fail(e, "admit descriptor");
     }
-    // the staging vector is released on return: wait for its copy
+    // the staging buffer is reused by the next admission: wait for its copy
     if ((e = hipStreamSynchronize(b->ustream)) != hipSuccess) return hip_fail(e, "admit sync");
     SlotHost &sl = b->slots[(size_t)slot];
     sl.used = true;
