@@ -326,6 +326,27 @@ def test_single_instance_path_agrees(ctx, inst_set):
             assert np.array_equal(s["assign"], r["assign"])
 
 
+def test_large_instance_admission(ctx):
+    """An instance whose arena range (~20 MB) exceeds the admission staging
+    buffer's first size, admitted between two small ones: the staging grows,
+    and each instance's fixed-step trajectory equals the single-instance FP32
+    path's, bit for bit (the large one runs with its state in device memory)."""
+    inst = []
+    for i, (n, m) in enumerate([(300, 1350), (60000, 258000), (500, 2300)]):
+        inst.append((n, planted_cp(m), dnx.generate_planted(900 + i, n, m), dnx.init_uniform(900 + i, n, m)))
+    out = run_batch(ctx, inst, 40, spl=20, stop=False)
+    for r, (n, cp, lits, x0) in zip(out, inst):
+        gp = ctx.problem_from_lits(n, cp, lits)
+        st = gp.state(dnx.params(len(cp) - 1), x0, dnx.FP32)
+        st.step(40)
+        s = st.read()
+        st.free()
+        gp.free()
+        assert r["status"] == dnx.UNKNOWN and r["steps"] == 40 and s["steps"] == 40
+        assert r["t"] == s["t"]
+        assert same_bits(r["x"], s["x"].astype(np.float32))
+
+
 def test_retire_readmit_and_budget(ctx):
     inst = make_set(13, 10, nmin=200, nmax=2000)
     b = dnx.Batch(ctx, slots=4, arena_bytes=8 << 20)
