@@ -93,6 +93,95 @@ __device__ __forceinline__ void vdw(const GInst &I, int i, BF2 o)
     I.vd[i] = (unsigned long long)__float_as_uint(o.v) | ((unsigned long long)__float_as_uint(o.d) << 32);
 }
 
+// Four consecutive clauses 4g .. 4g+3 of a mode-0 3-SAT instance in the
+// steady state: packed literal words, memories, stored C.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+struct CGroup {
+    uint32_t W[4][3];
+    float XS[4], XL[4], CX[4];
+};
+
+// Group g's clauses inside [c_lo, c_hi): a whole group moves as 16-byte
+// accesses at one 32-bit offset from each array's base (every array starts
+// 256-byte aligned), which leaves almost no address arithmetic; a group cut
+// by the range's ends takes its clauses one at a time (zeros outside).
+__device__ __forceinline__ void load_group(const GInst &I, int g, int c_lo, int c_hi, CGroup &G)
+{
+    const int c4 = g << 2;
+    if (c4 >= c_lo && c4 + 4 <= c_hi) {
+        const uint32_t o16 = (uint32_t)g * 16u, o48 = (uint32_t)g * 48u;
+        GLB const char *pkb = (GLB const char *)I.pk;
+        const u32x4 w0 = *(GLB const u32x4 *)(pkb + o48);
+        const u32x4 w1 = *(GLB const u32x4 *)(pkb + o48 + 16u);
+        const u32x4 w2 = *(GLB const u32x4 *)(pkb + o48 + 32u);
+        const f32x4 a = *(GLB const f32x4 *)((GLB const char *)I.xs + o16);
+        const f32x4 b = *(GLB const f32x4 *)((GLB const char *)I.xl + o16);
+        const f32x4 q = *(GLB const f32x4 *)((GLB const char *)I.cst + o16);
+        G.W[0][0] = w0.x;
+        G.W[0][1] = w0.y;
+        G.W[0][2] = w0.z;
+        G.W[1][0] = w0.w;
+        G.W[1][1] = w1.x;
+        G.W[1][2] = w1.y;
+        G.W[2][0] = w1.z;
+        G.W[2][1] = w1.w;
+        G.W[2][2] = w2.x;
+        G.W[3][0] = w2.y;
+        G.W[3][1] = w2.z;
+        G.W[3][2] = w2.w;
+        G.XS[0] = a.x;
+        G.XS[1] = a.y;
+        G.XS[2] = a.z;
+        G.XS[3] = a.w;
+        G.XL[0] = b.x;
+        G.XL[1] = b.y;
+        G.XL[2] = b.z;
+        G.XL[3] = b.w;
+        G.CX[0] = q.x;
+        G.CX[1] = q.y;
+        G.CX[2] = q.z;
+        G.CX[3] = q.w;
+    } else {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int c = c4 + u;
+            G.W[u][0] = G.W[u][1] = G.W[u][2] = 0u;
+            G.XS[u] = G.XL[u] = G.CX[u] = 0.0f;
+            if (c >= c_lo && c < c_hi) {
+                G.W[u][0] = I.pk[3 * c];
+                G.W[u][1] = I.pk[3 * c + 1];
+                G.W[u][2] = I.pk[3 * c + 2];
+                G.XS[u] = I.xs[c];
+                G.XL[u] = I.xl[c];
+                G.CX[u] = I.cst[c];
+            }
+        }
+    }
+}
+
+// Writes back group g's memories and stored C inside [c_lo, c_hi).
+__device__ __forceinline__ void store_group(const GInst &I, int g, int c_lo, int c_hi, const CGroup &G)
+{
+    const int c4 = g << 2;
+    if (c4 >= c_lo && c4 + 4 <= c_hi) {
+        const uint32_t o16 = (uint32_t)g * 16u;
+        *(GLB f32x4 *)((GLB char *)I.xs + o16) = f32x4{G.XS[0], G.XS[1], G.XS[2], G.XS[3]};
+        *(GLB f32x4 *)((GLB char *)I.xl + o16) = f32x4{G.XL[0], G.XL[1], G.XL[2], G.XL[3]};
+        *(GLB f32x4 *)((GLB char *)I.cst + o16) = f32x4{G.CX[0], G.CX[1], G.CX[2], G.CX[3]};
+    } else {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int c = c4 + u;
+            if (c >= c_lo && c < c_hi) {
+                I.xs[c] = G.XS[u];
+                I.xl[c] = G.XL[u];
+                I.cst[c] = G.CX[u];
+            }
+        }
+    }
+}
+
 template <int MODE, int BS>
 __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, BDyn *__restrict__ dyn,
                                               const int *__restrict__ work, int nwork, int *__restrict__ head,
@@ -211,6 +300,10 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                 BF2 a = pvr(v);
                 return pend ? clampV(a.v + h * a.d) : a.v;
             };
+            // mode 0, 3-SAT steady state: a thread's first clause group of
+            // the next chunk, requested before the current chunk's fold
+            CGroup pre;
+            bool have_pre = false;
             for (int ch = 0; ch < nch; ch++) {
                 const int c_lo = MODE == BM_LDS_ALL ? s_cb[ch] : 0;
                 const int c_hi = MODE == BM_LDS_ALL ? s_cb[ch + 1] : m;
@@ -224,101 +317,47 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                     // measured slower than these global accesses.)
                     float cmax = 0.0f;  // C >= 1/2 anywhere means unsolved
                     // Each thread takes four consecutive clauses 4g .. 4g+3 at
-                    // a time: their packed words (48 bytes) and their
-                    // memories and stored C (16 bytes each) move as 16-byte
-                    // accesses at one 32-bit offset from each array's base
-                    // (every array starts 256-byte aligned), which leaves
-                    // almost no address arithmetic; a group cut by the
-                    // chunk's ends takes its clauses one at a time.
-                    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-                    typedef float f32x4 __attribute__((ext_vector_type(4)));
-                    GLB const char *pkb = (GLB const char *)I.pk;
-                    GLB char *xsb = (GLB char *)I.xs, *xlb = (GLB char *)I.xl, *csb = (GLB char *)I.cst;
+                    // a time (load_group / store_group).
                     const int g_end = (c_hi + 3) >> 2;
+                    bool first = true;
                     for (int g = (c_lo >> 2) + tid; g < g_end; g += BS) {
                         const int c4 = g << 2;
-                        const bool full = c4 >= c_lo && c4 + 4 <= c_hi;
-                        const uint32_t o16 = (uint32_t)g * 16u, o48 = (uint32_t)g * 48u;
-                        uint32_t W[4][3];
-                        float XS[4], XL[4], CX[4];
-                        if (full) {
-                            const u32x4 w0 = *(GLB const u32x4 *)(pkb + o48);
-                            const u32x4 w1 = *(GLB const u32x4 *)(pkb + o48 + 16u);
-                            const u32x4 w2 = *(GLB const u32x4 *)(pkb + o48 + 32u);
-                            const f32x4 a = *(GLB const f32x4 *)(xsb + o16);
-                            const f32x4 b = *(GLB const f32x4 *)(xlb + o16);
-                            const f32x4 q = *(GLB const f32x4 *)(csb + o16);
-                            W[0][0] = w0.x;
-                            W[0][1] = w0.y;
-                            W[0][2] = w0.z;
-                            W[1][0] = w0.w;
-                            W[1][1] = w1.x;
-                            W[1][2] = w1.y;
-                            W[2][0] = w1.z;
-                            W[2][1] = w1.w;
-                            W[2][2] = w2.x;
-                            W[3][0] = w2.y;
-                            W[3][1] = w2.z;
-                            W[3][2] = w2.w;
-                            XS[0] = a.x;
-                            XS[1] = a.y;
-                            XS[2] = a.z;
-                            XS[3] = a.w;
-                            XL[0] = b.x;
-                            XL[1] = b.y;
-                            XL[2] = b.z;
-                            XL[3] = b.w;
-                            CX[0] = q.x;
-                            CX[1] = q.y;
-                            CX[2] = q.z;
-                            CX[3] = q.w;
-                        } else {
-#pragma unroll
-                            for (int u = 0; u < 4; u++) {
-                                const int c = c4 + u;
-                                W[u][0] = W[u][1] = W[u][2] = 0u;
-                                XS[u] = XL[u] = CX[u] = 0.0f;
-                                if (c >= c_lo && c < c_hi) {
-                                    W[u][0] = I.pk[3 * c];
-                                    W[u][1] = I.pk[3 * c + 1];
-                                    W[u][2] = I.pk[3 * c + 2];
-                                    XS[u] = I.xs[c];
-                                    XL[u] = I.xl[c];
-                                    CX[u] = I.cst[c];
-                                }
-                            }
-                        }
+                        CGroup G;
+                        if (first && have_pre)
+                            G = pre;
+                        else
+                            load_group(I, g, c_lo, c_hi, G);
+                        first = false;
 #pragma unroll
                         for (int u = 0; u < 4; u++) {
                             const int c = c4 + u;
                             if (c < c_lo || c >= c_hi) continue;
-                            const int32_t l0 = (int32_t)W[u][0], l1 = (int32_t)W[u][1], l2 = (int32_t)W[u][2];
-                            float xs = XS[u], xl = XL[u];
-                            const float V0 = pvv(__builtin_amdgcn_ubfe(W[u][0], 16, 15));
-                            const float V1 = pvv(__builtin_amdgcn_ubfe(W[u][1], 16, 15));
-                            const float V2 = pvv(__builtin_amdgcn_ubfe(W[u][2], 16, 15));
-                            memory_update_steady(p, CX[u], h, xs, xl);
-                            XS[u] = xs;
-                            XL[u] = xl;
-                            auto emit = [&](int j, float val) { se[W[u][j] & 0xffffu] = val; };
+                            const int32_t l0 = (int32_t)G.W[u][0], l1 = (int32_t)G.W[u][1], l2 = (int32_t)G.W[u][2];
+                            float xs = G.XS[u], xl = G.XL[u];
+                            const float V0 = pvv(__builtin_amdgcn_ubfe(G.W[u][0], 16, 15));
+                            const float V1 = pvv(__builtin_amdgcn_ubfe(G.W[u][1], 16, 15));
+                            const float V2 = pvv(__builtin_amdgcn_ubfe(G.W[u][2], 16, 15));
+                            memory_update_steady(p, G.CX[u], h, xs, xl);
+                            G.XS[u] = xs;
+                            G.XL[u] = xl;
+                            auto emit = [&](int j, float val) { se[G.W[u][j] & 0xffffu] = val; };
                             const float C = clause3_steady(p, l0, l1, l2, V0, V1, V2, xs, xl, emit);
-                            CX[u] = C;
+                            G.CX[u] = C;
                             cmax = dmax(cmax, C);
                         }
-                        if (full) {
-                            *(GLB f32x4 *)(xsb + o16) = f32x4{XS[0], XS[1], XS[2], XS[3]};
-                            *(GLB f32x4 *)(xlb + o16) = f32x4{XL[0], XL[1], XL[2], XL[3]};
-                            *(GLB f32x4 *)(csb + o16) = f32x4{CX[0], CX[1], CX[2], CX[3]};
-                        } else {
-#pragma unroll
-                            for (int u = 0; u < 4; u++) {
-                                const int c = c4 + u;
-                                if (c >= c_lo && c < c_hi) {
-                                    I.xs[c] = XS[u];
-                                    I.xl[c] = XL[u];
-                                    I.cst[c] = CX[u];
-                                }
-                            }
+                        store_group(I, g, c_lo, c_hi, G);
+                    }
+                    // the next chunk's first group of this thread, requested
+                    // before this chunk's fold so that its loads travel
+                    // during the fold (chunk ch does not touch the next
+                    // chunk's clauses)
+                    have_pre = false;
+                    if (ch + 1 < nch) {
+                        const int lo1 = s_cb[ch + 1], hi1 = s_cb[ch + 2];
+                        const int g1 = (lo1 >> 2) + tid;
+                        if (g1 < ((hi1 + 3) >> 2)) {
+                            load_group(I, g1, lo1, hi1, pre);
+                            have_pre = true;
                         }
                     }
                     unsolved |= !(cmax < 0.5f);
