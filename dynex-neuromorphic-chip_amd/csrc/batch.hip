@@ -219,39 +219,30 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
         p.zeta = I.zeta;
         p.xlmax = I.xlmax;
 
-        // the (V, dV) pairs: LDS in modes 0 and 1, device memory in mode 2
-        BF2 *lpv = (BF2 *)smem;
-        auto pvr = [&](int i) -> BF2 {
-            if constexpr (MODE == BM_GLOBAL) {
-                return vdr(I, i);
-            } else {
-                const float2 t = ((const float2 *)lpv)[i];  // one 8-byte LDS read
-                BF2 r;
-                r.v = t.x;
-                r.d = t.y;
-                return r;
-            }
-        };
+        // Modes 0 and 1 keep V of every variable in LDS (lv) and beside it
+        // one float per variable (ld): with several chunks (mode 0) the
+        // variable's partial fold between chunks, and after the variable pass
+        // its new dV, which puts V of the next state in place and is written
+        // back with V, as the pending form, when the launch ends.  The dV of
+        // the state a launch starts from stays in device memory, which this
+        // kernel does not write before the launch ends: only a launch's first
+        // step (V of the state not yet in place) reads it, through pvr.  Mode
+        // 2 keeps the (V, dV) pairs in device memory.
+        float *lv = (float *)smem, *ld = (float *)(smem + lds_var_bytes(n) / 2);
+        auto pvr = [&](int i) -> BF2 { return vdr(I, i); };
         auto pvv = [&](int i) -> float {
             if constexpr (MODE == BM_GLOBAL) return vdr(I, i).v;
-            else return lpv[i].v;
-        };
-        auto pvw = [&](int i, BF2 val) {
-            if constexpr (MODE == BM_GLOBAL) vdw(I, i, val);
-            else lpv[i] = val;
+            else return lv[i];
         };
         // mode 0 keeps the contributions of one chunk of consecutive clauses
-        // at a time in LDS (one chunk when they all fit); with several chunks
-        // each variable's partial fold waits in sacc between chunks
+        // at a time in LDS (one chunk when they all fit)
         const int nch = MODE == BM_LDS_ALL ? I.nchunk : 1;
         constexpr int UV = 2;  // chunk folds: variables whose slot ranges one thread loads at once
-        const size_t pvb = ((size_t)n * sizeof(BF2) + 15) / 16 * 16;
-        float *sacc = (float *)(smem + pvb);
-        float *se = (float *)(smem + pvb + (nch > 1 ? ((size_t)n * 4 + 15) / 16 * 16 : 0));
+        float *se = (float *)(smem + lds_var_bytes(n));
         if (MODE == BM_LDS_ALL)
             for (int i = tid; i <= nch; i += BS) s_cb[i] = I.cbound[i];
         if (MODE != BM_GLOBAL) {
-            for (int v = tid; v < n; v += BS) lpv[v] = vdr(I, v);
+            for (int v = tid; v < n; v += BS) lv[v] = vdr(I, v).v;
             __syncthreads();
         }
         // the variable pass's slot ranges (the last chunk's with several
@@ -522,7 +513,7 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                         for (int u = 0; u < UV; u++) {
                             const int v = vb + u * BS;
                             if (v >= n) break;
-                            sacc[v] = lds_fold(se, RB[u], RE[u], ch ? sacc[v] : 0.0f);
+                            ld[v] = lds_fold(se, RB[u], RE[u], ch ? ld[v] : 0.0f);
                         }
                     }
                     __syncthreads();
@@ -538,22 +529,26 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
             // loads of several variables measured slower)
             int fmax = 0;
             auto var_one = [&](int v, int rb, int re, bool pinned) {
-                BF2 a = pvr(v);
-                const float V = pend && !mat ? clampV(a.v + h * a.d) : a.v;
+                const float V = pend && !mat ? curV(v) : pvv(v);
                 float dV = 0.0f;
                 if (!done) {
                     if (MODE == BM_LDS_ALL) {
-                        dV = lds_fold(se, rb, re, nch > 1 ? sacc[v] : 0.0f);
+                        dV = lds_fold(se, rb, re, nch > 1 ? ld[v] : 0.0f);
                     } else {
                         dV = fold_gather<float>(I.e, I.eidx, rb, re);
                     }
                     if (pinned) dV = 0.0f;  // dynex.cc:306
                     fmax = max(fmax, fail_level(V, dV));
                 }
-                BF2 o;
-                o.v = V;
-                o.d = dV;
-                pvw(v, o);
+                if constexpr (MODE == BM_GLOBAL) {
+                    BF2 o;
+                    o.v = V;
+                    o.d = dV;
+                    vdw(I, v, o);
+                } else {
+                    lv[v] = V;
+                    ld[v] = dV;
+                }
             };
             if (n <= 2 * BS) {
                 // slot ranges and pins from registers: in mode 0 this pass
@@ -580,9 +575,13 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
                 // form, V and dV with the level, which every path resumes from)
                 const float hn = level_h<float>(plev);
                 for (int v = tid; v < n; v += BS) {
-                    BF2 a = pvr(v);
-                    a.v = clampV(a.v + hn * a.d);
-                    pvw(v, a);
+                    if constexpr (MODE == BM_GLOBAL) {
+                        BF2 a = vdr(I, v);
+                        a.v = clampV(a.v + hn * a.d);
+                        vdw(I, v, a);
+                    } else {
+                        lv[v] = clampV(lv[v] + hn * ld[v]);
+                    }
                 }
                 __syncthreads();
                 mat = true;
@@ -590,7 +589,12 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
         }
         __syncthreads();
         if (MODE != BM_GLOBAL)
-            for (int v = tid; v < n; v += BS) vdw(I, v, lpv[v]);
+            for (int v = tid; v < n; v += BS) {
+                BF2 o;
+                o.v = lv[v];
+                o.d = ld[v];
+                vdw(I, v, o);
+            }
         if (tid == 0) {
             BDyn o = d;
             o.steps = steps;

@@ -2,6 +2,7 @@
 // engine (batch_engine.cpp) and its kernels (batch.hip).  No HIP headers.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 
 #include "kernels.hpp"
@@ -9,11 +10,16 @@
 namespace dnx {
 
 // Execution shape of one instance inside the fused step kernel.
-//   0  V/dV pairs and the contribution array live in LDS
-//   1  V/dV pairs live in LDS, contributions in device memory
+//   0  V (and dV) and the contribution array live in LDS
+//   1  V (and dV) live in LDS, contributions in device memory
 //   2  both in device memory (instances too large for LDS)
 enum BatchMode { BM_LDS_ALL = 0, BM_LDS_V = 1, BM_GLOBAL = 2, BM_COUNT = 3 };
 constexpr int KC_MAX = 16;  // mode 0: chunks of consecutive clauses per instance at most
+
+// LDS bytes of the fused kernel's per-variable arrays in modes 0 and 1: V,
+// and one float beside it (the partial fold between chunks, then the new dV),
+// each rounded to 16 bytes; mode 0's chunk contributions follow them.
+constexpr size_t lds_var_bytes(int n) { return 2 * (((size_t)n * 4 + 15) / 16 * 16); }
 
 struct alignas(8) BF2 {
     float v, d;

@@ -230,7 +230,7 @@ struct Mode0Plan {
 static bool plan_mode0(const dnx_batch *b, const HostProblem &h, Mode0Plan &P)
 {
     const size_t cap = (size_t)b->shape[BM_LDS_ALL].lds_cap;
-    const size_t pv = ((size_t)h.n * sizeof(BF2) + 15) / 16 * 16;
+    const size_t pv = lds_var_bytes(h.n);
     const int64_t nnz = h.nnz();
     P.bound.clear();
     // packed literal words hold a 15-bit variable and a 16-bit LDS slot
@@ -241,9 +241,10 @@ static bool plan_mode0(const dnx_batch *b, const HostProblem &h, Mode0Plan &P)
         P.bound = {0, h.m};
         return true;
     }
-    const size_t acc = ((size_t)h.n * 4 + 15) / 16 * 16;
-    if (pv + acc >= cap) return false;
-    const int64_t slots = (int64_t)((cap - pv - acc) / 4);
+    // with several chunks each variable's partial fold lives in the float
+    // beside its V (lds_var_bytes)
+    if (pv >= cap) return false;
+    const int64_t slots = (int64_t)((cap - pv) / 4);
     if (slots < DNX_MAX_LITS || slots * KC_MAX < nnz) return false;
     P.bound.push_back(0);
     int64_t used = 0, most = 0;
@@ -260,7 +261,7 @@ static bool plan_mode0(const dnx_batch *b, const HostProblem &h, Mode0Plan &P)
     most = std::max(most, used);
     P.nchunk = (int)P.bound.size() - 1;
     if (P.nchunk > KC_MAX) return false;
-    P.lds = pv + acc + (size_t)most * 4;
+    P.lds = pv + (size_t)most * 4;
     return true;
 }
 
@@ -269,7 +270,7 @@ static int choose_mode(const dnx_batch *b, const HostProblem &h, Mode0Plan &P)
     if (plan_mode0(b, h, P)) return BM_LDS_ALL;
     P.nchunk = 1;
     P.bound = {0, h.m};
-    const size_t pv = ((size_t)h.n * sizeof(BF2) + 15) / 16 * 16;
+    const size_t pv = lds_var_bytes(h.n);
     P.lds = pv;
     if (pv <= (size_t)b->shape[BM_LDS_V].lds_cap) return BM_LDS_V;
     P.lds = 0;
