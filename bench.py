@@ -30,8 +30,8 @@ HBM_PEAK_GBPS = 8000.0
 # WRITE_SIZE, per dispatch of the batched solve), each scaled by what the
 # counter reads on the copy kernel (256 MiB each way: FETCH_SIZE x 2.000,
 # WRITE_SIZE x 1.000).  Keyed by bucket; k_batch<mode 0>, the flow launch:
-# 475.46 GB read + 282.18 GB written.
-TRAFFIC = {0: 757.64e9}
+# 484.51 GB read + 287.04 GB written.
+TRAFFIC = {0: 771.55e9}
 METRIC = "solved instances/s + p50/p99 time-to-solution (batched FP32 vs plain FP64, 1 GPU)"
 
 
