@@ -1154,7 +1154,17 @@ int dnx_batch_run(dnx_batch *b, int steps_per_launch, double walltime_s, int *re
     long long start_clauses = 0;
     for (int s = 0; s < b->nslots; s++)
         if (b->slots[(size_t)s].used && b->h_dyn[s].status == DNX_RUNNING) start_clauses += b->slots[(size_t)s].m;
-    const long long threshold = std::max(b->wide_clauses, (long long)(b->wide_frac * (double)start_clauses));
+    // the fraction of the run's clauses, but no more than 75% of what gang
+    // mode can hold (its workgroup budget times GANG_CPW clauses; the rest is
+    // slack for each instance's gang rounding up), so that the tail starts
+    // where gang mode takes it over whole instead of in wide mode
+    long long frac_clauses = (long long)(b->wide_frac * (double)start_clauses);
+    if (b->gang_blocks > 0) {
+        if (!b->gang_cap) b->gang_cap = std::max(0, gang_occupancy()) * b->cus;
+        const long long fit = (long long)std::min(b->gang_blocks, b->gang_cap) * GANG_CPW * 3 / 4;
+        if (fit > 0) frac_clauses = std::min(frac_clauses, fit);
+    }
+    const long long threshold = std::max(b->wide_clauses, frac_clauses);
     for (;;) {
         const BDyn *known = b->h_dyn;
         running = 0;

@@ -248,8 +248,10 @@ int dnx_batch_set_flow(dnx_batch *b, int enable);
 
 /* The tail threshold relative to the run: once the running instances hold
  * at most `fraction` of the clauses that were running when dnx_batch_run
- * began, gang mode (when the running set fits it) or else wide mode takes
- * over (the default: 0.0125, 300 steps per wide-mode sync). */
+ * began -- and, with gang mode on, at most 75% of the clauses its workgroup
+ * budget holds (2048 per workgroup) -- gang mode (when the running set fits
+ * it) or else wide mode takes over (the default: 0.0125, 300 steps per
+ * wide-mode sync). */
 int dnx_batch_set_wide_fraction(dnx_batch *b, double fraction, int steps_per_sync);
 
 /* Serving: requests with at least `clauses` clauses run in the wide lane
