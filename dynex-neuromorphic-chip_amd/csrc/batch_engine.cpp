@@ -216,11 +216,11 @@ namespace dnx {
 
 int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *prm, const double *x0,
                          int64_t step_cap, int stop_on_solve, int *slot_out);
-// Mode 0 holds the (V, dV) pairs and the contributions in LDS.  When the
-// contributions do not fit beside the pairs, the clauses are cut into at
-// most KC_MAX chunks of consecutive clauses whose contributions fit; a step
-// then evaluates and folds one chunk at a time, keeping each variable's
-// partial fold in LDS as well.
+// Mode 0 holds V, one more float per variable (lds_var_bytes) and the
+// contributions in LDS.  When the contributions do not fit beside them, the
+// clauses are cut into at most KC_MAX chunks of consecutive clauses whose
+// contributions fit; a step then evaluates and folds one chunk at a time,
+// keeping each variable's partial fold in that float.
 struct Mode0Plan {
     int nchunk = 0;
     size_t lds = 0;                 // dynamic LDS bytes the instance needs
