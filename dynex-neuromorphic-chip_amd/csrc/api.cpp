@@ -461,6 +461,7 @@ int dnx_state_create(dnx_problem *p, int prec, const dnx_params *prm, const doub
     }
     Ctl c;
     init_ctl(c);
+    c.loc_min = c.loc_last = h.m;
     std::vector<char> stage;
     pack_host(prec, x0, N, stage);
     hipStream_t st = p->ctx->stream;
@@ -498,6 +499,35 @@ static int set_mode(dnx_state *s, int stop, long long cap)
         return hip_fail(e, "mode upload");
     // the source values live on this stack frame: wait before returning
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "mode sync");
+    return DNX_OK;
+}
+
+int dnx_state_track_loc(dnx_state *s, int enable)
+{
+    if (!s) return DNX_E_ARG;
+    (void)hipSetDevice(s->p->ctx->device);
+    hipStream_t st = s->p->ctx->stream;
+    // track_loc, loc_min and loc_last are consecutive ints of the block
+    const int v[3] = {enable ? 1 : 0, s->p->hp.m, s->p->hp.m};
+    hipError_t e;
+    if ((e = hipMemcpyAsync(&s->ctl->track_loc, v, sizeof v, hipMemcpyHostToDevice, st)) != hipSuccess)
+        return hip_fail(e, "track_loc upload");
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "track_loc sync");
+    return DNX_OK;
+}
+
+int dnx_state_best_loc(dnx_state *s, int *best_loc, int *last_loc)
+{
+    if (!s) return DNX_E_ARG;
+    (void)hipSetDevice(s->p->ctx->device);
+    hipStream_t st = s->p->ctx->stream;
+    int v[3] = {0, 0, 0};
+    hipError_t e;
+    if ((e = hipMemcpyAsync(v, &s->ctl->track_loc, sizeof v, hipMemcpyDeviceToHost, st)) != hipSuccess)
+        return hip_fail(e, "loc download");
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "loc sync");
+    if (best_loc) *best_loc = v[1];
+    if (last_loc) *last_loc = v[2];
     return DNX_OK;
 }
 
@@ -581,8 +611,10 @@ int dnx_state_solve(dnx_state *s, int64_t step_cap, double walltime_s, int8_t *a
     const HostProblem &h = s->p->hp;
     hipStream_t st = s->p->ctx->stream;
     hipError_t e;
-    // the device step index equals the number of enqueued steps while running
-    int rc = set_mode(s, 1, (long long)s->enq + step_cap);
+    // the device step index equals the number of enqueued steps while running;
+    // a cap that would pass LLONG_MAX is no cap
+    const bool no_cap = step_cap >= LLONG_MAX - 1 - s->enq;
+    int rc = set_mode(s, 1, no_cap ? LLONG_MAX : (long long)s->enq + step_cap);
     if (rc) return rc;
     auto t0 = std::chrono::steady_clock::now();
     hipEvent_t ev[2];
@@ -600,7 +632,7 @@ int dnx_state_solve(dnx_state *s, int64_t step_cap, double walltime_s, int8_t *a
     };
     int st_final = DNX_RUNNING;
     // states 0..step_cap are evaluated: step_cap+1 step pairs reach the cap
-    const long long need = (long long)step_cap + 1;
+    const long long need = no_cap ? LLONG_MAX : (long long)step_cap + 1;
     long long issued = 0;
     int inflight[2], nin = 0, nextb = 0;
     while (rc == DNX_OK) {

@@ -649,6 +649,7 @@ __global__ __launch_bounds__(256) void k_wide_enter(const BDyn *__restrict__ dyn
         c->cap = d.cap;
         c->t = d.t;
         c->t_done = 0;
+        c->track_loc = 0;
         for (int s = 0; s < 3; s++) {
             c->stepped[s] = 0;
             c->done[s] = 0;

@@ -1436,6 +1436,11 @@ struct ServeOut {
     int *status;
     int64_t *steps;
     double *tts, *wait;
+    // optional (may be null, as may their entries): simulated time, sign
+    // assignment and final state (reference layout) of each request
+    double *t = nullptr;
+    int8_t *const *assign = nullptr;
+    double *const *x = nullptr;
 };
 
 // The continuous-batching loop.  Requests become visible at their arrival
@@ -1513,12 +1518,37 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
     // Free slots are in no work list, so the device status record is left
     // as it is.
     std::vector<BF2> vbuf;
+    DevBuf xbuf;
+    std::vector<float> xf;
     auto finish = [&](int s, const BDyn &d, hipStream_t ls) -> int {
         const int r = slot_req[(size_t)s];
         int stt = d.status;
         out.steps[r] = d.steps;
+        if (out.t) out.t[r] = d.t;
         out.tts[r] = host0 + (double)(d.t_done - tick0) / rate - arrival[r];
         SlotHost &sl = b->slots[(size_t)s];
+        if ((out.x && out.x[r]) || (out.assign && out.assign[r])) {
+            // the final state as the instance leaves (the verified state when
+            // solved: an ended instance has no pending step), on the lane's
+            // own stream, which is idle
+            const size_t N = (size_t)sl.n + 2 * (size_t)sl.m;
+            if (xbuf.size < sizeof(float) * (N ? N : 1)) {
+                const int ra = xbuf.alloc(sizeof(float) * (N ? N : 1));
+                if (ra) return ra;
+            }
+            launch_batch_materialize((const BInst *)b->b_inst.ptr + s, (const BDyn *)b->b_dyn.ptr + s, sl.n, sl.m,
+                                     (float *)xbuf.ptr, ls);
+            xf.resize(N ? N : 1);
+            hipError_t ee = hipGetLastError();
+            if (ee == hipSuccess && N)
+                ee = hipMemcpyAsync(xf.data(), xbuf.ptr, sizeof(float) * N, hipMemcpyDeviceToHost, ls);
+            if (ee == hipSuccess) ee = hipStreamSynchronize(ls);
+            if (ee != hipSuccess) return hip_fail(ee, "served state copy");
+            if (out.x && out.x[r])
+                for (size_t i = 0; i < N; i++) out.x[r][i] = (double)xf[i];
+            if (out.assign && out.assign[r])
+                for (int v = 0; v < sl.n; v++) out.assign[r][v] = xf[(size_t)v] >= 0.0f ? 1 : -1;
+        }
         if (stt == DNX_SAT) {
             vbuf.resize((size_t)(sl.n ? sl.n : 1));
             hipError_t ee = sl.n ? hipMemcpyAsync(vbuf.data(), b->h_inst[(size_t)s].vd, sizeof(BF2) * (size_t)sl.n,
@@ -1739,7 +1769,8 @@ extern "C" {
 
 int dnx_serve_trace(dnx_batch *b, int count, dnx_problem *const *probs, const dnx_params *prms,
                     const double *const *x0s, const double *arrival_s, int64_t step_cap, int steps_per_launch,
-                    double deadline_s, int *status, int64_t *steps, double *tts_s, double *wait_s, double *makespan_s)
+                    double deadline_s, int *status, int64_t *steps, double *tts_s, double *wait_s, double *makespan_s,
+                    double *t_out, int8_t *const *assign_out, double *const *x_out)
 {
     if (!b || count < 0 || (count && (!probs || !x0s || !arrival_s || !status || !steps || !tts_s || !wait_s)) ||
         steps_per_launch <= 0)
@@ -1760,6 +1791,9 @@ int dnx_serve_trace(dnx_batch *b, int count, dnx_problem *const *probs, const dn
         return q;
     };
     ServeOut out{status, steps, tts_s, wait_s};
+    out.t = t_out;
+    out.assign = assign_out;
+    out.x = x_out;
     return serve_loop(b, count, arrival_s, step_cap, steps_per_launch, deadline_s, get, [](int) {}, out, makespan_s);
 }
 

@@ -225,8 +225,11 @@ __device__ __forceinline__ T clauseK(const PT<T> &p, CL cl, int k, T Xs, T Xl, G
             min2 = tmin(min2, I);
         }
     }
+    // dynex.cc:391-392 and 431-432 clamp C for k = 3 and k = 2; the k-SAT
+    // branch (dynex.cc:457-470) does not, so the empty clause keeps
+    // C = INT_MAX / 2
     T C = min1 / T(2.0);
-    if (k <= 3) C = clamp01(C);
+    if (k == 2 || k == 3) C = clamp01(C);
     for (int j = 0; j < k; j++) {
         T q = cl[j] > 0 ? T(1) : T(-1);
         T V = clampV(getV(abs(cl[j]) - 1));
@@ -248,7 +251,7 @@ __device__ __forceinline__ T clauseK_C(CL cl, int k, GetV getV)
         min1 = tmin(min1, I);
     }
     T C = min1 / T(2.0);
-    if (k <= 3) C = clamp01(C);
+    if (k == 2 || k == 3) C = clamp01(C);  // as clauseK
     return C;
 }
 
@@ -399,6 +402,32 @@ __device__ __forceinline__ bool any_flag(const int *group)
     const int l = threadIdx.x & 63;
     return __ballot(group[(l & 7) * 32 + (l >> 3)] != 0) != 0ull;
 }
+// loc tracking: workgroup b adds its count of clauses with C >= 1/2 to
+// entry (b & 7) * 32 + ((b >> 3) & 7) of the slot's counters (a workgroup
+// whose count is 0 adds nothing); called by every thread of the workgroup
+template <int BS>
+__device__ __forceinline__ void add_loc(Ctl *c, int s, int unsat, int *sh)
+{
+    const int cnt = block_sum_int<BS>(unsat, sh);
+    if (threadIdx.x == 0 && cnt > 0) {
+        const int b = blockIdx.x;
+        atomicAdd(&c->locc[s][(b & 7) * 32 + ((b >> 3) & 7)], cnt);
+    }
+}
+// sums slot s's counters into loc_last and loc_min (one wave of one
+// workgroup calls it)
+__device__ __forceinline__ void fold_loc(Ctl *c, int s)
+{
+    const int l = threadIdx.x & 63;
+    int v = c->locc[s][(l & 7) * 32 + (l >> 3)];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (l == 0) {
+        c->loc_last = v;
+        if (v < c->loc_min) c->loc_min = v;
+    }
+}
+
 __device__ __forceinline__ int dev_level(const Ctl *c, int s)
 {
     int L = 0;
@@ -413,6 +442,7 @@ __device__ __forceinline__ void clear_slot(Ctl *c, int s)
         const int g = i / FLAG_USED, l = i % FLAG_USED;
         c->flags[s][g][(l & 7) * 32 + (l >> 3)] = 0;
     }
+    for (int l = threadIdx.x; l < FLAG_USED; l += blockDim.x) c->locc[s][(l & 7) * 32 + (l >> 3)] = 0;
     if (threadIdx.x == 0) {
         c->stepped[s] = 0;
         c->done[s] = 0;

@@ -59,7 +59,7 @@ __global__ __launch_bounds__(CBS) void k_step_clause(DevProb P, PT<T> p, const P
         ctl->steps += 1;
     }
     const int c = blockIdx.x * CBS + threadIdx.x;
-    int unsolved = 0;
+    int unsolved = 0, locu = 0;
     if (U3 && c < P.m) {
         // one gather of each [V, dV] pair serves both the previous state's C
         // (lazy memory update) and the new state's V
@@ -79,7 +79,8 @@ __global__ __launch_bounds__(CBS) void k_step_clause(DevProb P, PT<T> p, const P
         auto emit = [&](int j, T val) { ep[j] = val; };
         bool sat;
         T C = clause3_vals<T>(p, l0, l1, l2, r0, r1, r2, clamp01(xs), clampXl(xl, p.xlmax), emit, sat);
-        unsolved = !(C < T(0.5)) || !sat;
+        locu = !(C < T(0.5));
+        unsolved = locu || !sat;
     } else if (!U3 && c < P.m) {
         const int b = P.cp[c];
         const int k = P.cp[c + 1] - b;
@@ -101,7 +102,12 @@ __global__ __launch_bounds__(CBS) void k_step_clause(DevProb P, PT<T> p, const P
         auto emit = [&](int j, T val) { ep[j] = val; };
         bool sat;
         T C = clauseK<T>(p, cl, k, cxs, cxl, curV, emit, sat);
-        unsolved = !(C < T(0.5)) || !sat;
+        locu = !(C < T(0.5));  // dynex.cc:499
+        unsolved = locu || !sat;
+    }
+    if (ctl->track_loc) {
+        __shared__ int shl[CBS / 64];
+        add_loc<CBS>(ctl, slot, locu, shl);
     }
     if (__syncthreads_or(unsolved) && threadIdx.x == 0) raise_flag(ctl->flags[slot][FG_UNSOLVED]);
 }
@@ -122,6 +128,7 @@ __global__ __launch_bounds__(BS) void k_step_var(DevProb P, Pair<T> *__restrict_
     const T h = pend ? level_h<T>(dev_level(ctl, ps)) : T(0.0);
     const bool solved = ctl->stop_on_solve && !any_flag(ctl->flags[slot][FG_UNSOLVED]);
     const int done = solved ? DNX_SAT : (ctl->steps >= ctl->cap ? DNX_UNKNOWN : 0);
+    if (blockIdx.x == 0 && threadIdx.x < 64 && ctl->track_loc) fold_loc(ctl, slot);
     const int w = threadIdx.x >> 6;
     const int v0 = (blockIdx.x * (BS / 64) + w) * 64;
     const int v = v0 + (threadIdx.x & 63);

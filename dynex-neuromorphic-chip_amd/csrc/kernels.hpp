@@ -49,6 +49,16 @@ struct Ctl {
     int stepped[3];     // the variable pass produced a level (a step is pending)
     int done[3];        // DNX_SAT / DNX_UNKNOWN when the state ended the run
     unsigned long long t_done;  // device clock when a run ended (batched wide mode)
+    // loc tracking (single-instance path, when track_loc != 0): the clause
+    // pass adds each workgroup's count of clauses with C >= 1/2 to one of 64
+    // counters of its slot (spread over 8 lines like a flag group), and the
+    // variable pass sums them into loc_last and keeps the lowest in loc_min
+    // (the reference's `global`, dynex.cc:1188, 584-600)
+    int track_loc;
+    int loc_min;
+    int loc_last;
+    int pad_loc;
+    int locc[3][FLAG_GROUP];
     int flags[3][FG_COUNT][FLAG_GROUP];
 };
 

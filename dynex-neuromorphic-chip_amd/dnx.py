@@ -101,7 +101,9 @@ def lib():
     L.dnx_batch_get_shape.argtypes = [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)]
     L.dnx_state_time.argtypes = [vp, ip, C.POINTER(C.c_double)]
     L.dnx_serve_trace.argtypes = [vp, ip, vp, vp, vp, F64P, C.c_int64, ip, dp, vp, vp, vp, vp,
-                                  C.POINTER(C.c_double)]
+                                  C.POINTER(C.c_double), vp, vp, vp]
+    L.dnx_state_track_loc.argtypes = [vp, ip]
+    L.dnx_state_best_loc.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.dnx_serve_planted.argtypes = [vp, C.c_uint64, dp, dp, ip, ip, dp, dp, dp, dp, C.c_int64, ip, dp, ip, ip,
                                     C.POINTER(C.c_int), vp, vp, vp, vp, vp, vp, vp, C.POINTER(C.c_double)]
     _LIB = L
@@ -303,6 +305,15 @@ class State:
         check(lib().dnx_state_time(self.h, int(nsteps), C.byref(ms)))
         return ms.value
 
+    def track_loc(self, enable=True):
+        check(lib().dnx_state_track_loc(self.h, int(bool(enable))))
+
+    def best_loc(self):
+        """(lowest loc since tracking began, loc of the last evaluated state)"""
+        b, l = C.c_int(), C.c_int()
+        check(lib().dnx_state_best_loc(self.h, C.byref(b), C.byref(l)))
+        return b.value, l.value
+
     def solve(self, cap=100000, walltime=0.0):
         a = np.zeros(max(self.prob.n, 1), np.int8)
         steps, t, st = C.c_int64(), C.c_double(), C.c_int()
@@ -433,8 +444,9 @@ class Batch:
         check(lib().dnx_batch_memory(self.h, C.byref(a), C.byref(f)))
         return a.value, f.value
 
-    def serve(self, probs, x0s, arrivals, cap=100000, steps_per_launch=64, deadline=0.0):
-        """Continuous batching over a request trace; see dnx_serve_trace."""
+    def serve(self, probs, x0s, arrivals, cap=100000, steps_per_launch=64, deadline=0.0, results=False):
+        """Continuous batching over a request trace; see dnx_serve_trace.  With results=True each request's
+        simulated time, sign assignment and final state come back too (t, assign, x)."""
         k = len(probs)
         ph = (C.c_void_p * max(k, 1))(*[p.h for p in probs])
         xs = [np.ascontiguousarray(x, np.float64) for x in x0s]
@@ -444,11 +456,21 @@ class Batch:
         steps = np.zeros(max(k, 1), np.int64)
         tts = np.zeros(max(k, 1), np.float64)
         wait = np.zeros(max(k, 1), np.float64)
+        tt = np.zeros(max(k, 1), np.float64)
         mk = C.c_double()
+        asg = [np.zeros(max(p.n, 1), np.int8) for p in probs] if results else []
+        xo = [np.zeros(max(p.n + 2 * p.m, 1), np.float64) for p in probs] if results else []
+        ap = (C.c_void_p * max(k, 1))(*[a.ctypes.data for a in asg]) if results else None
+        xop = (C.c_void_p * max(k, 1))(*[x.ctypes.data for x in xo]) if results else None
         check(lib().dnx_serve_trace(self.h, k, ph, None, xp, arr if k else np.zeros(1), int(cap), int(steps_per_launch),
                                     float(deadline), st.ctypes.data_as(C.c_void_p), steps.ctypes.data_as(C.c_void_p),
-                                    tts.ctypes.data_as(C.c_void_p), wait.ctypes.data_as(C.c_void_p), C.byref(mk)))
-        return dict(status=st[:k], steps=steps[:k], tts=tts[:k], wait=wait[:k], makespan=mk.value)
+                                    tts.ctypes.data_as(C.c_void_p), wait.ctypes.data_as(C.c_void_p), C.byref(mk),
+                                    tt.ctypes.data_as(C.c_void_p), ap, xop))
+        out = dict(status=st[:k], steps=steps[:k], tts=tts[:k], wait=wait[:k], makespan=mk.value, t=tt[:k])
+        if results:
+            out["assign"] = [a[:p.n] for a, p in zip(asg, probs)]
+            out["x"] = [x[:p.n + 2 * p.m] for x, p in zip(xo, probs)]
+        return out
 
     def serve_planted(self, seed, rate, duration, n_min=100, n_max=100000, r_min=4.3, r_max=8.0, p1=0.46, p2=0.46,
                       cap=100000, steps_per_launch=64, max_requests=100000, gen_threads=8, deadline=0.0):

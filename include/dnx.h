@@ -117,6 +117,18 @@ int dnx_state_create(dnx_problem *p, int prec, const dnx_params *prm,
                      const double *x0, dnx_state **out);
 int dnx_state_free(dnx_state *s);
 
+/* loc tracking, for the reference's `global` (the lowest loc reached,
+ * dynex.cc:190, 1188, 584-600, printed at 1500-1504 and 2864): with enable
+ * != 0 every state the run evaluates has its loc (clauses with C >= 1/2,
+ * dynex.cc:499) counted on the device and the lowest kept; enabling resets
+ * the lowest to m.  Off by default (it adds one atomic per clause workgroup
+ * per step). */
+int dnx_state_track_loc(dnx_state *s, int enable);
+
+/* The lowest loc over the states evaluated since tracking was enabled (m if
+ * none) and the loc of the last evaluated state. */
+int dnx_state_best_loc(dnx_state *s, int *best_loc, int *last_loc);
+
 /* Advances nsteps adaptive steps without stopping at a solution (fixed-step
  * trajectory mode).  No host synchronisation inside. */
 int dnx_state_step(dnx_state *s, int nsteps);
@@ -313,11 +325,16 @@ int dnx_batch_memory(dnx_batch *b, int64_t *arena_bytes, int64_t *free_bytes);
  * retirement by the device clock, and queueing delay before admission.
  * prms may be NULL (reference defaults).  With deadline_s > 0 the loop
  * stops deadline_s seconds after the call starts; requests not finished by
- * then stay DNX_UNKNOWN with time-to-solution -1. */
+ * then stay DNX_UNKNOWN with time-to-solution -1.  Optional outputs (NULL to
+ * skip; a NULL entry skips one request): t_out[i] the simulated time of the
+ * request's last state, assign_out[i] (n entries) its sign assignment and
+ * x_out[i] (n + 2m entries) its final state in the reference layout, read
+ * as the request retires. */
 int dnx_serve_trace(dnx_batch *b, int count, dnx_problem *const *probs, const dnx_params *prms,
                     const double *const *x0s, const double *arrival_s, int64_t step_cap,
                     int steps_per_launch, double deadline_s, int *status, int64_t *steps,
-                    double *tts_s, double *wait_s, double *makespan_s);
+                    double *tts_s, double *wait_s, double *makespan_s, double *t_out,
+                    int8_t *const *assign_out, double *const *x_out);
 
 /* The served planted stream: Poisson arrivals at rate_per_s for duration_s
  * (at most max_requests), n log-uniform in [n_min, n_max], clause ratio

@@ -39,6 +39,8 @@ def make_set(seed, count, nmin=60, nmax=2500, rmin=4.6, rmax=8.0, with_general=T
             cl[0] = cl[0][:2]
             cl[1] = cl[1][:1]
             cl[2] = cl[2] + [-cl[3][0]]
+            if i % 14 == 3:
+                cl[4] = []  # the empty clause: C = INT_MAX/2, never solved (dynex.cc:457-470)
             cp = np.cumsum([0] + [len(c) for c in cl]).astype(np.int32)
             lits = np.array(sum(cl, []), np.int32)
         x0 = dnx.init_uniform(seed * 1000 + i, n, m)
@@ -380,11 +382,22 @@ def test_retire_readmit_and_budget(ctx):
     b.close()
 
 
+def check_served(r, i, steps, t, status, xo, n):
+    """Request i of a served trace against the instance solved alone: status,
+    steps, simulated time, the final state bit for bit and the assignment."""
+    assert r["status"][i] == (dnx.SAT if status == 1 else dnx.UNKNOWN), i
+    assert r["steps"][i] == steps and r["t"][i] == t, i
+    assert same_bits(r["x"][i], xo), i
+    assert np.array_equal(r["assign"][i], np.where(np.asarray(xo[:n]) >= 0, 1, -1).astype(np.int8)), i
+    assert r["tts"][i] > 0 and r["wait"][i] >= 0
+
+
 @pytest.mark.parametrize("big", [1 << 40, 8000])
 def test_serving_results_equal_solo_runs(ctx, inst_set, oracle_results, big):
     """Requests arriving while others run are admitted as slots and memory
-    free up; every served result equals the instance solved alone, whether it
-    ran in the narrow lane or (clauses >= big) in the wide lane."""
+    free up; every served result equals the instance solved alone (the float
+    oracle: steps, time, final state bits, assignment), whether it ran in the
+    narrow lane or (clauses >= big) in the wide lane."""
     k = 24
     probs = [dnx.host_problem(n, cp, lits) for n, cp, lits, _ in inst_set[:k]]
     x0s = [x0 for _, _, _, x0 in inst_set[:k]]
@@ -392,17 +405,35 @@ def test_serving_results_equal_solo_runs(ctx, inst_set, oracle_results, big):
     # few slots and a small arena force queueing and slot reuse
     b = dnx.Batch(ctx, slots=6, arena_bytes=6 << 20)
     b.set_serve_big(big)
-    r = b.serve(probs, x0s, arrivals, cap=1500, steps_per_launch=16)
+    r = b.serve(probs, x0s, arrivals, cap=1500, steps_per_launch=16, results=True)
     b.close()
     for p in probs:
         p.free()
     for i in range(k):
-        steps, t, status, _ = oracle_results[i]
-        assert r["status"][i] == (dnx.SAT if status == 1 else dnx.UNKNOWN)
-        assert r["steps"][i] == steps
-        assert r["tts"][i] > 0 and r["wait"][i] >= 0
+        steps, t, status, xo = oracle_results[i]
+        check_served(r, i, steps, t, status, xo, inst_set[i][0])
     assert r["makespan"] >= arrivals[-1]
     assert np.max(r["wait"]) > 0  # some requests had to queue
+
+
+def test_serving_above_default_wide_threshold(ctx, inst_set, oracle_results):
+    """The default policy: a request of 63,000 clauses (above the 60,000-clause
+    default) runs in the wide lane while small requests arrive around it in
+    the narrow lane; each result equals its solo run (the float oracle)."""
+    n, m = 9000, 63000
+    big = (n, planted_cp(m), dnx.generate_planted(6300, n, m), dnx.init_uniform(6300, n, m))
+    insts = list(inst_set[:6]) + [big] + list(inst_set[6:12])
+    probs = [dnx.host_problem(nn, cp, lits) for nn, cp, lits, _ in insts]
+    arrivals = np.sort(np.random.default_rng(4).uniform(0.0, 0.1, len(insts)))
+    b = dnx.Batch(ctx, slots=8)
+    r = b.serve(probs, [x0 for *_, x0 in insts], arrivals, cap=1500, steps_per_launch=16, results=True)
+    b.close()
+    for p in probs:
+        p.free()
+    ref = oracle_results[:6] + [oracle_solve(*big, 1500)] + oracle_results[6:12]
+    for i, ((nn, *_), (steps, t, status, xo)) in enumerate(zip(insts, ref)):
+        check_served(r, i, steps, t, status, xo, nn)
+    assert r["status"][6] == dnx.SAT
 
 
 def test_serve_planted_stream(ctx):

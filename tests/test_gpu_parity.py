@@ -30,7 +30,10 @@ def gen(n, m, seed):
 
 
 def mixed_problem(seed=4):
-    """k in {1,2,3,4,5}, a duplicated literal, an unused variable, unit pins."""
+    """k in {0,1,2,3,4,5}, a duplicated literal, an unused variable, unit
+    pins.  The empty clause (a lone "0" in DIMACS, dynex.cc:2369-2386) takes
+    the reference's k-SAT branch, whose C = INT_MAX/2 is not clamped
+    (dynex.cc:457-470)."""
     rng = np.random.default_rng(seed)
     n = 60
     cls = []
@@ -39,6 +42,8 @@ def mixed_problem(seed=4):
         vs = rng.choice(np.arange(1, n), k, replace=False)  # variable n never used
         cls.append(list(vs * rng.choice([-1, 1], k)))
     cls.append([5, 5, -7])
+    cls.insert(17, [])
+    cls.insert(200, [])
     cp = np.cumsum([0] + [len(c) for c in cls]).astype(np.int32)
     return n, cp, np.array(sum(cls, []), np.int32)
 
@@ -187,31 +192,59 @@ def test_trajectory_c2_fp64_1000_steps(ctx):
 
 @pytest.mark.parametrize("prec", [64, 32])
 def test_trajectory_c3_full_size(ctx, prec):
-    """n=100,000, m=800,000 (the largest single-instance size): the first
-    steps bit-identical to the oracle."""
+    """n=100,000, m=800,000 (the largest single-instance size): 100 steps
+    bit-identical to the oracle, compared after 1, 4, 30 and 100 steps."""
     cp, lits = gen(100000, 800000, 23)
     gp, op = both(ctx, 100000, cp, lits)
     x0 = dnx.init_uniform(23, 100000, 800000)
-    run_traj(gp, op, x0, [1, 4], prec)
+    run_traj(gp, op, x0, [1, 4, 30, 100], prec)
     gp.free()
 
 
 def test_trajectory_c2_fp32_vs_fp64(ctx):
-    """FP32 is bit-identical to the float oracle; its distance from the FP64
-    trajectory is reported (not a parity bound)."""
+    """FP32 is bit-identical to the float oracle for 1,000 steps; its
+    distance from the FP64 trajectory is reported (not a parity bound)."""
     cp, lits = gen(10000, 80000, 22)
     gp, op = both(ctx, 10000, cp, lits)
     x0 = dnx.init_uniform(22, 10000, 80000)
-    x32 = run_traj(gp, op, x0, [10, 100, 200], 32)
+    marks = [10, 100, 1000]
+    x32 = run_traj(gp, op, x0, marks, 32)
     s64 = gp.state(dnx.params(80000), x0, dnx.FP64)
     try:
-        for k, x in zip([10, 100, 200], x32):
-            s64.step(k - (0 if k == 10 else {100: 10, 200: 100}[k]))
+        done = 0
+        for k, x in zip(marks, x32):
+            s64.step(k - done)
+            done = k
             ref = s64.read()["x"]
             dv = np.max(np.abs(x[:10000] - ref[:10000]))
             print(f"step {k}: max |V32 - V64| = {dv:.3e}")
     finally:
         s64.free()
+    gp.free()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_loc_tracking_matches_oracle(ctx, prec):
+    """The device-side loc count of every evaluated state (the reference's
+    `global`, dynex.cc:1188, 584-600) against the oracle's per-state loc,
+    including an empty clause, which is never satisfied."""
+    n, cp, lits = mixed_problem(8)
+    gp, op = both(ctx, n, cp, lits)
+    m = gp.m
+    x0 = random_state(np.random.default_rng(9), n, m)
+    st = gp.state(dnx.params(m), x0, dnx.FP64 if prec == 64 else dnx.FP32)
+    xo = x0.copy() if prec == 64 else x0.astype(np.float32)
+    xo[:n] = np.where(op.unit != 0, op.unit, xo[:n])
+    try:
+        assert st.best_loc() == (m, m)  # nothing evaluated yet
+        st.track_loc(True)
+        _, _, loc = O.trajectory(op, O.params(m), xo, 90, prec=prec)
+        for k in (1, 2, 30, 90):
+            st.step(k - (0 if k == 1 else {2: 1, 30: 2, 90: 30}[k]))
+            assert st.best_loc() == (int(loc[:k].min()), int(loc[k - 1])), k
+        assert loc.min() >= 2  # the two empty clauses
+    finally:
+        st.free()
     gp.free()
 
 
@@ -301,16 +334,28 @@ def test_cli_portfolio(cnf_dir, tmp_path):
     assert r.stdout.rstrip().endswith("0")
 
 
-def test_cli_solves_fixture(cnf_dir, tmp_path):
+@pytest.mark.parametrize("quiet", [0, 1])
+def test_cli_solves_fixture(cnf_dir, tmp_path, quiet):
+    """The reference's solved output (dynex.cc:662-690, 1499-1514, 2864):
+    SOLUTION FOUND, the signed literals of the verified state, VERIFYING,
+    SAT (VERIFIED), TIME SPENT, BEST LOC, the v lines and global = 0; the
+    reference prints all of it under -q 1 as well (it clears quiet when it
+    finds the solution, dynex.cc:663, 716)."""
     src = cnf_path(cnf_dir, 200)
     path = str(tmp_path / os.path.basename(src))
     with open(src) as f, open(path, "w") as g:
         g.write(f.read())
     exe = os.path.join(dnx.HERE, "lib", "dynex")
-    r = subprocess.run([exe, "-i", path], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, "-i", path, "-q", str(quiet)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert "s [0] SATISFIABLE" in r.stdout and "SAT (VERIFIED)" in r.stdout
-    vals = [int(t) for ln in r.stdout.splitlines() if ln.startswith("v ") for t in ln[2:].split()]
+    lines = r.stdout.splitlines()
+    assert "s [0] SATISFIABLE" in r.stdout and "c [0] SAT (VERIFIED)" in lines
+    k = [i for i, ln in enumerate(lines) if ln.startswith("c [0] T=") and "SOLUTION FOUND" in ln]
+    assert len(k) == 1
+    listing = [int(t) for t in lines[k[0] + 1].split()]
+    assert lines[k[0] + 2] == "c [0] VERIFYING..."
+    assert "c [0] BEST LOC = 0 (GLOBAL 0) **" in lines
+    vals = [int(t) for ln in lines if ln.startswith("v ") for t in ln[2:].split()]
     assert vals[-1] == 0 and len(vals) == 201
     sol = open(path + ".solution.txt").read().replace(",", " ").split()
     op = O.Problem.from_dimacs(path)
@@ -318,4 +363,52 @@ def test_cli_solves_fixture(cnf_dir, tmp_path):
     for t in sol:
         a[abs(int(t)) - 1] = 1.0 if int(t) > 0 else -1.0
     assert O.verify(op, np.concatenate([a, np.zeros(2 * op.m)])) == 0
-    assert r.stdout.rstrip().endswith("0")
+    # the listing names each variable whose V is not zero with its sign, and
+    # agrees with the solution file and the v lines
+    assert len(set(abs(v) for v in listing)) == len(listing)
+    for v in listing:
+        assert np.sign(v) == a[abs(v) - 1] and vals[abs(v) - 1] == v
+    assert r.stdout.rstrip().endswith("0") and lines[-1].endswith("0")
+
+
+def unsat_file(cnf_dir, tmp_path):
+    """The n=100 fixture plus an empty clause: never satisfiable, and its
+    loc never drops below 1."""
+    src = cnf_path(cnf_dir, 100)
+    txt = open(src).read().replace("p cnf 100 800", "p cnf 100 801")
+    path = str(tmp_path / "with_empty_clause.cnf")
+    with open(path, "w") as g:
+        g.write(txt.rstrip("\n") + "\n0\n")
+    return path
+
+
+def test_cli_step_cap_unknown_and_global(cnf_dir, tmp_path):
+    """-z caps the adaptive loop: UNKNOWN, BEST LOC and `global` equal the
+    lowest loc the oracle's trajectory reaches over states 0..cap."""
+    path = unsat_file(cnf_dir, tmp_path)
+    exe = os.path.join(dnx.HERE, "lib", "dynex")
+    r = subprocess.run([exe, "-i", path, "-z", "60"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    op = O.Problem.from_dimacs(path)
+    assert op.m == 801 and op.cp[800] == op.cp[801]
+    x = O.init_ref(op, 0)
+    _, _, loc = O.trajectory(op, O.params(op.m), x, 61)
+    g = int(loc.min())
+    assert g >= 1
+    lines = r.stdout.splitlines()
+    assert "s [0] UNKNOWN" in lines and "SOLUTION FOUND" not in r.stdout
+    assert f"c [0] BEST LOC = {g} (GLOBAL {g}) **" in lines
+    assert lines[-1].endswith(str(g)) and r.stdout.rstrip().split()[-1] == str(g)
+
+
+def test_cli_walltime_ends_silently(cnf_dir, tmp_path):
+    """-u: the reference's RHS ends the thread once the wall clock passes the
+    limit (dynex.cc:335-345), so no TIME SPENT or s line follows; only
+    `global`, the lowest loc reached, is printed (dynex.cc:2864)."""
+    path = unsat_file(cnf_dir, tmp_path)
+    exe = os.path.join(dnx.HERE, "lib", "dynex")
+    r = subprocess.run([exe, "-i", path, "-u", "0.5"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "TIME SPENT" not in r.stdout and "s [0]" not in r.stdout and "SOLUTION" not in r.stdout
+    g = int(r.stdout.rstrip().split()[-1])
+    assert 1 <= g <= 801
