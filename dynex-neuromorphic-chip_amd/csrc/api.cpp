@@ -32,7 +32,58 @@ DevProb dev_prob(const dnx_problem *p)
     d.eidx = p->d_eidx;
     d.row_ptr = p->d_row_ptr;
     d.unit = p->hp.has_unit ? p->d_unit : nullptr;
+    d.gw = nullptr;
+    d.gbase = nullptr;
+    d.geidx = nullptr;
     return d;
+}
+
+DevProb dev_prob_state(const dnx_state *s)
+{
+    DevProb d = dev_prob(s->p);
+    if (s->grouped) {
+        const int i = s->prec == DNX_FP64 ? 0 : 1;
+        d.gw = (const uint32_t *)s->p->b_gw[i].ptr;
+        d.gbase = (const int32_t *)s->p->b_gbase[i].ptr;
+        d.geidx = (const int32_t *)s->p->b_geidx[i].ptr;
+    }
+    return d;
+}
+
+// The grouped-exchange tables of a problem for one precision (its
+// variable-pass workgroup size), built and uploaded once.  Returns whether
+// they exist.
+static int ensure_grouped(dnx_problem *p, int prec, bool *have)
+{
+    const int i = prec == DNX_FP64 ? 0 : 1;
+    *have = false;
+    if (p->gx_state[i] < 0) return DNX_OK;
+    if (p->gx_state[i] == 0) {
+        std::vector<uint32_t> gw;
+        std::vector<int32_t> gb, ge;
+        if (!grouped_tables(p->hp, gx_var_block(prec), gw, gb, ge)) {
+            p->gx_state[i] = -1;
+            return DNX_OK;
+        }
+        int rc;
+        if ((rc = p->b_gw[i].alloc(sizeof(uint32_t) * gw.size())) ||
+            (rc = p->b_gbase[i].alloc(sizeof(int32_t) * gb.size())) ||
+            (rc = p->b_geidx[i].alloc(sizeof(int32_t) * ge.size())))
+            return rc;
+        hipStream_t st = p->ctx->stream;
+        hipError_t e;
+        if ((!gw.empty() && (e = hipMemcpyAsync(p->b_gw[i].ptr, gw.data(), p->b_gw[i].size, hipMemcpyHostToDevice,
+                                                st)) != hipSuccess) ||
+            (!gb.empty() && (e = hipMemcpyAsync(p->b_gbase[i].ptr, gb.data(), p->b_gbase[i].size,
+                                                hipMemcpyHostToDevice, st)) != hipSuccess) ||
+            (!ge.empty() && (e = hipMemcpyAsync(p->b_geidx[i].ptr, ge.data(), p->b_geidx[i].size,
+                                                hipMemcpyHostToDevice, st)) != hipSuccess))
+            return hip_fail(e, "exchange tables upload");
+        if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "exchange tables sync");
+        p->gx_state[i] = 1;
+    }
+    *have = true;
+    return DNX_OK;
 }
 
 DevParams dev_params(const dnx_params *p)
@@ -135,7 +186,7 @@ static void init_ctl(Ctl &c)
 static int state_enqueue_steps(dnx_state *s, long long count)
 {
     hipStream_t st = s->p->ctx->stream;
-    DevProb P = dev_prob(s->p);
+    DevProb P = dev_prob_state(s);
     // align to the graph's slot phase, then replay whole chunks
     while (count > 0 && (s->enq % 3) != 0) {
         launch_step(s->prec, P, s->prm, s->b_vd.ptr, s->b_xs.ptr, s->b_xl.ptr, s->b_e.ptr, s->ctl,
@@ -174,6 +225,9 @@ static int state_enqueue_steps(dnx_state *s, long long count)
     if (e != hipSuccess) return hip_fail(e, "step launch");
     return DNX_OK;
 }
+
+// the grouped exchange applies to 3-SAT problems of up to GX_MAX_VARS variables
+static bool m_ok_grouped(const HostProblem &h) { return h.uniform3 && h.n <= GX_MAX_VARS && h.m > 0; }
 
 // clauses of the state x (reference layout) with C >= 1/2, i.e. whose
 // literals all have q*clamp(V) <= 0 (dynex.cc:384-387, 499)
@@ -327,6 +381,13 @@ int dnx_problem_from_lits(dnx_ctx *ctx, int n, int m, const int32_t *clause_ptr,
     return finish_problem(ctx, p, out);
 }
 
+int dnx_problem_set_exchange(dnx_problem *p, int exchange)
+{
+    if (!p || (exchange != DNX_EXCHANGE_LITERAL && exchange != DNX_EXCHANGE_GROUPED)) return DNX_E_ARG;
+    p->exchange = exchange;
+    return DNX_OK;
+}
+
 int dnx_problem_info(const dnx_problem *p, int *n, int *m, int64_t *nnz)
 {
     if (!p) return DNX_E_ARG;
@@ -443,6 +504,10 @@ int dnx_state_create(dnx_problem *p, int prec, const dnx_params *prm, const doub
     s->prec = prec;
     s->prm = dev_params(prm);
     int rc;
+    if (p->exchange == DNX_EXCHANGE_GROUPED && m_ok_grouped(h) && (rc = ensure_grouped(p, prec, &s->grouped))) {
+        delete s;
+        return rc;
+    }
     DevBuf bx;
     if ((rc = s->b_vd.alloc(2 * es * (size_t)(h.n > 0 ? h.n : 1))) ||
         (rc = s->b_xs.alloc(es * (size_t)(h.m > 0 ? h.m : 1))) ||

@@ -43,11 +43,17 @@ struct dnx_problem {
     dnx::DevBuf b_cp, b_lits, b_eidx, b_row, b_unit;
     int32_t *d_cp = nullptr, *d_lits = nullptr, *d_eidx = nullptr, *d_row_ptr = nullptr;
     int8_t *d_unit = nullptr;
+    // grouped-exchange tables per precision (index 0 FP64, 1 FP32), built on
+    // the first state that uses them: 0 not yet, 1 built, -1 not applicable
+    dnx::DevBuf b_gw[2], b_gbase[2], b_geidx[2];
+    int gx_state[2] = {0, 0};
+    int exchange = DNX_EXCHANGE_GROUPED;  // for states created from now on
 };
 
 struct dnx_state {
     dnx_problem *p = nullptr;
     int prec = DNX_FP64;
+    bool grouped = false;  // steps use the grouped exchange
     dnx::DevParams prm{};
     dnx::DevBuf b_vd, b_xs, b_xl, b_e, b_ctl;
     dnx::Ctl *ctl = nullptr;     // device control block
@@ -63,6 +69,8 @@ struct dnx_state {
 
 namespace dnx {
 DevProb dev_prob(const dnx_problem *p);
+// the problem as a state's steps see it (with its exchange tables)
+DevProb dev_prob_state(const dnx_state *s);
 // dnx_portfolio_solve in DNX_FP32: the copies run as slots of one batch.
 int portfolio_batched(dnx_problem *p, const dnx_params *prm, int seed, int threads, int64_t step_cap,
                       double walltime_s, int8_t *assign_out, int64_t *steps_out, double *t_out, int *status,

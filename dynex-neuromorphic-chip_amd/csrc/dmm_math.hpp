@@ -414,6 +414,26 @@ __device__ __forceinline__ void add_loc(Ctl *c, int s, int unsat, int *sh)
         atomicAdd(&c->locc[s][(b & 7) * 32 + ((b >> 3) & 7)], cnt);
     }
 }
+// the same for a per-thread count of clauses (a thread that evaluates
+// several clauses)
+template <int BS>
+__device__ __forceinline__ void add_loc_count(Ctl *c, int s, int count, int *sh)
+{
+    int v = count;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int cnt = 0;
+        for (int i = 0; i < BS / 64; i++) cnt += sh[i];
+        if (cnt > 0) {
+            const int b = blockIdx.x;
+            atomicAdd(&c->locc[s][(b & 7) * 32 + ((b >> 3) & 7)], cnt);
+        }
+    }
+}
 // sums slot s's counters into loc_last and loc_min (one wave of one
 // workgroup calls it)
 __device__ __forceinline__ void fold_loc(Ctl *c, int s)

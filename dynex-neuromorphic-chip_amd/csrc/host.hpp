@@ -69,6 +69,40 @@ void init_uniform(uint64_t seed, int n, int m, double xs0, double xl0, double *x
 
 int count_unsat(const HostProblem &p, const int8_t *assign);
 
+// Tables of the grouped exchange (3-SAT, n <= GX_MAX_VARS): variable v is
+// folded by the variable-pass workgroup v / vb, which runs on XCD
+// (v / vb) % 8 (workgroups are dealt to the 8 XCDs in turn), so group
+// g = (v / vb) & 7.  Contributions are stored group by group, each group in
+// literal order; gw/gbase let the clause pass find a literal's position,
+// geidx gives the fold each slot's position.  False when not 3-SAT or too
+// many variables.
+bool grouped_tables(const HostProblem &p, int vb, std::vector<uint32_t> &gw, std::vector<int32_t> &gbase,
+                    std::vector<int32_t> &geidx);
+
+// Tables of the sliced exchange (3-SAT).  The variables are cut into slices
+// of S; every clause touches at most three.  Clauses are regrouped into
+// units, each holding clauses whose variables all lie in the same (at most
+// three) slices, about m / cus clauses per unit and at most cus units, so
+// one workgroup per unit can hold the unit's slices of V in LDS and read V
+// from there.  A unit's clauses are taken SUB at a time (a sub-batch); a
+// sub-batch's contributions are staged in LDS ordered by destination fold
+// block (F variables) and written out as one contiguous run per block.  e
+// is block-major: fold block b owns positions [row_ptr[bF], row_ptr[bF+F])
+// and its runs arrive there in processing order; q gives, for each
+// position, the variable-major slot (relative to the block's first slot)
+// of the literal stored there.
+struct SlicedPlan {
+    int S = 0, SUB = 0, F = 0;
+    int max_region = 0;             // slots of the largest fold block
+    std::vector<uint32_t> gw;       // 3 per clause, processing order: sign | LDS index << 15 | staging slot
+    std::vector<int32_t> units;     // 8 per unit: c0, c1, nslices, slice0..2, first sub-batch, 0
+    std::vector<int32_t> sb_run;    // per sub-batch, its first run; one more entry at the end
+    std::vector<int32_t> runs;      // 2 per run: staging start << 16 | length, e position
+    std::vector<uint16_t> q;        // per e position
+    std::vector<int32_t> pidx;      // per clause, its processing position
+};
+bool sliced_tables(const HostProblem &p, int S, int SUB, int F, int cus, int region_cap, SlicedPlan &out);
+
 // Thread-local error message used by dnx_last_error().
 void set_error(const std::string &msg);
 

@@ -1,5 +1,6 @@
 // host_problem.cpp -- DIMACS I/O, CSR transpose, initial states and the
 // planted-instance generator.
+#include <algorithm>
 #include <cerrno>
 #include <climits>
 #include <cstdio>
@@ -8,6 +9,7 @@
 #include <random>
 
 #include "host.hpp"
+#include "kernels.hpp"
 
 namespace dnx {
 
@@ -52,6 +54,236 @@ int finalize_problem(HostProblem &p)
     p.eidx.assign((size_t)p.nnz(), 0);
     for (int64_t q = 0; q < p.nnz(); q++) p.eidx[(size_t)p.lpos[(size_t)q]] = (int32_t)q;
     return DNX_OK;
+}
+
+bool grouped_tables(const HostProblem &p, int vb, std::vector<uint32_t> &gw, std::vector<int32_t> &gbase,
+                    std::vector<int32_t> &geidx)
+{
+    if (!p.uniform3 || p.n > GX_MAX_VARS || vb <= 0) return false;
+    const int64_t nnz = p.nnz();
+    const int64_t nchunk = ((int64_t)p.m + GX_CHUNK - 1) / GX_CHUNK;
+    auto group = [&](int64_t q) { return (int)(((std::abs(p.lits[(size_t)q]) - 1) / vb) & (GX_GROUPS - 1)); };
+    int64_t cur[GX_GROUPS] = {0};
+    for (int64_t q = 0; q < nnz; q++) cur[group(q)]++;
+    for (int g = GX_GROUPS - 1, s = (int)nnz; g >= 0; g--) {  // group starts
+        s -= (int)cur[g];
+        cur[g] = s;
+    }
+    gw.assign((size_t)nnz, 0u);
+    gbase.assign((size_t)(nchunk * GX_GROUPS), 0);
+    std::vector<int32_t> epos((size_t)nnz);
+    for (int64_t k = 0; k < nchunk; k++) {
+        int cnt[GX_GROUPS] = {0};
+        for (int g = 0; g < GX_GROUPS; g++) gbase[(size_t)(k * GX_GROUPS + g)] = (int32_t)cur[g];
+        const int64_t q1 = std::min<int64_t>(nnz, (k + 1) * 3 * GX_CHUNK);
+        for (int64_t q = k * 3 * GX_CHUNK; q < q1; q++) {
+            const int32_t l = p.lits[(size_t)q];
+            const int g = group(q), off = cnt[g]++;
+            gw[(size_t)q] = (l < 0 ? 0x80000000u : 0u) | ((uint32_t)(std::abs(l) - 1) << 11) | ((uint32_t)g << 8) |
+                            (uint32_t)off;
+            epos[(size_t)q] = (int32_t)(cur[g] + off);
+        }
+        for (int g = 0; g < GX_GROUPS; g++) cur[g] += cnt[g];
+    }
+    geidx.resize((size_t)nnz);
+    for (int64_t s = 0; s < nnz; s++) geidx[(size_t)s] = epos[(size_t)p.eidx[(size_t)s]];
+    return true;
+}
+
+bool sliced_tables(const HostProblem &p, int S, int SUB, int F, int cus, int region_cap, SlicedPlan &out)
+{
+    if (!p.uniform3 || p.m <= 0 || S <= 0 || SUB <= 0 || SUB > 1365 || F <= 0 || cus <= 0) return false;
+    const int n = p.n, m = p.m;
+    const int NS = (n + S - 1) / S;
+    // fold blocks: every block's slots must fit the fold's LDS and q's 16 bits
+    int max_region = 0;
+    for (int v0 = 0; v0 < n; v0 += F) {
+        const int r = p.row_ptr[(size_t)std::min(n, v0 + F)] - p.row_ptr[(size_t)v0];
+        max_region = std::max(max_region, r);
+    }
+    if (max_region > region_cap || max_region > 65535) return false;
+    // the distinct slices of each clause, as a sorted triple (-1 padded)
+    struct Key {
+        int s[3];
+        bool operator<(const Key &o) const
+        {
+            return s[0] != o.s[0] ? s[0] < o.s[0] : s[1] != o.s[1] ? s[1] < o.s[1] : s[2] < o.s[2];
+        }
+        bool operator==(const Key &o) const { return s[0] == o.s[0] && s[1] == o.s[1] && s[2] == o.s[2]; }
+    };
+    std::vector<Key> key((size_t)m);
+    for (int c = 0; c < m; c++) {
+        int a[3];
+        for (int j = 0; j < 3; j++) a[j] = (std::abs(p.lits[(size_t)3 * c + j]) - 1) / S;
+        std::sort(a, a + 3);
+        Key k{{a[0], -1, -1}};
+        int ns = 1;
+        for (int j = 1; j < 3; j++)
+            if (a[j] != k.s[ns - 1]) k.s[ns++] = a[j];
+        key[(size_t)c] = k;
+    }
+    // clause sets by slice set, each in clause order; largest first
+    std::vector<int> ord((size_t)m);
+    for (int c = 0; c < m; c++) ord[(size_t)c] = c;
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return key[(size_t)x] < key[(size_t)y]; });
+    struct Set {
+        Key k;
+        int begin, size;
+    };
+    std::vector<Set> sets;
+    for (int i = 0; i < m;) {
+        int j = i;
+        while (j < m && key[(size_t)ord[(size_t)j]] == key[(size_t)ord[(size_t)i]]) j++;
+        sets.push_back({key[(size_t)ord[(size_t)i]], i, j - i});
+        i = j;
+    }
+    std::stable_sort(sets.begin(), sets.end(), [](const Set &x, const Set &y) { return x.size > y.size; });
+    // units of at most T clauses whose slice union has at most three
+    // slices: each set first fills the free room of units it is compatible
+    // with, then opens new ones; T grows until at most `cus` units result
+    struct Unit {
+        int sl[3];
+        int ns = 0, size = 0;
+        std::vector<std::pair<int, int>> parts;  // (set, count)
+    };
+    auto merge = [](const Unit &u, const Key &k, int *out3) {
+        int nsl = u.ns;
+        for (int j = 0; j < u.ns; j++) out3[j] = u.sl[j];
+        for (int j = 0; j < 3 && k.s[j] >= 0; j++) {
+            bool have = false;
+            for (int i = 0; i < nsl; i++) have |= out3[i] == k.s[j];
+            if (!have) {
+                if (nsl == 3) return -1;
+                out3[nsl++] = k.s[j];
+            }
+        }
+        return nsl;
+    };
+    auto form = [&](int T, std::vector<Unit> &U) {
+        U.clear();
+        for (int si = 0; si < (int)sets.size(); si++) {
+            int rem = sets[(size_t)si].size;
+            for (auto &u : U) {
+                if (rem == 0) break;
+                if (u.size >= T) continue;
+                int sl[3];
+                const int ns = merge(u, sets[(size_t)si].k, sl);
+                if (ns < 0) continue;
+                const int take = std::min(rem, T - u.size);
+                for (int j = 0; j < ns; j++) u.sl[j] = sl[j];
+                u.ns = ns;
+                u.size += take;
+                u.parts.push_back({si, take});
+                rem -= take;
+            }
+            while (rem > 0) {
+                Unit u;
+                u.ns = merge(u, sets[(size_t)si].k, u.sl);
+                const int take = std::min(rem, T);
+                u.size = take;
+                u.parts.push_back({si, take});
+                rem -= take;
+                U.push_back(std::move(u));
+            }
+        }
+    };
+    // every three-slice set needs a unit of its own: with more of them than
+    // two rounds of units, staging V would cost more than it saves
+    int triples = 0;
+    for (const Set &st : sets) triples += st.k.s[2] >= 0;
+    if (triples > 2 * cus) return false;
+    std::vector<Unit> U;
+    int T = std::max((m + cus - 1) / cus, std::min(m, SUB / 4));
+    for (int it = 0; it < 400; it++) {
+        form(T, U);
+        if ((int)U.size() <= cus || T >= m) break;
+        T = std::min(m, T + std::max(1, T / 64));
+    }
+    if ((int)U.size() > 2 * cus) return false;
+    // processing order, units and sub-batches
+    out = SlicedPlan();
+    out.S = S;
+    out.SUB = SUB;
+    out.F = F;
+    out.max_region = max_region;
+    std::vector<int> cursor(sets.size(), 0);
+    std::vector<int> porder;
+    porder.reserve((size_t)m);
+    int nsb = 0;
+    for (const Unit &u : U) {
+        const int c0 = (int)porder.size();
+        for (auto &pt : u.parts) {
+            const Set &st = sets[(size_t)pt.first];
+            for (int k = 0; k < pt.second; k++)
+                porder.push_back(ord[(size_t)st.begin + cursor[(size_t)pt.first] + k]);
+            cursor[(size_t)pt.first] += pt.second;
+        }
+        const int c1 = (int)porder.size();
+        int sl[3] = {u.sl[0], u.ns > 1 ? u.sl[1] : -1, u.ns > 2 ? u.sl[2] : -1};
+        out.units.insert(out.units.end(), {c0, c1, u.ns, sl[0], sl[1], sl[2], nsb, 0});
+        nsb += (c1 - c0 + SUB - 1) / SUB;
+    }
+    out.pidx.assign((size_t)m, 0);
+    for (int i = 0; i < m; i++) out.pidx[(size_t)porder[(size_t)i]] = i;
+    // per sub-batch: literals ordered by destination fold block (then by
+    // processing position), one run per block
+    out.gw.assign((size_t)3 * m, 0u);
+    out.q.assign((size_t)p.nnz(), 0);
+    out.sb_run.clear();
+    out.runs.clear();
+    std::vector<int64_t> fill((size_t)(n + F - 1) / F + 1);
+    for (size_t b = 0; b + 1 < fill.size(); b++) fill[b] = p.row_ptr[std::min((size_t)n, b * (size_t)F)];
+    std::vector<int> lit_blk((size_t)3 * SUB), cnt, order3((size_t)3 * SUB);
+    for (size_t ui = 0; ui < U.size(); ui++) {
+        const int *ud = &out.units[ui * 8];
+        const int c0 = ud[0], c1 = ud[1], nsl = ud[2];
+        for (int cb = c0; cb < c1; cb += SUB) {
+            const int ce = std::min(c1, cb + SUB), cntl = 3 * (ce - cb);
+            out.sb_run.push_back((int)out.runs.size() / 2);
+            // stable counting sort of the sub-batch's literals by block
+            int bmin = INT_MAX, bmax = -1;
+            for (int i = 0; i < cntl; i++) {
+                const int c = porder[(size_t)cb + i / 3];
+                const int v = std::abs(p.lits[(size_t)3 * c + i % 3]) - 1;
+                lit_blk[(size_t)i] = v / F;
+                bmin = std::min(bmin, v / F);
+                bmax = std::max(bmax, v / F);
+            }
+            cnt.assign((size_t)(bmax - bmin + 2), 0);
+            for (int i = 0; i < cntl; i++) cnt[(size_t)(lit_blk[(size_t)i] - bmin + 1)]++;
+            std::vector<int> start(cnt.size());
+            for (size_t b = 1; b < cnt.size(); b++) start[b] = start[b - 1] + cnt[b];
+            std::vector<int> pos(start.begin(), start.end());
+            for (int i = 0; i < cntl; i++) order3[(size_t)(pos[(size_t)(lit_blk[(size_t)i] - bmin)]++)] = i;
+            // runs: (staging start, length) and their e positions
+            for (int b = bmin; b <= bmax; b++) {
+                const int ls = start[(size_t)(b - bmin)], len = cnt[(size_t)(b - bmin + 1)];
+                if (!len) continue;
+                out.runs.push_back((ls << 16) | len);
+                out.runs.push_back((int32_t)fill[(size_t)b]);
+                for (int k = 0; k < len; k++) {
+                    const int i = order3[(size_t)ls + k];
+                    const int c = porder[(size_t)cb + i / 3];
+                    const int64_t lq = (int64_t)3 * c + i % 3;  // literal position
+                    out.q[(size_t)fill[(size_t)b] + k] =
+                        (uint16_t)(p.lpos[(size_t)lq] - p.row_ptr[(size_t)b * F]);
+                    // the literal's word: sign, LDS index of its variable
+                    // (slice index within the unit, offset in the slice),
+                    // staging slot
+                    const int v = std::abs(p.lits[(size_t)lq]) - 1;
+                    int si = 0;
+                    while (si < nsl && ud[3 + si] != v / S) si++;
+                    const uint32_t lidx = (uint32_t)(si * S + v % S);
+                    out.gw[(size_t)3 * (cb + i / 3) + i % 3] =
+                        (p.lits[(size_t)lq] < 0 ? 0x80000000u : 0u) | (lidx << 15) | (uint32_t)(ls + k);
+                }
+                fill[(size_t)b] += len;
+            }
+        }
+    }
+    out.sb_run.push_back((int)out.runs.size() / 2);
+    (void)NS;
+    return true;
 }
 
 // Reads one whitespace-delimited token; returns false at EOF.

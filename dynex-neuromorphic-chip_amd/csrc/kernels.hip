@@ -26,6 +26,7 @@
 // IEEE build of the reference bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 
 #include "dmm_math.hpp"
@@ -108,6 +109,117 @@ __global__ __launch_bounds__(CBS) void k_step_clause(DevProb P, PT<T> p, const P
     if (ctl->track_loc) {
         __shared__ int shl[CBS / 64];
         add_loc<CBS>(ctl, slot, locu, shl);
+    }
+    if (__syncthreads_or(unsolved) && threadIdx.x == 0) raise_flag(ctl->flags[slot][FG_UNSOLVED]);
+}
+
+// ---------------------------------------------------------------------------
+// Resident-state step, 3-SAT with the grouped exchange: clause pass.
+//
+// Each wave takes chunks of 64 consecutive clauses (GX_CHUNK), GX_U chunks at
+// a time with every load of those chunks issued before any is used.  A
+// literal's contribution goes to e[gbase[chunk][g] + offset]: the
+// contributions are stored group by group (group g = the XCD whose
+// variable-pass workgroups fold them), so the fold on each XCD gathers from
+// a region of 1/8 of e that stays in that XCD's L2, instead of from all of e
+// (one line from beyond L2 per 4-byte value).  Inside a chunk the 64 lanes'
+// stores of one group land in one run of consecutive positions.  The control
+// words are read once per workgroup, all requested before any is used.
+// ---------------------------------------------------------------------------
+constexpr int GX_U = 2;
+constexpr int GX_BS = 256;
+
+__device__ __forceinline__ int gx_var(uint32_t w) { return (int)((w >> 11) & 0xFFFFFu); }
+__device__ __forceinline__ int gx_lit(uint32_t w) { return (w >> 31) ? -1 : 1; }  // sign as a +-1 literal
+
+template <typename T>
+__global__ __launch_bounds__(GX_BS) void k_gx_clause(int m, const uint32_t *__restrict__ gw,
+                                                     const int32_t *__restrict__ gbase, PT<T> p,
+                                                     const Pair<T> *__restrict__ vd, T *__restrict__ Xs,
+                                                     T *__restrict__ Xl, T *__restrict__ e, Ctl *__restrict__ ctl,
+                                                     int slot)
+{
+    const int ps = (slot + 2) % 3;
+    const int status = ctl->status, done = ctl->done[ps], stepped = ctl->stepped[ps], track = ctl->track_loc;
+    const int plev = dev_level(ctl, ps);
+    if (status != DNX_RUNNING) return;
+    if (done) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) ctl->status = done;
+        return;
+    }
+    const bool pend = stepped != 0;
+    const T h = pend ? level_h<T>(plev) : T(0.0);
+    if (pend && blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->t += level_dt(plev);
+        ctl->steps += 1;
+    }
+    const int lane = threadIdx.x & 63;
+    const int nw = gridDim.x * (GX_BS / 64);
+    const int nchunk = (m + GX_CHUNK - 1) / GX_CHUNK;
+    int unsolved = 0, locu = 0;
+    for (int k0 = blockIdx.x * (GX_BS / 64) + (threadIdx.x >> 6); k0 < nchunk; k0 += GX_U * nw) {
+        uint32_t W[GX_U][3];
+        T XS[GX_U], XL[GX_U];
+        int GB[GX_U];
+        bool ok[GX_U];
+#pragma unroll
+        for (int u = 0; u < GX_U; u++) {
+            const int k = k0 + u * nw;
+            const int c = k * GX_CHUNK + lane;
+            ok[u] = k < nchunk && c < m;
+            W[u][0] = W[u][1] = W[u][2] = 0u;
+            XS[u] = XL[u] = T(0.0);
+            GB[u] = 0;
+            if (ok[u]) {
+                W[u][0] = gw[3 * c];
+                W[u][1] = gw[3 * c + 1];
+                W[u][2] = gw[3 * c + 2];
+                XS[u] = Xs[c];
+                XL[u] = Xl[c];
+            }
+            if (k < nchunk && lane < GX_GROUPS) GB[u] = gbase[k * GX_GROUPS + lane];
+        }
+        Pair<T> A[GX_U][3];
+#pragma unroll
+        for (int u = 0; u < GX_U; u++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                A[u][j].v = A[u][j].d = T(0.0);
+                if (ok[u]) A[u][j] = vd[gx_var(W[u][j])];
+            }
+#pragma unroll
+        for (int u = 0; u < GX_U; u++) {
+            // positions of the three contributions: the run start of each
+            // literal's group in this chunk (lane g holds group g's), plus
+            // the literal's offset
+            int pos[3];
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                pos[j] = __shfl(GB[u], (int)((W[u][j] >> 8) & 7u), 64) + (int)(W[u][j] & 0xFFu);
+            if (!ok[u]) continue;
+            const int c = (k0 + u * nw) * GX_CHUNK + lane;
+            const int32_t l0 = gx_lit(W[u][0]), l1 = gx_lit(W[u][1]), l2 = gx_lit(W[u][2]);
+            T xs = XS[u], xl = XL[u];
+            T r0 = A[u][0].v, r1 = A[u][1].v, r2 = A[u][2].v;
+            if (pend) {
+                memory_update(p, clause3_C_vals<T>(l0, l1, l2, r0, r1, r2), h, xs, xl);
+                Xs[c] = xs;
+                Xl[c] = xl;
+                r0 = clampV(A[u][0].v + h * A[u][0].d);
+                r1 = clampV(A[u][1].v + h * A[u][1].d);
+                r2 = clampV(A[u][2].v + h * A[u][2].d);
+            }
+            auto emit = [&](int j, T val) { e[pos[j]] = val; };
+            bool sat;
+            const T C = clause3_vals<T>(p, l0, l1, l2, r0, r1, r2, clamp01(xs), clampXl(xl, p.xlmax), emit, sat);
+            const int lu = !(C < T(0.5));
+            locu += lu;
+            unsolved |= lu || !sat;
+        }
+    }
+    if (track) {
+        __shared__ int shl[GX_BS / 64];
+        add_loc_count<GX_BS>(ctl, slot, locu, shl);
     }
     if (__syncthreads_or(unsolved) && threadIdx.x == 0) raise_flag(ctl->flags[slot][FG_UNSOLVED]);
 }
@@ -356,6 +468,21 @@ static void step_impl(const DevProb &P, const DevParams &prm, void *vd, void *xs
 {
     PT<T> p = to_pt<T>(prm);
     constexpr bool SORT = sizeof(T) == 8;
+    constexpr int BS = SORT ? 128 : 256;
+    if (P.cp == nullptr && P.gw) {
+        // grouped exchange: GX_U chunks per wave and pass, at most 8
+        // workgroups per CU's worth of waves for the whole clause set
+        const long long nchunk = ((long long)P.m + GX_CHUNK - 1) / GX_CHUNK;
+        const long long waves = (nchunk + GX_U - 1) / GX_U;
+        const unsigned gc = (unsigned)std::max(1LL, (waves + GX_BS / 64 - 1) / (GX_BS / 64));
+        k_gx_clause<T><<<gc, GX_BS, 0, st>>>(P.m, P.gw, P.gbase, p, (const Pair<T> *)vd, (T *)xs, (T *)xl, (T *)e,
+                                             ctl, slot);
+        DevProb Q = P;
+        Q.eidx = P.geidx;
+        k_step_var<T, SORT, BS, 2048><<<nblk(P.n > 0 ? P.n : 1, BS), BS, 0, st>>>(Q, (Pair<T> *)vd, (const T *)e,
+                                                                                 ctl, slot);
+        return;
+    }
     unsigned gc = nblk(P.m > 0 ? P.m : 1, CBS);
     if (P.cp == nullptr)
         k_step_clause<T, true><<<gc, CBS, 0, st>>>(P, p, (const Pair<T> *)vd, (T *)xs, (T *)xl, (T *)e, ctl,
@@ -363,7 +490,6 @@ static void step_impl(const DevProb &P, const DevParams &prm, void *vd, void *xs
     else
         k_step_clause<T, false><<<gc, CBS, 0, st>>>(P, p, (const Pair<T> *)vd, (T *)xs, (T *)xl, (T *)e, ctl,
                                                     slot);
-    constexpr int BS = SORT ? 128 : 256;
     k_step_var<T, SORT, BS, 2048><<<nblk(P.n > 0 ? P.n : 1, BS), BS, 0, st>>>(P, (Pair<T> *)vd, (const T *)e, ctl,
                                                                              slot);
 }

@@ -17,7 +17,26 @@ struct DevProb {
     const int32_t *eidx;     // literal position of each variable-major slot
     const int32_t *row_ptr;  // n+1
     const int8_t *unit;      // n pins, or nullptr when there are none
+    // grouped exchange of the resident-state step (3-SAT only, else null):
+    // contributions are stored grouped by the XCD whose variable-pass
+    // workgroups fold them (see grouped_tables), so each XCD's fold gathers
+    // from a region that stays in its L2.  gw: one word per literal (sign,
+    // variable, group, offset in its chunk's run); gbase: per 64-clause chunk
+    // the start of its run in each group; geidx: the e position of each
+    // variable-major slot (replaces eidx in the fold).
+    const uint32_t *gw;
+    const int32_t *gbase;
+    const int32_t *geidx;
 };
+
+// Packed literal word of the grouped exchange: bit 31 the sign (set for a
+// negative literal), bits 30..11 the 0-based variable, bits 10..8 the group
+// (XCD) of the variable's fold, bits 7..0 the literal's offset in its chunk's
+// run of that group.
+constexpr int GX_CHUNK = 64;       // clauses per chunk (one wave's clauses)
+constexpr int GX_GROUPS = 8;       // XCDs
+constexpr int GX_MAX_VARS = 1 << 20;
+inline int gx_var_block(int prec) { return prec == 0 ? 128 : 256; }  // variable-pass workgroup size
 
 struct DevParams {
     double alpha, beta, gamma, delta, epsilon, zeta, xl_max;

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libdnx.so")
 INCLUDE_H = os.path.join(os.path.dirname(HERE), "include", "dnx.h")
 
 OK, E_ARG, E_IO, E_PARSE, E_TOOLONG, E_HIP, E_NOMEM, E_BUDGET, E_STATE = 0, -1, -2, -3, -4, -5, -6, -7, -8
+EXCHANGE_LITERAL, EXCHANGE_GROUPED = 0, 1
 FP64, FP32 = 0, 1
 RUNNING, SAT, UNKNOWN, FAILED = 0, 1, 2, 3
 
@@ -103,6 +104,7 @@ def lib():
     L.dnx_serve_trace.argtypes = [vp, ip, vp, vp, vp, F64P, C.c_int64, ip, dp, vp, vp, vp, vp,
                                   C.POINTER(C.c_double), vp, vp, vp]
     L.dnx_state_track_loc.argtypes = [vp, ip]
+    L.dnx_problem_set_exchange.argtypes = [vp, ip]
     L.dnx_state_best_loc.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.dnx_serve_planted.argtypes = [vp, C.c_uint64, dp, dp, ip, ip, dp, dp, dp, dp, C.c_int64, ip, dp, ip, ip,
                                     C.POINTER(C.c_int), vp, vp, vp, vp, vp, vp, vp, C.POINTER(C.c_double)]
@@ -220,6 +222,10 @@ class Problem:
         n, m, nnz = C.c_int(), C.c_int(), C.c_int64()
         check(lib().dnx_problem_info(h, C.byref(n), C.byref(m), C.byref(nnz)))
         self.n, self.m, self.nnz = n.value, m.value, nnz.value
+
+    def set_exchange(self, exchange):
+        """EXCHANGE_GROUPED (default) or EXCHANGE_LITERAL for states created afterwards."""
+        check(lib().dnx_problem_set_exchange(self.h, int(exchange)))
 
     def clauses(self):
         cp = np.zeros(self.m + 1, np.int32)

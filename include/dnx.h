@@ -95,6 +95,20 @@ int dnx_problem_from_dimacs(dnx_ctx *ctx, const char *path, dnx_problem **out);
 int dnx_problem_from_lits(dnx_ctx *ctx, int n, int m, const int32_t *clause_ptr,
                           const int32_t *lits, dnx_problem **out);
 int dnx_problem_info(const dnx_problem *p, int *n, int *m, int64_t *nnz);
+
+/* How the resident-state step of a 3-SAT problem moves each literal's
+ * contribution from its clause to its variable's fold (no counterpart in the
+ * reference, whose clause loop appends to per-variable vectors,
+ * dynex.cc:408-410, 527-530); the bits are the same either way.
+ *   DNX_EXCHANGE_GROUPED (default): stored grouped by the XCD that folds
+ *     the variable, so each XCD's fold reads a region that stays in its L2
+ *     (3-SAT with at most 2^20 variables; others use the literal order);
+ *   DNX_EXCHANGE_LITERAL: stored in literal order, gathered through the
+ *     transpose.
+ * Applies to states created afterwards. */
+#define DNX_EXCHANGE_LITERAL 0
+#define DNX_EXCHANGE_GROUPED 1
+int dnx_problem_set_exchange(dnx_problem *p, int exchange);
 /* Copies the clause table back out (clause_ptr[m+1], lits[nnz]). */
 int dnx_problem_clauses(const dnx_problem *p, int32_t *clause_ptr, int32_t *lits);
 int dnx_problem_free(dnx_problem *p);

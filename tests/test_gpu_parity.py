@@ -123,10 +123,11 @@ def test_euler_bitexact(ctx):
 # ---------------------------------------------------------------------------
 # resident trajectories: the mode-6 loop (dynex.cc:1387-1417), fixed steps
 # ---------------------------------------------------------------------------
-def run_traj(gp, op, x0, checkpoints, prec):
+def run_traj(gp, op, x0, checkpoints, prec, exchange=dnx.EXCHANGE_GROUPED):
     """Steps the GPU state and the oracle to each checkpoint and compares."""
     m = gp.m
     prm, oprm = dnx.params(m), O.params(m)
+    gp.set_exchange(exchange)
     st = gp.state(prm, x0, dnx.FP64 if prec == 64 else dnx.FP32)
     xo = x0.copy() if prec == 64 else x0.astype(np.float32)
     if op.has_unit:  # the reference pins V at initialisation (dynex.cc:2694-2699)
@@ -154,13 +155,29 @@ def run_traj(gp, op, x0, checkpoints, prec):
     return dev
 
 
+@pytest.mark.parametrize("exchange", [dnx.EXCHANGE_GROUPED, dnx.EXCHANGE_LITERAL])
 @pytest.mark.parametrize("prec", [64, 32])
-def test_trajectory_c1_1000_steps(ctx, prec):
+def test_trajectory_c1_1000_steps(ctx, prec, exchange):
+    """n=100, m=800: 1000 steps bit-identical to the oracle with either
+    exchange of the contributions (grouped by folding XCD, or literal order)."""
     cp, lits = gen(100, 800, 11)
     gp, op = both(ctx, 100, cp, lits)
     x0 = dnx.init_uniform(11, 100, 800)
-    run_traj(gp, op, x0, [1, 2, 10, 100, 500, 1000], prec)
+    run_traj(gp, op, x0, [1, 2, 10, 100, 500, 1000], prec, exchange)
     gp.free()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_trajectory_grouped_ragged_chunks(ctx, prec):
+    """Clause counts that end inside a 64-clause chunk and variable counts that
+    end inside a fold workgroup, a few variables with no clause, and seeds
+    whose groups are uneven: the grouped exchange against the oracle."""
+    for seed, (n, m) in enumerate([(3, 1), (7, 65), (300, 1031), (2051, 9001), (5000, 3000)]):
+        cp, lits = gen(n, m, 300 + seed)
+        gp, op = both(ctx, n, cp, lits)
+        x0 = dnx.init_uniform(300 + seed, n, m)
+        run_traj(gp, op, x0, [1, 3, 40], prec)
+        gp.free()
 
 
 @pytest.mark.parametrize("prec", [64, 32])
@@ -193,11 +210,13 @@ def test_trajectory_c2_fp64_1000_steps(ctx):
 @pytest.mark.parametrize("prec", [64, 32])
 def test_trajectory_c3_full_size(ctx, prec):
     """n=100,000, m=800,000 (the largest single-instance size): 100 steps
-    bit-identical to the oracle, compared after 1, 4, 30 and 100 steps."""
+    bit-identical to the oracle, compared after 1, 4, 30 and 100 steps, with
+    the grouped exchange; and the literal-order exchange to step 4."""
     cp, lits = gen(100000, 800000, 23)
     gp, op = both(ctx, 100000, cp, lits)
     x0 = dnx.init_uniform(23, 100000, 800000)
     run_traj(gp, op, x0, [1, 4, 30, 100], prec)
+    run_traj(gp, op, x0, [1, 4], prec, dnx.EXCHANGE_LITERAL)
     gp.free()
 
 
