@@ -35,12 +35,80 @@ DevProb dev_prob(const dnx_problem *p)
     d.gw = nullptr;
     d.gbase = nullptr;
     d.geidx = nullptr;
+    d.sw = nullptr;
+    d.units = d.sbrun = d.runs = nullptr;
+    d.sq = nullptr;
+    d.pidx = nullptr;
+    d.nunits = d.sx_region = 0;
     return d;
+}
+
+template <typename V>
+static int upload_vec(DevBuf &b, const std::vector<V> &v, hipStream_t st)
+{
+    int rc = b.alloc(sizeof(V) * (v.empty() ? 1 : v.size()));
+    if (rc) return rc;
+    hipError_t e;
+    if (!v.empty() && (e = hipMemcpyAsync(b.ptr, v.data(), sizeof(V) * v.size(), hipMemcpyHostToDevice, st)) !=
+                          hipSuccess)
+        return hip_fail(e, "table upload");
+    return DNX_OK;
+}
+
+// The sliced-exchange tables of a problem for one precision, planned for
+// the device's compute units and uploaded once.  Returns whether they exist.
+static int ensure_sliced(dnx_problem *p, int prec, bool *have)
+{
+    const int i = prec == DNX_FP64 ? 0 : 1;
+    *have = false;
+    if (p->sx_state[i] < 0) return DNX_OK;
+    if (p->sx_state[i] == 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->ctx->device) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+        SlicedPlan pl;
+        if (!sliced_tables(p->hp, sx_slice(prec), sx_sub(prec), SX_F, cus, sx_max_rounds(prec), sx_region_cap(prec),
+                           pl)) {
+            p->sx_state[i] = -1;
+            return DNX_OK;
+        }
+        for (size_t k = 0; k + 1 < pl.sb_run.size(); k++)
+            if (pl.sb_run[k + 1] - pl.sb_run[k] > SX_MAXRUNS) {
+                p->sx_state[i] = -1;
+                return DNX_OK;
+            }
+        hipStream_t st = p->ctx->stream;
+        int rc;
+        if ((rc = upload_vec(p->b_sw[i], pl.gw, st)) || (rc = upload_vec(p->b_units[i], pl.units, st)) ||
+            (rc = upload_vec(p->b_sbrun[i], pl.sb_run, st)) || (rc = upload_vec(p->b_runs[i], pl.runs, st)) ||
+            (rc = upload_vec(p->b_sq[i], pl.q, st)) || (rc = upload_vec(p->b_pidx[i], pl.pidx, st)))
+            return rc;
+        hipError_t e;
+        if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "exchange tables sync");
+        p->sx_nunits[i] = (int)pl.units.size() / 8;
+        p->sx_region[i] = pl.max_region;
+        p->sx_state[i] = 1;
+    }
+    *have = true;
+    return DNX_OK;
 }
 
 DevProb dev_prob_state(const dnx_state *s)
 {
     DevProb d = dev_prob(s->p);
+    if (s->sliced) {
+        const int i = s->prec == DNX_FP64 ? 0 : 1;
+        const dnx_problem *p = s->p;
+        d.sw = (const uint32_t *)p->b_sw[i].ptr;
+        d.units = (const int32_t *)p->b_units[i].ptr;
+        d.sbrun = (const int32_t *)p->b_sbrun[i].ptr;
+        d.runs = (const int32_t *)p->b_runs[i].ptr;
+        d.sq = (const uint16_t *)p->b_sq[i].ptr;
+        d.pidx = (const int32_t *)p->b_pidx[i].ptr;
+        d.nunits = p->sx_nunits[i];
+        d.sx_region = p->sx_region[i];
+    }
     if (s->grouped) {
         const int i = s->prec == DNX_FP64 ? 0 : 1;
         d.gw = (const uint32_t *)s->p->b_gw[i].ptr;
@@ -183,14 +251,22 @@ static void init_ctl(Ctl &c)
     c.cap = LLONG_MAX;
 }
 
+static void one_step(dnx_state *s, const DevProb &P, int slot, hipStream_t st)
+{
+    if (s->sliced)
+        launch_step_sliced(s->prec, P, s->prm, s->b_vd.ptr, s->b_xs.ptr, s->b_xl.ptr, s->b_e.ptr, s->b_vnext.ptr,
+                           s->b_cst.ptr, s->ctl, slot, st);
+    else
+        launch_step(s->prec, P, s->prm, s->b_vd.ptr, s->b_xs.ptr, s->b_xl.ptr, s->b_e.ptr, s->ctl, slot, st);
+}
+
 static int state_enqueue_steps(dnx_state *s, long long count)
 {
     hipStream_t st = s->p->ctx->stream;
     DevProb P = dev_prob_state(s);
     // align to the graph's slot phase, then replay whole chunks
     while (count > 0 && (s->enq % 3) != 0) {
-        launch_step(s->prec, P, s->prm, s->b_vd.ptr, s->b_xs.ptr, s->b_xl.ptr, s->b_e.ptr, s->ctl,
-                    (int)(s->enq % 3), st);
+        one_step(s, P, (int)(s->enq % 3), st);
         s->enq++;
         count--;
     }
@@ -200,8 +276,7 @@ static int state_enqueue_steps(dnx_state *s, long long count)
             hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
             if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
             for (int i = 0; i < kChunk; i++)
-                launch_step(s->prec, P, s->prm, s->b_vd.ptr, s->b_xs.ptr, s->b_xl.ptr, s->b_e.ptr, s->ctl, i % 3,
-                            st);
+                one_step(s, P, i % 3, st);
             if ((e = hipStreamEndCapture(st, &g)) != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
             e = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
@@ -216,8 +291,7 @@ static int state_enqueue_steps(dnx_state *s, long long count)
         count -= kChunk;
     }
     while (count > 0) {
-        launch_step(s->prec, P, s->prm, s->b_vd.ptr, s->b_xs.ptr, s->b_xl.ptr, s->b_e.ptr, s->ctl,
-                    (int)(s->enq % 3), st);
+        one_step(s, P, (int)(s->enq % 3), st);
         s->enq++;
         count--;
     }
@@ -383,7 +457,7 @@ int dnx_problem_from_lits(dnx_ctx *ctx, int n, int m, const int32_t *clause_ptr,
 
 int dnx_problem_set_exchange(dnx_problem *p, int exchange)
 {
-    if (!p || (exchange != DNX_EXCHANGE_LITERAL && exchange != DNX_EXCHANGE_GROUPED)) return DNX_E_ARG;
+    if (!p || exchange < DNX_EXCHANGE_LITERAL || exchange > DNX_EXCHANGE_SLICED) return DNX_E_ARG;
     p->exchange = exchange;
     return DNX_OK;
 }
@@ -504,15 +578,26 @@ int dnx_state_create(dnx_problem *p, int prec, const dnx_params *prm, const doub
     s->prec = prec;
     s->prm = dev_params(prm);
     int rc;
-    if (p->exchange == DNX_EXCHANGE_GROUPED && m_ok_grouped(h) && (rc = ensure_grouped(p, prec, &s->grouped))) {
+    if (p->exchange == DNX_EXCHANGE_SLICED && h.uniform3 && h.m > 0 && (rc = ensure_sliced(p, prec, &s->sliced))) {
+        delete s;
+        return rc;
+    }
+    if (!s->sliced && p->exchange != DNX_EXCHANGE_LITERAL && m_ok_grouped(h) &&
+        (rc = ensure_grouped(p, prec, &s->grouped))) {
+        delete s;
+        return rc;
+    }
+    if (s->sliced && ((rc = s->b_vnext.alloc(4 * es * (size_t)sx_vstride(h.n > 0 ? h.n : 1))) ||
+                      (rc = s->b_cst.alloc(es * ((size_t)h.m + 1))))) {
         delete s;
         return rc;
     }
     DevBuf bx;
+    // one entry past the end of Xs, Xl, e (and the stored C): the sliced
+    // clause pass sends the stores of lanes without a clause there
     if ((rc = s->b_vd.alloc(2 * es * (size_t)(h.n > 0 ? h.n : 1))) ||
-        (rc = s->b_xs.alloc(es * (size_t)(h.m > 0 ? h.m : 1))) ||
-        (rc = s->b_xl.alloc(es * (size_t)(h.m > 0 ? h.m : 1))) ||
-        (rc = s->b_e.alloc(es * (size_t)(h.nnz() > 0 ? h.nnz() : 1))) || (rc = s->b_ctl.alloc(sizeof(Ctl))) ||
+        (rc = s->b_xs.alloc(es * ((size_t)h.m + 1))) || (rc = s->b_xl.alloc(es * ((size_t)h.m + 1))) ||
+        (rc = s->b_e.alloc(es * ((size_t)h.nnz() + 1))) || (rc = s->b_ctl.alloc(sizeof(Ctl))) ||
         (rc = bx.alloc(es * (N ? N : 1)))) {
         delete s;
         return rc;
@@ -535,7 +620,7 @@ int dnx_state_create(dnx_problem *p, int prec, const dnx_params *prm, const doub
         delete s;
         return hip_fail(e, "state upload");
     }
-    launch_load_state(prec, dev_prob(p), bx.ptr, s->b_vd.ptr, s->b_xs.ptr, s->b_xl.ptr, st);
+    launch_load_state(prec, dev_prob_state(s), bx.ptr, s->b_vd.ptr, s->b_xs.ptr, s->b_xl.ptr, st);
     if ((e = hipStreamSynchronize(st)) != hipSuccess) {
         delete s;
         return hip_fail(e, "state load");
@@ -646,7 +731,7 @@ int dnx_state_read(dnx_state *s, double *x, double *t, int64_t *steps, int *last
     if (x && N) {
         int rc = bo.alloc(N * es);
         if (rc) return rc;
-        launch_materialize(s->prec, dev_prob(s->p), s->prm, s->b_vd.ptr, s->b_xs.ptr, s->b_xl.ptr, s->ctl, bo.ptr,
+        launch_materialize(s->prec, dev_prob_state(s), s->prm, s->b_vd.ptr, s->b_xs.ptr, s->b_xl.ptr, s->ctl, bo.ptr,
                            st);
         if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "materialize");
         stage.resize(N * es);

@@ -47,13 +47,19 @@ struct dnx_problem {
     // the first state that uses them: 0 not yet, 1 built, -1 not applicable
     dnx::DevBuf b_gw[2], b_gbase[2], b_geidx[2];
     int gx_state[2] = {0, 0};
-    int exchange = DNX_EXCHANGE_GROUPED;  // for states created from now on
+    // sliced-exchange tables per precision, the same way
+    dnx::DevBuf b_sw[2], b_units[2], b_sbrun[2], b_runs[2], b_sq[2], b_pidx[2];
+    int sx_state[2] = {0, 0};
+    int sx_nunits[2] = {0, 0}, sx_region[2] = {0, 0};
+    int exchange = DNX_EXCHANGE_SLICED;  // for states created from now on
 };
 
 struct dnx_state {
     dnx_problem *p = nullptr;
     int prec = DNX_FP64;
     bool grouped = false;  // steps use the grouped exchange
+    bool sliced = false;   // steps use the sliced exchange (b_vnext, b_cst)
+    dnx::DevBuf b_vnext, b_cst;
     dnx::DevParams prm{};
     dnx::DevBuf b_vd, b_xs, b_xl, b_e, b_ctl;
     dnx::Ctl *ctl = nullptr;     // device control block

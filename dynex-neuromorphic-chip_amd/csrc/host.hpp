@@ -82,7 +82,8 @@ bool grouped_tables(const HostProblem &p, int vb, std::vector<uint32_t> &gw, std
 // Tables of the sliced exchange (3-SAT).  The variables are cut into slices
 // of S; every clause touches at most three.  Clauses are regrouped into
 // units, each holding clauses whose variables all lie in the same (at most
-// three) slices, about m / cus clauses per unit and at most cus units, so
+// three) slices, about m / cus clauses per unit and at most max_rounds x cus
+// units (one workgroup per compute unit at a time), so
 // one workgroup per unit can hold the unit's slices of V in LDS and read V
 // from there.  A unit's clauses are taken SUB at a time (a sub-batch); a
 // sub-batch's contributions are staged in LDS ordered by destination fold
@@ -94,14 +95,17 @@ bool grouped_tables(const HostProblem &p, int vb, std::vector<uint32_t> &gw, std
 struct SlicedPlan {
     int S = 0, SUB = 0, F = 0;
     int max_region = 0;             // slots of the largest fold block
-    std::vector<uint32_t> gw;       // 3 per clause, processing order: sign | LDS index << 15 | staging slot
-    std::vector<int32_t> units;     // 8 per unit: c0, c1, nslices, slice0..2, first sub-batch, 0
+    // 3 per clause, processing order: sign << 31 | LDS index << 15 | run << 7
+    // | offset in the run (the run's index within its sub-batch)
+    std::vector<uint32_t> gw;
+    std::vector<int32_t> units;     // 8 per unit: c0, c1, nslices, slice0..2, first sub-batch, first run
     std::vector<int32_t> sb_run;    // per sub-batch, its first run; one more entry at the end
     std::vector<int32_t> runs;      // 2 per run: staging start << 16 | length, e position
     std::vector<uint16_t> q;        // per e position
     std::vector<int32_t> pidx;      // per clause, its processing position
 };
-bool sliced_tables(const HostProblem &p, int S, int SUB, int F, int cus, int region_cap, SlicedPlan &out);
+bool sliced_tables(const HostProblem &p, int S, int SUB, int F, int cus, int max_rounds, int region_cap,
+                   SlicedPlan &out);
 
 // Thread-local error message used by dnx_last_error().
 void set_error(const std::string &msg);

@@ -90,7 +90,8 @@ bool grouped_tables(const HostProblem &p, int vb, std::vector<uint32_t> &gw, std
     return true;
 }
 
-bool sliced_tables(const HostProblem &p, int S, int SUB, int F, int cus, int region_cap, SlicedPlan &out)
+bool sliced_tables(const HostProblem &p, int S, int SUB, int F, int cus, int max_rounds, int region_cap,
+                   SlicedPlan &out)
 {
     if (!p.uniform3 || p.m <= 0 || S <= 0 || SUB <= 0 || SUB > 1365 || F <= 0 || cus <= 0) return false;
     const int n = p.n, m = p.m;
@@ -188,18 +189,22 @@ bool sliced_tables(const HostProblem &p, int S, int SUB, int F, int cus, int reg
         }
     };
     // every three-slice set needs a unit of its own: with more of them than
-    // two rounds of units, staging V would cost more than it saves
+    // max_rounds rounds of units, staging V would cost more than it saves
     int triples = 0;
     for (const Set &st : sets) triples += st.k.s[2] >= 0;
-    if (triples > 2 * cus) return false;
+    if (triples > max_rounds * cus) return false;
+    // as few rounds of units as the three-slice sets allow
+    const int target = std::max(1, (triples + cus - 1) / cus) * cus;
     std::vector<Unit> U;
-    int T = std::max((m + cus - 1) / cus, std::min(m, SUB / 4));
+    int T = std::max((m + target - 1) / target, std::min(m, SUB / 4));
     for (int it = 0; it < 400; it++) {
         form(T, U);
-        if ((int)U.size() <= cus || T >= m) break;
+        if ((int)U.size() <= target || T >= m) break;
         T = std::min(m, T + std::max(1, T / 64));
     }
-    if ((int)U.size() > 2 * cus) return false;
+    if ((int)U.size() > max_rounds * cus) return false;
+    for (const Unit &u : U)
+        if ((u.size + SUB - 1) / SUB > 63) return false;  // the clause pass keeps a unit's sub-batch table in LDS
     // processing order, units and sub-batches
     out = SlicedPlan();
     out.S = S;
@@ -220,7 +225,7 @@ bool sliced_tables(const HostProblem &p, int S, int SUB, int F, int cus, int reg
         }
         const int c1 = (int)porder.size();
         int sl[3] = {u.sl[0], u.ns > 1 ? u.sl[1] : -1, u.ns > 2 ? u.sl[2] : -1};
-        out.units.insert(out.units.end(), {c0, c1, u.ns, sl[0], sl[1], sl[2], nsb, 0});
+        out.units.insert(out.units.end(), {c0, c1, u.ns, sl[0], sl[1], sl[2], nsb, 0});  // [7]: see below
         nsb += (c1 - c0 + SUB - 1) / SUB;
     }
     out.pidx.assign((size_t)m, 0);
@@ -239,6 +244,7 @@ bool sliced_tables(const HostProblem &p, int S, int SUB, int F, int cus, int reg
         const int c0 = ud[0], c1 = ud[1], nsl = ud[2];
         for (int cb = c0; cb < c1; cb += SUB) {
             const int ce = std::min(c1, cb + SUB), cntl = 3 * (ce - cb);
+            if (cb == c0) out.units[ui * 8 + 7] = (int)out.runs.size() / 2;  // the unit's first run
             out.sb_run.push_back((int)out.runs.size() / 2);
             // stable counting sort of the sub-batch's literals by block
             int bmin = INT_MAX, bmax = -1;
@@ -255,10 +261,15 @@ bool sliced_tables(const HostProblem &p, int S, int SUB, int F, int cus, int reg
             for (size_t b = 1; b < cnt.size(); b++) start[b] = start[b - 1] + cnt[b];
             std::vector<int> pos(start.begin(), start.end());
             for (int i = 0; i < cntl; i++) order3[(size_t)(pos[(size_t)(lit_blk[(size_t)i] - bmin)]++)] = i;
-            // runs: (staging start, length) and their e positions
+            // runs: (staging start, length) and their e positions; a
+            // block's run is cut into pieces of at most 128 (a literal's word
+            // holds its piece and its offset in the piece)
             for (int b = bmin; b <= bmax; b++) {
-                const int ls = start[(size_t)(b - bmin)], len = cnt[(size_t)(b - bmin + 1)];
-                if (!len) continue;
+                const int bls = start[(size_t)(b - bmin)], blen = cnt[(size_t)(b - bmin + 1)];
+                for (int p0 = 0; p0 < blen; p0 += 128) {
+                const int ls = bls + p0, len = std::min(128, blen - p0);
+                const int rid = (int)out.runs.size() / 2 - out.sb_run.back();
+                if (rid >= SX_MAXRUNS) return false;
                 out.runs.push_back((ls << 16) | len);
                 out.runs.push_back((int32_t)fill[(size_t)b]);
                 for (int k = 0; k < len; k++) {
@@ -274,14 +285,19 @@ bool sliced_tables(const HostProblem &p, int S, int SUB, int F, int cus, int reg
                     int si = 0;
                     while (si < nsl && ud[3 + si] != v / S) si++;
                     const uint32_t lidx = (uint32_t)(si * S + v % S);
-                    out.gw[(size_t)3 * (cb + i / 3) + i % 3] =
-                        (p.lits[(size_t)lq] < 0 ? 0x80000000u : 0u) | (lidx << 15) | (uint32_t)(ls + k);
+                    out.gw[(size_t)3 * (cb + i / 3) + i % 3] = (p.lits[(size_t)lq] < 0 ? 0x80000000u : 0u) |
+                                                             (lidx << 15) | ((uint32_t)rid << 7) | (uint32_t)k;
                 }
                 fill[(size_t)b] += len;
+                }
             }
         }
     }
     out.sb_run.push_back((int)out.runs.size() / 2);
+    // padding: the clause pass also reads the tables one sub-batch past a
+    // unit's last and a whole workgroup's worth of runs (and uses neither)
+    out.sb_run.push_back(out.sb_run.back());
+    out.runs.insert(out.runs.end(), (size_t)2 * 256, 0);
     (void)NS;
     return true;
 }

@@ -225,6 +225,332 @@ __global__ __launch_bounds__(GX_BS) void k_gx_clause(int m, const uint32_t *__re
 }
 
 // ---------------------------------------------------------------------------
+// Resident-state step, 3-SAT with the sliced exchange (see sliced_tables).
+//
+// Clause pass: one workgroup per unit.  The unit's clauses read V only from
+// its (at most three) slices, which the workgroup first copies into LDS --
+// V of the state being evaluated: the candidate of the pending step's level
+// that the fold wrote (vnext), or the stored V at a launch's very first
+// state -- so the three V reads of a clause are LDS reads instead of
+// gathers from device memory.  The memory update uses the clause's C of the
+// previous state, which the previous pass stored (cst).  Each sub-batch's
+// contributions are staged in LDS ordered by destination fold block and
+// leave as one contiguous run per block (coalesced stores).
+//
+// Fold: one workgroup per block of SX_F variables.  Its slots form one
+// contiguous region of e; the workgroup reads the region (coalesced),
+// places each value at its variable-major slot in LDS (sq), then each
+// thread folds its variable's slots in literal order -- sorted ascending
+// first in FP64 -- and writes (V, dV) and the four candidates of V of the
+// next state.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct SXShape;
+template <>
+struct SXShape<float> {
+    static constexpr int BS = 1024, S = 10000, SUB = 1024;
+};
+template <>
+struct SXShape<double> {
+    static constexpr int BS = 512, S = 5000, SUB = 512;
+};
+
+template <typename T>
+__global__ __launch_bounds__(SXShape<T>::BS) void k_sx_clause(int n, int vstride, int m, long long nnz,
+                                                             const uint32_t *__restrict__ sw,
+                                                             const int32_t *__restrict__ units,
+                                                             const int32_t *__restrict__ sbrun,
+                                                             const int2 *__restrict__ runs, PT<T> p,
+                                                             const Pair<T> *__restrict__ vd,
+                                                             const T *__restrict__ vnext, T *__restrict__ Xs,
+                                                             T *__restrict__ Xl, T *__restrict__ cst,
+                                                             T *__restrict__ e, Ctl *__restrict__ ctl, int slot)
+{
+    constexpr int BS = SXShape<T>::BS, S = SXShape<T>::S, SUB = SXShape<T>::SUB;
+    static_assert(SUB == BS, "one clause per thread and sub-batch");
+    static_assert((S * sizeof(T)) % 16 == 0, "slices of whole 16-byte words");
+    // V of the unit's slices; two staging buffers (the write-out of one
+    // sub-batch runs beside the evaluation of the next) and three run tables
+    __shared__ __align__(16) T sv[3 * S];
+    __shared__ T st[2][3 * SUB];
+    __shared__ unsigned char smap[2][3 * SUB];
+    __shared__ int2 sr[3][SX_MAXRUNS];
+    __shared__ int s_sbr[65];
+    const int tid = threadIdx.x;
+    // The control words are requested first: loads complete in the order
+    // they were issued, so the test of the status waits for them alone.
+    const int ps = (slot + 2) % 3;
+    const int status = ctl->status, done = ctl->done[ps], stepped = ctl->stepped[ps], track = ctl->track_loc;
+    const int plev = dev_level(ctl, ps);
+    const long long steps0 = ctl->steps;
+    const int32_t *U = units + 8 * blockIdx.x;
+    const int c0 = U[0], c1 = U[1], nsl = U[2], sl0 = U[3], sl1 = U[4], sl2 = U[5], sb0 = U[6], run0 = U[7];
+    const int nsb = (c1 - c0 + SUB - 1) / SUB;
+    // Every device-memory access of the sub-batch loop is made on every path
+    // (indices clamped into the arrays, stores of lanes without a clause sent
+    // to the sink entry one past the end of Xs, Xl, cst and e; the run and
+    // sub-batch tables carry padding), so the waits for the next sub-batch's
+    // operands count a fixed number of later accesses and leave stores in
+    // flight.
+    struct Ops {
+        uint32_t w[3];
+        T xs, xl, cp;
+        int2 run;
+    };
+    auto load_ops = [&](int cb, int r0, Ops &o) {
+        const int li = min(cb + tid, m - 1);
+        o.run = runs[r0 + min(tid, SX_MAXRUNS - 1)];
+        o.w[0] = sw[3 * li];
+        o.w[1] = sw[3 * li + 1];
+        o.w[2] = sw[3 * li + 2];
+        o.xs = Xs[li];
+        o.xl = Xl[li];
+        o.cp = cst[li];
+    };
+    // the operands of the first sub-batch
+    Ops cur;
+    load_ops(c0, run0, cur);
+    // the unit's sub-batch table (first run of each sub-batch), into LDS
+    const int sbr = sbrun[sb0 + min(tid, nsb + 1)];
+    const int nr0 = sbrun[sb0 + 1] - run0;
+    if (status != DNX_RUNNING) return;
+    if (done) {
+        if (blockIdx.x == 0 && tid == 0) ctl->status = done;
+        return;
+    }
+    const bool pend = stepped != 0;
+    const T h = pend ? level_h<T>(plev) : T(0.0);
+    // FP32 from the third state on: V comes clamped from the fold and the
+    // memories from a memory update, so the steady-state forms apply (the
+    // same bits with fewer instructions, see clause3_steady)
+    const bool steady = sizeof(T) == 4 && pend && steps0 >= 1;
+    if (pend && blockIdx.x == 0 && tid == 0) {
+        ctl->t += level_dt(plev);
+        ctl->steps += 1;
+    }
+    // V of this state over the unit's slices: every load of a thread's share
+    // requested before the first LDS store; 16-byte words from the fold's
+    // candidates (each level's array padded to whole words), the stored V
+    // at a launch's very first state
+    if (pend) {
+        typedef T T16 __attribute__((ext_vector_type(16 / sizeof(T))));
+        constexpr int WPS = S * (int)sizeof(T) / 16;  // 16-byte words per slice
+        constexpr int SR = (WPS + BS - 1) / BS;
+        const int wmax = vstride * (int)sizeof(T) / 16 - 1;
+        const T16 *src = (const T16 *)(vnext + (size_t)(plev < 3 ? plev : 3) * vstride);
+        T16 x[3][SR];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int base = (k == 0 ? sl0 : k == 1 ? sl1 : sl2);
+#pragma unroll
+            for (int u = 0; u < SR; u++) x[k][u] = src[min(max(base, 0) * WPS + tid + u * BS, wmax)];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int u = 0; u < SR; u++) {
+                const int i = tid + u * BS;
+                if (k < nsl && i < WPS) *(T16 *)(sv + k * S + i * (16 / (int)sizeof(T))) = x[k][u];
+            }
+    } else {
+        constexpr int SR = (S + BS - 1) / BS;
+        T x[3][SR];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int base = (k == 0 ? sl0 : k == 1 ? sl1 : sl2) * S;
+#pragma unroll
+            for (int u = 0; u < SR; u++) x[k][u] = vd[min(max(base, 0) + tid + u * BS, n - 1)].v;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int u = 0; u < SR; u++) {
+                const int i = tid + u * BS;
+                if (k < nsl && i < S) sv[k * S + i] = x[k][u];
+            }
+    }
+    if (tid <= nsb + 1) s_sbr[tid] = sbr;
+    if (tid < nr0) sr[0][tid] = cur.run;
+    __syncthreads();  // V staged, the sub-batch table and the first runs in LDS
+    int unsolved = 0, locu = 0;
+    // Iteration j evaluates sub-batch j into staging buffer j & 1 and writes
+    // out sub-batch j - 1 from the other buffer; one barrier per iteration.
+    // The operands of sub-batch j + 1 are requested at iteration j.
+    for (int j = 0; j <= nsb; j++) {
+        const int b = j & 1;
+        Ops nxt;
+        load_ops(c0 + (j + 1) * SUB, s_sbr[min(j + 1, nsb)], nxt);
+        if (j < nsb) {
+            const int cb = c0 + j * SUB, ce = min(c1, cb + SUB);
+            const int2 *R = sr[j % 3];
+            const bool ok = cb + tid < ce;
+            T xs = cur.xs, xl = cur.xl, C = T(0.0);
+            if (ok) {
+                const uint32_t w0 = cur.w[0], w1 = cur.w[1], w2 = cur.w[2];
+                // staging slot of each literal: its run's start plus its
+                // offset; the slot also records the run for the write-out
+                int sp[3];
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    const uint32_t w = q == 0 ? w0 : q == 1 ? w1 : w2;
+                    const int rid = (int)((w >> 7) & 0xFFu);
+                    sp[q] = (R[rid].x >> 16) + (int)(w & 0x7Fu);
+                    smap[b][sp[q]] = (unsigned char)rid;
+                }
+                const int32_t l0 = (w0 >> 31) ? -1 : 1, l1 = (w1 >> 31) ? -1 : 1, l2 = (w2 >> 31) ? -1 : 1;
+                const T r0v = sv[(w0 >> 15) & 0xFFFFu], r1v = sv[(w1 >> 15) & 0xFFFFu],
+                        r2v = sv[(w2 >> 15) & 0xFFFFu];
+                auto emit = [&](int q, T val) { st[b][sp[q]] = val; };
+                bool sat = true;
+                if constexpr (sizeof(T) == 4) {
+                    if (steady) {
+                        memory_update_steady(p, cur.cp, h, xs, xl);
+                        // only bit 31 (the sign) of the words is read
+                        C = clause3_steady(p, (int32_t)w0, (int32_t)w1, (int32_t)w2, r0v, r1v, r2v, xs, xl, emit);
+                    }
+                }
+                if (!steady) {
+                    if (pend) memory_update(p, cur.cp, h, xs, xl);
+                    C = clause3_vals<T>(p, l0, l1, l2, r0v, r1v, r2v, clamp01(xs), clampXl(xl, p.xlmax), emit,
+                                        sat);
+                }
+                const int lu = !(C < T(0.5));
+                locu += lu;
+                unsolved |= lu || !sat;
+            }
+            // a state without a pending step writes its memories back unchanged
+            const int si = ok ? cb + tid : m;
+            Xs[si] = xs;
+            Xl[si] = xl;
+            cst[si] = C;
+        }
+        if (j >= 1) {
+            // sub-batch j - 1: one contiguous run per fold block
+            const int cb = c0 + (j - 1) * SUB, cnt = 3 * (min(c1, cb + SUB) - cb);
+            const int2 *R = sr[(j - 1) % 3];
+#pragma unroll
+            for (int u = 0; u < 3; u++) {
+                const int k = tid + u * BS;
+                const int kk = k < cnt ? k : 0;
+                const int2 r = R[smap[b ^ 1][kk]];
+                e[k < cnt ? (long long)r.y + (kk - (r.x >> 16)) : nnz] = st[b ^ 1][kk];
+            }
+        }
+        // the runs of sub-batch j + 1 (sub-batch j - 1's table is free now
+        // that this iteration's write-out has read it)
+        if (j + 1 < nsb && tid < s_sbr[j + 2] - s_sbr[j + 1]) sr[(j + 1) % 3][tid] = nxt.run;
+        __syncthreads();
+        cur = nxt;
+    }
+    if (track) {
+        __shared__ int shl[BS / 64];
+        add_loc_count<BS>(ctl, slot, locu, shl);
+    }
+    if (__syncthreads_or(unsolved) && tid == 0) raise_flag(ctl->flags[slot][FG_UNSOLVED]);
+}
+
+template <typename T, bool SORT>
+__global__ __launch_bounds__(SX_F) void k_sx_var(int n, int vstride, const int32_t *__restrict__ row_ptr,
+                                                 const uint16_t *__restrict__ sq, const T *__restrict__ e,
+                                                 Pair<T> *__restrict__ vd, T *__restrict__ vnext,
+                                                 const int8_t *__restrict__ unit, Ctl *__restrict__ ctl, int slot)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    T *sv = (T *)smem;
+    __shared__ int sh[SX_F / 64];
+    const int tid = threadIdx.x;
+    // everything that does not depend on another load is requested at once:
+    // the control words, the block's slot range, this thread's slot range
+    // and (V, dV) pair
+    const int ps = (slot + 2) % 3;
+    const int status = ctl->status, pdone = ctl->done[ps], stepped = ctl->stepped[ps];
+    const int plev = dev_level(ctl, ps);
+    const bool unsolved_any = any_flag(ctl->flags[slot][FG_UNSOLVED]);
+    const int stop = ctl->stop_on_solve, track = ctl->track_loc;
+    const long long steps = ctl->steps, cap = ctl->cap;
+    const int v0 = blockIdx.x * SX_F, v1 = min(n, v0 + SX_F);
+    const int v = v0 + tid;
+    const int s0 = row_ptr[v0], s1 = row_ptr[v1];
+    const int rb = v < n ? row_ptr[v] : 0, re = v < n ? row_ptr[v + 1] : 0;
+    Pair<T> a;
+    a.v = a.d = T(0.0);
+    if (v < n) a = vd[v];
+    if (status != DNX_RUNNING || pdone) return;
+    const bool pend = stepped != 0;
+    const T h = pend ? level_h<T>(plev) : T(0.0);
+    const bool solved = stop && !unsolved_any;
+    const int done = solved ? DNX_SAT : (steps >= cap ? DNX_UNKNOWN : 0);
+    if (blockIdx.x == 0 && tid < 64 && track) fold_loc(ctl, slot);
+    if (!done) {
+        // SU values per thread in flight (one round for the usual block
+        // sizes), then their LDS stores
+        constexpr int SU = sizeof(T) == 4 ? 32 : 16;
+        const int cnt = s1 - s0;
+        for (int k0 = tid; k0 < cnt; k0 += SU * SX_F) {
+            T x[SU];
+            int q[SU];
+#pragma unroll
+            for (int u = 0; u < SU; u++) {
+                const int k = k0 + u * SX_F;
+                q[u] = k < cnt ? (int)sq[s0 + k] : 0;
+                x[u] = k < cnt ? e[s0 + k] : T(0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < SU; u++)
+                if (k0 + u * SX_F < cnt) sv[q[u]] = x[u];
+        }
+    }
+    __syncthreads();
+    int f = 0;
+    if (v < n) {
+        const T V = pend ? clampV(a.v + h * a.d) : a.v;
+        T dV = T(0.0);
+        if (!done) {
+            T *sg = sv + (rb - s0);
+            const int deg = re - rb;
+            if (SORT) {
+                // dynex.cc:527-530: ascending sort, left fold from +0
+                for (int j = 1; j < deg; j++) {
+                    const T x = sg[j];
+                    int k = j;
+                    while (k > 0 && sg[k - 1] > x) {
+                        sg[k] = sg[k - 1];
+                        k--;
+                    }
+                    sg[k] = x;
+                }
+            }
+            T acc = T(0.0);
+            for (int j = 0; j < deg; j++) acc = acc + sg[j];
+            dV = acc;
+            if (unit && unit[v] != 0) dV = T(0.0);  // dynex.cc:306
+            f = fail_level(V, dV);
+#pragma unroll
+            for (int L = 0; L < 4; L++) vnext[(size_t)L * vstride + v] = clampV(V + level_h<T>(L) * dV);
+        }
+        Pair<T> o;
+        o.v = V;
+        o.d = dV;
+        vd[v] = o;
+    }
+    const int ns = (slot + 1) % 3;
+    if (done) {
+        if (blockIdx.x == 0) {
+            if (tid == 0) ctl->done[slot] = done;
+            clear_slot(ctl, ns);
+        }
+        return;
+    }
+    const int bm = block_max_int<SX_F>(f, sh);
+    if (tid == 0)
+        for (int L = 0; L < bm; L++) raise_flag(ctl->flags[slot][FG_LEVEL + L]);
+    if (blockIdx.x == 0) {
+        if (tid == 0) ctl->stepped[slot] = 1;
+        clear_slot(ctl, ns);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Resident-state step: variable pass.
 // ---------------------------------------------------------------------------
 template <typename T, bool SORT, int BS, int CAPW>
@@ -371,7 +697,8 @@ __global__ __launch_bounds__(256) void k_materialize(DevProb P, PT<T> p, const P
         xo[i] = pend ? clampV(a.v + h * a.d) : a.v;
     }
     if (i < m) {
-        T xs = Xs[i], xl = Xl[i];
+        const int pi = P.pidx ? P.pidx[i] : i;  // sliced exchange: processing order
+        T xs = Xs[pi], xl = Xl[pi];
         if (pend) {
             const int b = U3 ? 3 * i : P.cp[i];
             const int k = U3 ? 3 : P.cp[i + 1] - b;
@@ -397,8 +724,9 @@ __global__ __launch_bounds__(256) void k_load_state(DevProb P, const T *__restri
         vd[i] = a;
     }
     if (i < m) {
-        Xs[i] = x[n + i];
-        Xl[i] = x[n + m + i];
+        const int pi = P.pidx ? P.pidx[i] : i;  // sliced exchange: processing order
+        Xs[pi] = x[n + i];
+        Xl[pi] = x[n + m + i];
     }
 }
 
@@ -492,6 +820,36 @@ static void step_impl(const DevProb &P, const DevParams &prm, void *vd, void *xs
                                                     slot);
     k_step_var<T, SORT, BS, 2048><<<nblk(P.n > 0 ? P.n : 1, BS), BS, 0, st>>>(P, (Pair<T> *)vd, (const T *)e, ctl,
                                                                              slot);
+}
+
+template <typename T>
+static void step_sliced_impl(const DevProb &P, const DevParams &prm, void *vd, void *xs, void *xl, void *e,
+                             void *vnext, void *cst, Ctl *ctl, int slot, hipStream_t st)
+{
+    constexpr bool SORT = sizeof(T) == 8;
+    k_sx_clause<T><<<P.nunits, SXShape<T>::BS, 0, st>>>(P.n, sx_vstride(P.n), P.m, (long long)3 * P.m, P.sw,
+                                                        P.units, P.sbrun, (const int2 *)P.runs,
+                                                        to_pt<T>(prm), (const Pair<T> *)vd, (const T *)vnext,
+                                                        (T *)xs, (T *)xl, (T *)cst, (T *)e, ctl, slot);
+    const int lds = (int)sizeof(T) * (P.sx_region > 0 ? P.sx_region : 1);
+    static int lds_set = 0;
+    if (lds > 65536 && lds > lds_set) {
+        (void)hipFuncSetAttribute((const void *)k_sx_var<T, SORT>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        lds_set = lds;
+    }
+    k_sx_var<T, SORT><<<nblk(P.n > 0 ? P.n : 1, SX_F), SX_F, lds, st>>>(P.n, sx_vstride(P.n), P.row_ptr, P.sq,
+                                                                      (const T *)e, (Pair<T> *)vd, (T *)vnext,
+                                                                      P.unit, ctl, slot);
+}
+
+void launch_step_sliced(int prec, const DevProb &P, const DevParams &prm, void *vd, void *xs, void *xl, void *e,
+                        void *vnext, void *cst, Ctl *ctl, int slot, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    if (prec == DNX_FP64)
+        step_sliced_impl<double>(P, prm, vd, xs, xl, e, vnext, cst, ctl, slot, st);
+    else
+        step_sliced_impl<float>(P, prm, vd, xs, xl, e, vnext, cst, ctl, slot, st);
 }
 
 void launch_step(int prec, const DevProb &P, const DevParams &prm, void *vd, void *xs, void *xl, void *e, Ctl *ctl,

@@ -27,7 +27,33 @@ struct DevProb {
     const uint32_t *gw;
     const int32_t *gbase;
     const int32_t *geidx;
+    // sliced exchange (3-SAT only, else null; see sliced_tables): sw one
+    // word per literal in processing order (sign, LDS index of its variable
+    // among the unit's slices, staging slot), units 8 ints per unit, sbrun
+    // the first run of each sub-batch, runs (staging start << 16 | length,
+    // e position), sq the variable-major slot of each e position, pidx the
+    // processing position of each clause (Xs, Xl and the stored C are kept
+    // in processing order)
+    const uint32_t *sw;
+    const int32_t *units;
+    const int32_t *sbrun;
+    const int32_t *runs;
+    const uint16_t *sq;
+    const int32_t *pidx;
+    int nunits;
+    int sx_region;  // slots of the largest fold block
 };
+
+// Sliced exchange shapes: variables per slice (three slices of V in LDS),
+// clauses per sub-batch, variables per fold block.
+constexpr int SX_F = 256;
+constexpr int SX_MAXRUNS = 255;  // runs of one sub-batch (three slices of fold blocks)
+inline int sx_slice(int prec) { return prec == 0 ? 5000 : 10000; }
+inline int sx_sub(int prec) { return prec == 0 ? 512 : 1024; }
+inline int sx_region_cap(int prec) { return prec == 0 ? 20000 : 40000; }
+inline int sx_max_rounds(int) { return 2; }
+// elements per ladder level in the fold's candidate array: whole 16-byte words
+inline int sx_vstride(int n) { return (n + 3) / 4 * 4; }
 
 // Packed literal word of the grouped exchange: bit 31 the sign (set for a
 // negative literal), bits 30..11 the 0-based variable, bits 10..8 the group
@@ -111,6 +137,10 @@ void launch_euler(int prec, const DevProb &P, const DevParams &prm, const void *
 // one adaptive step of the resident state (clause pass + variable pass)
 void launch_step(int prec, const DevProb &P, const DevParams &prm, void *vd, void *xs,
                  void *xl, void *e, Ctl *ctl, int slot, void *stream);
+// the same with the sliced exchange: vnext holds 4 n candidate V values (one
+// per ladder level) written by the fold, cst each clause's last C
+void launch_step_sliced(int prec, const DevProb &P, const DevParams &prm, void *vd, void *xs, void *xl,
+                        void *e, void *vnext, void *cst, Ctl *ctl, int slot, void *stream);
 
 // state -> x_out (reference layout, precision T) with any pending update
 // applied; pins applied.  `tmpv` holds n values of T.

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libdnx.so")
 INCLUDE_H = os.path.join(os.path.dirname(HERE), "include", "dnx.h")
 
 OK, E_ARG, E_IO, E_PARSE, E_TOOLONG, E_HIP, E_NOMEM, E_BUDGET, E_STATE = 0, -1, -2, -3, -4, -5, -6, -7, -8
-EXCHANGE_LITERAL, EXCHANGE_GROUPED = 0, 1
+EXCHANGE_LITERAL, EXCHANGE_GROUPED, EXCHANGE_SLICED = 0, 1, 2
 FP64, FP32 = 0, 1
 RUNNING, SAT, UNKNOWN, FAILED = 0, 1, 2, 3
 
@@ -224,7 +224,7 @@ class Problem:
         self.n, self.m, self.nnz = n.value, m.value, nnz.value
 
     def set_exchange(self, exchange):
-        """EXCHANGE_GROUPED (default) or EXCHANGE_LITERAL for states created afterwards."""
+        """EXCHANGE_SLICED (default), EXCHANGE_GROUPED or EXCHANGE_LITERAL for states created afterwards."""
         check(lib().dnx_problem_set_exchange(self.h, int(exchange)))
 
     def clauses(self):

@@ -96,18 +96,26 @@ int dnx_problem_from_lits(dnx_ctx *ctx, int n, int m, const int32_t *clause_ptr,
                           const int32_t *lits, dnx_problem **out);
 int dnx_problem_info(const dnx_problem *p, int *n, int *m, int64_t *nnz);
 
-/* How the resident-state step of a 3-SAT problem moves each literal's
- * contribution from its clause to its variable's fold (no counterpart in the
- * reference, whose clause loop appends to per-variable vectors,
- * dynex.cc:408-410, 527-530); the bits are the same either way.
- *   DNX_EXCHANGE_GROUPED (default): stored grouped by the XCD that folds
- *     the variable, so each XCD's fold reads a region that stays in its L2
- *     (3-SAT with at most 2^20 variables; others use the literal order);
- *   DNX_EXCHANGE_LITERAL: stored in literal order, gathered through the
- *     transpose.
+/* How the resident-state step of a 3-SAT problem brings V to its clauses
+ * and each literal's contribution to its variable's fold (no counterpart in
+ * the reference, whose clause loop reads x directly and appends to
+ * per-variable vectors, dynex.cc:362-410, 527-530); the bits are the same
+ * either way.
+ *   DNX_EXCHANGE_SLICED (default): clauses regrouped into units whose
+ *     variables lie in at most three slices of V, which the unit's
+ *     workgroup holds in LDS; contributions leave as one contiguous run per
+ *     fold block (3-SAT whose slice sets need at most two rounds of units;
+ *     otherwise the grouped exchange);
+ *   DNX_EXCHANGE_GROUPED: V gathered from device memory, contributions
+ *     stored grouped by the XCD that folds the variable, so each XCD's fold
+ *     reads a region that stays in its L2 (3-SAT with at most 2^20
+ *     variables; otherwise the literal order);
+ *   DNX_EXCHANGE_LITERAL: contributions in literal order, gathered through
+ *     the transpose.
  * Applies to states created afterwards. */
 #define DNX_EXCHANGE_LITERAL 0
 #define DNX_EXCHANGE_GROUPED 1
+#define DNX_EXCHANGE_SLICED 2
 int dnx_problem_set_exchange(dnx_problem *p, int exchange);
 /* Copies the clause table back out (clause_ptr[m+1], lits[nnz]). */
 int dnx_problem_clauses(const dnx_problem *p, int32_t *clause_ptr, int32_t *lits);

@@ -123,7 +123,7 @@ def test_euler_bitexact(ctx):
 # ---------------------------------------------------------------------------
 # resident trajectories: the mode-6 loop (dynex.cc:1387-1417), fixed steps
 # ---------------------------------------------------------------------------
-def run_traj(gp, op, x0, checkpoints, prec, exchange=dnx.EXCHANGE_GROUPED):
+def run_traj(gp, op, x0, checkpoints, prec, exchange=dnx.EXCHANGE_SLICED):
     """Steps the GPU state and the oracle to each checkpoint and compares."""
     m = gp.m
     prm, oprm = dnx.params(m), O.params(m)
@@ -155,11 +155,12 @@ def run_traj(gp, op, x0, checkpoints, prec, exchange=dnx.EXCHANGE_GROUPED):
     return dev
 
 
-@pytest.mark.parametrize("exchange", [dnx.EXCHANGE_GROUPED, dnx.EXCHANGE_LITERAL])
+@pytest.mark.parametrize("exchange", [dnx.EXCHANGE_SLICED, dnx.EXCHANGE_GROUPED, dnx.EXCHANGE_LITERAL])
 @pytest.mark.parametrize("prec", [64, 32])
 def test_trajectory_c1_1000_steps(ctx, prec, exchange):
-    """n=100, m=800: 1000 steps bit-identical to the oracle with either
-    exchange of the contributions (grouped by folding XCD, or literal order)."""
+    """n=100, m=800: 1000 steps bit-identical to the oracle with each exchange
+    (V from LDS slices and contributions in runs per fold block; grouped by
+    folding XCD; literal order)."""
     cp, lits = gen(100, 800, 11)
     gp, op = both(ctx, 100, cp, lits)
     x0 = dnx.init_uniform(11, 100, 800)
@@ -167,16 +168,20 @@ def test_trajectory_c1_1000_steps(ctx, prec, exchange):
     gp.free()
 
 
+@pytest.mark.parametrize("exchange", [dnx.EXCHANGE_SLICED, dnx.EXCHANGE_GROUPED])
 @pytest.mark.parametrize("prec", [64, 32])
-def test_trajectory_grouped_ragged_chunks(ctx, prec):
-    """Clause counts that end inside a 64-clause chunk and variable counts that
-    end inside a fold workgroup, a few variables with no clause, and seeds
-    whose groups are uneven: the grouped exchange against the oracle."""
-    for seed, (n, m) in enumerate([(3, 1), (7, 65), (300, 1031), (2051, 9001), (5000, 3000)]):
+def test_trajectory_exchange_ragged_shapes(ctx, prec, exchange):
+    """Clause counts that end inside a chunk or sub-batch, variable counts that
+    end inside a fold block or slice, variables with no clause, many units
+    and slice sets, a duplicated literal: each exchange against the oracle."""
+    cases = [(3, 1), (7, 65), (300, 1031), (2051, 9001), (5000, 3000), (30000, 130000)]
+    for seed, (n, m) in enumerate(cases):
         cp, lits = gen(n, m, 300 + seed)
+        if seed == 2:
+            lits[3] = lits[4]
         gp, op = both(ctx, n, cp, lits)
         x0 = dnx.init_uniform(300 + seed, n, m)
-        run_traj(gp, op, x0, [1, 3, 40], prec)
+        run_traj(gp, op, x0, [1, 3, 40], prec, exchange)
         gp.free()
 
 
@@ -211,11 +216,14 @@ def test_trajectory_c2_fp64_1000_steps(ctx):
 def test_trajectory_c3_full_size(ctx, prec):
     """n=100,000, m=800,000 (the largest single-instance size): 100 steps
     bit-identical to the oracle, compared after 1, 4, 30 and 100 steps, with
-    the grouped exchange; and the literal-order exchange to step 4."""
+    the default exchange (sliced in FP32; FP64 needs more slices than two
+    rounds of units allow and takes the grouped one); and the grouped and
+    literal-order exchanges to step 4."""
     cp, lits = gen(100000, 800000, 23)
     gp, op = both(ctx, 100000, cp, lits)
     x0 = dnx.init_uniform(23, 100000, 800000)
     run_traj(gp, op, x0, [1, 4, 30, 100], prec)
+    run_traj(gp, op, x0, [1, 4], prec, dnx.EXCHANGE_GROUPED)
     run_traj(gp, op, x0, [1, 4], prec, dnx.EXCHANGE_LITERAL)
     gp.free()
 
