@@ -32,6 +32,10 @@ HBM_PEAK_GBPS = 8000.0
 # WRITE_SIZE x 1.000).  Keyed by bucket; k_batch<mode 0>, the flow launch:
 # 484.51 GB read + 287.04 GB written.
 TRAFFIC = {0: 771.55e9}
+# Bytes past L2 per step of the n=100k FP32 single-instance step with the sliced exchange, from two
+# rocprofv3 --pmc runs of `bench.py --profile-single` (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, calibrated on
+# the copy kernel): k_sx_clause 39.4 MB read + 21.3 MB written, k_sx_var 15.9 MB read + 2.4 MB written.
+SINGLE_TRAFFIC = dict(total=79.0e6, k_sx_clause=60.7e6, k_sx_var=18.3e6, vs_algorithmic=round(79.0 / 23.2, 2))
 METRIC = "solved instances/s + p50/p99 time-to-solution (batched FP32 vs plain FP64, 1 GPU)"
 
 
@@ -123,10 +127,14 @@ def plain_fp64(ctx, work, idx, cap, budget_s):
                 idx=list(idx[:done]), steps=steps, ok=oks)
 
 
-def single_instance(ctx, n, m, prec, nsteps):
+def single_instance(ctx, n, m, prec, nsteps, exchange=None):
+    """Per-step device time of the resident single-instance state (graph chunks of 30 steps), with the
+    problem's default exchange or the one named."""
     lits = dnx.generate_planted(90001, n, m)
     cp = np.arange(0, 3 * m + 1, 3, dtype=np.int32)
     gp = ctx.problem_from_lits(n, cp, lits)
+    if exchange is not None:
+        gp.set_exchange(exchange)
     st = gp.state(dnx.params(m), dnx.init_uniform(90001, n, m), prec)
     st.time_steps(30)
     ms = st.time_steps(nsteps)
@@ -134,8 +142,23 @@ def single_instance(ctx, n, m, prec, nsteps):
     gp.free()
     us = ms * 1e3 / nsteps
     bpe = bytes_alg(n, m) if prec == dnx.FP32 else 44 * m + 16 * n
-    return dict(n=n, m=m, us_per_step=us, achieved_GBps=bpe / (us * 1e-6) / 1e9,
-                frac=bpe / (us * 1e-6) / 1e9 / HBM_PEAK_GBPS, bytes_per_step=bpe)
+    return dict(n=n, m=m, us_per_step=round(us, 3), achieved_GBps=round(bpe / (us * 1e-6) / 1e9, 1),
+                frac=round(bpe / (us * 1e-6) / 1e9 / HBM_PEAK_GBPS, 4), bytes_per_step=bpe)
+
+
+def single_report(ctx, n=100000, m=800000):
+    """The n=100k single-instance step in both precisions, with every exchange of the contributions:
+    sliced (the FP32 default: V from LDS slices, runs per fold block), grouped by folding XCD (the FP64
+    default at this size), literal order (the transpose gather)."""
+    names = {dnx.EXCHANGE_SLICED: "sliced", dnx.EXCHANGE_GROUPED: "grouped", dnx.EXCHANGE_LITERAL: "literal"}
+    out = {}
+    for prec, tag, ns in ((dnx.FP32, "fp32", 300), (dnx.FP64, "fp64", 90)):
+        r = single_instance(ctx, n, m, prec, ns)
+        r["exchanges_us_per_step"] = {names[x]: single_instance(ctx, n, m, prec, ns, x)["us_per_step"]
+                                      for x in (dnx.EXCHANGE_SLICED, dnx.EXCHANGE_GROUPED, dnx.EXCHANGE_LITERAL)}
+        out[tag] = r
+    out["traffic_per_step_fp32"] = SINGLE_TRAFFIC
+    return out
 
 
 def served(ctx, seconds, rate, cap, spl, shape, seed=77):
@@ -333,7 +356,8 @@ def main():
     ap.add_argument("--skip-single", action="store_true")
     ap.add_argument("--served-seconds", type=float, default=60.0)
     ap.add_argument("--served-rate", type=float, default=200.0)
-    ap.add_argument("--served-fp64-sample", type=int, default=12)
+    ap.add_argument("--served-fp64-sample", type=int, default=64)
+    ap.add_argument("--served-fp64-seconds", type=float, default=60.0)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--profile-pass", action="store_true",
                     help="only the copy-kernel calibration and one batched solve (for rocprofv3 runs)")
@@ -433,7 +457,7 @@ def main():
         log(f"served: {srv}")
         if args.served_fp64_sample > 0:
             ss = served_sample(args.served_fp64_sample, 5)
-            p64s = plain_fp64(ctx, ss, np.arange(len(ss)), args.cap, args.fp64_seconds)
+            p64s = plain_fp64(ctx, ss, np.arange(len(ss)), args.cap, args.served_fp64_seconds)
             srv["plain_fp64_same_distribution"] = dict(instances=p64s["instances"], solved=p64s["solved"],
                                                        seconds=round(p64s["seconds"], 3),
                                                        instances_per_s=round(p64s["ips"], 3))
@@ -451,17 +475,21 @@ def main():
     single = None
     if not args.skip_single:
         log("single instance n=100k ...")
-        single = dict(fp32=single_instance(ctx, 100000, 800000, dnx.FP32, 300),
-                      fp64=single_instance(ctx, 100000, 800000, dnx.FP64, 100))
+        single = single_report(ctx)
     ctx.close()
 
     log("CPU oracle baseline ...")
-    procs = max(1, min(16, os.cpu_count() or 1))
+    # one process per host core, capped at 16: the host's share of a one-GPU box
+    host_cores = os.cpu_count() or 1
+    procs = max(1, min(16, host_cores))
     cpu_sample = np.random.default_rng(8).permutation(len(work))
     cpu = cpu_baseline_cores(work, cpu_sample, args.cap, args.cpu_seconds, procs)
     cpu1 = cpu_baseline(work, sample, args.cap, min(args.cpu_seconds, 10.0))
     ref_points = cpu_reference_points(args.cap)
-    cpu_line = dict(value=round(cpu["value"], 3), unit="instances/s", cores=procs, kind="port",
+    cpu_line = dict(value=round(cpu["value"], 3), unit="instances/s", cores=procs, host_cores=host_cores,
+                    cores_note=f"{procs} worker processes: min(16, os.cpu_count() = {host_cores}), 16 being the "
+                               f"host share of a one-GPU box",
+                    kind="port",
                     sample=f"{cpu['instances']} instances of the same workload (random sample), FP64 oracle, "
                            f"{procs} processes x {cpu['seconds']:.1f}s",
                     one_thread=dict(value=round(cpu1["value"], 3), instances=cpu1["instances"],
