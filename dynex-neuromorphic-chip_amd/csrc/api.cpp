@@ -371,6 +371,7 @@ int dnx_open(int device, dnx_ctx **out)
         return DNX_E_ARG;
     }
     if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+    (void)noalloc_init(device);  // a failed lookup is reported by the entry point that needs it
     dnx_ctx *c = new dnx_ctx();
     c->device = device;
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {

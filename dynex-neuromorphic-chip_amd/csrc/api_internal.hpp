@@ -29,6 +29,8 @@ size_t prec_size(int prec);
 void pack_host(int prec, const double *src, size_t count, std::vector<char> &out);
 void unpack_host(int prec, const std::vector<char> &in, size_t count, double *dst);
 DevParams dev_params(const dnx_params *p);
+// looks up the no-allocation scratch array on `device` (current), once
+int noalloc_init(int device);
 
 }  // namespace dnx
 

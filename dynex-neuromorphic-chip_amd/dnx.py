@@ -59,6 +59,7 @@ def lib():
     L.dnx_problem_free.argtypes = [vp]
     L.dnx_rhs.argtypes = [vp, ip, C.POINTER(Params), F64P, F64P, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.dnx_euler.argtypes = [vp, ip, C.POINTER(Params), F64P, F64P, dp, F64P]
+    L.dnx_rhs_device_noalloc.argtypes = [vp, ip, C.POINTER(Params), vp, vp, vp, vp, vp]
     L.dnx_state_create.argtypes = [vp, ip, C.POINTER(Params), F64P, pp]
     L.dnx_state_free.argtypes = [vp]
     L.dnx_state_step.argtypes = [vp, ip]
@@ -244,6 +245,12 @@ class Problem:
         loc, en = C.c_int(), C.c_double()
         check(lib().dnx_rhs(self.h, prec, C.byref(prm), x, d, C.byref(loc), C.byref(en)))
         return d, loc.value, en.value
+
+    def rhs_device_noalloc(self, prm, x_ptr, dxdt_ptr, loc_ptr=None, energy_ptr=None, stream=None, prec=FP64):
+        """dnx_rhs_device_noalloc: device addresses (ints) in the compute type;
+        queued on `stream` (a hipStream_t address, None: the context's stream)."""
+        check(lib().dnx_rhs_device_noalloc(self.h, prec, C.byref(prm), x_ptr, dxdt_ptr, loc_ptr, energy_ptr,
+                                           stream))
 
     def euler(self, prm, x, dxdt, h, prec=FP64):
         x = np.ascontiguousarray(x, np.float64)

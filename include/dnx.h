@@ -132,6 +132,24 @@ int dnx_rhs(dnx_problem *p, int prec, const dnx_params *prm, const double *x,
 int dnx_euler(dnx_problem *p, int prec, const dnx_params *prm, const double *x,
               const double *dxdt, double h, double *x_out);
 
+/* dnx_rhs on device buffers with no device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, serving loops that must not call hipMalloc).  x (n+2m values,
+ * reference layout) and dxdt are device pointers in the compute type (double
+ * for DNX_FP64, float for DNX_FP32); loc (int) and energy (double) are device
+ * pointers or NULL and receive what dnx_rhs returns through its pointers,
+ * bit-identical.  The work is queued on `stream` (a hipStream_t; NULL: the
+ * context's stream) and the call returns without waiting.  The per-literal
+ * contributions and per-workgroup partial sums live in one device array of
+ * DNX_NOALLOC_SCRATCH_BYTES per GPU (looked up when a context is opened on
+ * it, held until the process ends): a problem needing more is refused with
+ * DNX_E_BUDGET (about nnz <= 8.3M in DNX_FP64, 16.7M in DNX_FP32).  Every call
+ * on one GPU shares that array and it keeps the last call's contents: calls
+ * that overlap on one GPU (two streams, two host threads) race. */
+#define DNX_NOALLOC_SCRATCH_BYTES 67108864 /* 64 MiB */
+int dnx_rhs_device_noalloc(dnx_problem *p, int prec, const dnx_params *prm, const void *x,
+                           void *dxdt, int *loc, double *energy, void *stream);
+
 /* ---- resident state (the mode-6 loop of dynex.cc:1381-1427) ---------- */
 /* Uploads x0 (reference layout; unit-clause pins are applied to V as the
  * reference does at dynex.cc:2694-2699) and keeps the state on the device. */
