@@ -82,7 +82,7 @@ def batched_solve(ctx, work, hps, cap, spl, timing, shape=None, order=0):
     b.set_order(order)
     try:
         ta = time.perf_counter()
-        slots = [b.admit(hp, dnx.params(w["m"]), w["x0"], cap=cap) for w, hp in zip(work, hps)]
+        slots = b.admit_many(hps, [w["x0"] for w in work], cap=cap)
         admit_s = time.perf_counter() - ta
         if timing:
             b.timing(1)
@@ -98,8 +98,9 @@ def batched_solve(ctx, work, hps, cap, spl, timing, shape=None, order=0):
         verified = np.zeros(len(slots), bool)
         for k, s in enumerate(slots):
             if st[k] == dnx.SAT:
-                _, unsat = b.assignment(s, work[k]["n"])
-                verified[k] = unsat == 0
+                # exact integer check against the caller's own clause table
+                a, _ = b.assignment(s, work[k]["n"])
+                verified[k] = hps[k].check_assignment(a) == 0
         modes = np.array([b.info(s)[2] for s in slots])
     finally:
         b.close()

@@ -1306,6 +1306,62 @@ void launch_batch_stamp(unsigned long long *p, void *stream)
     k_batch_stamp<<<1, 64, 0, (hipStream_t)stream>>>(p);
 }
 
+// ADMIT_PARTS workgroups per record, each taking every ADMIT_PARTS-th
+// 256-lane stripe of the record's arrays.
+constexpr int ADMIT_PARTS = 16;
+
+__global__ __launch_bounds__(256) void k_admit_place(const AdmitRec *__restrict__ recs,
+                                                     const unsigned char *__restrict__ stage,
+                                                     unsigned char *__restrict__ arena, BInst *__restrict__ inst,
+                                                     BDyn *__restrict__ dyn)
+{
+    const AdmitRec *R = recs + blockIdx.x / ADMIT_PARTS;
+    const long long t = (long long)(blockIdx.x % ADMIT_PARTS) * 256 + threadIdx.x;
+    constexpr long long S = (long long)ADMIT_PARTS * 256;
+    const unsigned char *src = stage + R->src;
+    unsigned char *dst = arena + R->dst;
+    const long long n16 = (long long)(R->prefix / 16), nnz = R->nnz;
+    for (long long i = t; i < n16; i += S) ((uint4 *)dst)[i] = ((const uint4 *)src)[i];
+    const int32_t *lits = (const int32_t *)(src + R->lits);
+    const int32_t *pos = (const int32_t *)(src + R->pos);
+    const int32_t *lp = (const int32_t *)(src + R->lpos);
+    int32_t *eidx = (int32_t *)(dst + R->eidx);
+    // pos is a permutation of [0, nnz): the scatter covers eidx[0, nnz)
+    // once, the rest of the region is zeroed
+    for (long long q = t; q < (long long)R->eidx_words; q += S) {
+        if (q < nnz)
+            eidx[pos[q]] = (int32_t)q;
+        else
+            eidx[q] = 0;
+    }
+    if (R->pk) {
+        uint32_t *pk = (uint32_t *)(dst + R->pk);
+        for (long long q = t; q < (long long)R->pk_words; q += S) {
+            uint32_t w = 0u;
+            if (R->fill_pk && q < nnz) {
+                const int32_t l = lits[q];
+                w = (l < 0 ? 0x80000000u : 0u) | ((uint32_t)(abs(l) - 1) << 16) | (uint32_t)lp[q];
+            }
+            pk[q] = w;
+        }
+    }
+    float *e = (float *)(dst + R->e);
+    for (long long q = t; q < (long long)R->e_words; q += S) e[q] = 0.0f;
+    float *cst = (float *)(dst + R->cst);
+    for (long long q = t; q < (long long)R->cst_words; q += S) cst[q] = 0.0f;
+    if (t == 0) {
+        inst[R->slot] = R->inst;
+        dyn[R->slot] = R->dyn;
+    }
+}
+
+void launch_admit_place(const AdmitRec *recs, int nrec, const unsigned char *stage, unsigned char *arena,
+                        BInst *inst, BDyn *dyn, void *stream)
+{
+    if (nrec > 0)
+        k_admit_place<<<(unsigned)nrec * ADMIT_PARTS, 256, 0, (hipStream_t)stream>>>(recs, stage, arena, inst, dyn);
+}
+
 void launch_batch_materialize(const BInst *inst, const BDyn *dyn, int n, int m, float *x_out, void *stream)
 {
     int mx = n > m ? n : m;

@@ -88,6 +88,28 @@ void launch_batch_zero(int *p, int count, void *stream);
 void launch_batch_stamp(unsigned long long *p, void *stream);
 // materialised state of one slot in the reference layout (float)
 void launch_batch_materialize(const BInst *inst, const BDyn *dyn, int n, int m, float *x_out, void *stream);
+
+// Admission: one instance's staged tables and where they go.  The host stages
+// a prefix of the instance's arena range (tables only the host can build);
+// the device copies it into place and derives the rest: the variable-major
+// literal index (eidx[pos[q]] = q), mode 0's packed literal words, and
+// zeroed contribution and stored-C arrays.  Offsets are bytes; `src` is
+// relative to the staging buffer, `dst` to the arena, the others to the
+// instance's range; word counts cover whole 256-byte-aligned regions.
+struct AdmitRec {
+    unsigned long long src, dst, prefix;  // prefix: staged bytes (a multiple of 256)
+    long long nnz;
+    int slot, fill_pk;                    // fill_pk: mode-0 3-SAT packed words
+    unsigned long long lits, pos, lpos;   // staged: literals, global transpose slots, pk's LDS slots
+    unsigned long long pk, eidx, e, cst;  // derived (pk 0: none)
+    unsigned long long pk_words, eidx_words, e_words, cst_words;
+    BInst inst;
+    BDyn dyn;
+};
+// places `nrec` records (each record's tables, then its descriptor and
+// status entry at its slot)
+void launch_admit_place(const AdmitRec *recs, int nrec, const unsigned char *stage, unsigned char *arena,
+                        BInst *inst, BDyn *dyn, void *stream);
 int batch_max_threads();
 
 // wide mode (see batch.hip); maps hold (slot, local block) pairs

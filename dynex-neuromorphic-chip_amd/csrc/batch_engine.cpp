@@ -34,6 +34,8 @@
 namespace dnx {
 
 static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+// bytes per half of the admission staging buffers
+static constexpr size_t ADMIT_CHUNK = (size_t)16 << 20;
 
 // First-fit allocator over [0, cap) with coalescing on release.
 struct Arena {
@@ -90,7 +92,7 @@ struct SlotHost {
     int nchunk = 1;  // mode 0: LDS chunks of its contributions
     size_t lds = 0;  // dynamic LDS bytes the fused kernel needs for it
     size_t off = 0, size = 0;
-    std::vector<int32_t> cp, lits;  // kept for the host-side clause check
+    bool general = false;  // clause_ptr in the arena (else 3 literals per clause)
 };
 
 // Compulsory bytes of one FP32 step of an instance: literals read once
@@ -185,14 +187,21 @@ struct dnx_batch {
     unsigned long long gang_epoch = 0;
     int gang_arrive = 0;      // arrivals counted on the device so far
     DevBuf b_gmap, b_gwords, b_gerr;
-    // pinned staging of one instance's arena range at admission (reused: an
-    // admission waits for its upload before it returns)
-    unsigned char *h_upload = nullptr;
-    size_t h_upload_size = 0;
+    // admission staging: two halves of adm_chunk bytes, pinned on the host
+    // (h_adm) and on the device (b_adm); an admission waits for its uploads
+    // before it returns
+    unsigned char *h_adm = nullptr;
+    size_t adm_chunk = 0;
+    DevBuf b_adm;
+    hipEvent_t adm_ev[2] = {nullptr, nullptr};
+    bool adm_pending[2] = {false, false};
     ~dnx_batch()
     {
+        if (ustream) (void)hipStreamSynchronize(ustream);
         if (arena_base) (void)hipFree(arena_base);
-        if (h_upload) (void)hipHostFree(h_upload);
+        if (h_adm) (void)hipHostFree(h_adm);
+        for (auto &e : adm_ev)
+            if (e) (void)hipEventDestroy(e);
         if (h_dyn) (void)hipHostFree(h_dyn);
         if (h_stage) (void)hipHostFree(h_stage);
         if (h_work) (void)hipHostFree(h_work);
@@ -216,6 +225,8 @@ namespace dnx {
 
 int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *prm, const double *x0,
                          int64_t step_cap, int stop_on_solve, int *slot_out);
+int admit_many_host(dnx_batch *b, int count, const HostProblem *const *hs, const dnx_params *prms,
+                    const double *const *x0s, int64_t step_cap, int stop_on_solve, int *slots_out);
 // Mode 0 holds V, one more float per variable (lds_var_bytes) and the
 // contributions in LDS.  When the contributions do not fit beside them, the
 // clauses are cut into at most KC_MAX chunks of consecutive clauses whose
@@ -280,6 +291,7 @@ static int choose_mode(const dnx_batch *b, const HostProblem &h, Mode0Plan &P)
 // Byte layout of one instance inside its arena range.
 struct Layout {
     size_t lits, cp, lpos, pk, gpos, eidx, row, unit, xs, xl, vd, e, cbound, cbase, cst, total;
+    size_t upload;  // bytes [0, upload) are built on the host
 };
 static Layout layout_of(int n, int m, int64_t nnz, bool general, bool has_unit, int nchunk)
 {
@@ -290,24 +302,27 @@ static Layout layout_of(int n, int m, int64_t nnz, bool general, bool has_unit, 
         o += al256(bytes);
         return at;
     };
+    // tables the host builds (uploaded at admission) ...
     L.lits = take((size_t)nnz * 4);
     L.cp = general ? take(((size_t)m + 1) * 4) : 0;
-    // both transposes and the device contribution array exist for every
-    // instance, so any instance can also run in wide mode
     L.lpos = take((size_t)nnz * 4);
-    L.pk = general ? 0 : take((size_t)nnz * 4);  // filled for mode-0 3-SAT instances
     // transpose slots of the literals; with one chunk (or outside mode 0)
     // lpos holds them already
     L.gpos = nchunk > 1 ? take((size_t)nnz * 4) : 0;
-    L.eidx = take((size_t)nnz * 4);
     L.row = take(((size_t)n + 1) * 4);
     L.unit = has_unit ? take((size_t)n) : 0;
     L.xs = take((size_t)m * 4);
     L.xl = take((size_t)m * 4);
     L.vd = take((size_t)n * sizeof(BF2));
-    L.e = take((size_t)nnz * 4);
     L.cbound = take(((size_t)nchunk + 1) * 4);
     L.cbase = nchunk > 1 ? take((size_t)nchunk * ((size_t)n + 1) * 4) : 0;
+    L.upload = o;
+    // ... and the ones the device derives or clears at admission.  Both
+    // transposes and the contribution array exist for every instance, so
+    // any instance can also run in wide mode.
+    L.pk = general ? 0 : take((size_t)nnz * 4);  // filled for mode-0 3-SAT instances
+    L.eidx = take((size_t)nnz * 4);
+    L.e = take((size_t)nnz * 4);
     L.cst = take((size_t)m * 4);
     L.total = o ? o : 256;
     return L;
@@ -466,10 +481,22 @@ int dnx_batch_open(dnx_ctx *ctx, int prec, int slots, int64_t arena_bytes, dnx_b
         return hip_fail(e, "hipHostMalloc");
     }
     for (int i = 0; i < 2; i++)
-        if ((e = hipEventCreateWithFlags(&b->ev[i], hipEventDisableTiming)) != hipSuccess) {
+        if ((e = hipEventCreateWithFlags(&b->ev[i], hipEventDisableTiming)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&b->adm_ev[i], hipEventDisableTiming)) != hipSuccess) {
             delete b;
             return hip_fail(e, "event");
         }
+    // admission staging (grown by an admission that needs more)
+    b->adm_chunk = ADMIT_CHUNK;
+    if ((e = hipHostMalloc((void **)&b->h_adm, 2 * b->adm_chunk, hipHostMallocDefault)) != hipSuccess) {
+        b->h_adm = nullptr;
+        delete b;
+        return hip_fail(e, "admission staging");
+    }
+    if ((rc = b->b_adm.alloc(2 * b->adm_chunk))) {
+        delete b;
+        return rc;
+    }
     if ((e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&b->side_done, hipEventDisableTiming)) != hipSuccess ||
         (e = hipStreamCreateWithFlags(&b->ustream, hipStreamNonBlocking)) != hipSuccess) {
@@ -557,52 +584,128 @@ int dnx_batch_admit(dnx_batch *b, const dnx_problem *p, const dnx_params *prm, c
     return dnx_batch_admit_host(b, p->hp, prm, x0, step_cap, stop_on_solve, slot_out);
 }
 
+int dnx_batch_admit_many(dnx_batch *b, int count, const dnx_problem *const *probs, const dnx_params *prms,
+                         const double *const *x0s, int64_t step_cap, int stop_on_solve, int *slots_out)
+{
+    if (!b || count < 0 || (count && (!probs || !x0s || !slots_out)) || step_cap < 0) return DNX_E_ARG;
+    std::vector<const HostProblem *> hs((size_t)count);
+    for (int i = 0; i < count; i++) {
+        if (!probs[i] || !x0s[i]) return DNX_E_ARG;
+        hs[(size_t)i] = &probs[i]->hp;
+    }
+    std::vector<dnx_params> pv;
+    if (!prms) {
+        pv.resize((size_t)count);
+        for (int i = 0; i < count; i++) dnx_params_default(hs[(size_t)i]->m, 10000, &pv[(size_t)i]);
+    }
+    return admit_many_host(b, count, hs.data(), prms ? prms : pv.data(), x0s, step_cap, stop_on_solve, slots_out);
+}
+
 }  // extern "C"
 
-namespace dnx {
+namespace {
 
-int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *prm, const double *x0,
-                         int64_t step_cap, int stop_on_solve, int *slot_out)
-{
-    if (b->free_slots.empty()) {
-        set_error("no free slot");
-        return DNX_E_BUDGET;
+// Runs fn(i) for i in [0, n) on the calling thread and a fixed set of helper
+// threads; run() returns when every index is done.
+class ForkJoin {
+  public:
+    explicit ForkJoin(int helpers)
+    {
+        for (int i = 0; i < helpers; i++) th_.emplace_back([this] { loop(); });
     }
-    (void)hipSetDevice(b->ctx->device);
-    const int n = h.n, m = h.m;
-    const int64_t nnz = h.nnz();
-    const bool general = !h.uniform3;
-    Mode0Plan plan;
-    const int mode = choose_mode(b, h, plan);
-    const int nchunk = plan.nchunk;
-    const Layout L = layout_of(n, m, nnz, general, h.has_unit, nchunk);
-    size_t off;
-    if (!b->arena.alloc(L.total, off)) {
-        set_error("instance needs " + std::to_string(L.total) + " bytes; arena has " +
-                  std::to_string(b->arena.free_bytes()) + " free in fragments");
-        return DNX_E_BUDGET;
-    }
-    const int slot = b->free_slots.back();
-    // stage the whole range in pinned host memory, then one copy
-    hipError_t e;
-    if (b->h_upload_size < L.total) {
-        if (b->h_upload) (void)hipHostFree(b->h_upload);
-        b->h_upload = nullptr;
-        b->h_upload_size = 0;
-        const size_t want = std::max(L.total, (size_t)8 << 20);
-        if ((e = hipHostMalloc((void **)&b->h_upload, want, hipHostMallocDefault)) != hipSuccess) {
-            b->arena.release(off, L.total);
-            return hip_fail(e, "admit staging");
+    ~ForkJoin()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            quit_ = true;
+            gen_++;
         }
-        b->h_upload_size = want;
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
     }
-    unsigned char *stage = b->h_upload;
-    memset(stage, 0, L.total);
+    void run(int n, const std::function<void(int)> &fn)
+    {
+        if (th_.empty() || n <= 1) {
+            for (int i = 0; i < n; i++) fn(i);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            fn_ = &fn;
+            n_ = n;
+            next_.store(0);
+            active_ = (int)th_.size();
+            gen_++;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [&] { return active_ == 0; });
+        fn_ = nullptr;
+    }
+
+  private:
+    void work()
+    {
+        for (int i; (i = next_.fetch_add(1)) < n_;) (*fn_)(i);
+    }
+    void loop()
+    {
+        unsigned long long seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return gen_ != seen; });
+                seen = gen_;
+                if (quit_) return;
+            }
+            work();
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--active_ == 0) done_.notify_one();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(int)> *fn_ = nullptr;
+    int n_ = 0, active_ = 0;
+    std::atomic<int> next_{0};
+    unsigned long long gen_ = 0;
+    bool quit_ = false;
+};
+
+struct AdmitItem {
+    const HostProblem *h = nullptr;
+    const double *x0 = nullptr;
+    dnx_params prm{};
+    int mode = 0, slot = -1;
+    Mode0Plan plan;
+    Layout L{};
+    BInst inst{};
+    size_t off = 0;    // arena offset
+    size_t soff = 0;   // offset of the staged prefix in its staging buffer
+    int chunk = 0;
+};
+
+// Writes the part of an instance's arena range that only the host builds
+// (everything below L.upload) to `stage`, with every region's alignment
+// padding zeroed.
+void stage_prefix(const AdmitItem &a, unsigned char *stage)
+{
+    const HostProblem &h = *a.h;
+    const Layout &L = a.L;
+    const int n = h.n, m = h.m, nchunk = a.plan.nchunk;
+    const int64_t nnz = h.nnz();
+    auto tail = [&](size_t at, size_t bytes) {
+        const size_t end = al256(bytes);
+        if (end > bytes) memset(stage + at + bytes, 0, end - bytes);
+    };
     auto put = [&](size_t at, const void *src, size_t bytes) {
         if (bytes) memcpy(stage + at, src, bytes);
+        tail(at, bytes);
     };
     put(L.lits, h.lits.data(), (size_t)nnz * 4);
-    if (general) put(L.cp, h.cp.data(), ((size_t)m + 1) * 4);
+    if (!h.uniform3) put(L.cp, h.cp.data(), ((size_t)m + 1) * 4);
     if (nchunk == 1) {
         put(L.lpos, h.lpos.data(), (size_t)nnz * 4);
     } else {
@@ -614,109 +717,263 @@ int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *p
         for (int k = 0; k < nchunk; k++) {
             int32_t *base = cb + (size_t)k * ((size_t)n + 1);
             std::fill(base, base + n + 1, 0);
-            const int q0 = h.cp[(size_t)plan.bound[(size_t)k]], q1 = h.cp[(size_t)plan.bound[(size_t)k + 1]];
+            const int q0 = h.cp[(size_t)a.plan.bound[(size_t)k]], q1 = h.cp[(size_t)a.plan.bound[(size_t)k + 1]];
             for (int q = q0; q < q1; q++) base[std::abs(h.lits[(size_t)q]) - 1 + 1]++;
             for (int v = 0; v < n; v++) base[v + 1] += base[v];
             std::copy(base, base + n, run.begin());
             for (int q = q0; q < q1; q++) lp[q] = run[(size_t)std::abs(h.lits[(size_t)q]) - 1]++;
         }
+        tail(L.lpos, (size_t)nnz * 4);
+        tail(L.cbase, (size_t)nchunk * ((size_t)n + 1) * 4);
+        put(L.gpos, h.lpos.data(), (size_t)nnz * 4);
     }
-    if (mode == BM_LDS_ALL && !general) {
-        // one word per literal for the fused kernel's steady state: sign in
-        // bit 31, variable in bits 16-30, LDS slot in bits 0-15 (plan_mode0
-        // admits only instances whose variables and slots fit)
-        const int32_t *lp = (const int32_t *)(stage + L.lpos);
-        uint32_t *pk = (uint32_t *)(stage + L.pk);
-        for (int64_t q = 0; q < nnz; q++) {
-            const int32_t l = h.lits[(size_t)q];
-            pk[q] = (l < 0 ? 0x80000000u : 0u) | ((uint32_t)(std::abs(l) - 1) << 16) | (uint32_t)lp[q];
-        }
-    }
-    if (nchunk > 1) put(L.gpos, h.lpos.data(), (size_t)nnz * 4);
-    put(L.cbound, plan.bound.data(), ((size_t)nchunk + 1) * 4);
-    put(L.eidx, h.eidx.data(), (size_t)nnz * 4);
     put(L.row, h.row_ptr.data(), ((size_t)n + 1) * 4);
     if (h.has_unit) put(L.unit, h.unit.data(), (size_t)n);
     float *xs = (float *)(stage + L.xs), *xl = (float *)(stage + L.xl);
+    const double *x0 = a.x0;
     for (int c = 0; c < m; c++) {
         xs[c] = (float)x0[(size_t)n + c];
         xl[c] = (float)x0[(size_t)n + m + c];
     }
+    tail(L.xs, (size_t)m * 4);
+    tail(L.xl, (size_t)m * 4);
     BF2 *vd = (BF2 *)(stage + L.vd);
     for (int v = 0; v < n; v++) {  // unit pins at admission (dynex.cc:2694-2699)
         vd[v].v = h.unit[(size_t)v] != 0 ? (float)h.unit[(size_t)v] : (float)x0[v];
         vd[v].d = 0.0f;
     }
-    char *base = (char *)b->arena_base + off;
-    if ((e = hipMemcpyAsync(base, stage, L.total, hipMemcpyHostToDevice, b->ustream)) != hipSuccess) {
-        b->arena.release(off, L.total);
-        return hip_fail(e, "admit upload");
+    tail(L.vd, (size_t)n * sizeof(BF2));
+    put(L.cbound, a.plan.bound.data(), ((size_t)nchunk + 1) * 4);
+}
+
+}  // namespace
+
+namespace dnx {
+
+int dnx_batch_admit_host(dnx_batch *b, const HostProblem &h, const dnx_params *prm, const double *x0,
+                         int64_t step_cap, int stop_on_solve, int *slot_out)
+{
+    const HostProblem *hp = &h;
+    return admit_many_host(b, 1, &hp, prm, &x0, step_cap, stop_on_solve, slot_out);
+}
+
+// Admission of `count` instances: plans and arena ranges first (all or
+// none), then the host-built tables are staged by worker threads into
+// pinned chunks, each chunk uploaded with one copy and placed by one
+// kernel (which also derives the tables the device can build), chunks
+// alternating between two buffers so staging overlaps the uploads; one wait
+// at the end.
+int admit_many_host(dnx_batch *b, int count, const HostProblem *const *hs, const dnx_params *prms,
+                    const double *const *x0s, int64_t step_cap, int stop_on_solve, int *slots_out)
+{
+    if (count < 0) return DNX_E_ARG;
+    if (count == 0) return DNX_OK;
+    if ((size_t)count > b->free_slots.size()) {
+        set_error(count == 1 ? "no free slot" : "fewer free slots than instances");
+        return DNX_E_BUDGET;
     }
-    BInst I{};
-    I.n = n;
-    I.m = m;
-    I.mode = mode;
-    I.flags = (general ? BF_GENERAL : 0) | (h.has_unit ? BF_UNIT : 0);
-    I.lits = (const int32_t *)(base + L.lits);
-    I.cp = general ? (const int32_t *)(base + L.cp) : nullptr;
-    I.lpos = (const int32_t *)(base + L.lpos);
-    I.pk = mode == BM_LDS_ALL && !general ? (const uint32_t *)(base + L.pk) : nullptr;
-    I.gpos = nchunk > 1 ? (const int32_t *)(base + L.gpos) : I.lpos;
-    I.eidx = (const int32_t *)(base + L.eidx);
-    I.row_ptr = (const int32_t *)(base + L.row);
-    I.unit = h.has_unit ? (const int8_t *)(base + L.unit) : nullptr;
-    I.xs = (float *)(base + L.xs);
-    I.xl = (float *)(base + L.xl);
-    I.vd = (BF2 *)(base + L.vd);
-    I.e = (float *)(base + L.e);
-    I.cbound = (const int32_t *)(base + L.cbound);
-    I.cbase = nchunk > 1 ? (const int32_t *)(base + L.cbase) : I.row_ptr;
-    I.cst = (float *)(base + L.cst);
-    I.nchunk = nchunk;
-    I.alpha = (float)prm->alpha;
-    I.beta = (float)prm->beta;
-    I.gamma = (float)prm->gamma;
-    I.delta = (float)prm->delta;
-    I.eps = (float)prm->epsilon;
-    I.zeta = (float)prm->zeta;
-    I.xlmax = (float)(double)prm->xl_max;
-    BDyn D;
-    memset(&D, 0, sizeof(D));
-    D.status = DNX_RUNNING;
-    D.plevel = -1;
-    D.stop_on_solve = stop_on_solve ? 1 : 0;
-    D.cap = step_cap;
-    b->h_inst[(size_t)slot] = I;
-    // the descriptor and status copies read from host memory that outlives
-    // the stream: h_inst is owned by the batch; D is staged in pinned memory
-    // the upload's source is the slot's staging record: the snapshot buffers
-    // may be the target of status copies still in flight
-    b->h_stage[slot] = D;
-    b->h_dyn[slot] = D;
-    b->h_dyn[(size_t)b->nslots + slot] = D;
-    if ((e = hipMemcpyAsync((BInst *)b->b_inst.ptr + slot, &b->h_inst[(size_t)slot], sizeof(BInst),
-                            hipMemcpyHostToDevice, b->ustream)) != hipSuccess ||
-        (e = hipMemcpyAsync((BDyn *)b->b_dyn.ptr + slot, &b->h_stage[slot], sizeof(BDyn), hipMemcpyHostToDevice,
-                            b->ustream)) != hipSuccess) {
-        b->arena.release(off, L.total);
-        return hip_fail(e, "admit descriptor");
+    (void)hipSetDevice(b->ctx->device);
+    std::vector<AdmitItem> items((size_t)count);
+    const size_t fs = b->free_slots.size();
+    auto release_upto = [&](int k) {
+        for (int i = 0; i < k; i++) b->arena.release(items[(size_t)i].off, items[(size_t)i].L.total);
+    };
+    const size_t rec_bytes = sizeof(AdmitRec);
+    size_t chunk_need = 0;
+    int helpers = 0;
+    if (count >= 8) {
+        const unsigned hc = std::thread::hardware_concurrency();
+        helpers = (int)std::min<unsigned>(hc ? hc : 1, 16) - 1;
     }
-    // the staging buffer is reused by the next admission: wait for its copy
-    if ((e = hipStreamSynchronize(b->ustream)) != hipSuccess) return hip_fail(e, "admit sync");
-    SlotHost &sl = b->slots[(size_t)slot];
-    sl.used = true;
-    sl.n = n;
-    sl.m = m;
-    sl.nnz = nnz;
-    sl.mode = mode;
-    sl.nchunk = nchunk;
-    sl.lds = plan.lds;
-    sl.off = off;
-    sl.size = L.total;
-    sl.cp = h.cp;
-    sl.lits = h.lits;
-    b->free_slots.pop_back();
-    *slot_out = slot;
+    ForkJoin pool(helpers);
+    pool.run(count, [&](int i) {
+        AdmitItem &a = items[(size_t)i];
+        a.h = hs[i];
+        a.x0 = x0s[i];
+        a.prm = prms[i];
+        a.mode = choose_mode(b, *a.h, a.plan);
+        a.L = layout_of(a.h->n, a.h->m, a.h->nnz(), !a.h->uniform3, a.h->has_unit, a.plan.nchunk);
+    });
+    for (int i = 0; i < count; i++) {
+        AdmitItem &a = items[(size_t)i];
+        if (!b->arena.alloc(a.L.total, a.off)) {
+            release_upto(i);
+            set_error("instance needs " + std::to_string(a.L.total) + " bytes; arena has " +
+                      std::to_string(b->arena.free_bytes()) + " free in fragments");
+            return DNX_E_BUDGET;
+        }
+        a.slot = b->free_slots[fs - 1 - (size_t)i];
+        chunk_need = std::max(chunk_need, al256(rec_bytes) + a.L.upload);
+    }
+    // staging buffers: two halves of `ch` bytes, pinned on the host and on
+    // the device; kept by the batch for later admissions
+    const size_t need = std::max(chunk_need, ADMIT_CHUNK);
+    hipError_t e;
+    if (b->adm_chunk < need) {
+        const size_t ch = need;
+        if (b->h_adm) (void)hipHostFree(b->h_adm);
+        b->h_adm = nullptr;
+        b->adm_chunk = 0;
+        int rc;
+        if ((e = hipHostMalloc((void **)&b->h_adm, 2 * ch, hipHostMallocDefault)) != hipSuccess) {
+            release_upto(count);
+            return hip_fail(e, "admission staging");
+        }
+        if ((rc = b->b_adm.alloc(2 * ch))) {
+            release_upto(count);
+            return rc;
+        }
+        b->adm_chunk = ch;
+    }
+    const size_t ch = b->adm_chunk;
+    // cut the list into chunks: records first, then each prefix
+    std::vector<int> cstart{0};
+    {
+        int k = 0;
+        size_t used = 0;
+        for (int i = 0; i < count; i++) {
+            const size_t up = items[(size_t)i].L.upload;
+            if (i > cstart.back() && al256((size_t)(i - cstart.back() + 1) * rec_bytes) + used + up > ch) {
+                cstart.push_back(i);
+                k++;
+                used = 0;
+            }
+            items[(size_t)i].chunk = k;
+            items[(size_t)i].soff = used;  // relative to the end of the records
+            used += up;
+        }
+        cstart.push_back(count);
+    }
+    unsigned char *arena = (unsigned char *)b->arena_base;
+    const int nchunks = (int)cstart.size() - 1;
+    int rc = DNX_OK;
+    for (int k = 0; k < nchunks && rc == DNX_OK; k++) {
+        const int hf = k & 1, i0 = cstart[(size_t)k], i1 = cstart[(size_t)k + 1];
+        if (b->adm_pending[hf]) {
+            if ((e = hipEventSynchronize(b->adm_ev[hf])) != hipSuccess) {
+                rc = hip_fail(e, "admission wait");
+                break;
+            }
+            b->adm_pending[hf] = false;
+        }
+        unsigned char *hbuf = b->h_adm + (size_t)hf * ch;
+        unsigned char *dbuf = (unsigned char *)b->b_adm.ptr + (size_t)hf * ch;
+        const size_t hdr = al256((size_t)(i1 - i0) * rec_bytes);
+        AdmitRec *recs = (AdmitRec *)hbuf;
+        pool.run(i1 - i0, [&](int j) {
+            const AdmitItem &a = items[(size_t)(i0 + j)];
+            stage_prefix(a, hbuf + hdr + a.soff);
+        });
+        size_t bytes = hdr;
+        for (int i = i0; i < i1; i++) {
+            AdmitItem &a = items[(size_t)i];
+            const HostProblem &h = *a.h;
+            const Layout &L = a.L;
+            const bool general = !h.uniform3;
+            const int64_t nnz = h.nnz();
+            unsigned char *base = arena + a.off;
+            AdmitRec R;
+            memset(&R, 0, sizeof(R));
+            R.src = hdr + a.soff;
+            R.dst = a.off;
+            R.prefix = L.upload;
+            R.nnz = nnz;
+            R.slot = a.slot;
+            R.fill_pk = a.mode == BM_LDS_ALL && !general;
+            R.lits = L.lits;
+            R.pos = a.plan.nchunk > 1 ? L.gpos : L.lpos;
+            R.lpos = L.lpos;
+            R.pk = general ? 0 : L.pk;
+            R.eidx = L.eidx;
+            R.e = L.e;
+            R.cst = L.cst;
+            R.pk_words = general ? 0 : al256((size_t)nnz * 4) / 4;
+            R.eidx_words = al256((size_t)nnz * 4) / 4;
+            R.e_words = al256((size_t)nnz * 4) / 4;
+            R.cst_words = al256((size_t)h.m * 4) / 4;
+            BInst &I = R.inst;
+            I.n = h.n;
+            I.m = h.m;
+            I.mode = a.mode;
+            I.flags = (general ? BF_GENERAL : 0) | (h.has_unit ? BF_UNIT : 0);
+            I.lits = (const int32_t *)(base + L.lits);
+            I.cp = general ? (const int32_t *)(base + L.cp) : nullptr;
+            I.lpos = (const int32_t *)(base + L.lpos);
+            I.pk = R.fill_pk ? (const uint32_t *)(base + L.pk) : nullptr;
+            I.gpos = a.plan.nchunk > 1 ? (const int32_t *)(base + L.gpos) : I.lpos;
+            I.eidx = (const int32_t *)(base + L.eidx);
+            I.row_ptr = (const int32_t *)(base + L.row);
+            I.unit = h.has_unit ? (const int8_t *)(base + L.unit) : nullptr;
+            I.xs = (float *)(base + L.xs);
+            I.xl = (float *)(base + L.xl);
+            I.vd = (BF2 *)(base + L.vd);
+            I.e = (float *)(base + L.e);
+            I.cbound = (const int32_t *)(base + L.cbound);
+            I.cbase = a.plan.nchunk > 1 ? (const int32_t *)(base + L.cbase) : I.row_ptr;
+            I.cst = (float *)(base + L.cst);
+            I.nchunk = a.plan.nchunk;
+            I.alpha = (float)a.prm.alpha;
+            I.beta = (float)a.prm.beta;
+            I.gamma = (float)a.prm.gamma;
+            I.delta = (float)a.prm.delta;
+            I.eps = (float)a.prm.epsilon;
+            I.zeta = (float)a.prm.zeta;
+            I.xlmax = (float)(double)a.prm.xl_max;
+            BDyn &D = R.dyn;
+            D.status = DNX_RUNNING;
+            D.plevel = -1;
+            D.stop_on_solve = stop_on_solve ? 1 : 0;
+            D.cap = step_cap;
+            recs[i - i0] = R;
+            a.inst = I;
+            bytes = std::max(bytes, (size_t)R.src + L.upload);
+        }
+        if ((e = hipMemcpyAsync(dbuf, hbuf, bytes, hipMemcpyHostToDevice, b->ustream)) != hipSuccess) {
+            rc = hip_fail(e, "admit upload");
+            break;
+        }
+        launch_admit_place((const AdmitRec *)dbuf, i1 - i0, dbuf, arena, (BInst *)b->b_inst.ptr,
+                           (BDyn *)b->b_dyn.ptr, b->ustream);
+        if ((e = hipGetLastError()) != hipSuccess || (e = hipEventRecord(b->adm_ev[hf], b->ustream)) != hipSuccess) {
+            rc = hip_fail(e, "admit placement");
+            break;
+        }
+        b->adm_pending[hf] = true;
+    }
+    // the staging buffers are reused by the next admission, and the
+    // instances must be in place before anything launches: wait here
+    if ((e = hipStreamSynchronize(b->ustream)) != hipSuccess && rc == DNX_OK) rc = hip_fail(e, "admit sync");
+    b->adm_pending[0] = b->adm_pending[1] = false;
+    if (rc != DNX_OK) {
+        release_upto(count);
+        return rc;
+    }
+    for (int i = 0; i < count; i++) {
+        const AdmitItem &a = items[(size_t)i];
+        const int slot = a.slot;
+        b->h_inst[(size_t)slot] = a.inst;
+        BDyn D;
+        memset(&D, 0, sizeof(D));
+        D.status = DNX_RUNNING;
+        D.plevel = -1;
+        D.stop_on_solve = stop_on_solve ? 1 : 0;
+        D.cap = step_cap;
+        b->h_dyn[slot] = D;
+        b->h_dyn[(size_t)b->nslots + slot] = D;
+        SlotHost &sl = b->slots[(size_t)slot];
+        sl.used = true;
+        sl.n = a.h->n;
+        sl.m = a.h->m;
+        sl.nnz = a.h->nnz();
+        sl.mode = a.mode;
+        sl.nchunk = a.plan.nchunk;
+        sl.lds = a.plan.lds;
+        sl.off = a.off;
+        sl.size = a.L.total;
+        sl.general = !a.h->uniform3;
+        slots_out[i] = slot;
+    }
+    b->free_slots.resize(fs - (size_t)count);
     return DNX_OK;
 }
 
@@ -1284,11 +1541,23 @@ int dnx_batch_assignment(dnx_batch *b, int slot, int8_t *assign, int *unsat)
     if ((e = hipStreamSynchronize(b->ctx->stream)) != hipSuccess) return hip_fail(e, "assignment sync");
     for (int v = 0; v < sl.n; v++) assign[v] = vd[(size_t)v].v >= 0.0f ? 1 : -1;
     if (unsat) {
+        // the clause table the slot holds (uploaded verbatim at admission)
+        std::vector<int32_t> lits((size_t)(sl.nnz ? sl.nnz : 1)), cp;
+        const BInst &I = b->h_inst[(size_t)slot];
+        if (sl.nnz && (e = hipMemcpy(lits.data(), I.lits, sizeof(int32_t) * (size_t)sl.nnz,
+                                     hipMemcpyDeviceToHost)) != hipSuccess)
+            return hip_fail(e, "clause table copy");
+        if (sl.general) {
+            cp.resize((size_t)sl.m + 1);
+            if ((e = hipMemcpy(cp.data(), I.cp, sizeof(int32_t) * cp.size(), hipMemcpyDeviceToHost)) != hipSuccess)
+                return hip_fail(e, "clause table copy");
+        }
         int bad = 0;
         for (int c = 0; c < sl.m; c++) {
             bool ok = false;
-            for (int q = sl.cp[(size_t)c]; q < sl.cp[(size_t)c + 1] && !ok; q++) {
-                int32_t lit = sl.lits[(size_t)q];
+            const int q0 = sl.general ? cp[(size_t)c] : 3 * c, q1 = sl.general ? cp[(size_t)c + 1] : 3 * c + 3;
+            for (int q = q0; q < q1 && !ok; q++) {
+                int32_t lit = lits[(size_t)q];
                 int8_t a = assign[std::abs(lit) - 1];
                 ok = (lit > 0 && a > 0) || (lit < 0 && a < 0);
             }
@@ -1556,11 +1825,13 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
                                  : hipSuccess;
             if (ee == hipSuccess) ee = hipStreamSynchronize(ls);
             if (ee != hipSuccess) return hip_fail(ee, "assignment copy");
+            // against the request's own clause table (released below)
+            const HostProblem &hq = *get(r).hp;
             int bad = 0;
             for (int c = 0; c < sl.m; c++) {
                 bool ok = false;
-                for (int q = sl.cp[(size_t)c]; q < sl.cp[(size_t)c + 1] && !ok; q++) {
-                    const int32_t lit = sl.lits[(size_t)q];
+                for (int q = hq.cp[(size_t)c]; q < hq.cp[(size_t)c + 1] && !ok; q++) {
+                    const int32_t lit = hq.lits[(size_t)q];
                     const bool pos = vbuf[(size_t)std::abs(lit) - 1].v >= 0.0f;
                     ok = (lit > 0) == pos;
                 }

@@ -82,6 +82,7 @@ def lib():
     L.dnx_batch_close.argtypes = [vp]
     L.dnx_batch_set_shape.argtypes = [vp, ip, ip, ip]
     L.dnx_batch_admit.argtypes = [vp, vp, C.POINTER(Params), F64P, C.c_int64, ip, C.POINTER(C.c_int)]
+    L.dnx_batch_admit_many.argtypes = [vp, ip, pp, vp, pp, C.c_int64, ip, I32P]
     L.dnx_batch_advance.argtypes = [vp, ip]
     L.dnx_batch_poll.argtypes = [vp, vp, vp, vp, vp]
     L.dnx_batch_run.argtypes = [vp, ip, dp, C.POINTER(C.c_int)]
@@ -362,6 +363,20 @@ class Batch:
         check(lib().dnx_batch_admit(self.h, prob.h, C.byref(prm), np.ascontiguousarray(x0, np.float64), int(cap),
                                     int(bool(stop_on_solve)), C.byref(s)))
         return s.value
+
+    def admit_many(self, probs, x0s, cap=100000, stop_on_solve=True, prms=None):
+        """dnx_batch_admit_many: the slots of probs[i] in order; prms None
+        means each instance's defaults."""
+        k = len(probs)
+        x0s = [np.ascontiguousarray(x, np.float64) for x in x0s]
+        if len(x0s) != k or (prms is not None and len(prms) != k):
+            raise DnxError(E_ARG, "probs, x0s and prms differ in length")
+        ph = (C.c_void_p * k)(*[p.h for p in probs])
+        xh = (C.c_void_p * k)(*[x.ctypes.data for x in x0s])
+        pa = (Params * k)(*prms) if prms is not None else None
+        slots = np.zeros(k, np.int32)
+        check(lib().dnx_batch_admit_many(self.h, k, ph, pa, xh, int(cap), int(bool(stop_on_solve)), slots))
+        return [int(s) for s in slots]
 
     def advance(self, steps):
         check(lib().dnx_batch_advance(self.h, int(steps)))

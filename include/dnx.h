@@ -237,6 +237,18 @@ int dnx_batch_set_shape(dnx_batch *b, int mode, int block, int lds_cap);
 int dnx_batch_admit(dnx_batch *b, const dnx_problem *p, const dnx_params *prm, const double *x0,
                     int64_t step_cap, int stop_on_solve, int *slot_out);
 
+/* `count` admissions in one call: the same slots and contents as count
+ * dnx_batch_admit calls in order (slots_out[i] is instance i's), with the
+ * tables staged by host threads into pinned chunks, each uploaded in one
+ * copy and placed by one kernel that also derives the transposed literal
+ * index, and a single wait at the end.  prms may be NULL (defaults for each
+ * instance's m, dnx_params_default with -m 10000).  All or none: on
+ * DNX_E_BUDGET (too few slots, or the arena cannot hold them all) nothing
+ * is admitted. */
+int dnx_batch_admit_many(dnx_batch *b, int count, const dnx_problem *const *probs,
+                         const dnx_params *prms, const double *const *x0s, int64_t step_cap,
+                         int stop_on_solve, int *slots_out);
+
 /* Enqueues one launch per execution shape that advances every running slot
  * by up to `steps` steps.  Returns without waiting. */
 int dnx_batch_advance(dnx_batch *b, int steps);
@@ -258,7 +270,9 @@ int dnx_batch_stamp(dnx_batch *b, uint64_t *ticks, double *ticks_per_s);
 int dnx_batch_read_state(dnx_batch *b, int slot, double *x, int *status, int64_t *steps, double *t);
 
 /* Sign assignment (+1/-1, x >= 0 positive) of a slot and, when unsat is
- * not NULL, the number of clauses it leaves unsatisfied (exact check). */
+ * not NULL, the number of clauses it leaves unsatisfied (exact integer check
+ * over the clause table the slot holds, read back from the device; the
+ * batch keeps no host copy of admitted problems). */
 int dnx_batch_assignment(dnx_batch *b, int slot, int8_t *assign, int *unsat);
 
 /* Frees a slot and its arena range. */

@@ -57,13 +57,16 @@ def oracle_solve(n, cp, lits, x0, cap):
     return steps, t, status, x
 
 
-def run_batch(ctx, inst, cap, spl, shapes=None, stop=True, wide=0, gang=None, split=False, flow=True, order=0):
+def run_batch(ctx, inst, cap, spl, shapes=None, stop=True, wide=0, gang=None, split=False, flow=True, order=0,
+              many=False):
     """gang = (max_blocks, clauses_per_workgroup, steps_per_launch) enables
     gang mode for the tail (the instances left once wide mode applies);
     split keeps the single-chunk LDS instances of the tail in the fused
     kernel beside wide mode; flow runs the fused launch until the wide-mode
     threshold instead of spl steps per launch; order > 0 starts instances of
-    at least that many clauses first and the others smallest first."""
+    at least that many clauses first and the others smallest first; many
+    admits the whole set in one dnx_batch_admit_many call ("defaults": with
+    prms NULL)."""
     b = dnx.Batch(ctx, slots=len(inst) + 2, arena_bytes=1 << 30)
     b.set_wide(wide, 30)
     b.set_tail_split(split)
@@ -77,10 +80,17 @@ def run_batch(ctx, inst, cap, spl, shapes=None, stop=True, wide=0, gang=None, sp
         for md, (bs, lds) in shapes.items():
             b.set_shape(md, bs, lds)
     slots = []
-    for n, cp, lits, x0 in inst:
-        hp = dnx.host_problem(n, cp, lits)
-        slots.append(b.admit(hp, dnx.params(len(cp) - 1), x0, cap=cap, stop_on_solve=stop))
-        hp.free()
+    if many:
+        hps = [dnx.host_problem(n, cp, lits) for n, cp, lits, x0 in inst]
+        prms = None if many == "defaults" else [dnx.params(len(cp) - 1) for n, cp, lits, x0 in inst]
+        slots = b.admit_many(hps, [x0 for n, cp, lits, x0 in inst], cap=cap, stop_on_solve=stop, prms=prms)
+        for hp in hps:
+            hp.free()
+    else:
+        for n, cp, lits, x0 in inst:
+            hp = dnx.host_problem(n, cp, lits)
+            slots.append(b.admit(hp, dnx.params(len(cp) - 1), x0, cap=cap, stop_on_solve=stop))
+            hp.free()
     b.run(steps_per_launch=spl)
     res = b.poll()
     out = []
@@ -104,8 +114,9 @@ def oracle_results(inst_set):
     return [oracle_solve(n, cp, lits, x0, 1500) for n, cp, lits, x0 in inst_set]
 
 
-def test_batch_matches_oracle(ctx, inst_set, oracle_results):
-    out = run_batch(ctx, inst_set, 1500, spl=37)
+@pytest.mark.parametrize("many", [False, True, "defaults"])
+def test_batch_matches_oracle(ctx, inst_set, oracle_results, many):
+    out = run_batch(ctx, inst_set, 1500, spl=37, many=many)
     modes = set()
     for r, (steps, t, status, xo), (n, cp, lits, x0) in zip(out, oracle_results, inst_set):
         modes.add(r["mode"])
@@ -242,10 +253,9 @@ def test_full_mixed_workload(ctx):
         s = 4096 * 100003 + i
         inst.append((n, planted_cp(m), dnx.generate_planted(s, n, m), dnx.init_uniform(s, n, m)))
     b = dnx.Batch(ctx, slots=count)
-    slots = []
-    for n, cp, lits, x0 in inst:
-        hp = dnx.host_problem(n, cp, lits)
-        slots.append(b.admit(hp, dnx.params(len(cp) - 1), x0, cap=100000))
+    hps = [dnx.host_problem(n, cp, lits) for n, cp, lits, x0 in inst]
+    slots = b.admit_many(hps, [x0 for n, cp, lits, x0 in inst], cap=100000)
+    for hp in hps:
         hp.free()
     b.run(steps_per_launch=2048)
     res = b.poll()
@@ -328,15 +338,18 @@ def test_single_instance_path_agrees(ctx, inst_set):
             assert np.array_equal(s["assign"], r["assign"])
 
 
-def test_large_instance_admission(ctx):
-    """An instance whose arena range (~20 MB) exceeds the admission staging
-    buffer's first size, admitted between two small ones: the staging grows,
-    and each instance's fixed-step trajectory equals the single-instance FP32
-    path's, bit for bit (the large one runs with its state in device memory)."""
+@pytest.mark.parametrize("many", [False, True])
+def test_large_instance_admission(ctx, many):
+    """Instances of ~20 MB and ~45 MB arena ranges between small ones (the
+    larger's host-built part, ~22 MB, exceeds the 16 MB admission staging
+    chunk, which then grows; admitted together, the set spans several
+    chunks): each instance's fixed-step trajectory equals the single-instance
+    FP32 path's, bit for bit (the large ones run with their state in device
+    memory)."""
     inst = []
-    for i, (n, m) in enumerate([(300, 1350), (60000, 258000), (500, 2300)]):
+    for i, (n, m) in enumerate([(300, 1350), (60000, 258000), (500, 2300), (150000, 640000), (700, 3100)]):
         inst.append((n, planted_cp(m), dnx.generate_planted(900 + i, n, m), dnx.init_uniform(900 + i, n, m)))
-    out = run_batch(ctx, inst, 40, spl=20, stop=False)
+    out = run_batch(ctx, inst, 40, spl=20, stop=False, many=many)
     for r, (n, cp, lits, x0) in zip(out, inst):
         gp = ctx.problem_from_lits(n, cp, lits)
         st = gp.state(dnx.params(len(cp) - 1), x0, dnx.FP32)
@@ -380,6 +393,49 @@ def test_retire_readmit_and_budget(ctx):
     assert e.value.code == dnx.E_BUDGET  # larger than the arena
     hp.free()
     b.close()
+
+
+def test_admit_many_all_or_none(ctx):
+    """dnx_batch_admit_many admits all or nothing: with too few free slots or
+    too little arena for the whole set it fails with DNX_E_BUDGET and leaves
+    the slots and memory as they were; the set then fits after a retire."""
+    inst = make_set(21, 6, nmin=200, nmax=1500)
+    hps = [dnx.host_problem(n, cp, lits) for n, cp, lits, x0 in inst]
+    x0s = [x0 for n, cp, lits, x0 in inst]
+    b = dnx.Batch(ctx, slots=5, arena_bytes=64 << 20)
+    try:
+        arena, free0 = b.memory()
+        with pytest.raises(dnx.DnxError) as e:
+            b.admit_many(hps, x0s, cap=1500)
+        assert e.value.code == dnx.E_BUDGET  # 6 instances, 5 slots
+        assert b.memory()[1] == free0
+        first = b.admit_many(hps[:1], x0s[:1], cap=1500)
+        with pytest.raises(dnx.DnxError) as e:
+            b.admit_many(hps[1:], x0s[1:], cap=1500)
+        assert e.value.code == dnx.E_BUDGET  # 5 more, 4 free slots
+        b.retire(first[0])
+        assert b.memory()[1] == free0
+        slots = b.admit_many(hps[1:], x0s[1:], cap=1500)
+        assert len(set(slots)) == 5
+        b.run(64)
+        res = b.poll()
+        for (n, cp, lits, x0), s in zip(inst[1:], slots):
+            steps, t, status, xo = oracle_solve(n, cp, lits, x0, 1500)
+            assert res["steps"][s] == steps and res["t"][s] == t
+            assert same_bits(b.read_state(s, n, len(cp) - 1)["x"], xo)
+    finally:
+        b.close()
+    small = dnx.Batch(ctx, slots=8, arena_bytes=2 << 20)
+    try:
+        free1 = small.memory()[1]
+        with pytest.raises(dnx.DnxError) as e:
+            small.admit_many(hps, x0s, cap=1500)
+        assert e.value.code == dnx.E_BUDGET  # the set needs more than 2 MiB
+        assert small.memory()[1] == free1
+    finally:
+        small.close()
+        for hp in hps:
+            hp.free()
 
 
 def check_served(r, i, steps, t, status, xo, n):
