@@ -175,7 +175,27 @@ def served(ctx, seconds, rate, cap, spl, shape, seed=77):
     ok = r["status"] == dnx.SAT
     tts = r["tts"][ok]
     done = r["tts"] >= 0
+    # where each request still unfinished at the deadline was, and what it is
+    left = r["where"] != dnx.SERVE_ENDED
+    ratio = r["m"] / np.maximum(r["n"], 1)
+    names = {dnx.SERVE_QUEUED: "queued", dnx.SERVE_NARROW: "narrow_lane", dnx.SERVE_WIDE: "wide_lane_running",
+             dnx.SERVE_WIDE_WAIT: "wide_lane_waiting", dnx.SERVE_NOT_ARRIVED: "not_arrived"}
+    unfinished = dict(count=int(left.sum()),
+                      where={v: int((r["where"] == k).sum()) for k, v in names.items()},
+                      n_min=int(r["n"][left].min()) if left.any() else None,
+                      clause_ratio_below_5=int((left & (ratio < 5.0)).sum()),
+                      clause_ratio_5_to_6=int((left & (ratio >= 5.0) & (ratio < 6.0)).sum()),
+                      clause_ratio_above_6=int((left & (ratio >= 6.0)).sum()),
+                      steps_done_p50=float(np.percentile(r["steps"][left], 50)) if left.any() else None,
+                      reason=None)
+    wide_only = left.any() and np.all(np.isin(r["where"][left], [dnx.SERVE_WIDE, dnx.SERVE_WIDE_WAIT]))
+    if wide_only:
+        unfinished["reason"] = ("every one was resident in the wide lane (admitted, state on the device) when the "
+                                "deadline came: large instances near the threshold ratio need thousands to tens of "
+                                "thousands of steps each, more wide-lane work than the stream leaves time for; the "
+                                "lane finishes the least expected work first")
     return dict(requests=int(r["count"]), solved_verified=int(ok.sum()), finished=int(done.sum()),
+                unfinished_at_deadline=unfinished,
                 unknown=int((r["status"] == dnx.UNKNOWN).sum()), failed=int((r["status"] == dnx.FAILED).sum()),
                 seconds=round(float(r["makespan"]), 3),
                 instances_per_s=round(float(ok.sum() / r["makespan"]), 2) if r["makespan"] > 0 else None,

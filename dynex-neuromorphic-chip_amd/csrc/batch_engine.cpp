@@ -162,6 +162,10 @@ struct dnx_batch {
     long long serve_big = 60000;
     // serving: clause budget of the wide lane's active set per chunk
     long long serve_active_clauses = 6000000;
+    // serving: wide-lane members below this many steps are scheduled by
+    // least attained service, the others by least expected work (0: all by
+    // least attained service)
+    long long serve_las_steps = 1024;
     // portfolio: dnx_batch_run returns at the first status sync at which
     // any slot is satisfied
     bool group_stop = false;
@@ -1659,6 +1663,13 @@ int dnx_batch_set_serve_big(dnx_batch *b, int64_t clauses, int64_t active_clause
     return DNX_OK;
 }
 
+int dnx_batch_set_serve_order(dnx_batch *b, int64_t las_steps)
+{
+    if (!b || las_steps < 0) return DNX_E_ARG;
+    b->serve_las_steps = las_steps;
+    return DNX_OK;
+}
+
 int dnx_batch_timing(dnx_batch *b, int enable, double *ms, int64_t *launches, double *alg_bytes)
 {
     if (!b) return DNX_E_ARG;
@@ -1710,6 +1721,7 @@ struct ServeOut {
     double *t = nullptr;
     int8_t *const *assign = nullptr;
     double *const *x = nullptr;
+    int *where = nullptr;  // optional: DNX_SERVE_* per request
 };
 
 // The continuous-batching loop.  Requests become visible at their arrival
@@ -1840,6 +1852,7 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
             if (bad != 0) stt = DNX_FAILED;
         }
         out.status[r] = stt;
+        if (out.where) out.where[r] = DNX_SERVE_ENDED;
         b->arena.release(sl.off, sl.size);
         sl = SlotHost();
         b->free_slots.push_back(s);
@@ -1854,18 +1867,26 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
         out.steps[i] = 0;
         out.tts[i] = -1.0;
         out.wait[i] = -1.0;
+        if (out.where) out.where[i] = DNX_SERVE_NOT_ARRIVED;
     }
     while (finished < count) {
         bool progressed = false;
         const double now = now_s();
         if (deadline > 0 && now > deadline) {
             // requests still queued, running or not yet arrived stay
-            // DNX_UNKNOWN with no time-to-solution
+            // DNX_UNKNOWN with no time-to-solution; where they were and the
+            // steps they had completed are reported
             if (wst) (void)hipStreamSynchronize(wst);
             (void)hipStreamSynchronize(st);
             for (int s = 0; s < b->nslots; s++) {
                 if (slot_req[(size_t)s] < 0) continue;
                 const int r = slot_req[(size_t)s];
+                out.steps[r] = attained[(size_t)s];
+                if (out.where)
+                    out.where[r] = !slot_wide[(size_t)s] ? DNX_SERVE_NARROW
+                                   : std::binary_search(wide_active.begin(), wide_active.end(), s)
+                                       ? DNX_SERVE_WIDE
+                                       : DNX_SERVE_WIDE_WAIT;
                 SlotHost &sl = b->slots[(size_t)s];
                 b->arena.release(sl.off, sl.size);
                 sl = SlotHost();
@@ -1873,11 +1894,17 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
                 slot_req[(size_t)s] = -1;
                 release(r);
             }
-            for (int r : queue) release(r);
+            for (int r : queue) {
+                if (out.where) out.where[r] = DNX_SERVE_QUEUED;
+                release(r);
+            }
             for (int r = next; r < count; r++) release(r);
             break;
         }
-        while (next < count && arrival[next] <= now) queue.push_back(next++);
+        while (next < count && arrival[next] <= now) {
+            if (out.where) out.where[next] = DNX_SERVE_QUEUED;
+            queue.push_back(next++);
+        }
         // ---- admission -------------------------------------------------
         while (!queue.empty()) {
             const int r = queue.front();
@@ -1892,6 +1919,7 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
                     out.steps[r] = 0;
                     out.tts[r] = -1.0;
                     out.wait[r] = now_s() - arrival[r];
+                    if (out.where) out.where[r] = DNX_SERVE_ENDED;
                     release(r);
                     finished++;
                     continue;
@@ -1918,7 +1946,9 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
             narrow_busy = false;
             progressed = true;
             for (int s : narrow_launched) {
-                if (slot_req[(size_t)s] < 0 || hn[s].status == DNX_RUNNING) continue;
+                if (slot_req[(size_t)s] < 0) continue;
+                attained[(size_t)s] = hn[s].steps;
+                if (hn[s].status == DNX_RUNNING) continue;
                 if ((rc = finish(s, hn[s], st))) return rc;
                 narrow_run.erase(std::find(narrow_run.begin(), narrow_run.end(), s));
             }
@@ -1946,13 +1976,27 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
             }
         }
         if (!wide_busy && !wide_set.empty()) {
-            // least attained service: the members with the fewest steps so
-            // far run, up to the lane's clause budget; the others keep
-            // their state and wait.  New arrivals start at once, and under
-            // overload the longest-running instances yield.
+            // Two classes, then the lane's clause budget in that order; the
+            // members left out keep their state and wait.  Members below
+            // serve_las_steps steps: least attained service (new arrivals
+            // start at once).  Members past it: least expected work first,
+            // m * exp(-1.5 m/n) -- random 3-SAT step counts fall about 4.5x
+            // per unit of clause ratio above the threshold (~4.27) -- so that
+            // under overload the lane completes what it can instead of
+            // advancing every long run equally.
             std::vector<int> active = wide_set;
-            std::stable_sort(active.begin(), active.end(),
-                             [&](int x, int y) { return attained[(size_t)x] < attained[(size_t)y]; });
+            const long long T = b->serve_las_steps;
+            auto fg = [&](int x) { return T <= 0 || attained[(size_t)x] < T; };
+            auto work = [&](int x) {
+                const SlotHost &sl = b->slots[(size_t)x];
+                return (double)sl.m * std::exp(-1.5 * (double)sl.m / (double)std::max(sl.n, 1));
+            };
+            std::stable_sort(active.begin(), active.end(), [&](int x, int y) {
+                const bool fx = fg(x), fy = fg(y);
+                if (fx != fy) return fx;
+                if (fx) return attained[(size_t)x] < attained[(size_t)y];
+                return work(x) < work(y);
+            });
             long long budget = 0;
             size_t keep = 0;
             while (keep < active.size() &&
@@ -2041,7 +2085,7 @@ extern "C" {
 int dnx_serve_trace(dnx_batch *b, int count, dnx_problem *const *probs, const dnx_params *prms,
                     const double *const *x0s, const double *arrival_s, int64_t step_cap, int steps_per_launch,
                     double deadline_s, int *status, int64_t *steps, double *tts_s, double *wait_s, double *makespan_s,
-                    double *t_out, int8_t *const *assign_out, double *const *x_out)
+                    double *t_out, int8_t *const *assign_out, double *const *x_out, int *where_out)
 {
     if (!b || count < 0 || (count && (!probs || !x0s || !arrival_s || !status || !steps || !tts_s || !wait_s)) ||
         steps_per_launch <= 0)
@@ -2065,6 +2109,7 @@ int dnx_serve_trace(dnx_batch *b, int count, dnx_problem *const *probs, const dn
     out.t = t_out;
     out.assign = assign_out;
     out.x = x_out;
+    out.where = where_out;
     return serve_loop(b, count, arrival_s, step_cap, steps_per_launch, deadline_s, get, [](int) {}, out, makespan_s);
 }
 
@@ -2076,7 +2121,7 @@ int dnx_serve_planted(dnx_batch *b, uint64_t seed, double rate_per_s, double dur
                       double r_min, double r_max, double p1, double p2, int64_t step_cap, int steps_per_launch,
                       double deadline_s, int max_requests, int gen_threads, int *count_out, double *arrival_out,
                       int *n_out, int *m_out, int *status, int64_t *steps, double *tts_s, double *wait_s,
-                      double *makespan_s)
+                      double *makespan_s, int *where_out)
 {
     if (!b || rate_per_s <= 0 || duration_s <= 0 || n_min < 3 || n_max < n_min || r_min <= 0 || r_max < r_min ||
         max_requests <= 0 || !count_out || !arrival_out || !n_out || !m_out || !status || !steps || !tts_s ||
@@ -2161,6 +2206,7 @@ int dnx_serve_planted(dnx_batch *b, uint64_t seed, double rate_per_s, double dur
         released++;
     };
     ServeOut out{status, steps, tts_s, wait_s};
+    out.where = where_out;
     int rc = serve_loop(b, count, arrival_out, step_cap, steps_per_launch, deadline_s, get, release, out, makespan_s);
     stop = true;
     for (auto &x : th) x.join();

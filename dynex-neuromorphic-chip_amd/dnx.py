@@ -17,6 +17,7 @@ OK, E_ARG, E_IO, E_PARSE, E_TOOLONG, E_HIP, E_NOMEM, E_BUDGET, E_STATE = 0, -1, 
 EXCHANGE_LITERAL, EXCHANGE_GROUPED, EXCHANGE_SLICED = 0, 1, 2
 FP64, FP32 = 0, 1
 RUNNING, SAT, UNKNOWN, FAILED = 0, 1, 2, 3
+SERVE_ENDED, SERVE_QUEUED, SERVE_NARROW, SERVE_WIDE, SERVE_WIDE_WAIT, SERVE_NOT_ARRIVED = 0, 1, 2, 3, 4, 5
 
 I32P = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 I8P = np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")
@@ -83,6 +84,7 @@ def lib():
     L.dnx_batch_set_shape.argtypes = [vp, ip, ip, ip]
     L.dnx_batch_admit.argtypes = [vp, vp, C.POINTER(Params), F64P, C.c_int64, ip, C.POINTER(C.c_int)]
     L.dnx_batch_admit_many.argtypes = [vp, ip, pp, vp, pp, C.c_int64, ip, I32P]
+    L.dnx_batch_set_serve_order.argtypes = [vp, C.c_int64]
     L.dnx_batch_advance.argtypes = [vp, ip]
     L.dnx_batch_poll.argtypes = [vp, vp, vp, vp, vp]
     L.dnx_batch_run.argtypes = [vp, ip, dp, C.POINTER(C.c_int)]
@@ -104,12 +106,12 @@ def lib():
     L.dnx_batch_get_shape.argtypes = [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)]
     L.dnx_state_time.argtypes = [vp, ip, C.POINTER(C.c_double)]
     L.dnx_serve_trace.argtypes = [vp, ip, vp, vp, vp, F64P, C.c_int64, ip, dp, vp, vp, vp, vp,
-                                  C.POINTER(C.c_double), vp, vp, vp]
+                                  C.POINTER(C.c_double), vp, vp, vp, vp]
     L.dnx_state_track_loc.argtypes = [vp, ip]
     L.dnx_problem_set_exchange.argtypes = [vp, ip]
     L.dnx_state_best_loc.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.dnx_serve_planted.argtypes = [vp, C.c_uint64, dp, dp, ip, ip, dp, dp, dp, dp, C.c_int64, ip, dp, ip, ip,
-                                    C.POINTER(C.c_int), vp, vp, vp, vp, vp, vp, vp, C.POINTER(C.c_double)]
+                                    C.POINTER(C.c_int), vp, vp, vp, vp, vp, vp, vp, C.POINTER(C.c_double), vp]
     _LIB = L
     return L
 
@@ -441,6 +443,9 @@ class Batch:
     def set_serve_big(self, clauses, active_clauses=6000000):
         check(lib().dnx_batch_set_serve_big(self.h, int(clauses), int(active_clauses)))
 
+    def set_serve_order(self, las_steps):
+        check(lib().dnx_batch_set_serve_order(self.h, int(las_steps)))
+
     def set_wide(self, max_clauses, steps_per_sync=300):
         check(lib().dnx_batch_set_wide(self.h, int(max_clauses), int(steps_per_sync)))
 
@@ -485,6 +490,7 @@ class Batch:
         tts = np.zeros(max(k, 1), np.float64)
         wait = np.zeros(max(k, 1), np.float64)
         tt = np.zeros(max(k, 1), np.float64)
+        wh = np.zeros(max(k, 1), np.int32)
         mk = C.c_double()
         asg = [np.zeros(max(p.n, 1), np.int8) for p in probs] if results else []
         xo = [np.zeros(max(p.n + 2 * p.m, 1), np.float64) for p in probs] if results else []
@@ -493,8 +499,9 @@ class Batch:
         check(lib().dnx_serve_trace(self.h, k, ph, None, xp, arr if k else np.zeros(1), int(cap), int(steps_per_launch),
                                     float(deadline), st.ctypes.data_as(C.c_void_p), steps.ctypes.data_as(C.c_void_p),
                                     tts.ctypes.data_as(C.c_void_p), wait.ctypes.data_as(C.c_void_p), C.byref(mk),
-                                    tt.ctypes.data_as(C.c_void_p), ap, xop))
-        out = dict(status=st[:k], steps=steps[:k], tts=tts[:k], wait=wait[:k], makespan=mk.value, t=tt[:k])
+                                    tt.ctypes.data_as(C.c_void_p), ap, xop, wh.ctypes.data_as(C.c_void_p)))
+        out = dict(status=st[:k], steps=steps[:k], tts=tts[:k], wait=wait[:k], makespan=mk.value, t=tt[:k],
+                   where=wh[:k])
         if results:
             out["assign"] = [a[:p.n] for a, p in zip(asg, probs)]
             out["x"] = [x[:p.n + 2 * p.m] for x, p in zip(xo, probs)]
@@ -511,13 +518,14 @@ class Batch:
         steps = np.zeros(M, np.int64)
         tts = np.zeros(M, np.float64)
         wait = np.zeros(M, np.float64)
+        wh = np.zeros(M, np.int32)
         mk = C.c_double()
         ptr = lambda a: a.ctypes.data_as(C.c_void_p)
         check(lib().dnx_serve_planted(self.h, int(seed), float(rate), float(duration), int(n_min), int(n_max),
                                       float(r_min), float(r_max), float(p1), float(p2), int(cap),
                                       int(steps_per_launch), float(deadline), M, int(gen_threads), C.byref(cnt),
                                       ptr(arr), ptr(ns),
-                                      ptr(ms), ptr(st), ptr(steps), ptr(tts), ptr(wait), C.byref(mk)))
+                                      ptr(ms), ptr(st), ptr(steps), ptr(tts), ptr(wait), C.byref(mk), ptr(wh)))
         k = cnt.value
         return dict(count=k, arrival=arr[:k], n=ns[:k], m=ms[:k], status=st[:k], steps=steps[:k], tts=tts[:k],
-                    wait=wait[:k], makespan=mk.value)
+                    wait=wait[:k], makespan=mk.value, where=wh[:k])

@@ -322,11 +322,18 @@ int dnx_batch_set_wide_fraction(dnx_batch *b, double fraction, int steps_per_syn
 
 /* Serving: requests with at least `clauses` clauses run in the wide lane
  * (default 60,000; needs wide mode enabled).  Each wide chunk (30 steps,
- * then a status sync) advances the members with the fewest steps so far, up
- * to active_clauses clauses in total (default 6,000,000):
- * least-attained-service order, so new requests start at once and, under
- * overload, the longest-running ones wait. */
+ * then a status sync) advances members in the order dnx_batch_set_serve_order
+ * sets, up to active_clauses clauses in total (default 6,000,000). */
 int dnx_batch_set_serve_big(dnx_batch *b, int64_t clauses, int64_t active_clauses);
+
+/* Serving: order of the wide lane's members for its active set.  Members
+ * that have run fewer than las_steps steps come first, least attained
+ * service among them; the rest follow by least expected work, m times
+ * exp(-1.5 m/n) (random 3-SAT needs fewer steps the further its clause
+ * ratio lies above the threshold), so an overloaded lane finishes what it
+ * can instead of advancing every long run equally.  las_steps = 0 orders
+ * every member by least attained service.  Default 1024. */
+int dnx_batch_set_serve_order(dnx_batch *b, int64_t las_steps);
 
 /* Autotuner: times candidate execution shapes on `sample` synthetic planted
  * instances (n log-uniform in [n_min, n_max], clause ratio uniform in
@@ -388,7 +395,17 @@ int dnx_serve_trace(dnx_batch *b, int count, dnx_problem *const *probs, const dn
                     const double *const *x0s, const double *arrival_s, int64_t step_cap,
                     int steps_per_launch, double deadline_s, int *status, int64_t *steps,
                     double *tts_s, double *wait_s, double *makespan_s, double *t_out,
-                    int8_t *const *assign_out, double *const *x_out);
+                    int8_t *const *assign_out, double *const *x_out, int *where_out);
+
+/* where_out[i] (optional, both serving calls): where request i was when the
+ * loop ended.  For a request still DNX_UNKNOWN at the deadline, steps[i] is
+ * then the steps it had completed by the last status the loop read. */
+#define DNX_SERVE_ENDED 0       /* retired: SAT, step cap, or DNX_FAILED */
+#define DNX_SERVE_QUEUED 1      /* arrived, never admitted (no slot or arena range free) */
+#define DNX_SERVE_NARROW 2      /* running in the fused-kernel lane */
+#define DNX_SERVE_WIDE 3        /* running in the wide lane, in its active set */
+#define DNX_SERVE_WIDE_WAIT 4   /* resident in the wide lane, waiting for its active set */
+#define DNX_SERVE_NOT_ARRIVED 5 /* arrival after the deadline */
 
 /* The served planted stream: Poisson arrivals at rate_per_s for duration_s
  * (at most max_requests), n log-uniform in [n_min, n_max], clause ratio
@@ -400,7 +417,8 @@ int dnx_serve_planted(dnx_batch *b, uint64_t seed, double rate_per_s, double dur
                       int n_max, double r_min, double r_max, double p1, double p2, int64_t step_cap,
                       int steps_per_launch, double deadline_s, int max_requests, int gen_threads,
                       int *count_out, double *arrival_out, int *n_out, int *m_out, int *status,
-                      int64_t *steps, double *tts_s, double *wait_s, double *makespan_s);
+                      int64_t *steps, double *tts_s, double *wait_s, double *makespan_s,
+                      int *where_out);
 
 /* ---- measurement ------------------------------------------------------ */
 /* Streams `bytes` (rounded down to a multiple of 16) from one device buffer

@@ -492,6 +492,41 @@ def test_serving_above_default_wide_threshold(ctx, inst_set, oracle_results):
     assert r["status"][6] == dnx.SAT
 
 
+def unsolvable(n, m, seed):
+    """A planted instance plus one empty clause: it never verifies, so it
+    runs until its step cap."""
+    lits = dnx.generate_planted(seed, n, m)
+    cp = np.concatenate([planted_cp(m), [3 * m]]).astype(np.int32)
+    return n, cp, lits, dnx.init_uniform(seed, n, m + 1)
+
+
+def test_serve_deadline_reports_where(ctx):
+    """Requests still unfinished at the deadline stay DNX_UNKNOWN with no
+    time-to-solution, and report where they were: three in the narrow lane,
+    two in the wide lane (a budget that holds one at a time, so one runs and
+    one waits), one never admitted (no slot), one arriving after the
+    deadline; the steps they had completed come back too."""
+    insts = [unsolvable(1500, 6000, 70 + i) for i in range(3)]
+    insts += [unsolvable(2500, 9000, 80 + i) for i in range(2)]
+    insts += [unsolvable(800, 3000, 90), unsolvable(800, 3000, 91)]
+    arrivals = np.array([0.0, 0.0, 0.0, 0.01, 0.01, 0.05, 5.0])
+    probs = [dnx.host_problem(n, cp, lits) for n, cp, lits, _ in insts]
+    b = dnx.Batch(ctx, slots=5)
+    b.set_serve_big(8000, 10000)
+    r = b.serve(probs, [x0 for *_, x0 in insts], arrivals, cap=10 ** 9, steps_per_launch=64, deadline=1.0)
+    b.close()
+    for p in probs:
+        p.free()
+    assert np.all(r["status"] == dnx.UNKNOWN) and np.all(r["tts"] == -1.0)
+    w = r["where"]
+    assert list(w[:3]) == [dnx.SERVE_NARROW] * 3
+    assert set(w[3:5]) <= {dnx.SERVE_WIDE, dnx.SERVE_WIDE_WAIT} and dnx.SERVE_WIDE in set(w[3:5])
+    assert w[5] == dnx.SERVE_QUEUED and w[6] == dnx.SERVE_NOT_ARRIVED
+    assert np.all(r["steps"][:3] > 0) and r["steps"][5] == 0 and r["steps"][6] == 0
+    assert r["steps"][3:5].max() > 0
+    print("where", w, "steps", r["steps"])
+
+
 def test_serve_planted_stream(ctx):
     b = dnx.Batch(ctx, slots=64)
     r = b.serve_planted(seed=5, rate=200.0, duration=0.5, n_min=100, n_max=3000, cap=20000, steps_per_launch=32,
@@ -501,6 +536,7 @@ def test_serve_planted_stream(ctx):
     assert np.all(np.diff(r["arrival"]) >= 0)
     assert np.all((r["status"] == dnx.SAT) | (r["status"] == dnx.UNKNOWN))
     assert np.all(r["tts"] > 0)
+    assert np.all(r["where"] == dnx.SERVE_ENDED)
     print("served", r["count"], "p50 tts ms", np.percentile(r["tts"], 50) * 1e3,
           "p99", np.percentile(r["tts"], 99) * 1e3)
 
