@@ -533,6 +533,67 @@ __device__ __forceinline__ T fold_var(EP e, IP eidx, int b, int deg, T *buf)
     return acc;
 }
 
+// Ascending left fold from +0 of s[0, deg) with deg <= NB (a power of two),
+// through registers: the values, +0 past deg, go through a bitonic sorting
+// network (every compare-exchange pair fixed at compile time), then are
+// summed in order.  This is dynex.cc:527-530's std::sort then accumulate:
+// equal values are the same bits except for signed zeros, and zeros -- the
+// pads included -- never change an ascending fold from +0 (the running sum
+// is never -0), so every ascending order gives the same bits.
+template <typename T, int NB>
+__device__ __forceinline__ T sorted_sum_net(const T *s, int deg)
+{
+    T r[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++) r[j] = j < deg ? s[j] : T(0.0);
+#pragma unroll
+    for (int k = 2; k <= NB; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < NB; i++) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const T lo = fmin(r[i], r[l]), hi = fmax(r[i], r[l]);
+                    if ((i & k) == 0) {
+                        r[i] = lo;
+                        r[l] = hi;
+                    } else {
+                        r[i] = hi;
+                        r[l] = lo;
+                    }
+                }
+            }
+    T acc = T(0.0);
+#pragma unroll
+    for (int j = 0; j < NB; j++) acc = acc + r[j];
+    return acc;
+}
+
+// Ascending fold of a lane's LDS segment s[0, deg): a sorting network sized
+// for the wave's largest segment (dmax, uniform across the wave), insertion
+// sort in place beyond 64.
+template <typename T>
+__device__ __forceinline__ T sorted_sum_lds(T *s, int deg, int dmax)
+{
+    if (dmax <= 8) return sorted_sum_net<T, 8>(s, deg);
+    if (dmax <= 16) return sorted_sum_net<T, 16>(s, deg);
+    if (dmax <= 32) return sorted_sum_net<T, 32>(s, deg);
+    if (dmax <= 64) return sorted_sum_net<T, 64>(s, deg);
+    for (int j = 1; j < deg; j++) {
+        const T x = s[j];
+        int k = j;
+        while (k > 0 && s[k - 1] > x) {
+            s[k] = s[k - 1];
+            k--;
+        }
+        s[k] = x;
+    }
+    T acc = T(0.0);
+    for (int j = 0; j < deg; j++) acc = acc + s[j];
+    return acc;
+}
+
 // Wave-cooperative fold: a wave owns 64 consecutive variables, whose
 // transpose slots are contiguous.  The wave stages all their contributions
 // in LDS (coalesced index loads, independent gathers, eight per lane in
@@ -568,24 +629,19 @@ __device__ __forceinline__ T wave_fold(EP e, IP eidx, IP row_ptr, int n, int v0,
     }
     __syncthreads();
     T acc = T(0.0);
+    const int rb = v < n ? row_ptr[v] : 0, re = v < n ? row_ptr[v + 1] : 0;
+    // the wave's largest segment, from every lane (before any lane branches)
+    const int dmax = SORT ? wave_max(re - rb) : 0;
     if (active && v < n) {
-        const int rb = row_ptr[v], re = row_ptr[v + 1];
         if (fits) {
             T *s = stage + (rb - sb);
             const int deg = re - rb;
             if (SORT) {
                 // dynex.cc:527-530: ascending sort, left fold from +0
-                for (int j = 1; j < deg; j++) {
-                    T x = s[j];
-                    int k = j;
-                    while (k > 0 && s[k - 1] > x) {
-                        s[k] = s[k - 1];
-                        k--;
-                    }
-                    s[k] = x;
-                }
+                acc = sorted_sum_lds<T>(s, deg, dmax);
+            } else {
+                for (int j = 0; j < deg; j++) acc = acc + s[j];
             }
-            for (int j = 0; j < deg; j++) acc = acc + s[j];
         } else if (SORT) {
             acc = fold_var<T, true, 64, 0>(e, eidx, rb, re - rb, nullptr);
         } else {

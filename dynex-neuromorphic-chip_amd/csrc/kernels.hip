@@ -502,26 +502,21 @@ __global__ __launch_bounds__(SX_F) void k_sx_var(int n, int vstride, const int32
     }
     __syncthreads();
     int f = 0;
+    // the wave's largest segment, from every lane (before any lane branches)
+    const int dmax = SORT ? wave_max(re - rb) : 0;
     if (v < n) {
         const T V = pend ? clampV(a.v + h * a.d) : a.v;
         T dV = T(0.0);
         if (!done) {
             T *sg = sv + (rb - s0);
             const int deg = re - rb;
+            T acc = T(0.0);
             if (SORT) {
                 // dynex.cc:527-530: ascending sort, left fold from +0
-                for (int j = 1; j < deg; j++) {
-                    const T x = sg[j];
-                    int k = j;
-                    while (k > 0 && sg[k - 1] > x) {
-                        sg[k] = sg[k - 1];
-                        k--;
-                    }
-                    sg[k] = x;
-                }
+                acc = sorted_sum_lds<T>(sg, deg, dmax);
+            } else {
+                for (int j = 0; j < deg; j++) acc = acc + sg[j];
             }
-            T acc = T(0.0);
-            for (int j = 0; j < deg; j++) acc = acc + sg[j];
             dV = acc;
             if (unit && unit[v] != 0) dV = T(0.0);  // dynex.cc:306
             f = fail_level(V, dV);
