@@ -383,7 +383,10 @@ def main():
     ap.add_argument("--profile-pass", action="store_true",
                     help="only the copy-kernel calibration and one batched solve (for rocprofv3 runs)")
     ap.add_argument("--profile-single", action="store_true",
-                    help="only the copy-kernel calibration and 300 FP32 steps of the n=100k instance")
+                    help="only the copy-kernel calibration and the steps of the n=100k instance (300 in FP32, 90 "
+                         "in FP64)")
+    ap.add_argument("--profile-prec", type=int, choices=(32, 64), default=32,
+                    help="precision of --profile-single")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only (--gpus 1); a batch would shard by instance across GPUs, "
@@ -394,8 +397,10 @@ def main():
         ctx = dnx.Context(0)
         copy_bytes = 256 << 20
         ms = ctx.device_copy(copy_bytes, repeats=4)
-        r = single_instance(ctx, 100000, 800000, dnx.FP32, 300)
-        print(json.dumps(dict(profile_single=True, copy_bytes=copy_bytes, copy_ms=ms, **r)), flush=True)
+        prec = dnx.FP64 if args.profile_prec == 64 else dnx.FP32
+        r = single_instance(ctx, 100000, 800000, prec, 300 if prec == dnx.FP32 else 90)
+        print(json.dumps(dict(profile_single=True, precision=args.profile_prec, copy_bytes=copy_bytes, copy_ms=ms,
+                              **r)), flush=True)
         ctx.close()
         return
     t0 = time.perf_counter()
