@@ -34,8 +34,8 @@ HBM_PEAK_GBPS = 8000.0
 TRAFFIC = {0: 771.55e9}
 # Bytes past L2 per step of the n=100k FP32 single-instance step with the sliced exchange, from two
 # rocprofv3 --pmc runs of `bench.py --profile-single` (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, calibrated on
-# the copy kernel): k_sx_clause 39.4 MB read + 21.3 MB written, k_sx_var 15.9 MB read + 2.4 MB written.
-SINGLE_TRAFFIC = dict(total=79.0e6, k_sx_clause=60.7e6, k_sx_var=18.3e6, vs_algorithmic=round(79.0 / 23.2, 2))
+# the copy kernel): k_sx_clause 24.1 MB read + 20.5 MB written, k_sx_var 15.9 MB read + 2.4 MB written.
+SINGLE_TRAFFIC = dict(total=62.9e6, k_sx_clause=44.6e6, k_sx_var=18.3e6, vs_algorithmic=round(62.9 / 23.2, 2))
 METRIC = "solved instances/s + p50/p99 time-to-solution (batched FP32 vs plain FP64, 1 GPU)"
 
 

@@ -297,8 +297,13 @@ __global__ __launch_bounds__(SXShape<T>::BS) void k_sx_clause(int n, int vstride
         T xs, xl, cp;
         int2 run;
     };
+    // indices past the unit's last clause are clamped to it (not to the
+    // last clause of the problem): the lanes without a clause, and the
+    // look-ahead past the last sub-batch, then re-read the unit's own lines
+    // instead of pulling the next unit's, which another XCD reads anyway
+    const int c_last = min(max(c0, c1 - 1), m - 1);
     auto load_ops = [&](int cb, int r0, Ops &o) {
-        const int li = min(cb + tid, m - 1);
+        const int li = min(cb + tid, c_last);
         o.run = runs[r0 + min(tid, SX_MAXRUNS - 1)];
         o.w[0] = sw[3 * li];
         o.w[1] = sw[3 * li + 1];
