@@ -170,6 +170,7 @@ def served(ctx, seconds, rate, cap, spl, shape, seed=77):
         r = b.serve_planted(seed=seed, rate=rate, duration=seconds, n_min=100, n_max=100000, r_min=4.3, r_max=8.0,
                             cap=cap, steps_per_launch=spl, max_requests=int(rate * seconds * 2 + 100),
                             gen_threads=8, deadline=seconds + 30.0)
+        lanes = b.served_by()
     finally:
         b.close()
     ok = r["status"] == dnx.SAT
@@ -179,7 +180,8 @@ def served(ctx, seconds, rate, cap, spl, shape, seed=77):
     left = r["where"] != dnx.SERVE_ENDED
     ratio = r["m"] / np.maximum(r["n"], 1)
     names = {dnx.SERVE_QUEUED: "queued", dnx.SERVE_NARROW: "narrow_lane", dnx.SERVE_WIDE: "wide_lane_running",
-             dnx.SERVE_WIDE_WAIT: "wide_lane_waiting", dnx.SERVE_NOT_ARRIVED: "not_arrived"}
+             dnx.SERVE_WIDE_WAIT: "wide_lane_waiting", dnx.SERVE_NOT_ARRIVED: "not_arrived",
+             dnx.SERVE_SOLO: "solo_lane_running", dnx.SERVE_SOLO_WAIT: "solo_lane_waiting"}
     unfinished = dict(count=int(left.sum()),
                       where={v: int((r["where"] == k).sum()) for k, v in names.items()},
                       n_min=int(r["n"][left].min()) if left.any() else None,
@@ -188,14 +190,15 @@ def served(ctx, seconds, rate, cap, spl, shape, seed=77):
                       clause_ratio_above_6=int((left & (ratio >= 6.0)).sum()),
                       steps_done_p50=float(np.percentile(r["steps"][left], 50)) if left.any() else None,
                       reason=None)
-    wide_only = left.any() and np.all(np.isin(r["where"][left], [dnx.SERVE_WIDE, dnx.SERVE_WIDE_WAIT]))
-    if wide_only:
-        unfinished["reason"] = ("every one was resident in the wide lane (admitted, state on the device) when the "
+    big_only = left.any() and np.all(np.isin(r["where"][left], [dnx.SERVE_WIDE, dnx.SERVE_WIDE_WAIT, dnx.SERVE_SOLO,
+                                                                 dnx.SERVE_SOLO_WAIT]))
+    if big_only:
+        unfinished["reason"] = ("every one was a large request (n >= 10,000) in the wide or solo lane when the "
                                 "deadline came: large instances near the threshold ratio need thousands to tens of "
-                                "thousands of steps each, more wide-lane work than the stream leaves time for; the "
-                                "lane finishes the least expected work first")
+                                "thousands of steps each, more work than the stream leaves time for; the wide lane "
+                                "finishes the least expected work first, the solo lane runs requests in arrival order")
     return dict(requests=int(r["count"]), solved_verified=int(ok.sum()), finished=int(done.sum()),
-                unfinished_at_deadline=unfinished,
+                retired_by_lane=lanes, unfinished_at_deadline=unfinished,
                 unknown=int((r["status"] == dnx.UNKNOWN).sum()), failed=int((r["status"] == dnx.FAILED).sum()),
                 seconds=round(float(r["makespan"]), 3),
                 instances_per_s=round(float(ok.sum() / r["makespan"]), 2) if r["makespan"] > 0 else None,

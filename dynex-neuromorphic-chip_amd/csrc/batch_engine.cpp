@@ -21,6 +21,7 @@
 #include <functional>
 #include <limits>
 #include <map>
+#include <set>
 #include <memory>
 #include <mutex>
 #include <random>
@@ -29,6 +30,7 @@
 #include <vector>
 
 #include "api_internal.hpp"
+#include "solo.hpp"
 #include "batch.hpp"
 
 namespace dnx {
@@ -166,6 +168,15 @@ struct dnx_batch {
     // least attained service, the others by least expected work (0: all by
     // least attained service)
     long long serve_las_steps = 1024;
+    // serving: 3-SAT requests with at least this many clauses run alone on
+    // the whole device with the sliced exchange (the solo lane), at most
+    // serve_solo_active resident at a time (0 clauses, the default: no solo
+    // lane)
+    long long serve_solo = 0;
+    int serve_solo_active = 1024;
+    // requests the last serving call retired in each lane: narrow, wide,
+    // solo, and those that could never run (larger than the arena)
+    long long served_by[4] = {0, 0, 0, 0};
     // portfolio: dnx_batch_run returns at the first status sync at which
     // any slot is satisfied
     bool group_stop = false;
@@ -1663,6 +1674,21 @@ int dnx_batch_set_serve_big(dnx_batch *b, int64_t clauses, int64_t active_clause
     return DNX_OK;
 }
 
+int dnx_batch_set_serve_solo(dnx_batch *b, int64_t min_clauses, int max_active)
+{
+    if (!b || min_clauses < 0 || max_active < 1 || max_active > 4096) return DNX_E_ARG;
+    b->serve_solo = min_clauses;
+    b->serve_solo_active = max_active;
+    return DNX_OK;
+}
+
+int dnx_batch_served_by(dnx_batch *b, int64_t *counts)
+{
+    if (!b || !counts) return DNX_E_ARG;
+    for (int i = 0; i < 4; i++) counts[i] = b->served_by[i];
+    return DNX_OK;
+}
+
 int dnx_batch_set_serve_order(dnx_batch *b, int64_t las_steps)
 {
     if (!b || las_steps < 0) return DNX_E_ARG;
@@ -1724,6 +1750,305 @@ struct ServeOut {
     int *where = nullptr;  // optional: DNX_SERVE_* per request
 };
 
+// The serving loop's solo lane: requests of at least serve_solo clauses run
+// alone on the whole device with the single-instance path's sliced exchange
+// (solo.hpp), their tables and state in an arena range, a 30-step graph per
+// request replayed on the lane's own stream, two chunks in flight per
+// request (each followed by a copy of the control block's head and an
+// event).  Plans are built by planner threads from arrival on; a request
+// whose plan does not apply goes back to the other lanes.
+struct SoloLane {
+    static constexpr int CHUNK = 30;  // steps per graph (a multiple of 3)
+    struct Run {
+        int r = -1;
+        size_t off = 0, size = 0;
+        std::unique_ptr<SlicedPlan> pl;
+        SoloLayout L{};
+        DevProb P{};
+        DevParams prm{};
+        unsigned char *base = nullptr;
+        hipGraphExec_t graph = nullptr;
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        Ctl *head = nullptr;  // pinned: two control-block heads
+        int issued = 0, polled = 0;  // chunks issued / polled
+        long long attained = 0;
+        bool ended = false;
+    };
+    dnx_batch *b;
+    const std::function<ServeReq(int)> &get;
+    bool on = false;
+    hipStream_t st = nullptr;
+    hipStream_t cap_st = nullptr;  // idle stream the request graphs are captured on
+    std::vector<Run> runs;
+    std::deque<int> queue;  // routed requests, arrival order
+    // planner threads
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<int> jobs;
+    std::map<int, std::unique_ptr<SlicedPlan>> plans;  // done: null when the plan does not apply
+    std::set<int> failed;
+    bool stop = false;
+    std::vector<std::thread> planners;
+    // pinned staging of one admission, free again once `up_ev` has passed
+    unsigned char *stage = nullptr;
+    size_t stage_size = 0;
+    hipEvent_t up_ev = nullptr;
+    bool up_pending = false;
+
+    SoloLane(dnx_batch *bb, const std::function<ServeReq(int)> &g) : b(bb), get(g) {}
+    int init()
+    {
+        on = b->serve_solo > 0;
+        if (!on) return DNX_OK;
+        hipError_t e;
+        if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess ||
+            (e = hipStreamCreateWithFlags(&cap_st, hipStreamNonBlocking)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&up_ev, hipEventDisableTiming)) != hipSuccess)
+            return hip_fail(e, "solo lane setup");
+        runs.resize((size_t)b->serve_solo_active);
+        for (auto &R : runs) {
+            if ((e = hipEventCreateWithFlags(&R.ev[0], hipEventDisableTiming)) != hipSuccess ||
+                (e = hipEventCreateWithFlags(&R.ev[1], hipEventDisableTiming)) != hipSuccess ||
+                (e = hipHostMalloc((void **)&R.head, 2 * sizeof(Ctl), hipHostMallocDefault)) != hipSuccess)
+                return hip_fail(e, "solo lane setup");
+        }
+        const unsigned hc = std::thread::hardware_concurrency();
+        const int np = std::max(1, std::min(6, (int)(hc ? hc : 2) / 2));
+        for (int i = 0; i < np; i++) planners.emplace_back([this] { plan_loop(); });
+        return DNX_OK;
+    }
+    // plans held at once (ready or being built): a plan of an n=100,000
+    // request holds tens of MB of host memory
+    static constexpr int AHEAD = 8;
+    int building = 0;
+    void plan_loop()
+    {
+        for (;;) {
+            int r;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return stop || (!jobs.empty() && (int)plans.size() + building < AHEAD); });
+                if (stop) return;
+                r = jobs.front();
+                jobs.pop_front();
+                building++;
+            }
+            const ServeReq q = get(r);
+            auto pl = std::make_unique<SlicedPlan>();
+            const bool ok = solo_plan(*q.hp, b->cus, *pl);
+            std::lock_guard<std::mutex> lk(mu);
+            building--;
+            if (ok) {
+                plans[r] = std::move(pl);
+            } else {
+                failed.insert(r);
+                cv.notify_one();  // one fewer held
+            }
+        }
+    }
+    // planners must be done with every request before any is released
+    void stop_planners()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+        }
+        cv.notify_all();
+        for (auto &t : planners) t.join();
+        planners.clear();
+    }
+    bool wants(const HostProblem &h) const { return on && h.uniform3 && h.m >= b->serve_solo; }
+    void route(int r)
+    {
+        queue.push_back(r);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            jobs.push_back(r);
+        }
+        cv.notify_one();
+    }
+    int active() const
+    {
+        int k = 0;
+        for (auto &R : runs) k += R.r >= 0;
+        return k;
+    }
+    // whether the plan of the queue's first request is done: 1 ready, -1 does
+    // not apply, 0 not yet
+    int plan_state(int r, std::unique_ptr<SlicedPlan> &out)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (failed.count(r)) {
+            failed.erase(r);
+            return -1;
+        }
+        auto it = plans.find(r);
+        if (it == plans.end()) return 0;
+        out = std::move(it->second);
+        plans.erase(it);
+        cv.notify_one();  // room for one more plan
+        return 1;
+    }
+    // places request r (plan pl) in run slot R; DNX_E_BUDGET when the arena
+    // has no room for it now
+    int start(Run &R, int r, std::unique_ptr<SlicedPlan> &pl, int64_t cap)
+    {
+        const ServeReq q = get(r);
+        const HostProblem &h = *q.hp;
+        const SoloLayout L = solo_layout(h, *pl);
+        size_t off;
+        if (!b->arena.alloc(L.total, off)) return DNX_E_BUDGET;
+        hipError_t e;
+        if (up_pending) {
+            if ((e = hipEventSynchronize(up_ev)) != hipSuccess) return hip_fail(e, "solo staging wait");
+            up_pending = false;
+        }
+        if (stage_size < L.upload) {
+            if (stage) (void)hipHostFree(stage);
+            stage = nullptr;
+            stage_size = 0;
+            if ((e = hipHostMalloc((void **)&stage, L.upload, hipHostMallocDefault)) != hipSuccess) {
+                b->arena.release(off, L.total);
+                return hip_fail(e, "solo staging");
+            }
+            stage_size = L.upload;
+        }
+        solo_stage(h, *pl, q.x0, cap, L, stage);
+        unsigned char *base = (unsigned char *)b->arena_base + off;
+        R.r = r;
+        R.off = off;
+        R.size = L.total;
+        R.L = L;
+        R.base = base;
+        R.P = solo_devprob(h, *pl, L, base);
+        R.prm = dev_params(&q.prm);
+        R.pl = std::move(pl);
+        R.issued = R.polled = 0;
+        R.attained = 0;
+        R.ended = false;
+        // one 30-step graph for the request (slots rotate with the step),
+        // captured on a stream that never holds work
+        hipGraph_t g;
+        if ((e = hipStreamBeginCapture(cap_st, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+            return hip_fail(e, "solo capture");
+        for (int i = 0; i < CHUNK; i++)
+            launch_step_sliced(DNX_FP32, R.P, R.prm, base + L.vd, base + L.xs, base + L.xl, base + L.e,
+                               base + L.vnext, base + L.cst, (Ctl *)(base + L.ctl), i % 3, cap_st);
+        if ((e = hipStreamEndCapture(cap_st, &g)) != hipSuccess) return hip_fail(e, "solo capture end");
+        e = hipGraphInstantiate(&R.graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            R.graph = nullptr;
+            return hip_fail(e, "solo graph");
+        }
+        if ((e = hipMemcpyAsync(base, stage, L.upload, hipMemcpyHostToDevice, st)) != hipSuccess ||
+            (e = hipEventRecord(up_ev, st)) != hipSuccess)
+            return hip_fail(e, "solo upload");
+        up_pending = true;
+        launch_load_state(DNX_FP32, R.P, base + L.xstage, base + L.vd, base + L.xs, base + L.xl, st);
+        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "solo state load");
+        return DNX_OK;
+    }
+    // Keeps two chunks queued on the lane, each for the resident request
+    // that comes first: those below serve_las_steps steps by least attained
+    // service (new requests start at once), then by least expected work,
+    // m exp(-1.5 m/n), as in the wide lane -- so a near-threshold request
+    // that needs tens of thousands of steps does not hold the lane.
+    int inflight = 0;
+    int issue()
+    {
+        hipError_t e;
+        const long long T = b->serve_las_steps;
+        while (inflight < 2) {
+            Run *best = nullptr;
+            bool bfg = false;
+            double bkey = 0.0;
+            for (auto &R : runs) {
+                if (R.r < 0 || R.ended || R.issued - R.polled >= 2) continue;
+                const long long att = R.attained + (long long)CHUNK * (R.issued - R.polled);
+                const bool fg = T <= 0 || att < T;
+                const double key = fg ? (double)att
+                                      : (double)R.P.m * std::exp(-1.5 * (double)R.P.m / (double)std::max(R.P.n, 1));
+                if (!best || (fg && !bfg) || (fg == bfg && key < bkey)) {
+                    best = &R;
+                    bfg = fg;
+                    bkey = key;
+                }
+            }
+            if (!best) break;
+            Run &R = *best;
+            const int k = R.issued & 1;
+            if ((e = hipGraphLaunch(R.graph, st)) != hipSuccess ||
+                (e = hipMemcpyAsync(&R.head[k], R.base + R.L.ctl, solo_ctl_head(), hipMemcpyDeviceToHost, st)) !=
+                    hipSuccess ||
+                (e = hipEventRecord(R.ev[k], st)) != hipSuccess)
+                return hip_fail(e, "solo chunk");
+            R.issued++;
+            inflight++;
+        }
+        return DNX_OK;
+    }
+    static int status_of(const Ctl &c)
+    {
+        if (c.status != DNX_RUNNING) return c.status;
+        return c.done[c.steps % 3];
+    }
+    // polls run R: 1 when it has ended (its queued chunks drained), else 0
+    int poll(Run &R, int *rc)
+    {
+        *rc = DNX_OK;
+        while (R.polled < R.issued) {
+            const int k = R.polled & 1;
+            const hipError_t q = hipEventQuery(R.ev[k]);
+            if (q == hipErrorNotReady) return 0;
+            if (q != hipSuccess) {
+                *rc = hip_fail(q, "solo poll");
+                return 0;
+            }
+            R.polled++;
+            inflight--;
+            const Ctl &c = R.head[k];
+            R.attained = c.steps;
+            if (status_of(c) != DNX_RUNNING) {
+                R.ended = true;
+                // the later chunk does nothing on an ended run; let it pass
+                // before the range is read back and freed
+                if (R.polled < R.issued) {
+                    const hipError_t w = hipEventSynchronize(R.ev[(R.issued - 1) & 1]);
+                    if (w != hipSuccess) *rc = hip_fail(w, "solo drain");
+                    inflight -= R.issued - R.polled;
+                    R.polled = R.issued;
+                }
+                return 1;
+            }
+        }
+        return 0;
+    }
+    void release_run(Run &R)
+    {
+        if (R.graph) (void)hipGraphExecDestroy(R.graph);
+        R.graph = nullptr;
+        b->arena.release(R.off, R.size);
+        R.r = -1;
+        R.pl.reset();
+    }
+    ~SoloLane()
+    {
+        stop_planners();
+        if (st) (void)hipStreamSynchronize(st);
+        for (auto &R : runs) {
+            if (R.graph) (void)hipGraphExecDestroy(R.graph);
+            for (auto &ev : R.ev)
+                if (ev) (void)hipEventDestroy(ev);
+            if (R.head) (void)hipHostFree(R.head);
+        }
+        if (stage) (void)hipHostFree(stage);
+        if (up_ev) (void)hipEventDestroy(up_ev);
+        if (st) (void)hipStreamDestroy(st);
+        if (cap_st) (void)hipStreamDestroy(cap_st);
+    }
+};
+
 // The continuous-batching loop.  Requests become visible at their arrival
 // time and are admitted first come, first served as soon as a slot and an
 // arena range are free.  Two lanes advance what is resident, each on its own
@@ -1744,12 +2069,24 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
     using clk = std::chrono::steady_clock;
     const clk::time_point T0 = clk::now();
     auto now_s = [&]() { return std::chrono::duration<double>(clk::now() - T0).count(); };
+    // host time of a device clock reading: of three stamps, the one whose
+    // call returned soonest; the reading is taken near the call's end (a
+    // copy and a wait follow it), so the end is used: times to solution are
+    // then never early, at most a few tens of microseconds late
     uint64_t tick0 = 0;
-    double rate = 1e8;
-    const double hb = now_s();
-    int rc = dnx_batch_stamp(b, &tick0, &rate);
-    if (rc) return rc;
-    const double host0 = 0.5 * (hb + now_s());
+    double rate = 1e8, host0 = 0.0, best = 1e30;
+    int rc;
+    for (int k = 0; k < 3; k++) {
+        uint64_t tk = 0;
+        const double hb = now_s();
+        if ((rc = dnx_batch_stamp(b, &tk, &rate))) return rc;
+        const double he = now_s();
+        if (he - hb < best) {
+            best = he - hb;
+            tick0 = tk;
+            host0 = he;
+        }
+    }
     const long long big =
         b->wide_clauses > 0 || b->wide_frac > 0 ? std::max<long long>(b->serve_big, 1) : LLONG_MAX;
     hipStream_t st = b->ctx->stream;
@@ -1789,6 +2126,9 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
         (e = hipEventCreateWithFlags(&ev_adm, hipEventDisableTiming)) != hipSuccess)
         return hip_fail(e, "serve setup");
     BDyn *hn = b->h_dyn, *hw = b->h_dyn + b->nslots;  // lane snapshots
+    for (auto &c : b->served_by) c = 0;
+    SoloLane solo(b, get);
+    if ((rc = solo.init())) return rc;
     std::deque<int> queue;
     int next = 0, finished = 0;
     std::vector<int8_t> assign;
@@ -1853,10 +2193,51 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
         }
         out.status[r] = stt;
         if (out.where) out.where[r] = DNX_SERVE_ENDED;
+        b->served_by[slot_wide[(size_t)s] ? 1 : 0]++;
         b->arena.release(sl.off, sl.size);
         sl = SlotHost();
         b->free_slots.push_back(s);
         slot_req[(size_t)s] = -1;
+        release(r);
+        finished++;
+        return DNX_OK;
+    };
+    // ends a solo run: its final state read back on the lane's stream (the
+    // verified state when solved), the exact clause check against the
+    // request's own table, then its range is freed.  Time to solution is
+    // taken by the host clock when the lane sees the end.
+    auto finish_solo = [&](SoloLane::Run &R) -> int {
+        const int r = R.r;
+        const Ctl &c = R.head[(R.polled - 1) & 1];
+        int stt = SoloLane::status_of(c);
+        const HostProblem &hq = *get(r).hp;
+        const size_t n = (size_t)hq.n, N = n + 2 * (size_t)hq.m;
+        const bool want_x = out.x && out.x[r];
+        launch_materialize(DNX_FP32, R.P, R.prm, R.base + R.L.vd, R.base + R.L.xs, R.base + R.L.xl,
+                           (Ctl *)(R.base + R.L.ctl), R.base + R.L.xstage, solo.st);
+        xf.resize(N ? N : 1);
+        const size_t rd = want_x ? N : n;
+        hipError_t ee = hipGetLastError();
+        if (ee == hipSuccess && rd)
+            ee = hipMemcpyAsync(xf.data(), R.base + R.L.xstage, sizeof(float) * rd, hipMemcpyDeviceToHost, solo.st);
+        if (ee == hipSuccess) ee = hipStreamSynchronize(solo.st);
+        if (ee != hipSuccess) return hip_fail(ee, "solo state copy");
+        if (stt == DNX_SAT) {
+            std::vector<int8_t> a(n ? n : 1);
+            for (size_t v = 0; v < n; v++) a[v] = xf[v] >= 0.0f ? 1 : -1;
+            if (count_unsat(hq, a.data()) != 0) stt = DNX_FAILED;
+        }
+        out.status[r] = stt;
+        out.steps[r] = c.steps;
+        if (out.t) out.t[r] = c.t;
+        out.tts[r] = now_s() - arrival[r];
+        if (want_x)
+            for (size_t i = 0; i < N; i++) out.x[r][i] = (double)xf[i];
+        if (out.assign && out.assign[r])
+            for (size_t v = 0; v < n; v++) out.assign[r][v] = xf[v] >= 0.0f ? 1 : -1;
+        if (out.where) out.where[r] = DNX_SERVE_ENDED;
+        b->served_by[2]++;
+        solo.release_run(R);
         release(r);
         finished++;
         return DNX_OK;
@@ -1878,6 +2259,21 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
             // steps they had completed are reported
             if (wst) (void)hipStreamSynchronize(wst);
             (void)hipStreamSynchronize(st);
+            if (solo.st) (void)hipStreamSynchronize(solo.st);
+            solo.stop_planners();
+            for (auto &R : solo.runs) {
+                if (R.r < 0) continue;
+                out.steps[R.r] = R.attained;
+                if (out.where) out.where[R.r] = DNX_SERVE_SOLO;
+                const int r = R.r;
+                solo.release_run(R);
+                release(r);
+            }
+            for (int r : solo.queue) {
+                if (out.where) out.where[r] = DNX_SERVE_SOLO_WAIT;
+                release(r);
+            }
+            solo.queue.clear();
             for (int s = 0; s < b->nslots; s++) {
                 if (slot_req[(size_t)s] < 0) continue;
                 const int r = slot_req[(size_t)s];
@@ -1902,8 +2298,64 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
             break;
         }
         while (next < count && arrival[next] <= now) {
+            if (solo.on && solo.wants(*get(next).hp)) {
+                if (out.where) out.where[next] = DNX_SERVE_SOLO_WAIT;
+                solo.route(next++);
+                continue;
+            }
             if (out.where) out.where[next] = DNX_SERVE_QUEUED;
             queue.push_back(next++);
+        }
+        // ---- solo lane: start requests whose plans are ready, keep two
+        // chunks of each running one queued, retire the ones that ended
+        if (solo.on) {
+            for (auto &R : solo.runs) {
+                if (R.r >= 0 || solo.queue.empty()) continue;
+                const int r = solo.queue.front();
+                std::unique_ptr<SlicedPlan> pl;
+                const int ps = solo.plan_state(r, pl);
+                if (ps == 0) break;
+                solo.queue.pop_front();
+                progressed = true;
+                if (ps < 0) {  // the exchange does not apply: the other lanes take it
+                    if (out.where) out.where[r] = DNX_SERVE_QUEUED;
+                    queue.push_back(r);
+                    continue;
+                }
+                rc = solo.start(R, r, pl, cap);
+                if (rc == DNX_E_BUDGET) {
+                    rc = DNX_OK;
+                    if (narrow_run.empty() && wide_set.empty() && solo.active() == 0) {
+                        // larger than the whole arena: it can never run
+                        out.status[r] = DNX_FAILED;
+                        out.wait[r] = now_s() - arrival[r];
+                        if (out.where) out.where[r] = DNX_SERVE_ENDED;
+                        b->served_by[3]++;
+                        release(r);
+                        finished++;
+                        continue;
+                    }
+                    // no room now: back to the front with its plan kept
+                    solo.queue.push_front(r);
+                    std::lock_guard<std::mutex> lk(solo.mu);
+                    solo.plans[r] = std::move(pl);
+                    break;
+                }
+                if (rc) return rc;
+                out.wait[r] = now_s() - arrival[r];
+                if (out.where) out.where[r] = DNX_SERVE_SOLO;
+            }
+            for (auto &R : solo.runs) {
+                if (R.r < 0) continue;
+                int prc = DNX_OK;
+                const int endd = solo.poll(R, &prc);
+                if (prc) return prc;
+                if (endd) {
+                    progressed = true;
+                    if ((rc = finish_solo(R))) return rc;
+                }
+            }
+            if ((rc = solo.issue())) return rc;
         }
         // ---- admission -------------------------------------------------
         while (!queue.empty()) {
@@ -1912,7 +2364,7 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
             int slot = -1;
             rc = dnx_batch_admit_host(b, *q.hp, &q.prm, q.x0, cap, 1, &slot);
             if (rc == DNX_E_BUDGET) {
-                if (narrow_run.empty() && wide_set.empty()) {
+                if (narrow_run.empty() && wide_set.empty() && solo.active() == 0) {
                     // larger than the whole arena: it can never run
                     queue.pop_front();
                     out.status[r] = DNX_FAILED;
@@ -1920,6 +2372,7 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
                     out.tts[r] = -1.0;
                     out.wait[r] = now_s() - arrival[r];
                     if (out.where) out.where[r] = DNX_SERVE_ENDED;
+                    b->served_by[3]++;
                     release(r);
                     finished++;
                     continue;
@@ -2069,7 +2522,8 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
         if (!progressed) {
             // nothing completed: sleep a little, or until the next arrival
             double dt = 50e-6;
-            if (narrow_run.empty() && wide_set.empty() && next < count) dt = std::min(0.01, arrival[next] - now_s());
+            if (narrow_run.empty() && wide_set.empty() && solo.active() == 0 && solo.queue.empty() && next < count)
+                dt = std::min(0.01, arrival[next] - now_s());
             if (dt > 0) std::this_thread::sleep_for(std::chrono::duration<double>(dt));
         }
     }

@@ -112,10 +112,16 @@ bool sliced_tables(const HostProblem &p, int S, int SUB, int F, int cus, int max
         }
         bool operator==(const Key &o) const { return s[0] == o.s[0] && s[1] == o.s[1] && s[2] == o.s[2]; }
     };
+    // each variable's slice and offset in it (no divisions per literal)
+    std::vector<int32_t> vsl((size_t)n), voff((size_t)n);
+    for (int v = 0; v < n; v++) {
+        vsl[(size_t)v] = v / S;
+        voff[(size_t)v] = v % S;
+    }
     std::vector<Key> key((size_t)m);
     for (int c = 0; c < m; c++) {
         int a[3];
-        for (int j = 0; j < 3; j++) a[j] = (std::abs(p.lits[(size_t)3 * c + j]) - 1) / S;
+        for (int j = 0; j < 3; j++) a[j] = vsl[(size_t)(std::abs(p.lits[(size_t)3 * c + j]) - 1)];
         std::sort(a, a + 3);
         Key k{{a[0], -1, -1}};
         int ns = 1;
@@ -123,10 +129,21 @@ bool sliced_tables(const HostProblem &p, int S, int SUB, int F, int cus, int max
             if (a[j] != k.s[ns - 1]) k.s[ns++] = a[j];
         key[(size_t)c] = k;
     }
-    // clause sets by slice set, each in clause order; largest first
+    // clause sets by slice set, each in clause order; largest first.  The
+    // clauses are ordered by key, stably: a counting sort over the key's
+    // lexicographic rank when the key space is small, else a stable sort
     std::vector<int> ord((size_t)m);
-    for (int c = 0; c < m; c++) ord[(size_t)c] = c;
-    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return key[(size_t)x] < key[(size_t)y]; });
+    const int64_t R = (int64_t)NS + 1, space = R * R * R;
+    if (space <= ((int64_t)1 << 22)) {
+        auto rank = [&](const Key &k) { return ((int64_t)(k.s[0] + 1) * R + (k.s[1] + 1)) * R + (k.s[2] + 1); };
+        std::vector<int> start((size_t)space + 1, 0);
+        for (int c = 0; c < m; c++) start[(size_t)rank(key[(size_t)c]) + 1]++;
+        for (int64_t r = 0; r < space; r++) start[(size_t)r + 1] += start[(size_t)r];
+        for (int c = 0; c < m; c++) ord[(size_t)start[(size_t)rank(key[(size_t)c])]++] = c;
+    } else {
+        for (int c = 0; c < m; c++) ord[(size_t)c] = c;
+        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return key[(size_t)x] < key[(size_t)y]; });
+    }
     struct Set {
         Key k;
         int begin, size;
@@ -239,6 +256,7 @@ bool sliced_tables(const HostProblem &p, int S, int SUB, int F, int cus, int max
     std::vector<int64_t> fill((size_t)(n + F - 1) / F + 1);
     for (size_t b = 0; b + 1 < fill.size(); b++) fill[b] = p.row_ptr[std::min((size_t)n, b * (size_t)F)];
     std::vector<int> lit_blk((size_t)3 * SUB), cnt, order3((size_t)3 * SUB);
+    std::vector<int64_t> lit_q((size_t)3 * SUB);  // literal position of each staged literal
     for (size_t ui = 0; ui < U.size(); ui++) {
         const int *ud = &out.units[ui * 8];
         const int c0 = ud[0], c1 = ud[1], nsl = ud[2];
@@ -250,7 +268,9 @@ bool sliced_tables(const HostProblem &p, int S, int SUB, int F, int cus, int max
             int bmin = INT_MAX, bmax = -1;
             for (int i = 0; i < cntl; i++) {
                 const int c = porder[(size_t)cb + i / 3];
-                const int v = std::abs(p.lits[(size_t)3 * c + i % 3]) - 1;
+                const int64_t lq = (int64_t)3 * c + i % 3;
+                const int v = std::abs(p.lits[(size_t)lq]) - 1;
+                lit_q[(size_t)i] = lq;
                 lit_blk[(size_t)i] = v / F;
                 bmin = std::min(bmin, v / F);
                 bmax = std::max(bmax, v / F);
@@ -272,19 +292,18 @@ bool sliced_tables(const HostProblem &p, int S, int SUB, int F, int cus, int max
                 if (rid >= SX_MAXRUNS) return false;
                 out.runs.push_back((ls << 16) | len);
                 out.runs.push_back((int32_t)fill[(size_t)b]);
+                const int rb0 = p.row_ptr[(size_t)b * F];
                 for (int k = 0; k < len; k++) {
                     const int i = order3[(size_t)ls + k];
-                    const int c = porder[(size_t)cb + i / 3];
-                    const int64_t lq = (int64_t)3 * c + i % 3;  // literal position
-                    out.q[(size_t)fill[(size_t)b] + k] =
-                        (uint16_t)(p.lpos[(size_t)lq] - p.row_ptr[(size_t)b * F]);
+                    const int64_t lq = lit_q[(size_t)i];  // literal position
+                    out.q[(size_t)fill[(size_t)b] + k] = (uint16_t)(p.lpos[(size_t)lq] - rb0);
                     // the literal's word: sign, LDS index of its variable
                     // (slice index within the unit, offset in the slice),
                     // staging slot
                     const int v = std::abs(p.lits[(size_t)lq]) - 1;
                     int si = 0;
-                    while (si < nsl && ud[3 + si] != v / S) si++;
-                    const uint32_t lidx = (uint32_t)(si * S + v % S);
+                    while (si < nsl && ud[3 + si] != vsl[(size_t)v]) si++;
+                    const uint32_t lidx = (uint32_t)(si * S + voff[(size_t)v]);
                     out.gw[(size_t)3 * (cb + i / 3) + i % 3] = (p.lits[(size_t)lq] < 0 ? 0x80000000u : 0u) |
                                                              (lidx << 15) | ((uint32_t)rid << 7) | (uint32_t)k;
                 }

@@ -492,6 +492,38 @@ def test_serving_above_default_wide_threshold(ctx, inst_set, oracle_results):
     assert r["status"][6] == dnx.SAT
 
 
+def test_serving_solo_lane_equals_solo_runs(ctx, inst_set, oracle_results):
+    """3-SAT requests of at least 5,000 clauses run in the solo lane (alone on
+    the device, the single-instance path's sliced exchange) while the others
+    run in the narrow lane around them; every result equals the float oracle
+    (steps, time, final state bits, assignment), and the lane counts say
+    which lane retired each."""
+    insts = list(inst_set[:16])
+    probs = [dnx.host_problem(nn, cp, lits) for nn, cp, lits, _ in insts]
+    arrivals = np.sort(np.random.default_rng(8).uniform(0.0, 0.05, len(insts)))
+    b = dnx.Batch(ctx, slots=8)
+    b.set_serve_solo(5000, 2)
+    r = b.serve(probs, [x0 for *_, x0 in insts], arrivals, cap=1500, steps_per_launch=16, results=True)
+    lanes = b.served_by()
+    b.close()
+    for p in probs:
+        p.free()
+    for i, ((nn, *_), (steps, t, status, xo)) in enumerate(zip(insts, oracle_results[:16])):
+        check_served(r, i, steps, t, status, xo, nn)
+    expect = sum(1 for nn, cp, lits, _ in insts if len(cp) - 1 >= 5000 and len(lits) == 3 * (len(cp) - 1))
+    print("lanes", lanes, "expected solo", expect)
+    assert expect > 0 and lanes["solo"] == expect
+    assert lanes["narrow"] + lanes["solo"] == len(insts)
+
+
+def unsat3(n, m, seed):
+    """A planted 3-SAT instance made unsatisfiable by two 3-literal clauses,
+    (x1 x1 x1) and (-x1 -x1 -x1): every clause still has three literals."""
+    lits = dnx.generate_planted(seed, n, m - 2)
+    lits = np.concatenate([lits, [1, 1, 1, -1, -1, -1]]).astype(np.int32)
+    return n, planted_cp(m), lits, dnx.init_uniform(seed, n, m)
+
+
 def unsolvable(n, m, seed):
     """A planted instance plus one empty clause: it never verifies, so it
     runs until its step cap."""
@@ -505,14 +537,17 @@ def test_serve_deadline_reports_where(ctx):
     time-to-solution, and report where they were: three in the narrow lane,
     two in the wide lane (a budget that holds one at a time, so one runs and
     one waits), one never admitted (no slot), one arriving after the
-    deadline; the steps they had completed come back too."""
-    insts = [unsolvable(1500, 6000, 70 + i) for i in range(3)]
-    insts += [unsolvable(2500, 9000, 80 + i) for i in range(2)]
+    deadline, two in the solo lane (one at a time: one runs, one waits); the
+    steps they had completed come back too."""
+    # narrow x3, solo, wide x2, solo, queued, late (arrival order)
+    insts = [unsolvable(1500, 6000, 70 + i) for i in range(3)] + [unsat3(5000, 25000, 95)]
+    insts += [unsolvable(2500, 9000, 80 + i) for i in range(2)] + [unsat3(5000, 25000, 96)]
     insts += [unsolvable(800, 3000, 90), unsolvable(800, 3000, 91)]
-    arrivals = np.array([0.0, 0.0, 0.0, 0.01, 0.01, 0.05, 5.0])
+    arrivals = np.array([0.0, 0.0, 0.0, 0.0, 0.01, 0.01, 0.02, 0.05, 5.0])
     probs = [dnx.host_problem(n, cp, lits) for n, cp, lits, _ in insts]
     b = dnx.Batch(ctx, slots=5)
     b.set_serve_big(8000, 10000)
+    b.set_serve_solo(20000, 1)
     r = b.serve(probs, [x0 for *_, x0 in insts], arrivals, cap=10 ** 9, steps_per_launch=64, deadline=1.0)
     b.close()
     for p in probs:
@@ -520,10 +555,11 @@ def test_serve_deadline_reports_where(ctx):
     assert np.all(r["status"] == dnx.UNKNOWN) and np.all(r["tts"] == -1.0)
     w = r["where"]
     assert list(w[:3]) == [dnx.SERVE_NARROW] * 3
-    assert set(w[3:5]) <= {dnx.SERVE_WIDE, dnx.SERVE_WIDE_WAIT} and dnx.SERVE_WIDE in set(w[3:5])
-    assert w[5] == dnx.SERVE_QUEUED and w[6] == dnx.SERVE_NOT_ARRIVED
-    assert np.all(r["steps"][:3] > 0) and r["steps"][5] == 0 and r["steps"][6] == 0
-    assert r["steps"][3:5].max() > 0
+    assert set(w[4:6]) <= {dnx.SERVE_WIDE, dnx.SERVE_WIDE_WAIT} and dnx.SERVE_WIDE in set(w[4:6])
+    assert w[3] == dnx.SERVE_SOLO and w[6] == dnx.SERVE_SOLO_WAIT  # one solo run at a time
+    assert w[7] == dnx.SERVE_QUEUED and w[8] == dnx.SERVE_NOT_ARRIVED
+    assert np.all(r["steps"][:3] > 0) and r["steps"][7] == 0 and r["steps"][8] == 0
+    assert r["steps"][4:6].max() > 0 and r["steps"][3] > 0 and r["steps"][6] == 0
     print("where", w, "steps", r["steps"])
 
 
