@@ -1229,15 +1229,26 @@ __global__ __launch_bounds__(256) void k_batch_materialize(const BInst *__restri
     }
 }
 
+// Dynamic LDS above 64 KB must be allowed per kernel, and the setting is
+// per device: set[dev] keeps the largest size allowed on each device.
+constexpr int MAX_DEV_ATTR = 64;
+static void lds_attr(const void *fn, int lds, int (&set)[MAX_DEV_ATTR])
+{
+    if (lds <= 65536) return;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEV_ATTR) dev = 0;
+    if (lds > set[dev]) {
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        set[dev] = lds;
+    }
+}
+
 template <int MODE, int BS>
 static void launch_one(int lds, int grid, const BInst *insts, BDyn *dyn, const int *work, int nwork, int *head,
                        int K, BFlow *flow, hipStream_t st)
 {
-    static int lds_set = 0;
-    if (lds > 65536 && lds > lds_set) {
-        (void)hipFuncSetAttribute((const void *)k_batch<MODE, BS>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-        lds_set = lds;
-    }
+    static int lds_set[MAX_DEV_ATTR] = {};
+    lds_attr((const void *)k_batch<MODE, BS>, lds, lds_set);
     k_batch<MODE, BS><<<grid, BS, lds, st>>>(insts, dyn, work, nwork, head, K, flow);
 }
 

@@ -741,6 +741,20 @@ __global__ __launch_bounds__(256) void k_copy(const uint4 *__restrict__ src, uin
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+// Dynamic LDS above 64 KB must be allowed per kernel, and the setting is
+// per device: set[dev] keeps the largest size allowed on each device.
+constexpr int MAX_DEV_ATTR = 64;
+static void lds_attr(const void *fn, int lds, int (&set)[MAX_DEV_ATTR])
+{
+    if (lds <= 65536) return;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEV_ATTR) dev = 0;
+    if (lds > set[dev]) {
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        set[dev] = lds;
+    }
+}
+
 static inline unsigned nblk(long long count, int bs) { return (unsigned)((count + bs - 1) / bs); }
 
 int rhs_clause_blocks(int m) { return (int)nblk(m > 0 ? m : 1, CBS); }
@@ -832,11 +846,8 @@ static void step_sliced_impl(const DevProb &P, const DevParams &prm, void *vd, v
                                                         to_pt<T>(prm), (const Pair<T> *)vd, (const T *)vnext,
                                                         (T *)xs, (T *)xl, (T *)cst, (T *)e, ctl, slot);
     const int lds = (int)sizeof(T) * (P.sx_region > 0 ? P.sx_region : 1);
-    static int lds_set = 0;
-    if (lds > 65536 && lds > lds_set) {
-        (void)hipFuncSetAttribute((const void *)k_sx_var<T, SORT>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-        lds_set = lds;
-    }
+    static int lds_set[MAX_DEV_ATTR] = {};
+    lds_attr((const void *)k_sx_var<T, SORT>, lds, lds_set);
     k_sx_var<T, SORT><<<nblk(P.n > 0 ? P.n : 1, SX_F), SX_F, lds, st>>>(P.n, sx_vstride(P.n), P.row_ptr, P.sq,
                                                                       (const T *)e, (Pair<T> *)vd, (T *)vnext,
                                                                       P.unit, ctl, slot);
