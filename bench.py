@@ -69,11 +69,14 @@ def percentile_ms(vals, q):
 
 
 def apply_shape(b, shape):
-    """Autotuned fused-kernel shapes; the tail goes to gangs (or wide mode)
-    once 1.25% of the run's clauses are left (the library default)."""
+    """Autotuned fused-kernel shapes and gang clauses per workgroup; the tail
+    goes to gangs (or wide mode) once 1.25% of the run's clauses are left
+    (the library default)."""
     if shape:
         for md in (0, 1):
             b.set_shape(md, shape[md][0], shape[md][1])
+        if len(shape) > 2:
+            b.set_gang(*shape[2])
 
 
 def batched_solve(ctx, work, hps, cap, spl, timing, shape=None, order=0):
@@ -418,9 +421,9 @@ def main():
         t0 = time.perf_counter()
         tb = dnx.Batch(ctx, slots=1024, arena_bytes=8 << 30)
         tune_rows = tb.tune(n_min=100, n_max=5000, r_min=4.3, r_max=8.0, sample=512, steps=60, seed=11)
-        shape = (tb.get_shape(0)[:2], tb.get_shape(1)[:2])
+        shape = (tb.get_shape(0)[:2], tb.get_shape(1)[:2], tb.get_gang())
         tb.close()
-        log(f"autotuned in {time.perf_counter() - t0:.1f}s: mode0 {shape[0]}, mode1 {shape[1]}")
+        log(f"autotuned in {time.perf_counter() - t0:.1f}s: mode0 {shape[0]}, mode1 {shape[1]}, gang {shape[2]}")
     if args.profile_pass:
         copy_bytes = 256 << 20
         ms = ctx.device_copy(copy_bytes, repeats=4)
@@ -565,7 +568,8 @@ def main():
         "solve_rate_fp32_vs_fp64_on_sample": [fp32_rate, fp64_rate],
         "precision": precision,
         "served": srv,
-        "autotuned_shape": None if shape is None else dict(mode0=list(shape[0]), mode1=list(shape[1])),
+        "autotuned_shape": None if shape is None else dict(mode0=list(shape[0]), mode1=list(shape[1]),
+                                                           gang_clauses_per_workgroup=shape[2][1]),
         "single_instance_n100k": single,
         "roofline": roofline,
         "cpu_baseline": cpu_line,

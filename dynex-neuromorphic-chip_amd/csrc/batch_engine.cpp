@@ -2726,9 +2726,58 @@ static int tune_run(dnx_ctx *ctx, const std::vector<TuneInst> &inst, const std::
     return DNX_OK;
 }
 
+// Times `steps` fixed steps of the picked instances in gang mode from the
+// first step (the tail threshold set past every clause count), with gangs
+// asking for `cpw` clauses per workgroup; ms_out is the device time of the
+// gang (or, when gangs do not apply, wide) launches.
+static int tune_gang_run(dnx_ctx *ctx, const std::vector<TuneInst> &inst, const std::vector<int> &pick, int cpw,
+                         int steps, double *ms_out)
+{
+    size_t bytes = 1 << 20;
+    for (int i : pick) {
+        const HostProblem &h = inst[(size_t)i].hp;
+        bytes += layout_of(h.n, h.m, h.nnz(), !h.uniform3, h.has_unit, KC_MAX).total;
+    }
+    dnx_batch *t = nullptr;
+    int rc = dnx_batch_open(ctx, DNX_FP32, (int)pick.size(), (int64_t)bytes, &t);
+    if (rc) return rc;
+    struct Close {
+        dnx_batch *t;
+        ~Close() { dnx_batch_close(t); }
+    } closer{t};
+    if ((rc = dnx_batch_set_gang(t, 256, cpw, steps)) ||
+        (rc = dnx_batch_set_wide(t, std::numeric_limits<int64_t>::max(), 30)))
+        return rc;
+    for (int i : pick) {
+        dnx_params prm;
+        dnx_params_default(inst[(size_t)i].hp.m, 10000, &prm);
+        int slot;
+        if ((rc = dnx_batch_admit_host(t, inst[(size_t)i].hp, &prm, inst[(size_t)i].x0.data(), steps, 0, &slot)))
+            return rc;
+    }
+    double ms[NBUCKET];
+    int64_t nl[NBUCKET];
+    if ((rc = dnx_batch_timing(t, 1, nullptr, nullptr, nullptr))) return rc;
+    if ((rc = dnx_batch_run(t, steps, 0.0, nullptr))) return rc;
+    if ((rc = dnx_batch_timing(t, 0, ms, nl, nullptr))) return rc;
+    double tot = 0.0;
+    for (int md = 0; md < NBUCKET; md++) tot += ms[md];
+    *ms_out = tot;
+    return DNX_OK;
+}
+
 }  // namespace dnx
 
 extern "C" {
+
+int dnx_batch_get_gang(dnx_batch *b, int *max_blocks, int *clauses_per_workgroup, int *steps_per_launch)
+{
+    if (!b) return DNX_E_ARG;
+    if (max_blocks) *max_blocks = b->gang_blocks;
+    if (clauses_per_workgroup) *clauses_per_workgroup = b->gang_cpw;
+    if (steps_per_launch) *steps_per_launch = b->gang_steps;
+    return DNX_OK;
+}
 
 int dnx_batch_tune(dnx_batch *b, int n_min, int n_max, double r_min, double r_max, int sample, int steps,
                    uint64_t seed, double *report, int report_len, int *report_count)
@@ -2811,6 +2860,30 @@ int dnx_batch_tune(dnx_batch *b, int n_min, int n_max, double r_min, double r_ma
     b->shape[BM_LDS_ALL].bs = b0;
     b->shape[BM_LDS_ALL].lds_cap = c0;
     b->shape[BM_LDS_V].bs = b1;
+    // 2. the tail's shape: clauses per gang workgroup, timed on the sample's
+    // eight largest instances advanced in gangs from their first step
+    if (b->gang_blocks > 0) {
+        std::vector<int> big = all;
+        std::stable_sort(big.begin(), big.end(),
+                         [&](int x, int y) { return inst[(size_t)x].hp.m > inst[(size_t)y].hp.m; });
+        big.resize(std::min<size_t>(big.size(), 8));
+        double gbest = -1.0;
+        int cbest = b->gang_cpw;
+        double galg = 0.0;
+        for (int i : big) galg += (double)steps * (28.0 * inst[(size_t)i].hp.m + 8.0 * inst[(size_t)i].hp.n);
+        for (int cpw : {256, 384, 768}) {
+            double ms = 0.0;
+            int rc = tune_gang_run(b->ctx, inst, big, cpw, steps, &ms);
+            if (rc) return rc;
+            const double gbps = ms > 0 ? galg / (ms * 1e-3) / 1e9 : 0.0;
+            rep(2, cpw, 0, ms, gbps);
+            if (gbps > gbest) {
+                gbest = gbps;
+                cbest = cpw;
+            }
+        }
+        b->gang_cpw = cbest;
+    }
     if (report_count) *report_count = nrep;
     return DNX_OK;
 }

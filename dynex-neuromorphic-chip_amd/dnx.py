@@ -86,6 +86,7 @@ def lib():
     L.dnx_batch_admit.argtypes = [vp, vp, C.POINTER(Params), F64P, C.c_int64, ip, C.POINTER(C.c_int)]
     L.dnx_batch_admit_many.argtypes = [vp, ip, pp, vp, pp, C.c_int64, ip, I32P]
     L.dnx_batch_set_serve_order.argtypes = [vp, C.c_int64]
+    L.dnx_batch_get_gang.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.dnx_batch_set_serve_solo.argtypes = [vp, C.c_int64, ip]
     L.dnx_batch_served_by.argtypes = [vp, I64P]
     L.dnx_batch_advance.argtypes = [vp, ip]
@@ -439,6 +440,11 @@ class Batch:
         bs, cap, wide = C.c_int(), C.c_int(), C.c_int64()
         check(lib().dnx_batch_get_shape(self.h, int(mode), C.byref(bs), C.byref(cap), C.byref(wide)))
         return bs.value, cap.value, wide.value
+
+    def get_gang(self):
+        mb, cpw, k = C.c_int(), C.c_int(), C.c_int()
+        check(lib().dnx_batch_get_gang(self.h, C.byref(mb), C.byref(cpw), C.byref(k)))
+        return mb.value, cpw.value, k.value
 
     def set_gang(self, max_blocks=256, clauses_per_workgroup=384, steps_per_launch=512):
         check(lib().dnx_batch_set_gang(self.h, int(max_blocks), int(clauses_per_workgroup), int(steps_per_launch)))

@@ -354,10 +354,13 @@ int dnx_batch_served_by(dnx_batch *b, int64_t *counts);
 /* Autotuner: times candidate execution shapes on `sample` synthetic planted
  * instances (n log-uniform in [n_min, n_max], clause ratio uniform in
  * [r_min, r_max]) for `steps` fixed steps each -- workgroup size of modes 0
- * and 1 and the LDS budget of mode 0 -- and applies the fastest to the
- * empty batch b.  report (may be NULL) receives report_len doubles: one
- * 5-tuple per measurement (kind, block, lds_cap, ms, GB/s); *report_count is
- * the number of doubles written. */
+ * and 1 and the LDS budget of mode 0 (the fused kernel's clause tile), then
+ * the tail's clauses per gang workgroup (256 / 384 / 768, on the sample's
+ * eight largest instances in gangs) -- and applies the fastest to the empty
+ * batch b.  report (may be NULL) receives report_len doubles: one 5-tuple per
+ * measurement (kind 0/1/2 = mode-0 shape / mode-1 block / gang clauses per
+ * workgroup, block or clauses, lds_cap, ms, GB/s); *report_count is the
+ * number of doubles written. */
 int dnx_batch_tune(dnx_batch *b, int n_min, int n_max, double r_min, double r_max, int sample,
                    int steps, uint64_t seed, double *report, int report_len, int *report_count);
 
@@ -376,6 +379,8 @@ int dnx_batch_get_shape(dnx_batch *b, int mode, int *block, int *lds_cap, int64_
  * fit, wide mode runs until enough instances have ended.  Results are
  * bit-identical to the other modes. */
 int dnx_batch_set_gang(dnx_batch *b, int max_blocks, int clauses_per_workgroup, int steps_per_launch);
+/* The gang settings in force (what dnx_batch_set_gang or the autotuner set). */
+int dnx_batch_get_gang(dnx_batch *b, int *max_blocks, int *clauses_per_workgroup, int *steps_per_launch);
 
 /* Waits for queued work, then reports the summed device time (ms) and the
  * count of timed launches since the last reset, in six buckets: modes 0-2

@@ -584,7 +584,10 @@ def test_autotuner_applies_a_shape(ctx, inst_set, oracle_results):
     assert np.all(rep[:, 3] > 0)
     bs0, cap0, wide = b.get_shape(0)
     assert bs0 in (256, 512, 1024) and wide >= 0
-    print("tuned: mode0", b.get_shape(0), "mode1", b.get_shape(1))
+    gang_rows = rep[rep[:, 0] == 2]
+    assert len(gang_rows) == 3 and set(gang_rows[:, 1]) == {256, 384, 768}
+    assert b.get_gang()[1] in (256, 384, 768)
+    print("tuned: mode0", b.get_shape(0), "mode1", b.get_shape(1), "gang", b.get_gang())
     # the tuned batch computes the same bits
     slots = []
     for n, cp, lits, x0 in inst_set[:10]:
