@@ -395,6 +395,33 @@ def test_retire_readmit_and_budget(ctx):
     b.close()
 
 
+def test_admit_many_edge_instances(ctx):
+    """One call admitting degenerate and mixed instances -- a single variable
+    with a unit clause (pinned), k-SAT with an empty clause, unit pins among
+    3-SAT clauses, an instance with no clauses -- computes the float oracle's
+    bits, as single admissions do."""
+    insts = []
+    # one variable, one unit clause: V pinned, solved at once
+    insts.append((1, np.array([0, 1], np.int32), np.array([1], np.int32)))
+    # k-SAT with an empty clause (never solved) and a duplicated literal
+    cl = [[1, -2, 3], [], [2, 2, -4], [4, 5], [-1, -3, -5, 2]]
+    insts.append((5, np.cumsum([0] + [len(c) for c in cl]).astype(np.int32), np.array(sum(cl, []), np.int32)))
+    # 3-SAT plus unit clauses (pins)
+    lits = dnx.generate_planted(31, 200, 900)
+    cl = [list(lits[3 * c:3 * c + 3]) for c in range(900)] + [[7], [-11]]
+    insts.append((200, np.cumsum([0] + [len(c) for c in cl]).astype(np.int32), np.array(sum(cl, []), np.int32)))
+    # variables but no clauses: the empty formula is satisfied at state 0
+    insts.append((3, np.zeros(1, np.int32), np.zeros(0, np.int32)))
+    full = [(n, cp, lits, dnx.init_uniform(500 + i, n, len(cp) - 1)) for i, (n, cp, lits) in enumerate(insts)]
+    for many in (False, True):
+        out = run_batch(ctx, full, 300, spl=16, many=many)
+        for r, (n, cp, lits, x0) in zip(out, full):
+            steps, t, status, xo = oracle_solve(n, cp, lits, x0, 300)
+            assert r["status"] == (dnx.SAT if status == 1 else dnx.UNKNOWN)
+            assert r["steps"] == steps and r["t"] == t
+            assert same_bits(r["x"], xo)
+
+
 def test_admit_many_all_or_none(ctx):
     """dnx_batch_admit_many admits all or nothing: with too few free slots or
     too little arena for the whole set it fails with DNX_E_BUDGET and leaves
