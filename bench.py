@@ -184,7 +184,7 @@ def served(ctx, seconds, rate, cap, spl, shape, seed=77):
     ratio = r["m"] / np.maximum(r["n"], 1)
     names = {dnx.SERVE_QUEUED: "queued", dnx.SERVE_NARROW: "narrow_lane", dnx.SERVE_WIDE: "wide_lane_running",
              dnx.SERVE_WIDE_WAIT: "wide_lane_waiting", dnx.SERVE_NOT_ARRIVED: "not_arrived",
-             dnx.SERVE_SOLO: "solo_lane_running", dnx.SERVE_SOLO_WAIT: "solo_lane_waiting"}
+             dnx.SERVE_SLICED: "sliced_lane_running", dnx.SERVE_SLICED_WAIT: "sliced_lane_waiting"}
     unfinished = dict(count=int(left.sum()),
                       where={v: int((r["where"] == k).sum()) for k, v in names.items()},
                       n_min=int(r["n"][left].min()) if left.any() else None,
@@ -193,13 +193,13 @@ def served(ctx, seconds, rate, cap, spl, shape, seed=77):
                       clause_ratio_above_6=int((left & (ratio >= 6.0)).sum()),
                       steps_done_p50=float(np.percentile(r["steps"][left], 50)) if left.any() else None,
                       reason=None)
-    big_only = left.any() and np.all(np.isin(r["where"][left], [dnx.SERVE_WIDE, dnx.SERVE_WIDE_WAIT, dnx.SERVE_SOLO,
-                                                                 dnx.SERVE_SOLO_WAIT]))
+    big_only = left.any() and np.all(np.isin(r["where"][left], [dnx.SERVE_WIDE, dnx.SERVE_WIDE_WAIT, dnx.SERVE_SLICED,
+                                                                 dnx.SERVE_SLICED_WAIT]))
     if big_only:
-        unfinished["reason"] = ("every one was a large request (n >= 10,000) in the wide or solo lane when the "
-                                "deadline came: large instances near the threshold ratio need thousands to tens of "
-                                "thousands of steps each, more work than the stream leaves time for; the wide lane "
-                                "finishes the least expected work first, the solo lane runs requests in arrival order")
+        unfinished["reason"] = ("every one was a large request (n >= 10,000) resident in the sliced or wide lane "
+                                "when the deadline came: large instances near the threshold ratio need thousands to "
+                                "tens of thousands of steps each, more work than the stream leaves time for; both "
+                                "lanes advance the least expected work first once a request has run 1,024 steps")
     return dict(requests=int(r["count"]), solved_verified=int(ok.sum()), finished=int(done.sum()),
                 retired_by_lane=lanes, unfinished_at_deadline=unfinished,
                 unknown=int((r["status"] == dnx.UNKNOWN).sum()), failed=int((r["status"] == dnx.FAILED).sum()),

@@ -168,14 +168,13 @@ struct dnx_batch {
     // least attained service, the others by least expected work (0: all by
     // least attained service)
     long long serve_las_steps = 1024;
-    // serving: 3-SAT requests with at least this many clauses run alone on
-    // the whole device with the sliced exchange (the solo lane), at most
-    // serve_solo_active resident at a time (0 clauses, the default: no solo
-    // lane)
-    long long serve_solo = 0;
-    int serve_solo_active = 1024;
+    // serving: 3-SAT requests with at least this many clauses run in the
+    // sliced lane, at most serve_sliced_active resident at a time (0: no
+    // sliced lane)
+    long long serve_sliced = 60000;
+    int serve_sliced_active = 1024;
     // requests the last serving call retired in each lane: narrow, wide,
-    // solo, and those that could never run (larger than the arena)
+    // sliced, and those that could never run (larger than the arena)
     long long served_by[4] = {0, 0, 0, 0};
     // portfolio: dnx_batch_run returns at the first status sync at which
     // any slot is satisfied
@@ -1674,11 +1673,11 @@ int dnx_batch_set_serve_big(dnx_batch *b, int64_t clauses, int64_t active_clause
     return DNX_OK;
 }
 
-int dnx_batch_set_serve_solo(dnx_batch *b, int64_t min_clauses, int max_active)
+int dnx_batch_set_serve_sliced(dnx_batch *b, int64_t min_clauses, int max_active)
 {
     if (!b || min_clauses < 0 || max_active < 1 || max_active > 4096) return DNX_E_ARG;
-    b->serve_solo = min_clauses;
-    b->serve_solo_active = max_active;
+    b->serve_sliced = min_clauses;
+    b->serve_sliced_active = max_active;
     return DNX_OK;
 }
 
@@ -1750,15 +1749,19 @@ struct ServeOut {
     int *where = nullptr;  // optional: DNX_SERVE_* per request
 };
 
-// The serving loop's solo lane: requests of at least serve_solo clauses run
-// alone on the whole device with the single-instance path's sliced exchange
-// (solo.hpp), their tables and state in an arena range, a 30-step graph per
-// request replayed on the lane's own stream, two chunks in flight per
-// request (each followed by a copy of the control block's head and an
-// event).  Plans are built by planner threads from arrival on; a request
+// The serving loop's sliced lane: 3-SAT requests of at least serve_sliced
+// clauses run with the single-instance path's sliced exchange (solo.hpp),
+// their tables and state in arena ranges.  Every 30-step chunk advances an
+// active set of resident requests together -- one launch pair per step whose
+// blocks a map deals to (request, unit) and (request, fold block) -- chosen
+// in the wide lane's order up to the lane's clause budget; after the chunk a
+// kernel gathers the members' control-block heads for one copy and an event.
+// Two chunks are in flight.  Plans are built by planner threads from arrival
+// on (units of ~3,000 clauses, so many requests share the device); a request
 // whose plan does not apply goes back to the other lanes.
-struct SoloLane {
-    static constexpr int CHUNK = 30;  // steps per graph (a multiple of 3)
+struct SlicedLane {
+    static constexpr int CHUNK = 30;  // steps per chunk (a multiple of 3)
+    static constexpr int HEADW = 32;  // ints gathered per control block (covers its head)
     struct Run {
         int r = -1;
         size_t off = 0, size = 0;
@@ -1766,61 +1769,67 @@ struct SoloLane {
         SoloLayout L{};
         DevProb P{};
         DevParams prm{};
+        SXDesc desc{};
         unsigned char *base = nullptr;
-        hipGraphExec_t graph = nullptr;
-        hipEvent_t ev[2] = {nullptr, nullptr};
-        Ctl *head = nullptr;  // pinned: two control-block heads
-        int issued = 0, polled = 0;  // chunks issued / polled
         long long attained = 0;
+        int inflight = 0;   // chunks in flight that include it
         bool ended = false;
+        Ctl last{};         // head of its last polled control block
     };
+    struct Chunk {
+        std::vector<int> members;  // run indices
+        hipEvent_t ev = nullptr;
+        DevBuf desc, mapc, mapv, heads;
+        unsigned char *h_stage = nullptr;  // pinned: desc + maps going up, heads coming down
+        size_t h_size = 0;
+        size_t heads_off = 0;  // where the heads land in h_stage
+        bool busy = false;
+    };
+    static_assert(offsetof(Ctl, track_loc) <= HEADW * sizeof(int), "the gathered head covers the poll fields");
     dnx_batch *b;
     const std::function<ServeReq(int)> &get;
     bool on = false;
     hipStream_t st = nullptr;
-    hipStream_t cap_st = nullptr;  // idle stream the request graphs are captured on
     std::vector<Run> runs;
+    Chunk chunks[2];
+    int next_chunk = 0;
     std::deque<int> queue;  // routed requests, arrival order
     // planner threads
     std::mutex mu;
     std::condition_variable cv;
     std::deque<int> jobs;
-    std::map<int, std::unique_ptr<SlicedPlan>> plans;  // done: null when the plan does not apply
-    std::set<int> failed;
+    std::map<int, std::unique_ptr<SlicedPlan>> plans;  // done
+    std::set<int> failed;                              // the exchange does not apply
     bool stop = false;
     std::vector<std::thread> planners;
+    // plans held at once (ready or being built): a plan of an n=100,000
+    // request holds tens of MB of host memory
+    static constexpr int AHEAD = 16;
+    int building = 0;
     // pinned staging of one admission, free again once `up_ev` has passed
     unsigned char *stage = nullptr;
     size_t stage_size = 0;
     hipEvent_t up_ev = nullptr;
     bool up_pending = false;
 
-    SoloLane(dnx_batch *bb, const std::function<ServeReq(int)> &g) : b(bb), get(g) {}
+    SlicedLane(dnx_batch *bb, const std::function<ServeReq(int)> &g) : b(bb), get(g) {}
     int init()
     {
-        on = b->serve_solo > 0;
+        on = b->serve_sliced > 0;
         if (!on) return DNX_OK;
         hipError_t e;
         if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess ||
-            (e = hipStreamCreateWithFlags(&cap_st, hipStreamNonBlocking)) != hipSuccess ||
             (e = hipEventCreateWithFlags(&up_ev, hipEventDisableTiming)) != hipSuccess)
-            return hip_fail(e, "solo lane setup");
-        runs.resize((size_t)b->serve_solo_active);
-        for (auto &R : runs) {
-            if ((e = hipEventCreateWithFlags(&R.ev[0], hipEventDisableTiming)) != hipSuccess ||
-                (e = hipEventCreateWithFlags(&R.ev[1], hipEventDisableTiming)) != hipSuccess ||
-                (e = hipHostMalloc((void **)&R.head, 2 * sizeof(Ctl), hipHostMallocDefault)) != hipSuccess)
-                return hip_fail(e, "solo lane setup");
-        }
+            return hip_fail(e, "sliced lane setup");
+        for (auto &c : chunks)
+            if ((e = hipEventCreateWithFlags(&c.ev, hipEventDisableTiming)) != hipSuccess)
+                return hip_fail(e, "sliced lane setup");
+        runs.resize((size_t)b->serve_sliced_active);
         const unsigned hc = std::thread::hardware_concurrency();
         const int np = std::max(1, std::min(6, (int)(hc ? hc : 2) / 2));
         for (int i = 0; i < np; i++) planners.emplace_back([this] { plan_loop(); });
         return DNX_OK;
     }
-    // plans held at once (ready or being built): a plan of an n=100,000
-    // request holds tens of MB of host memory
-    static constexpr int AHEAD = 8;
-    int building = 0;
     void plan_loop()
     {
         for (;;) {
@@ -1835,7 +1844,7 @@ struct SoloLane {
             }
             const ServeReq q = get(r);
             auto pl = std::make_unique<SlicedPlan>();
-            const bool ok = solo_plan(*q.hp, b->cus, *pl);
+            const bool ok = solo_plan_units(*q.hp, b->cus, 3072, *pl);
             std::lock_guard<std::mutex> lk(mu);
             building--;
             if (ok) {
@@ -1857,7 +1866,7 @@ struct SoloLane {
         for (auto &t : planners) t.join();
         planners.clear();
     }
-    bool wants(const HostProblem &h) const { return on && h.uniform3 && h.m >= b->serve_solo; }
+    bool wants(const HostProblem &h) const { return on && h.uniform3 && h.m >= b->serve_sliced; }
     void route(int r)
     {
         queue.push_back(r);
@@ -1873,8 +1882,8 @@ struct SoloLane {
         for (auto &R : runs) k += R.r >= 0;
         return k;
     }
-    // whether the plan of the queue's first request is done: 1 ready, -1 does
-    // not apply, 0 not yet
+    // whether the plan of request r is done: 1 ready, -1 does not apply, 0
+    // not yet
     int plan_state(int r, std::unique_ptr<SlicedPlan> &out)
     {
         std::lock_guard<std::mutex> lk(mu);
@@ -1900,7 +1909,7 @@ struct SoloLane {
         if (!b->arena.alloc(L.total, off)) return DNX_E_BUDGET;
         hipError_t e;
         if (up_pending) {
-            if ((e = hipEventSynchronize(up_ev)) != hipSuccess) return hip_fail(e, "solo staging wait");
+            if ((e = hipEventSynchronize(up_ev)) != hipSuccess) return hip_fail(e, "sliced lane staging wait");
             up_pending = false;
         }
         if (stage_size < L.upload) {
@@ -1909,7 +1918,7 @@ struct SoloLane {
             stage_size = 0;
             if ((e = hipHostMalloc((void **)&stage, L.upload, hipHostMallocDefault)) != hipSuccess) {
                 b->arena.release(off, L.total);
-                return hip_fail(e, "solo staging");
+                return hip_fail(e, "sliced lane staging");
             }
             stage_size = L.upload;
         }
@@ -1922,70 +1931,18 @@ struct SoloLane {
         R.base = base;
         R.P = solo_devprob(h, *pl, L, base);
         R.prm = dev_params(&q.prm);
+        R.desc = solo_desc(h, *pl, L, base, R.prm);
         R.pl = std::move(pl);
-        R.issued = R.polled = 0;
         R.attained = 0;
+        R.inflight = 0;
         R.ended = false;
-        // one 30-step graph for the request (slots rotate with the step),
-        // captured on a stream that never holds work
-        hipGraph_t g;
-        if ((e = hipStreamBeginCapture(cap_st, hipStreamCaptureModeThreadLocal)) != hipSuccess)
-            return hip_fail(e, "solo capture");
-        for (int i = 0; i < CHUNK; i++)
-            launch_step_sliced(DNX_FP32, R.P, R.prm, base + L.vd, base + L.xs, base + L.xl, base + L.e,
-                               base + L.vnext, base + L.cst, (Ctl *)(base + L.ctl), i % 3, cap_st);
-        if ((e = hipStreamEndCapture(cap_st, &g)) != hipSuccess) return hip_fail(e, "solo capture end");
-        e = hipGraphInstantiate(&R.graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) {
-            R.graph = nullptr;
-            return hip_fail(e, "solo graph");
-        }
+        memset(&R.last, 0, sizeof(R.last));
         if ((e = hipMemcpyAsync(base, stage, L.upload, hipMemcpyHostToDevice, st)) != hipSuccess ||
             (e = hipEventRecord(up_ev, st)) != hipSuccess)
-            return hip_fail(e, "solo upload");
+            return hip_fail(e, "sliced lane upload");
         up_pending = true;
         launch_load_state(DNX_FP32, R.P, base + L.xstage, base + L.vd, base + L.xs, base + L.xl, st);
-        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "solo state load");
-        return DNX_OK;
-    }
-    // Keeps two chunks queued on the lane, each for the resident request
-    // that comes first: those below serve_las_steps steps by least attained
-    // service (new requests start at once), then by least expected work,
-    // m exp(-1.5 m/n), as in the wide lane -- so a near-threshold request
-    // that needs tens of thousands of steps does not hold the lane.
-    int inflight = 0;
-    int issue()
-    {
-        hipError_t e;
-        const long long T = b->serve_las_steps;
-        while (inflight < 2) {
-            Run *best = nullptr;
-            bool bfg = false;
-            double bkey = 0.0;
-            for (auto &R : runs) {
-                if (R.r < 0 || R.ended || R.issued - R.polled >= 2) continue;
-                const long long att = R.attained + (long long)CHUNK * (R.issued - R.polled);
-                const bool fg = T <= 0 || att < T;
-                const double key = fg ? (double)att
-                                      : (double)R.P.m * std::exp(-1.5 * (double)R.P.m / (double)std::max(R.P.n, 1));
-                if (!best || (fg && !bfg) || (fg == bfg && key < bkey)) {
-                    best = &R;
-                    bfg = fg;
-                    bkey = key;
-                }
-            }
-            if (!best) break;
-            Run &R = *best;
-            const int k = R.issued & 1;
-            if ((e = hipGraphLaunch(R.graph, st)) != hipSuccess ||
-                (e = hipMemcpyAsync(&R.head[k], R.base + R.L.ctl, solo_ctl_head(), hipMemcpyDeviceToHost, st)) !=
-                    hipSuccess ||
-                (e = hipEventRecord(R.ev[k], st)) != hipSuccess)
-                return hip_fail(e, "solo chunk");
-            R.issued++;
-            inflight++;
-        }
+        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "sliced lane state load");
         return DNX_OK;
     }
     static int status_of(const Ctl &c)
@@ -1993,59 +1950,136 @@ struct SoloLane {
         if (c.status != DNX_RUNNING) return c.status;
         return c.done[c.steps % 3];
     }
-    // polls run R: 1 when it has ended (its queued chunks drained), else 0
-    int poll(Run &R, int *rc)
+    // Issues a chunk if a chunk buffer is free and some resident request can
+    // advance: members in order (below serve_las_steps steps by least
+    // attained service, then by least expected work, m exp(-1.5 m/n)) up to
+    // the lane's clause budget, at least one.
+    int issue()
+    {
+        Chunk &C = chunks[next_chunk];
+        if (C.busy) return DNX_OK;
+        const long long T = b->serve_las_steps;
+        std::vector<int> cand;
+        for (int k = 0; k < (int)runs.size(); k++)
+            if (runs[(size_t)k].r >= 0 && !runs[(size_t)k].ended) cand.push_back(k);
+        if (cand.empty()) return DNX_OK;
+        auto fg = [&](int k) { return T <= 0 || runs[(size_t)k].attained < T; };
+        auto work = [&](int k) {
+            const Run &R = runs[(size_t)k];
+            return (double)R.P.m * std::exp(-1.5 * (double)R.P.m / (double)std::max(R.P.n, 1));
+        };
+        std::stable_sort(cand.begin(), cand.end(), [&](int x, int y) {
+            const bool fx = fg(x), fy = fg(y);
+            if (fx != fy) return fx;
+            if (fx) return runs[(size_t)x].attained < runs[(size_t)y].attained;
+            return work(x) < work(y);
+        });
+        long long budget = 0;
+        C.members.clear();
+        for (int k : cand) {
+            const long long mk = runs[(size_t)k].P.m;
+            if (!C.members.empty() && budget + mk > b->serve_active_clauses) break;
+            C.members.push_back(k);
+            budget += mk;
+        }
+        // descriptors and maps, staged in pinned memory, one copy up
+        const int nm = (int)C.members.size();
+        std::vector<int2> mc, mv;
+        int region = 1;
+        for (int j = 0; j < nm; j++) {
+            const Run &R = runs[(size_t)C.members[(size_t)j]];
+            for (int u = 0; u < R.P.nunits; u++) mc.push_back(make_int2(j, u));
+            for (int v = 0; v * SX_F < R.P.n; v++) mv.push_back(make_int2(j, v));
+            region = std::max(region, R.P.sx_region);
+        }
+        const size_t dbytes = al256(sizeof(SXDesc) * (size_t)nm), cbytes = al256(sizeof(int2) * mc.size()),
+                     vbytes = al256(sizeof(int2) * (mv.empty() ? 1 : mv.size())),
+                     hbytes = al256(sizeof(int) * HEADW * (size_t)nm);
+        const size_t need = dbytes + cbytes + vbytes + hbytes;
+        hipError_t e;
+        int rc;
+        if (C.h_size < need) {
+            if (C.h_stage) (void)hipHostFree(C.h_stage);
+            C.h_stage = nullptr;
+            C.h_size = 0;
+            const size_t want = std::max(need, (size_t)1 << 20);
+            if ((e = hipHostMalloc((void **)&C.h_stage, want, hipHostMallocDefault)) != hipSuccess)
+                return hip_fail(e, "sliced lane maps");
+            C.h_size = want;
+        }
+        auto grow = [&](DevBuf &d, size_t bytes) { return d.size >= bytes ? DNX_OK : d.alloc(std::max(bytes, d.size * 2)); };
+        if ((rc = grow(C.desc, dbytes)) || (rc = grow(C.mapc, cbytes)) || (rc = grow(C.mapv, vbytes)) ||
+            (rc = grow(C.heads, hbytes)))
+            return rc;
+        unsigned char *h = C.h_stage;
+        for (int j = 0; j < nm; j++) ((SXDesc *)h)[j] = runs[(size_t)C.members[(size_t)j]].desc;
+        memcpy(h + dbytes, mc.data(), sizeof(int2) * mc.size());
+        if (!mv.empty()) memcpy(h + dbytes + cbytes, mv.data(), sizeof(int2) * mv.size());
+        if ((e = hipMemcpyAsync(C.desc.ptr, h, sizeof(SXDesc) * (size_t)nm, hipMemcpyHostToDevice, st)) != hipSuccess ||
+            (e = hipMemcpyAsync(C.mapc.ptr, h + dbytes, sizeof(int2) * mc.size(), hipMemcpyHostToDevice, st)) !=
+                hipSuccess ||
+            (!mv.empty() && (e = hipMemcpyAsync(C.mapv.ptr, h + dbytes + cbytes, sizeof(int2) * mv.size(),
+                                                hipMemcpyHostToDevice, st)) != hipSuccess))
+            return hip_fail(e, "sliced lane maps");
+        for (int i = 0; i < CHUNK; i++)
+            launch_step_sliced_multi((const SXDesc *)C.desc.ptr, C.mapc.ptr, (int)mc.size(), C.mapv.ptr,
+                                     (int)mv.size(), region, i % 3, st);
+        launch_ctl_gather((const SXDesc *)C.desc.ptr, nm, HEADW, (int *)C.heads.ptr, st);
+        C.heads_off = dbytes + cbytes + vbytes;
+        if ((e = hipGetLastError()) != hipSuccess ||
+            (e = hipMemcpyAsync(h + C.heads_off, C.heads.ptr, sizeof(int) * HEADW * (size_t)nm,
+                                hipMemcpyDeviceToHost, st)) != hipSuccess ||
+            (e = hipEventRecord(C.ev, st)) != hipSuccess)
+            return hip_fail(e, "sliced lane chunk");
+        for (int k : C.members) runs[(size_t)k].inflight++;
+        C.busy = true;
+        next_chunk ^= 1;
+        return DNX_OK;
+    }
+    // Polls the oldest chunk in flight; fills `ended` with runs that ended
+    // and are in no chunk still in flight.  Returns 1 when a chunk completed.
+    int poll(std::vector<int> &ended, int *rc)
     {
         *rc = DNX_OK;
-        while (R.polled < R.issued) {
-            const int k = R.polled & 1;
-            const hipError_t q = hipEventQuery(R.ev[k]);
-            if (q == hipErrorNotReady) return 0;
-            if (q != hipSuccess) {
-                *rc = hip_fail(q, "solo poll");
-                return 0;
-            }
-            R.polled++;
-            inflight--;
-            const Ctl &c = R.head[k];
-            R.attained = c.steps;
-            if (status_of(c) != DNX_RUNNING) {
-                R.ended = true;
-                // the later chunk does nothing on an ended run; let it pass
-                // before the range is read back and freed
-                if (R.polled < R.issued) {
-                    const hipError_t w = hipEventSynchronize(R.ev[(R.issued - 1) & 1]);
-                    if (w != hipSuccess) *rc = hip_fail(w, "solo drain");
-                    inflight -= R.issued - R.polled;
-                    R.polled = R.issued;
-                }
-                return 1;
-            }
+        // the older chunk in flight: next_chunk when both are, else the other
+        Chunk &C = chunks[next_chunk].busy ? chunks[next_chunk] : chunks[next_chunk ^ 1];
+        if (!C.busy) return 0;
+        const hipError_t q = hipEventQuery(C.ev);
+        if (q == hipErrorNotReady) return 0;
+        if (q != hipSuccess) {
+            *rc = hip_fail(q, "sliced lane poll");
+            return 0;
         }
-        return 0;
+        const size_t nm = C.members.size();
+        const int *heads = (const int *)(C.h_stage + C.heads_off);
+        for (size_t j = 0; j < nm; j++) {
+            Run &R = runs[(size_t)C.members[j]];
+            memcpy(&R.last, heads + j * HEADW, solo_ctl_head());
+            R.attained = R.last.steps;
+            R.inflight--;
+            if (status_of(R.last) != DNX_RUNNING) R.ended = true;
+            if (R.ended && R.inflight == 0) ended.push_back(C.members[j]);
+        }
+        C.busy = false;
+        return 1;
     }
     void release_run(Run &R)
     {
-        if (R.graph) (void)hipGraphExecDestroy(R.graph);
-        R.graph = nullptr;
         b->arena.release(R.off, R.size);
         R.r = -1;
         R.pl.reset();
     }
-    ~SoloLane()
+    ~SlicedLane()
     {
         stop_planners();
         if (st) (void)hipStreamSynchronize(st);
-        for (auto &R : runs) {
-            if (R.graph) (void)hipGraphExecDestroy(R.graph);
-            for (auto &ev : R.ev)
-                if (ev) (void)hipEventDestroy(ev);
-            if (R.head) (void)hipHostFree(R.head);
+        for (auto &c : chunks) {
+            if (c.ev) (void)hipEventDestroy(c.ev);
+            if (c.h_stage) (void)hipHostFree(c.h_stage);
         }
         if (stage) (void)hipHostFree(stage);
         if (up_ev) (void)hipEventDestroy(up_ev);
         if (st) (void)hipStreamDestroy(st);
-        if (cap_st) (void)hipStreamDestroy(cap_st);
     }
 };
 
@@ -2127,8 +2161,8 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
         return hip_fail(e, "serve setup");
     BDyn *hn = b->h_dyn, *hw = b->h_dyn + b->nslots;  // lane snapshots
     for (auto &c : b->served_by) c = 0;
-    SoloLane solo(b, get);
-    if ((rc = solo.init())) return rc;
+    SlicedLane sliced(b, get);
+    if ((rc = sliced.init())) return rc;
     std::deque<int> queue;
     int next = 0, finished = 0;
     std::vector<int8_t> assign;
@@ -2202,26 +2236,26 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
         finished++;
         return DNX_OK;
     };
-    // ends a solo run: its final state read back on the lane's stream (the
+    // ends a sliced-lane run: its final state read back on the lane's stream (the
     // verified state when solved), the exact clause check against the
     // request's own table, then its range is freed.  Time to solution is
     // taken by the host clock when the lane sees the end.
-    auto finish_solo = [&](SoloLane::Run &R) -> int {
+    auto finish_sliced = [&](SlicedLane::Run &R) -> int {
         const int r = R.r;
-        const Ctl &c = R.head[(R.polled - 1) & 1];
-        int stt = SoloLane::status_of(c);
+        const Ctl &c = R.last;
+        int stt = SlicedLane::status_of(c);
         const HostProblem &hq = *get(r).hp;
         const size_t n = (size_t)hq.n, N = n + 2 * (size_t)hq.m;
         const bool want_x = out.x && out.x[r];
         launch_materialize(DNX_FP32, R.P, R.prm, R.base + R.L.vd, R.base + R.L.xs, R.base + R.L.xl,
-                           (Ctl *)(R.base + R.L.ctl), R.base + R.L.xstage, solo.st);
+                           (Ctl *)(R.base + R.L.ctl), R.base + R.L.xstage, sliced.st);
         xf.resize(N ? N : 1);
         const size_t rd = want_x ? N : n;
         hipError_t ee = hipGetLastError();
         if (ee == hipSuccess && rd)
-            ee = hipMemcpyAsync(xf.data(), R.base + R.L.xstage, sizeof(float) * rd, hipMemcpyDeviceToHost, solo.st);
-        if (ee == hipSuccess) ee = hipStreamSynchronize(solo.st);
-        if (ee != hipSuccess) return hip_fail(ee, "solo state copy");
+            ee = hipMemcpyAsync(xf.data(), R.base + R.L.xstage, sizeof(float) * rd, hipMemcpyDeviceToHost, sliced.st);
+        if (ee == hipSuccess) ee = hipStreamSynchronize(sliced.st);
+        if (ee != hipSuccess) return hip_fail(ee, "sliced-lane state copy");
         if (stt == DNX_SAT) {
             std::vector<int8_t> a(n ? n : 1);
             for (size_t v = 0; v < n; v++) a[v] = xf[v] >= 0.0f ? 1 : -1;
@@ -2237,7 +2271,7 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
             for (size_t v = 0; v < n; v++) out.assign[r][v] = xf[v] >= 0.0f ? 1 : -1;
         if (out.where) out.where[r] = DNX_SERVE_ENDED;
         b->served_by[2]++;
-        solo.release_run(R);
+        sliced.release_run(R);
         release(r);
         finished++;
         return DNX_OK;
@@ -2259,21 +2293,21 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
             // steps they had completed are reported
             if (wst) (void)hipStreamSynchronize(wst);
             (void)hipStreamSynchronize(st);
-            if (solo.st) (void)hipStreamSynchronize(solo.st);
-            solo.stop_planners();
-            for (auto &R : solo.runs) {
+            if (sliced.st) (void)hipStreamSynchronize(sliced.st);
+            sliced.stop_planners();
+            for (auto &R : sliced.runs) {
                 if (R.r < 0) continue;
                 out.steps[R.r] = R.attained;
-                if (out.where) out.where[R.r] = DNX_SERVE_SOLO;
+                if (out.where) out.where[R.r] = DNX_SERVE_SLICED;
                 const int r = R.r;
-                solo.release_run(R);
+                sliced.release_run(R);
                 release(r);
             }
-            for (int r : solo.queue) {
-                if (out.where) out.where[r] = DNX_SERVE_SOLO_WAIT;
+            for (int r : sliced.queue) {
+                if (out.where) out.where[r] = DNX_SERVE_SLICED_WAIT;
                 release(r);
             }
-            solo.queue.clear();
+            sliced.queue.clear();
             for (int s = 0; s < b->nslots; s++) {
                 if (slot_req[(size_t)s] < 0) continue;
                 const int r = slot_req[(size_t)s];
@@ -2298,34 +2332,34 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
             break;
         }
         while (next < count && arrival[next] <= now) {
-            if (solo.on && solo.wants(*get(next).hp)) {
-                if (out.where) out.where[next] = DNX_SERVE_SOLO_WAIT;
-                solo.route(next++);
+            if (sliced.on && sliced.wants(*get(next).hp)) {
+                if (out.where) out.where[next] = DNX_SERVE_SLICED_WAIT;
+                sliced.route(next++);
                 continue;
             }
             if (out.where) out.where[next] = DNX_SERVE_QUEUED;
             queue.push_back(next++);
         }
-        // ---- solo lane: start requests whose plans are ready, keep two
-        // chunks of each running one queued, retire the ones that ended
-        if (solo.on) {
-            for (auto &R : solo.runs) {
-                if (R.r >= 0 || solo.queue.empty()) continue;
-                const int r = solo.queue.front();
+        // ---- sliced lane: start requests whose plans are ready, retire the
+        // ones that ended, keep two chunks queued
+        if (sliced.on) {
+            for (auto &R : sliced.runs) {
+                if (R.r >= 0 || sliced.queue.empty()) continue;
+                const int r = sliced.queue.front();
                 std::unique_ptr<SlicedPlan> pl;
-                const int ps = solo.plan_state(r, pl);
+                const int ps = sliced.plan_state(r, pl);
                 if (ps == 0) break;
-                solo.queue.pop_front();
+                sliced.queue.pop_front();
                 progressed = true;
                 if (ps < 0) {  // the exchange does not apply: the other lanes take it
                     if (out.where) out.where[r] = DNX_SERVE_QUEUED;
                     queue.push_back(r);
                     continue;
                 }
-                rc = solo.start(R, r, pl, cap);
+                rc = sliced.start(R, r, pl, cap);
                 if (rc == DNX_E_BUDGET) {
                     rc = DNX_OK;
-                    if (narrow_run.empty() && wide_set.empty() && solo.active() == 0) {
+                    if (narrow_run.empty() && wide_set.empty() && sliced.active() == 0) {
                         // larger than the whole arena: it can never run
                         out.status[r] = DNX_FAILED;
                         out.wait[r] = now_s() - arrival[r];
@@ -2336,26 +2370,27 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
                         continue;
                     }
                     // no room now: back to the front with its plan kept
-                    solo.queue.push_front(r);
-                    std::lock_guard<std::mutex> lk(solo.mu);
-                    solo.plans[r] = std::move(pl);
+                    sliced.queue.push_front(r);
+                    std::lock_guard<std::mutex> lk(sliced.mu);
+                    sliced.plans[r] = std::move(pl);
                     break;
                 }
                 if (rc) return rc;
                 out.wait[r] = now_s() - arrival[r];
-                if (out.where) out.where[r] = DNX_SERVE_SOLO;
+                if (out.where) out.where[r] = DNX_SERVE_SLICED;
             }
-            for (auto &R : solo.runs) {
-                if (R.r < 0) continue;
+            for (;;) {
+                std::vector<int> ended;
                 int prc = DNX_OK;
-                const int endd = solo.poll(R, &prc);
+                const int got = sliced.poll(ended, &prc);
                 if (prc) return prc;
-                if (endd) {
-                    progressed = true;
-                    if ((rc = finish_solo(R))) return rc;
-                }
+                if (!got) break;
+                progressed = true;
+                for (int k : ended)
+                    if ((rc = finish_sliced(sliced.runs[(size_t)k]))) return rc;
             }
-            if ((rc = solo.issue())) return rc;
+            for (int i = 0; i < 2; i++)
+                if ((rc = sliced.issue())) return rc;
         }
         // ---- admission -------------------------------------------------
         while (!queue.empty()) {
@@ -2364,7 +2399,7 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
             int slot = -1;
             rc = dnx_batch_admit_host(b, *q.hp, &q.prm, q.x0, cap, 1, &slot);
             if (rc == DNX_E_BUDGET) {
-                if (narrow_run.empty() && wide_set.empty() && solo.active() == 0) {
+                if (narrow_run.empty() && wide_set.empty() && sliced.active() == 0) {
                     // larger than the whole arena: it can never run
                     queue.pop_front();
                     out.status[r] = DNX_FAILED;
@@ -2522,7 +2557,7 @@ static int serve_loop(dnx_batch *b, int count, const double *arrival, int64_t ca
         if (!progressed) {
             // nothing completed: sleep a little, or until the next arrival
             double dt = 50e-6;
-            if (narrow_run.empty() && wide_set.empty() && solo.active() == 0 && solo.queue.empty() && next < count)
+            if (narrow_run.empty() && wide_set.empty() && sliced.active() == 0 && sliced.queue.empty() && next < count)
                 dt = std::min(0.01, arrival[next] - now_s());
             if (dt > 0) std::this_thread::sleep_for(std::chrono::duration<double>(dt));
         }

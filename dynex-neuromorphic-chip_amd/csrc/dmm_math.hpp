@@ -28,7 +28,7 @@ struct PT {
 };
 
 template <typename T>
-static inline PT<T> to_pt(const DevParams &p)
+__host__ __device__ static inline PT<T> to_pt(const DevParams &p)
 {
     PT<T> r;
     r.alpha = (T)p.alpha;

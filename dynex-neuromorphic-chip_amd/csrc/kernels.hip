@@ -255,16 +255,17 @@ struct SXShape<double> {
     static constexpr int BS = 512, S = 5000, SUB = 512;
 };
 
+// The clause pass of one unit `ub` of one instance (the kernels below call
+// it with their block index, or with the block's entry of a map over many
+// instances).
 template <typename T>
-__global__ __launch_bounds__(SXShape<T>::BS) void k_sx_clause(int n, int vstride, int m, long long nnz,
-                                                             const uint32_t *__restrict__ sw,
-                                                             const int32_t *__restrict__ units,
-                                                             const int32_t *__restrict__ sbrun,
-                                                             const int2 *__restrict__ runs, PT<T> p,
-                                                             const Pair<T> *__restrict__ vd,
-                                                             const T *__restrict__ vnext, T *__restrict__ Xs,
-                                                             T *__restrict__ Xl, T *__restrict__ cst,
-                                                             T *__restrict__ e, Ctl *__restrict__ ctl, int slot)
+__device__ __forceinline__ void sx_clause_body(int n, int vstride, int m, long long nnz,
+                                               const uint32_t *__restrict__ sw, const int32_t *__restrict__ units,
+                                               const int32_t *__restrict__ sbrun, const int2 *__restrict__ runs,
+                                               PT<T> p, const Pair<T> *__restrict__ vd,
+                                               const T *__restrict__ vnext, T *__restrict__ Xs, T *__restrict__ Xl,
+                                               T *__restrict__ cst, T *__restrict__ e, Ctl *__restrict__ ctl,
+                                               int slot, int ub)
 {
     constexpr int BS = SXShape<T>::BS, S = SXShape<T>::S, SUB = SXShape<T>::SUB;
     static_assert(SUB == BS, "one clause per thread and sub-batch");
@@ -283,7 +284,7 @@ __global__ __launch_bounds__(SXShape<T>::BS) void k_sx_clause(int n, int vstride
     const int status = ctl->status, done = ctl->done[ps], stepped = ctl->stepped[ps], track = ctl->track_loc;
     const int plev = dev_level(ctl, ps);
     const long long steps0 = ctl->steps;
-    const int32_t *U = units + 8 * blockIdx.x;
+    const int32_t *U = units + 8 * ub;
     const int c0 = U[0], c1 = U[1], nsl = U[2], sl0 = U[3], sl1 = U[4], sl2 = U[5], sb0 = U[6], run0 = U[7];
     const int nsb = (c1 - c0 + SUB - 1) / SUB;
     // Every device-memory access of the sub-batch loop is made on every path
@@ -320,7 +321,7 @@ __global__ __launch_bounds__(SXShape<T>::BS) void k_sx_clause(int n, int vstride
     const int nr0 = sbrun[sb0 + 1] - run0;
     if (status != DNX_RUNNING) return;
     if (done) {
-        if (blockIdx.x == 0 && tid == 0) ctl->status = done;
+        if (ub == 0 && tid == 0) ctl->status = done;
         return;
     }
     const bool pend = stepped != 0;
@@ -329,7 +330,7 @@ __global__ __launch_bounds__(SXShape<T>::BS) void k_sx_clause(int n, int vstride
     // memories from a memory update, so the steady-state forms apply (the
     // same bits with fewer instructions, see clause3_steady)
     const bool steady = sizeof(T) == 4 && pend && steps0 >= 1;
-    if (pend && blockIdx.x == 0 && tid == 0) {
+    if (pend && ub == 0 && tid == 0) {
         ctl->t += level_dt(plev);
         ctl->steps += 1;
     }
@@ -454,11 +455,40 @@ __global__ __launch_bounds__(SXShape<T>::BS) void k_sx_clause(int n, int vstride
     if (__syncthreads_or(unsolved) && tid == 0) raise_flag(ctl->flags[slot][FG_UNSOLVED]);
 }
 
+template <typename T>
+__global__ __launch_bounds__(SXShape<T>::BS) void k_sx_clause(int n, int vstride, int m, long long nnz,
+                                                             const uint32_t *__restrict__ sw,
+                                                             const int32_t *__restrict__ units,
+                                                             const int32_t *__restrict__ sbrun,
+                                                             const int2 *__restrict__ runs, PT<T> p,
+                                                             const Pair<T> *__restrict__ vd,
+                                                             const T *__restrict__ vnext, T *__restrict__ Xs,
+                                                             T *__restrict__ Xl, T *__restrict__ cst,
+                                                             T *__restrict__ e, Ctl *__restrict__ ctl, int slot)
+{
+    sx_clause_body<T>(n, vstride, m, nnz, sw, units, sbrun, runs, p, vd, vnext, Xs, Xl, cst, e, ctl, slot,
+                      (int)blockIdx.x);
+}
+
+// Many instances in one launch: block b runs unit map[b].y of instance
+// map[b].x.
+template <typename T>
+__global__ __launch_bounds__(SXShape<T>::BS) void k_sx_clause_multi(const SXDesc *__restrict__ d,
+                                                                   const int2 *__restrict__ map, int slot)
+{
+    const int2 mb = map[blockIdx.x];
+    const SXDesc &D = d[mb.x];
+    sx_clause_body<T>(D.n, D.vstride, D.m, 3LL * D.m, D.sw, D.units, D.sbrun, (const int2 *)D.runs,
+                      to_pt<T>(D.prm), (const Pair<T> *)D.vd, (const T *)D.vnext, (T *)D.xs, (T *)D.xl,
+                      (T *)D.cst, (T *)D.e, D.ctl, slot, mb.y);
+}
+
+// The variable pass of fold block `vb` of one instance.
 template <typename T, bool SORT>
-__global__ __launch_bounds__(SX_F) void k_sx_var(int n, int vstride, const int32_t *__restrict__ row_ptr,
-                                                 const uint16_t *__restrict__ sq, const T *__restrict__ e,
-                                                 Pair<T> *__restrict__ vd, T *__restrict__ vnext,
-                                                 const int8_t *__restrict__ unit, Ctl *__restrict__ ctl, int slot)
+__device__ __forceinline__ void sx_var_body(int n, int vstride, const int32_t *__restrict__ row_ptr,
+                                            const uint16_t *__restrict__ sq, const T *__restrict__ e,
+                                            Pair<T> *__restrict__ vd, T *__restrict__ vnext,
+                                            const int8_t *__restrict__ unit, Ctl *__restrict__ ctl, int slot, int vb)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     T *sv = (T *)smem;
@@ -473,7 +503,7 @@ __global__ __launch_bounds__(SX_F) void k_sx_var(int n, int vstride, const int32
     const bool unsolved_any = any_flag(ctl->flags[slot][FG_UNSOLVED]);
     const int stop = ctl->stop_on_solve, track = ctl->track_loc;
     const long long steps = ctl->steps, cap = ctl->cap;
-    const int v0 = blockIdx.x * SX_F, v1 = min(n, v0 + SX_F);
+    const int v0 = vb * SX_F, v1 = min(n, v0 + SX_F);
     const int v = v0 + tid;
     const int s0 = row_ptr[v0], s1 = row_ptr[v1];
     const int rb = v < n ? row_ptr[v] : 0, re = v < n ? row_ptr[v + 1] : 0;
@@ -485,7 +515,7 @@ __global__ __launch_bounds__(SX_F) void k_sx_var(int n, int vstride, const int32
     const T h = pend ? level_h<T>(plev) : T(0.0);
     const bool solved = stop && !unsolved_any;
     const int done = solved ? DNX_SAT : (steps >= cap ? DNX_UNKNOWN : 0);
-    if (blockIdx.x == 0 && tid < 64 && track) fold_loc(ctl, slot);
+    if (vb == 0 && tid < 64 && track) fold_loc(ctl, slot);
     if (!done) {
         // SU values per thread in flight (one round for the usual block
         // sizes), then their LDS stores
@@ -535,7 +565,7 @@ __global__ __launch_bounds__(SX_F) void k_sx_var(int n, int vstride, const int32
     }
     const int ns = (slot + 1) % 3;
     if (done) {
-        if (blockIdx.x == 0) {
+        if (vb == 0) {
             if (tid == 0) ctl->done[slot] = done;
             clear_slot(ctl, ns);
         }
@@ -544,10 +574,31 @@ __global__ __launch_bounds__(SX_F) void k_sx_var(int n, int vstride, const int32
     const int bm = block_max_int<SX_F>(f, sh);
     if (tid == 0)
         for (int L = 0; L < bm; L++) raise_flag(ctl->flags[slot][FG_LEVEL + L]);
-    if (blockIdx.x == 0) {
+    if (vb == 0) {
         if (tid == 0) ctl->stepped[slot] = 1;
         clear_slot(ctl, ns);
     }
+}
+
+template <typename T, bool SORT>
+__global__ __launch_bounds__(SX_F) void k_sx_var(int n, int vstride, const int32_t *__restrict__ row_ptr,
+                                                 const uint16_t *__restrict__ sq, const T *__restrict__ e,
+                                                 Pair<T> *__restrict__ vd, T *__restrict__ vnext,
+                                                 const int8_t *__restrict__ unit, Ctl *__restrict__ ctl, int slot)
+{
+    sx_var_body<T, SORT>(n, vstride, row_ptr, sq, e, vd, vnext, unit, ctl, slot, (int)blockIdx.x);
+}
+
+// Many instances in one launch: block b folds block map[b].y of instance
+// map[b].x (dynamic LDS sized for the largest region among them).
+template <typename T, bool SORT>
+__global__ __launch_bounds__(SX_F) void k_sx_var_multi(const SXDesc *__restrict__ d, const int2 *__restrict__ map,
+                                                       int slot)
+{
+    const int2 mb = map[blockIdx.x];
+    const SXDesc &D = d[mb.x];
+    sx_var_body<T, SORT>(D.n, D.vstride, D.row_ptr, D.sq, (const T *)D.e, (Pair<T> *)D.vd, (T *)D.vnext, D.unit,
+                         D.ctl, slot, mb.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -851,6 +902,33 @@ static void step_sliced_impl(const DevProb &P, const DevParams &prm, void *vd, v
     k_sx_var<T, SORT><<<nblk(P.n > 0 ? P.n : 1, SX_F), SX_F, lds, st>>>(P.n, sx_vstride(P.n), P.row_ptr, P.sq,
                                                                       (const T *)e, (Pair<T> *)vd, (T *)vnext,
                                                                       P.unit, ctl, slot);
+}
+
+// out[k] <- the first `words` ints of instance k's control block
+__global__ __launch_bounds__(64) void k_ctl_gather(const SXDesc *__restrict__ d, int count, int words,
+                                                   int *__restrict__ out)
+{
+    const int k = blockIdx.x;
+    if (k >= count) return;
+    const int *src = (const int *)d[k].ctl;
+    for (int i = threadIdx.x; i < words; i += 64) out[(size_t)k * words + i] = src[i];
+}
+
+void launch_ctl_gather(const SXDesc *d, int count, int words, int *out, void *stream)
+{
+    if (count > 0) k_ctl_gather<<<count, 64, 0, (hipStream_t)stream>>>(d, count, words, out);
+}
+
+void launch_step_sliced_multi(const SXDesc *d, const void *map_c, int nclause, const void *map_v, int nvar,
+                              int lds_region, int slot, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    if (nclause > 0)
+        k_sx_clause_multi<float><<<nclause, SXShape<float>::BS, 0, st>>>(d, (const int2 *)map_c, slot);
+    const int lds = (int)sizeof(float) * (lds_region > 0 ? lds_region : 1);
+    static int lds_set[MAX_DEV_ATTR] = {};
+    lds_attr((const void *)k_sx_var_multi<float, false>, lds, lds_set);
+    if (nvar > 0) k_sx_var_multi<float, false><<<nvar, SX_F, lds, st>>>(d, (const int2 *)map_v, slot);
 }
 
 void launch_step_sliced(int prec, const DevProb &P, const DevParams &prm, void *vd, void *xs, void *xl, void *e,

@@ -68,6 +68,21 @@ struct DevParams {
     double alpha, beta, gamma, delta, epsilon, zeta, xl_max;
 };
 
+struct Ctl;
+
+// One instance of a sliced step that advances many instances in one launch
+// pair (the sliced tables and state of a resident instance; see DevProb).
+struct SXDesc {
+    int n, m, vstride, pad;
+    const uint32_t *sw;
+    const int32_t *units, *sbrun, *runs, *row_ptr;
+    const uint16_t *sq;
+    const int8_t *unit;
+    void *vd, *vnext, *xs, *xl, *cst, *e;
+    Ctl *ctl;
+    DevParams prm;
+};
+
 // Per-instance control block.  Three rotating slots carry one step's
 // verdicts: step s writes slot s%3 (the clause pass raises the `unsolved`
 // flag group, the variable pass the level groups and `stepped`); the clause
@@ -139,6 +154,14 @@ void launch_step(int prec, const DevProb &P, const DevParams &prm, void *vd, voi
                  void *xl, void *e, Ctl *ctl, int slot, void *stream);
 // the same with the sliced exchange: vnext holds 4 n candidate V values (one
 // per ladder level) written by the fold, cst each clause's last C
+// FP32 sliced step of many instances: nclause blocks (map_c: instance, unit)
+// then nvar blocks (map_v: instance, fold block), at step slot `slot`;
+// lds_region the largest fold-block region among them
+void launch_step_sliced_multi(const SXDesc *d, const void *map_c, int nclause, const void *map_v, int nvar,
+                              int lds_region, int slot, void *stream);
+// copies the first `words` ints of each instance's control block to out
+// (count x words ints)
+void launch_ctl_gather(const SXDesc *d, int count, int words, int *out, void *stream);
 void launch_step_sliced(int prec, const DevProb &P, const DevParams &prm, void *vd, void *xs, void *xl,
                         void *e, void *vnext, void *cst, Ctl *ctl, int slot, void *stream);
 

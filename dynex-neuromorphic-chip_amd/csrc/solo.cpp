@@ -1,6 +1,7 @@
 // solo.cpp -- layout, staging and device view of a solo instance (solo.hpp).
 #include "solo.hpp"
 
+#include <algorithm>
 #include <climits>
 #include <cstddef>
 #include <cstring>
@@ -18,6 +19,41 @@ bool solo_plan(const HostProblem &h, int cus, SlicedPlan &pl)
     for (size_t k = 0; k + 1 < pl.sb_run.size(); k++)
         if (pl.sb_run[k + 1] - pl.sb_run[k] > SX_MAXRUNS) return false;
     return true;
+}
+
+bool solo_plan_units(const HostProblem &h, int cus, int clauses_per_unit, SlicedPlan &pl)
+{
+    if (!h.uniform3 || h.m <= 0) return false;
+    const int want = std::max(1, std::min(cus, (h.m + clauses_per_unit - 1) / clauses_per_unit));
+    if (want < cus && solo_plan(h, want, pl)) return true;
+    return solo_plan(h, cus, pl);
+}
+
+SXDesc solo_desc(const HostProblem &h, const SlicedPlan &pl, const SoloLayout &L, unsigned char *base,
+                 const DevParams &prm)
+{
+    SXDesc d;
+    memset(&d, 0, sizeof(d));
+    d.n = h.n;
+    d.m = h.m;
+    d.vstride = sx_vstride(h.n > 0 ? h.n : 1);
+    d.sw = (const uint32_t *)(base + L.sw);
+    d.units = (const int32_t *)(base + L.units);
+    d.sbrun = (const int32_t *)(base + L.sbrun);
+    d.runs = (const int32_t *)(base + L.runs);
+    d.row_ptr = (const int32_t *)(base + L.row);
+    d.sq = (const uint16_t *)(base + L.sq);
+    d.unit = h.has_unit ? (const int8_t *)(base + L.unit) : nullptr;
+    d.vd = base + L.vd;
+    d.vnext = base + L.vnext;
+    d.xs = base + L.xs;
+    d.xl = base + L.xl;
+    d.cst = base + L.cst;
+    d.e = base + L.e;
+    d.ctl = (Ctl *)(base + L.ctl);
+    d.prm = prm;
+    (void)pl;
+    return d;
 }
 
 SoloLayout solo_layout(const HostProblem &h, const SlicedPlan &pl)

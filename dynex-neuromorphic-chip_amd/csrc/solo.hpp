@@ -20,6 +20,9 @@ namespace dnx {
 // The sliced plan with the single-instance path's parameters for FP32 on a
 // device of `cus` compute units; false when the exchange does not apply.
 bool solo_plan(const HostProblem &h, int cus, SlicedPlan &pl);
+// The same with units of about `clauses_per_unit` clauses (for many
+// instances sharing one launch), else the single-instance plan.
+bool solo_plan_units(const HostProblem &h, int cus, int clauses_per_unit, SlicedPlan &pl);
 
 // Byte layout of a solo instance inside its range.  [0, upload) is built on
 // the host and copied in one transfer; the rest is device-only state.
@@ -37,6 +40,10 @@ void solo_stage(const HostProblem &h, const SlicedPlan &pl, const double *x0, in
 
 // The device view of the instance placed at `base`.
 DevProb solo_devprob(const HostProblem &h, const SlicedPlan &pl, const SoloLayout &L, unsigned char *base);
+
+// The instance's entry of a many-instance sliced step.
+SXDesc solo_desc(const HostProblem &h, const SlicedPlan &pl, const SoloLayout &L, unsigned char *base,
+                 const DevParams &prm);
 
 // Bytes of the control block's head that carry the run's status, step count,
 // time and per-slot verdicts (what a poll needs).

@@ -18,7 +18,7 @@ EXCHANGE_LITERAL, EXCHANGE_GROUPED, EXCHANGE_SLICED = 0, 1, 2
 FP64, FP32 = 0, 1
 RUNNING, SAT, UNKNOWN, FAILED = 0, 1, 2, 3
 SERVE_ENDED, SERVE_QUEUED, SERVE_NARROW, SERVE_WIDE, SERVE_WIDE_WAIT, SERVE_NOT_ARRIVED = 0, 1, 2, 3, 4, 5
-SERVE_SOLO, SERVE_SOLO_WAIT = 6, 7
+SERVE_SLICED, SERVE_SLICED_WAIT = 6, 7
 
 I32P = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 I8P = np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")
@@ -87,7 +87,7 @@ def lib():
     L.dnx_batch_admit_many.argtypes = [vp, ip, pp, vp, pp, C.c_int64, ip, I32P]
     L.dnx_batch_set_serve_order.argtypes = [vp, C.c_int64]
     L.dnx_batch_get_gang.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
-    L.dnx_batch_set_serve_solo.argtypes = [vp, C.c_int64, ip]
+    L.dnx_batch_set_serve_sliced.argtypes = [vp, C.c_int64, ip]
     L.dnx_batch_served_by.argtypes = [vp, I64P]
     L.dnx_batch_advance.argtypes = [vp, ip]
     L.dnx_batch_poll.argtypes = [vp, vp, vp, vp, vp]
@@ -455,14 +455,14 @@ class Batch:
     def set_serve_order(self, las_steps):
         check(lib().dnx_batch_set_serve_order(self.h, int(las_steps)))
 
-    def set_serve_solo(self, min_clauses, max_active=2):
-        check(lib().dnx_batch_set_serve_solo(self.h, int(min_clauses), int(max_active)))
+    def set_serve_sliced(self, min_clauses, max_active=1024):
+        check(lib().dnx_batch_set_serve_sliced(self.h, int(min_clauses), int(max_active)))
 
     def served_by(self):
-        """Requests the last serving call retired per lane: narrow, wide, solo, never runnable."""
+        """Requests the last serving call retired per lane: narrow, wide, sliced, never runnable."""
         c = np.zeros(4, np.int64)
         check(lib().dnx_batch_served_by(self.h, c))
-        return dict(narrow=int(c[0]), wide=int(c[1]), solo=int(c[2]), too_large=int(c[3]))
+        return dict(narrow=int(c[0]), wide=int(c[1]), sliced=int(c[2]), too_large=int(c[3]))
 
     def set_wide(self, max_clauses, steps_per_sync=300):
         check(lib().dnx_batch_set_wide(self.h, int(max_clauses), int(steps_per_sync)))

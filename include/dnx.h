@@ -335,19 +335,22 @@ int dnx_batch_set_serve_big(dnx_batch *b, int64_t clauses, int64_t active_clause
  * every member by least attained service.  Default 1024. */
 int dnx_batch_set_serve_order(dnx_batch *b, int64_t las_steps);
 
-/* Serving: 3-SAT requests with at least min_clauses clauses (0, the
- * default, turns the lane off) run in the solo lane: alone on the whole
- * device, with the single-instance path's sliced exchange (its plan built on
- * host threads after arrival), their state resident in the batch's arena,
- * at most max_active (1-4096, default 1024) resident at a time.  30-step
- * chunks go to resident requests in the wide lane's order (least attained
- * service below dnx_batch_set_serve_order's las_steps, then least expected
- * work).  A request whose plan does not apply goes to the other lanes.
- * Results are the bits of solving it alone. */
-int dnx_batch_set_serve_solo(dnx_batch *b, int64_t min_clauses, int max_active);
+/* Serving: 3-SAT requests with at least min_clauses clauses (default
+ * 60,000; 0 turns the lane off) run in the sliced lane: the single-instance
+ * path's sliced exchange (each request's plan built on host threads after
+ * arrival, units of about 3,000 clauses), their state resident in the
+ * batch's arena, at most max_active (1-4096, default 1024) resident at a
+ * time.  Each 30-step chunk advances an active set of them together, one
+ * launch pair per step over all their units and fold blocks, chosen in the
+ * wide lane's order (least attained service below
+ * dnx_batch_set_serve_order's las_steps, then least expected work) up to
+ * dnx_batch_set_serve_big's active_clauses.  A request whose plan does not
+ * apply goes to the other lanes.  Results are the bits of solving it
+ * alone. */
+int dnx_batch_set_serve_sliced(dnx_batch *b, int64_t min_clauses, int max_active);
 
 /* Requests the last serving call (dnx_serve_trace / dnx_serve_planted)
- * retired in each lane: counts[0] narrow, [1] wide, [2] solo, [3] larger
+ * retired in each lane: counts[0] narrow, [1] wide, [2] sliced, [3] larger
  * than the arena (DNX_FAILED). */
 int dnx_batch_served_by(dnx_batch *b, int64_t *counts);
 
@@ -427,8 +430,8 @@ int dnx_serve_trace(dnx_batch *b, int count, dnx_problem *const *probs, const dn
 #define DNX_SERVE_WIDE 3        /* running in the wide lane, in its active set */
 #define DNX_SERVE_WIDE_WAIT 4   /* resident in the wide lane, waiting for its active set */
 #define DNX_SERVE_NOT_ARRIVED 5 /* arrival after the deadline */
-#define DNX_SERVE_SOLO 6        /* running in the solo lane (dnx_batch_set_serve_solo) */
-#define DNX_SERVE_SOLO_WAIT 7   /* routed to the solo lane, plan or turn not ready */
+#define DNX_SERVE_SLICED 6      /* resident in the sliced lane (dnx_batch_set_serve_sliced) */
+#define DNX_SERVE_SLICED_WAIT 7 /* routed to the sliced lane, plan or room not ready */
 
 /* The served planted stream: Poisson arrivals at rate_per_s for duration_s
  * (at most max_requests), n log-uniform in [n_min, n_max], clause ratio

@@ -499,18 +499,24 @@ def test_serving_results_equal_solo_runs(ctx, inst_set, oracle_results, big):
     assert np.max(r["wait"]) > 0  # some requests had to queue
 
 
-def test_serving_above_default_wide_threshold(ctx, inst_set, oracle_results):
-    """The default policy: a request of 63,000 clauses (above the 60,000-clause
-    default) runs in the wide lane while small requests arrive around it in
-    the narrow lane; each result equals its solo run (the float oracle)."""
+@pytest.mark.parametrize("lane", ["sliced", "wide"])
+def test_serving_above_default_wide_threshold(ctx, inst_set, oracle_results, lane):
+    """A request of 63,000 clauses (above the 60,000-clause defaults) runs in
+    the sliced lane by default, in the wide lane with the sliced lane off,
+    while small requests arrive around it in the narrow lane; each result
+    equals its solo run (the float oracle)."""
     n, m = 9000, 63000
     big = (n, planted_cp(m), dnx.generate_planted(6300, n, m), dnx.init_uniform(6300, n, m))
     insts = list(inst_set[:6]) + [big] + list(inst_set[6:12])
     probs = [dnx.host_problem(nn, cp, lits) for nn, cp, lits, _ in insts]
     arrivals = np.sort(np.random.default_rng(4).uniform(0.0, 0.1, len(insts)))
     b = dnx.Batch(ctx, slots=8)
+    if lane == "wide":
+        b.set_serve_sliced(0)
     r = b.serve(probs, [x0 for *_, x0 in insts], arrivals, cap=1500, steps_per_launch=16, results=True)
+    lanes = b.served_by()
     b.close()
+    assert lanes[lane] == 1 and lanes["narrow"] == len(insts) - 1
     for p in probs:
         p.free()
     ref = oracle_results[:6] + [oracle_solve(*big, 1500)] + oracle_results[6:12]
@@ -519,17 +525,17 @@ def test_serving_above_default_wide_threshold(ctx, inst_set, oracle_results):
     assert r["status"][6] == dnx.SAT
 
 
-def test_serving_solo_lane_equals_solo_runs(ctx, inst_set, oracle_results):
-    """3-SAT requests of at least 5,000 clauses run in the solo lane (alone on
-    the device, the single-instance path's sliced exchange) while the others
-    run in the narrow lane around them; every result equals the float oracle
-    (steps, time, final state bits, assignment), and the lane counts say
-    which lane retired each."""
+def test_serving_sliced_lane_equals_solo_runs(ctx, inst_set, oracle_results):
+    """3-SAT requests of at least 5,000 clauses run in the sliced lane (the
+    single-instance path's sliced exchange, several requests per launch)
+    while the others run in the narrow lane around them; every result equals
+    the float oracle (steps, time, final state bits, assignment), and the
+    lane counts say which lane retired each."""
     insts = list(inst_set[:16])
     probs = [dnx.host_problem(nn, cp, lits) for nn, cp, lits, _ in insts]
     arrivals = np.sort(np.random.default_rng(8).uniform(0.0, 0.05, len(insts)))
     b = dnx.Batch(ctx, slots=8)
-    b.set_serve_solo(5000, 2)
+    b.set_serve_sliced(5000, 2)
     r = b.serve(probs, [x0 for *_, x0 in insts], arrivals, cap=1500, steps_per_launch=16, results=True)
     lanes = b.served_by()
     b.close()
@@ -538,9 +544,9 @@ def test_serving_solo_lane_equals_solo_runs(ctx, inst_set, oracle_results):
     for i, ((nn, *_), (steps, t, status, xo)) in enumerate(zip(insts, oracle_results[:16])):
         check_served(r, i, steps, t, status, xo, nn)
     expect = sum(1 for nn, cp, lits, _ in insts if len(cp) - 1 >= 5000 and len(lits) == 3 * (len(cp) - 1))
-    print("lanes", lanes, "expected solo", expect)
-    assert expect > 0 and lanes["solo"] == expect
-    assert lanes["narrow"] + lanes["solo"] == len(insts)
+    print("lanes", lanes, "expected sliced", expect)
+    assert expect > 0 and lanes["sliced"] == expect
+    assert lanes["narrow"] + lanes["sliced"] == len(insts)
 
 
 def unsat3(n, m, seed):
@@ -564,9 +570,9 @@ def test_serve_deadline_reports_where(ctx):
     time-to-solution, and report where they were: three in the narrow lane,
     two in the wide lane (a budget that holds one at a time, so one runs and
     one waits), one never admitted (no slot), one arriving after the
-    deadline, two in the solo lane (one at a time: one runs, one waits); the
+    deadline, two in the sliced lane (room for one: one runs, one waits); the
     steps they had completed come back too."""
-    # narrow x3, solo, wide x2, solo, queued, late (arrival order)
+    # narrow x3, sliced, wide x2, sliced, queued, late (arrival order)
     insts = [unsolvable(1500, 6000, 70 + i) for i in range(3)] + [unsat3(5000, 25000, 95)]
     insts += [unsolvable(2500, 9000, 80 + i) for i in range(2)] + [unsat3(5000, 25000, 96)]
     insts += [unsolvable(800, 3000, 90), unsolvable(800, 3000, 91)]
@@ -574,7 +580,7 @@ def test_serve_deadline_reports_where(ctx):
     probs = [dnx.host_problem(n, cp, lits) for n, cp, lits, _ in insts]
     b = dnx.Batch(ctx, slots=5)
     b.set_serve_big(8000, 10000)
-    b.set_serve_solo(20000, 1)
+    b.set_serve_sliced(20000, 1)
     r = b.serve(probs, [x0 for *_, x0 in insts], arrivals, cap=10 ** 9, steps_per_launch=64, deadline=1.0)
     b.close()
     for p in probs:
@@ -583,7 +589,7 @@ def test_serve_deadline_reports_where(ctx):
     w = r["where"]
     assert list(w[:3]) == [dnx.SERVE_NARROW] * 3
     assert set(w[4:6]) <= {dnx.SERVE_WIDE, dnx.SERVE_WIDE_WAIT} and dnx.SERVE_WIDE in set(w[4:6])
-    assert w[3] == dnx.SERVE_SOLO and w[6] == dnx.SERVE_SOLO_WAIT  # one solo run at a time
+    assert w[3] == dnx.SERVE_SLICED and w[6] == dnx.SERVE_SLICED_WAIT  # room for one
     assert w[7] == dnx.SERVE_QUEUED and w[8] == dnx.SERVE_NOT_ARRIVED
     assert np.all(r["steps"][:3] > 0) and r["steps"][7] == 0 and r["steps"][8] == 0
     assert r["steps"][4:6].max() > 0 and r["steps"][3] > 0 and r["steps"][6] == 0
