@@ -459,7 +459,7 @@ def main():
     ab_sum = np.sum([r["alg"] for r in runs], axis=0)
     names = ["k_batch<mode 0>", "k_batch<mode 1>", "k_batch<mode 2>",
              "k_wide_clause+k_wide_var (one chunk of steps between status syncs)", "k_gang",
-             "k_batch<mode 0> tail launch beside wide mode"]
+             "k_batch<mode 0> tail launch beside wide mode", "k_epoch (gangs beside single-workgroup instances)"]
     dom = int(np.argmax(ms_sum))
     avg_us = float(ms_sum[dom] / nl_sum[dom] * 1e3) if nl_sum[dom] else 0.0
     per_launch = float(ab_sum[dom] / nl_sum[dom]) if nl_sum[dom] else 0.0
@@ -469,10 +469,10 @@ def main():
                     frac=round(achieved / HBM_PEAK_GBPS, 4),
                     traffic=TRAFFIC.get(dom) if (args.instances, args.seed) == (4096, 4096) else None,
                     kernel=names[dom], alg_bytes_per_launch=round(per_launch), avg_launch_us=round(avg_us, 2),
-                    device_ms={names[i]: round(float(ms_sum[i]), 2) for i in range(6)},
-                    launches={names[i]: int(nl_sum[i]) for i in range(6)},
+                    device_ms={names[i]: round(float(ms_sum[i]), 2) for i in range(len(names))},
+                    launches={names[i]: int(nl_sum[i]) for i in range(len(names))},
                     achieved_GBps={names[i]: round(float(ab_sum[i] / ms_sum[i] / 1e6), 1) if ms_sum[i] else None
-                                   for i in range(6)},
+                                   for i in range(len(names))},
                     all_kernels_GBps=round(float(ab_sum.sum()) / all_s / 1e9, 1) if all_s else None)
 
     rng = np.random.default_rng(7)

@@ -182,10 +182,12 @@ __device__ __forceinline__ void store_group(const GInst &I, int g, int c_lo, int
     }
 }
 
+// One workgroup's work loop: takes instances from the list and advances each
+// up to K steps (or, in a flow, until it ends or the flow stops).
 template <int MODE, int BS>
-__global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, BDyn *__restrict__ dyn,
-                                              const int *__restrict__ work, int nwork, int *__restrict__ head,
-                                              int K, BFlow *__restrict__ flow)
+__device__ __forceinline__ void flow_worker(const BInst *__restrict__ insts, BDyn *__restrict__ dyn,
+                                            const int *__restrict__ work, int nwork, int *__restrict__ head,
+                                            int K, BFlow *__restrict__ flow)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_item;
@@ -197,7 +199,15 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
         __syncthreads();  // previous item finished with LDS and s_item
         if (tid == 0) {
             const bool stopped = flow && __hip_atomic_load(&flow->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_item = stopped ? nwork : atomicAdd(head, 1);
+            const int item = stopped ? nwork : atomicAdd(head, 1);
+            // a workgroup that finds the list empty while the flow runs is
+            // idle from now on: once idle_trigger of them are, the flow stops
+            // (the idle CUs can then serve the running instances as gangs)
+            if (item >= nwork && !stopped && flow && flow->idle_trigger > 0 &&
+                __hip_atomic_fetch_add(&flow->idle, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 >=
+                    flow->idle_trigger)
+                __hip_atomic_store(&flow->stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_item = item;
             s_lev[0] = 0;
             s_lev[1] = 0;
         }
@@ -617,6 +627,14 @@ __global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, B
     }
 }
 
+template <int MODE, int BS>
+__global__ __launch_bounds__(BS) void k_batch(const BInst *__restrict__ insts, BDyn *__restrict__ dyn,
+                                              const int *__restrict__ work, int nwork, int *__restrict__ head,
+                                              int K, BFlow *__restrict__ flow)
+{
+    flow_worker<MODE, BS>(insts, dyn, work, nwork, head, K, flow);
+}
+
 // ---------------------------------------------------------------------------
 // Wide mode.  When few instances remain, one workgroup per instance leaves
 // most of the chip idle and each instance latency-bound.  Wide mode advances
@@ -951,43 +969,35 @@ __device__ __forceinline__ bool gang_barrier(unsigned long long *words, int gang
     return s_ok != 0;
 }
 
-// map[b] = (slot, rank, G, gang) for block b (slot -1: idle block).  Before
-// the first step every block checks that it runs on XCD b % 8; one that does
-// not sets *err = 2, and since every block passes a launch-wide arrival count
-// (*arrive, compared against arrive_base + nblk) before touching any state,
-// a misplaced launch ends with every instance unchanged.
-__global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, BDyn *__restrict__ dyn,
-                                             const int4 *__restrict__ map, int nblk,
-                                             unsigned long long *__restrict__ words, unsigned long long base, int K,
-                                             int *__restrict__ err, int *__restrict__ arrive, int arrive_base)
+// LDS of one gang workgroup: its clauses' literals, transpose slots and
+// memories, and the per-wave staging of the fold.
+template <int GB, int CW>
+struct GangLds {
+    int32_t lits[3 * GANG_CPW];
+    int32_t gpos[3 * GANG_CPW];
+    float xs[GANG_CPW], xl[GANG_CPW];
+    float cst[GANG_CPW];  // each clause's C of the state the previous step evaluated
+    float stage[GB / 64][CW];
+};
+constexpr int GANG_LDS = (int)sizeof(GangLds<GBS, WCAPW>);
+// the epoch launch's workgroups: 1024 threads (the fused kernel's mode-0
+// shape); its gang members stage up to 1024 contributions per wave (a wave
+// whose variables have more folds them from device memory, same order)
+constexpr int EBS = 1024, ECAPW = 1024;
+constexpr int EGANG_LDS = (int)sizeof(GangLds<EBS, ECAPW>);
+
+// Workgroup `rank` of gang `gang` (G workgroups) advances instance `slot` by
+// up to K steps.
+template <int GBS, int CW>
+__device__ __forceinline__ void gang_run(const BInst *__restrict__ insts, BDyn *__restrict__ dyn, int slot, int rank,
+                                         int G, int gang, unsigned long long *__restrict__ words,
+                                         unsigned long long base, int K, int *__restrict__ err, GangLds<GBS, CW> &L)
 {
-    __shared__ int32_t s_lits[3 * GANG_CPW];
-    __shared__ int32_t s_gpos[3 * GANG_CPW];
-    __shared__ float s_xs[GANG_CPW], s_xl[GANG_CPW];
-    __shared__ float stage[GBS / 64][WCAPW];
     __shared__ int sh[GBS / 64];
-    __shared__ int s_go;
-    const int b = blockIdx.x;
-    if (b >= nblk) return;
     const int tid = threadIdx.x;
-    if (tid == 0) {
-        if (xcc_id() != (b & 7)) __hip_atomic_store(err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        __hip_atomic_fetch_add(arrive, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        int it = 0;
-        while (__hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - arrive_base < nblk &&
-               ++it < GANG_SPIN)
-            __builtin_amdgcn_s_sleep(1);
-        __threadfence();
-        if (it >= GANG_SPIN) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_go = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
-    }
-    __syncthreads();
-    if (!s_go) return;
-    const int4 gm = map[b];
-    const int slot = __builtin_amdgcn_readfirstlane(gm.x), rank = __builtin_amdgcn_readfirstlane(gm.y);
-    const int G = __builtin_amdgcn_readfirstlane(gm.z), gang = __builtin_amdgcn_readfirstlane(gm.w);
-    if (slot < 0) return;
+    int32_t *s_lits = L.lits, *s_gpos = L.gpos;
+    float *s_xs = L.xs, *s_xl = L.xl, *s_cst = L.cst;
+    auto &stage = L.stage;
     const BDyn d = dyn[slot];
     if (d.status != DNX_RUNNING) return;
     const GInst I = global_view(insts[slot]);
@@ -1043,6 +1053,10 @@ __global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, B
         }
         // ---- clause pass ----------------------------------------------------
         int unsolved = 0;
+        // steady: a step of this launch before this one stored each clause's
+        // C, and the memories come out of a memory update (clamped), so the
+        // steady-state forms apply (V is clamped below) -- the same bits
+        const bool steady = k >= 1 && steps >= 2;
         if (!general) {
             // 3-SAT: the (V, dV) pairs of all of a thread's clauses (at most
             // four: GANG_CPW = 4 GBS) are requested before any is evaluated
@@ -1063,20 +1077,31 @@ __global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, B
                     const int32_t l0 = s_lits[3 * i], l1 = s_lits[3 * i + 1], l2 = s_lits[3 * i + 2];
                     const BF2 a0 = A[u][0], a1 = A[u][1], a2 = A[u][2];
                     float xs = s_xs[i], xl = s_xl[i];
-                    float r0 = a0.v, r1 = a1.v, r2 = a2.v;
-                    if (pend) {
-                        memory_update(p, clause3_C_vals<float>(l0, l1, l2, r0, r1, r2), h, xs, xl);
+                    auto emit = [&](int j, float val) { I.e[s_gpos[3 * i + j]] = val; };
+                    float C;
+                    if (steady) {
+                        memory_update_steady(p, s_cst[i], h, xs, xl);
                         s_xs[i] = xs;
                         s_xl[i] = xl;
-                        r0 = clampV(a0.v + h * a0.d);
-                        r1 = clampV(a1.v + h * a1.d);
-                        r2 = clampV(a2.v + h * a2.d);
+                        C = clause3_steady(p, l0, l1, l2, clampV(a0.v + h * a0.d), clampV(a1.v + h * a1.d),
+                                           clampV(a2.v + h * a2.d), xs, xl, emit);
+                        unsolved |= !(C < 0.5f);
+                    } else {
+                        float r0 = a0.v, r1 = a1.v, r2 = a2.v;
+                        if (pend) {
+                            memory_update(p, clause3_C_vals<float>(l0, l1, l2, r0, r1, r2), h, xs, xl);
+                            s_xs[i] = xs;
+                            s_xl[i] = xl;
+                            r0 = clampV(a0.v + h * a0.d);
+                            r1 = clampV(a1.v + h * a1.d);
+                            r2 = clampV(a2.v + h * a2.d);
+                        }
+                        bool sat;
+                        C = clause3_vals<float>(p, l0, l1, l2, r0, r1, r2, clamp01(xs), clampXl(xl, p.xlmax), emit,
+                                                sat);
+                        unsolved |= !(C < 0.5f) || !sat;
                     }
-                    auto emit = [&](int j, float val) { I.e[s_gpos[3 * i + j]] = val; };
-                    bool sat;
-                    const float C = clause3_vals<float>(p, l0, l1, l2, r0, r1, r2, clamp01(xs), clampXl(xl, p.xlmax),
-                                                        emit, sat);
-                    unsolved |= !(C < 0.5f) || !sat;
+                    s_cst[i] = C;
                 }
             }
         }
@@ -1118,7 +1143,7 @@ __global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, B
             const bool first = v0 == vb + w * 64;
             const int rb = first ? RB0 : vok ? I.row_ptr[v] : 0, re = first ? RE0 : vok ? I.row_ptr[v + 1] : 0;
             const BF2 a = first ? PA : vok ? vdr_l2(I, v) : BF2{0.0f, 0.0f};  // requested before the fold
-            const float dVf = wave_fold_vm<WCAPW>([&](int q) { return er_l2(I, q); }, rb, re,
+            const float dVf = wave_fold_vm<CW>([&](int q) { return er_l2(I, q); }, rb, re,
                                                   v0 < n ? min(63, n - 1 - v0) : 0, stage[w]);
             if (vok) {
                 const float V = pend ? clampV(a.v + h * a.d) : a.v;
@@ -1163,22 +1188,105 @@ __global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, B
     }
 }
 
-int gang_occupancy()
+// map[b] = (slot, rank, G, gang) for block b (slot -1: idle block).  Before
+// the first step every block checks that it runs on XCD b % 8; one that does
+// not sets *err = 2, and since every block passes a launch-wide arrival count
+// (*arrive, compared against arrive_base + nblk) before touching any state,
+// a misplaced launch ends with every instance unchanged.
+__global__ __launch_bounds__(GBS) void k_gang(const BInst *__restrict__ insts, BDyn *__restrict__ dyn,
+                                             const int4 *__restrict__ map, int nblk,
+                                             unsigned long long *__restrict__ words, unsigned long long base, int K,
+                                             int *__restrict__ err, int *__restrict__ arrive, int arrive_base)
 {
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gang, GBS, 0) != hipSuccess) return 0;
-    return nb;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_go;
+    const int b = blockIdx.x;
+    if (b >= nblk) return;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        if (xcc_id() != (b & 7)) __hip_atomic_store(err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        __hip_atomic_fetch_add(arrive, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int it = 0;
+        while (__hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - arrive_base < nblk &&
+               ++it < GANG_SPIN)
+            __builtin_amdgcn_s_sleep(1);
+        __threadfence();
+        if (it >= GANG_SPIN) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_go = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+    }
+    __syncthreads();
+    if (!s_go) return;
+    const int4 gm = map[b];
+    const int slot = __builtin_amdgcn_readfirstlane(gm.x), rank = __builtin_amdgcn_readfirstlane(gm.y);
+    const int G = __builtin_amdgcn_readfirstlane(gm.z), gang = __builtin_amdgcn_readfirstlane(gm.w);
+    if (slot < 0) return;
+    gang_run<GBS, WCAPW>(insts, dyn, slot, rank, G, gang, words, base, K, err, *(GangLds<GBS, WCAPW> *)smem);
 }
 
-int launch_gang(const BInst *insts, BDyn *dyn, const int *map, int nblk, unsigned long long *words,
-                unsigned long long base, int K, int *err, int *arrive, int arrive_base, void *stream)
+// ---------------------------------------------------------------------------
+// Epoch launch: the tail of a batch as one cooperative launch of one
+// workgroup per CU, in which gangs and single-workgroup instances share the
+// chip.  A workgroup takes its role from the XCD it runs on and the order in
+// which workgroups start there (flow->xcd_rank): the first gcount[x]
+// workgroups on XCD x are the gang members listed at gmap[x * GANG_MAX + r]
+// as (slot, rank, G, gang); every other workgroup is a flow worker, which
+// takes instances from the work list and runs each until it ends or the flow
+// stops.  Roles are thus placed by where the workgroups actually run, not by
+// their block index.  A gang member first waits (bounded) until all of its
+// XCD's gang members have started (*err = 3 if they do not: the gangs then
+// change nothing); the gangs advance their instances up to K steps, and the
+// last gang workgroup to finish raises the flow's stop, after which the flow
+// workers leave at the end of their current step.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(EBS) void k_epoch(const BInst *__restrict__ insts, BDyn *__restrict__ dyn,
+                                              const int4 *__restrict__ gmap, const int *__restrict__ gcount,
+                                              unsigned long long *__restrict__ words, unsigned long long base, int K,
+                                              int *__restrict__ err, const int *__restrict__ work, int nwork,
+                                              int *__restrict__ head, BFlow *__restrict__ flow)
 {
-    if (nblk <= 0) return 0;
-    const int4 *mp = (const int4 *)map;
-    void *args[] = {(void *)&insts, (void *)&dyn, (void *)&mp, (void *)&nblk, (void *)&words, (void *)&base,
-                    (void *)&K, (void *)&err, (void *)&arrive, (void *)&arrive_base};
-    return (int)hipLaunchCooperativeKernel((const void *)k_gang, dim3(nblk), dim3(GBS), args, 0,
-                                           (hipStream_t)stream);
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_role;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        const int x = xcc_id();
+        int role = -1;
+        if (x >= 0 && x < GANG_XCD) {
+            const int gx = gcount[x];
+            if (gx > 0) {
+                const int r = __hip_atomic_fetch_add(&flow->xcd_rank[x], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (r < gx) {
+                    role = x * GANG_MAX + r;
+                    // every gang member of this XCD has started before any
+                    // of them touches an instance
+                    int it = 0;
+                    while (__hip_atomic_load(&flow->xcd_rank[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gx &&
+                           ++it < GANG_SPIN)
+                        __builtin_amdgcn_s_sleep(1);
+                    if (it >= GANG_SPIN) __hip_atomic_store(err, 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) role = -2;
+                }
+            }
+        }
+        s_role = role;
+    }
+    __syncthreads();
+    const int role = s_role;
+    if (role == -1) {
+        flow_worker<BM_LDS_ALL, EBS>(insts, dyn, work, nwork, head, INT_MAX, flow);
+        return;
+    }
+    if (role >= 0) {
+        const int4 gm = gmap[role];
+        const int slot = __builtin_amdgcn_readfirstlane(gm.x), rank = __builtin_amdgcn_readfirstlane(gm.y);
+        const int G = __builtin_amdgcn_readfirstlane(gm.z), gang = __builtin_amdgcn_readfirstlane(gm.w);
+        if (slot >= 0)
+            gang_run<EBS, ECAPW>(insts, dyn, slot, rank, G, gang, words, base, K, err, *(GangLds<EBS, ECAPW> *)smem);
+    }
+    __syncthreads();
+    if (tid == 0 &&
+        __hip_atomic_fetch_add(&flow->gangs_done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 >= flow->gangs)
+        __hip_atomic_store(&flow->stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void k_batch_zero(int *p, int count)
@@ -1241,6 +1349,54 @@ static void lds_attr(const void *fn, int lds, int (&set)[MAX_DEV_ATTR])
         (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
         set[dev] = lds;
     }
+}
+
+int gang_occupancy()
+{
+    static int set[MAX_DEV_ATTR] = {};
+    lds_attr((const void *)k_gang, GANG_LDS, set);
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gang, GBS, GANG_LDS) != hipSuccess) return 0;
+    return nb;
+}
+
+int launch_gang(const BInst *insts, BDyn *dyn, const int *map, int nblk, unsigned long long *words,
+                unsigned long long base, int K, int *err, int *arrive, int arrive_base, void *stream)
+{
+    if (nblk <= 0) return 0;
+    static int set[MAX_DEV_ATTR] = {};
+    lds_attr((const void *)k_gang, GANG_LDS, set);
+    const int4 *mp = (const int4 *)map;
+    void *args[] = {(void *)&insts, (void *)&dyn, (void *)&mp, (void *)&nblk, (void *)&words, (void *)&base,
+                    (void *)&K, (void *)&err, (void *)&arrive, (void *)&arrive_base};
+    return (int)hipLaunchCooperativeKernel((const void *)k_gang, dim3(nblk), dim3(GBS), args, GANG_LDS,
+                                           (hipStream_t)stream);
+}
+
+int epoch_lds(int flow_lds) { return flow_lds > EGANG_LDS ? flow_lds : EGANG_LDS; }
+
+int epoch_occupancy(int flow_lds)
+{
+    static int set[MAX_DEV_ATTR] = {};
+    const int lds = epoch_lds(flow_lds);
+    lds_attr((const void *)k_epoch, lds, set);
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_epoch, EBS, lds) != hipSuccess) return 0;
+    return nb;
+}
+
+int launch_epoch(const BInst *insts, BDyn *dyn, const int *gmap, const int *gcount, unsigned long long *words,
+                 unsigned long long base, int K, int *err, const int *work, int nwork, int *head, BFlow *flow,
+                 int flow_lds, int grid, void *stream)
+{
+    static int set[MAX_DEV_ATTR] = {};
+    const int lds = epoch_lds(flow_lds);
+    lds_attr((const void *)k_epoch, lds, set);
+    const int4 *gm = (const int4 *)gmap;
+    void *args[] = {(void *)&insts, (void *)&dyn, (void *)&gm,   (void *)&gcount, (void *)&words, (void *)&base,
+                    (void *)&K,     (void *)&err, (void *)&work, (void *)&nwork,  (void *)&head,  (void *)&flow};
+    return (int)hipLaunchCooperativeKernel((const void *)k_epoch, dim3(grid), dim3(EBS), args, lds,
+                                           (hipStream_t)stream);
 }
 
 template <int MODE, int BS>

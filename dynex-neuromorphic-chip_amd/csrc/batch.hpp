@@ -71,13 +71,23 @@ struct BDyn {
 // until they end (instead of K steps each): an instance that ends takes its
 // clauses off `live`, and the one that brings live to `threshold` or below
 // raises `stop`; every workgroup then leaves after its current step, with
-// its instance's state written back, and takes no further instance.
+// its instance's state written back, and takes no further instance.  With
+// idle_trigger > 0, `idle` counts the workgroups that found the list empty,
+// and the one that brings it to idle_trigger raises `stop` too.  In an epoch
+// launch (see launch_epoch) the gang workgroups count themselves out in
+// `gangs_done`, and the last of `gangs` to finish raises `stop`.
 struct BFlow {
     long long live;
     long long threshold;
     int stop;
-    int pad[11];
+    int idle;
+    int idle_trigger;
+    int gangs;
+    int gangs_done;
+    int xcd_rank[8];  // epoch launch: workgroups that have started on each XCD
+    int pad[15];
 };
+static_assert(sizeof(BFlow) == 128, "two cache lines");
 
 // one launch of the fused kernel over `nwork` slots listed in `work`: K steps
 // per instance, or (flow != nullptr) each instance until it ends or the flow
@@ -135,5 +145,16 @@ constexpr int GANG_XCD = 8;      // XCDs the dispatcher deals workgroups to, in 
 // launches (this launch waits for arrive_base + nblk)
 int launch_gang(const BInst *insts, BDyn *dyn, const int *map, int nblk, unsigned long long *words,
                 unsigned long long base, int K, int *err, int *arrive, int arrive_base, void *stream);
+
+// epoch launch (see batch.hip): `grid` workgroups (one per CU), gangs listed
+// per XCD in gmap (GANG_XCD x GANG_MAX entries of (slot, rank, G, gang)) with
+// gcount[x] members on XCD x, flow workers over `work`; flow->gangs holds the
+// total of gcount, and flow's counters start at zero.  *err: 1 a gang barrier
+// timed out, 3 not every gang member started (the gangs changed nothing).
+// Returns the hipError_t of the launch.
+int epoch_occupancy(int flow_lds);
+int launch_epoch(const BInst *insts, BDyn *dyn, const int *gmap, const int *gcount, unsigned long long *words,
+                 unsigned long long base, int K, int *err, const int *work, int nwork, int *head, BFlow *flow,
+                 int flow_lds, int grid, void *stream);
 
 }  // namespace dnx

@@ -102,9 +102,10 @@ struct SlotHost {
 // pair read and written (8 per variable).
 static double step_bytes(const SlotHost &sl) { return 4.0 * (double)sl.nnz + 16.0 * sl.m + 8.0 * sl.n; }
 
-// timing buckets: the three fused-kernel modes, wide mode, gang mode, and the
-// fused kernel's tail launch beside wide mode
-constexpr int BK_WIDE = BM_COUNT, BK_GANG = BM_COUNT + 1, BK_TAIL = BM_COUNT + 2, NBUCKET = BM_COUNT + 3;
+// timing buckets: the three fused-kernel modes, wide mode, gang mode, the
+// fused kernel's tail launch beside wide mode, and epoch launches
+constexpr int BK_WIDE = BM_COUNT, BK_GANG = BM_COUNT + 1, BK_TAIL = BM_COUNT + 2, BK_EPOCH = BM_COUNT + 3,
+              NBUCKET = BM_COUNT + 4;
 
 struct Shape {
     int bs;
@@ -201,6 +202,15 @@ struct dnx_batch {
     unsigned long long gang_epoch = 0;
     int gang_arrive = 0;      // arrivals counted on the device so far
     DevBuf b_gmap, b_gwords, b_gerr;
+    // elastic tail: once the fused kernel has CUs to spare (fewer running
+    // instances than CUs), epoch launches give them to the largest running
+    // instances as gangs while the others keep one workgroup each
+    // (batch_epoch_run).  epoch_min_clauses: the smallest instance given a
+    // gang (0, the default, disables epochs); epoch_steps: gang steps per
+    // epoch launch.
+    long long epoch_min_clauses = 0;
+    int epoch_steps = 1024;
+    DevBuf b_egmap;  // GANG_XCD x GANG_MAX (slot, rank, G, gang), then GANG_XCD counts
     // admission staging: two halves of adm_chunk bytes, pinned on the host
     // (h_adm) and on the device (b_adm); an admission waits for its uploads
     // before it returns
@@ -1412,6 +1422,208 @@ static int batch_split_tail(dnx_batch *b, std::chrono::steady_clock::time_point 
     return DNX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Elastic tail.  While more instances run than the chip has CUs, the flow
+// launch keeps every CU busy with one instance each.  Once fewer run, CUs
+// fall idle while the longest instances still need thousands of steps at
+// the pace of one workgroup.  An epoch launch (launch_epoch) gives the spare
+// CUs to the largest running instances as gangs and keeps the others on
+// single workgroups, in one cooperative launch; each epoch ends when its
+// gangs have taken epoch_steps steps, and the next epoch is planned from the
+// statuses (more spare CUs as instances end: more or larger gangs).
+// ---------------------------------------------------------------------------
+struct EpochPlan {
+    std::vector<int> gmap;    // GANG_XCD x GANG_MAX entries of (slot, rank, G, gang)
+    int gcount[GANG_XCD] = {};
+    int members = 0;          // gang workgroups in total
+    int gangs = 0;
+    std::vector<int> flow;    // the other running instances, largest first
+};
+
+// Instances of at least epoch_min_clauses clauses, largest first, each get
+// the workgroups their clauses need in LDS (2048 per workgroup) on the XCD
+// with the most room, as long as one CU stays for each instance left to the
+// flow workers; then the gangs grow one workgroup at a time, largest
+// instance first, up to what they ask for (m / gang_cpw, at most GANG_MAX)
+// while CUs are to spare on their XCDs.  False when no gang forms.
+static bool epoch_plan(const dnx_batch *b, const std::vector<int> &run, EpochPlan &P)
+{
+    const int per_xcd = b->cus / GANG_XCD;
+    std::vector<int> order(run);
+    std::sort(order.begin(), order.end(), [&](int x, int y) {
+        const int mx = b->slots[(size_t)x].m, my = b->slots[(size_t)y].m;
+        return mx != my ? mx > my : x < y;
+    });
+    int spare = b->cus - (int)run.size();  // CUs left once every instance has one
+    int room[GANG_XCD];
+    for (int x = 0; x < GANG_XCD; x++) room[x] = per_xcd;
+    struct G {
+        int slot, x, size, hi;
+    };
+    std::vector<G> gs;
+    std::vector<char> in((size_t)b->nslots, 0);
+    int members = 0;
+    for (int s : order) {
+        const int m = b->slots[(size_t)s].m;
+        if (m < b->epoch_min_clauses) break;
+        const int lo = std::max(1, (m + GANG_CPW - 1) / GANG_CPW);
+        if (lo > GANG_MAX || lo - 1 > spare) continue;
+        int x = 0;
+        for (int y = 1; y < GANG_XCD; y++)
+            if (room[y] > room[x]) x = y;
+        if (room[x] < lo) continue;
+        const int hi = std::max(lo, std::min(GANG_MAX, (m + b->gang_cpw - 1) / b->gang_cpw));
+        if (members + lo > b->gang_blocks) continue;
+        gs.push_back({s, x, lo, hi});
+        room[x] -= lo;
+        spare -= lo - 1;
+        members += lo;
+        in[(size_t)s] = 1;
+    }
+    if (gs.empty()) return false;
+    for (bool grew = true; grew && spare > 0;) {
+        grew = false;
+        for (G &g : gs)
+            if (spare > 0 && g.size < g.hi && room[g.x] > 0 && members < b->gang_blocks) {
+                g.size++;
+                room[g.x]--;
+                spare--;
+                members++;
+                grew = true;
+            }
+    }
+    P.gmap.assign((size_t)GANG_XCD * GANG_MAX * 4, -1);
+    for (int x = 0; x < GANG_XCD; x++) P.gcount[x] = 0;
+    P.members = 0;
+    P.gangs = (int)gs.size();
+    for (size_t gi = 0; gi < gs.size(); gi++) {
+        const G &g = gs[gi];
+        for (int r = 0; r < g.size; r++) {
+            int *e = &P.gmap[((size_t)g.x * GANG_MAX + (size_t)P.gcount[g.x]) * 4];
+            e[0] = g.slot;
+            e[1] = r;
+            e[2] = g.size;
+            e[3] = (int)gi;
+            P.gcount[g.x]++;
+        }
+        P.members += g.size;
+    }
+    P.flow.clear();
+    for (int s : order)
+        if (!in[(size_t)s]) P.flow.push_back(s);
+    return true;
+}
+
+// One epoch (see above).  *used is false when it does not apply (epochs
+// off, an instance outside mode 0, or no gang forms); the status table in
+// h_dyn (buffer 0) is current on return when *used.
+static int batch_epoch_run(dnx_batch *b, bool *used)
+{
+    *used = false;
+    if (b->epoch_min_clauses <= 0 || b->gang_blocks <= 0 || b->cus <= 0 || b->cus % GANG_XCD != 0) return DNX_OK;
+    std::vector<int> run;
+    for (int s = 0; s < b->nslots; s++) {
+        if (!b->slots[(size_t)s].used || b->h_dyn[s].status != DNX_RUNNING) continue;
+        if (b->slots[(size_t)s].mode != BM_LDS_ALL) return DNX_OK;
+        run.push_back(s);
+    }
+    if (run.empty() || (int)run.size() >= b->cus) return DNX_OK;
+    EpochPlan P;
+    if (!epoch_plan(b, run, P)) return DNX_OK;
+    int lds = 0;
+    for (int s : P.flow) lds = std::max(lds, (int)b->slots[(size_t)s].lds);
+    if (epoch_occupancy(lds) < 1) return DNX_OK;
+    *used = true;
+    const int S = b->nslots;
+    hipStream_t st = b->ctx->stream;
+    hipError_t e;
+    int rc;
+    // gang table and counts, the flow work list, the flow control block
+    const size_t gbytes = sizeof(int) * (P.gmap.size() + GANG_XCD);
+    if (b->b_egmap.size < gbytes && (rc = b->b_egmap.alloc(gbytes))) return rc;
+    std::vector<int> up(P.gmap);
+    up.insert(up.end(), P.gcount, P.gcount + GANG_XCD);
+    if ((e = hipMemcpyAsync(b->b_egmap.ptr, up.data(), gbytes, hipMemcpyHostToDevice, st)) != hipSuccess)
+        return hip_fail(e, "epoch map upload");
+    int *hw = b->h_work + (size_t)b->buf * BM_COUNT * S;
+    int *dw = (int *)b->b_work.ptr + (size_t)b->buf * BM_COUNT * S;
+    int *dh = (int *)b->b_head.ptr + b->buf * BM_COUNT;
+    for (size_t i = 0; i < P.flow.size(); i++) hw[i] = P.flow[i];
+    if (!P.flow.empty() &&
+        (e = hipMemcpyAsync(dw, hw, sizeof(int) * P.flow.size(), hipMemcpyHostToDevice, st)) != hipSuccess)
+        return hip_fail(e, "epoch work list upload");
+    launch_batch_zero(dh, 1, st);
+    if (!b->b_flow.ptr && (rc = b->b_flow.alloc(sizeof(BFlow)))) return rc;
+    BFlow f{};
+    f.live = LLONG_MAX / 2;
+    f.threshold = -1;
+    f.gangs = P.members;
+    if ((e = hipMemcpyAsync(b->b_flow.ptr, &f, sizeof f, hipMemcpyHostToDevice, st)) != hipSuccess)
+        return hip_fail(e, "epoch flow upload");
+    const size_t wbytes = (size_t)P.gangs * 2 * GANG_MAX * GANG_LINE * sizeof(unsigned long long);
+    if (b->b_gwords.size < wbytes) {
+        // fresh lines are zero; epochs only grow, so lines never need clearing again
+        if ((rc = b->b_gwords.alloc(wbytes))) return rc;
+        if ((e = hipMemsetAsync(b->b_gwords.ptr, 0, wbytes, st)) != hipSuccess) return hip_fail(e, "gang words");
+    }
+    if (!b->b_gerr.ptr) {
+        if ((rc = b->b_gerr.alloc(2 * sizeof(int)))) return rc;
+        if ((e = hipMemsetAsync(b->b_gerr.ptr, 0, 2 * sizeof(int), st)) != hipSuccess) return hip_fail(e, "gang flag");
+        b->gang_arrive = 0;
+    }
+    // the host buffers above must not change before the uploads ran
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "epoch upload");
+    std::vector<std::pair<int, long long>> before;
+    if (b->timing)
+        for (int s : run) before.push_back({s, b->h_dyn[s].steps});
+    const int K = std::max(1, b->epoch_steps);
+    const bool timed = b->timing && !b->t_pending[BK_EPOCH];
+    if (timed && (e = hipEventRecord(b->t0[BK_EPOCH], st)) != hipSuccess) return hip_fail(e, "timing");
+    const int le = launch_epoch((const BInst *)b->b_inst.ptr, (BDyn *)b->b_dyn.ptr, (const int *)b->b_egmap.ptr,
+                                (const int *)b->b_egmap.ptr + P.gmap.size(), (unsigned long long *)b->b_gwords.ptr,
+                                b->gang_epoch, K, (int *)b->b_gerr.ptr, dw, (int)P.flow.size(), dh,
+                                (BFlow *)b->b_flow.ptr, lds, b->cus, st);
+    b->gang_epoch += 2 * (unsigned long long)K + 2;
+    if (le != 0) return hip_fail((hipError_t)le, "epoch launch");
+    if (timed) {
+        if ((e = hipEventRecord(b->t1[BK_EPOCH], st)) != hipSuccess) return hip_fail(e, "timing");
+        b->t_pending[BK_EPOCH] = true;
+    }
+    if ((rc = batch_snapshot(b, 0)) || (rc = batch_wait(b, 0))) return rc;
+    batch_collect_timing(b);
+    for (auto &q : before)
+        b->t_alg[BK_EPOCH] += (double)(b->h_dyn[q.first].steps - q.second) * step_bytes(b->slots[(size_t)q.first]);
+    int err = 0;
+    if ((e = hipMemcpy(&err, b->b_gerr.ptr, sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess)
+        return hip_fail(e, "epoch flag read");
+    if (err == 3) {
+        // not every gang member started: the gangs changed nothing (the flow
+        // workers' steps stand); epochs are switched off for this batch
+        b->epoch_min_clauses = 0;
+        if ((e = hipMemset(b->b_gerr.ptr, 0, sizeof(int))) != hipSuccess) return hip_fail(e, "gang flag");
+    } else if (err) {
+        set_error("epoch launch: a gang barrier wait timed out; the gang's instances are in an undefined state");
+        return DNX_E_STATE;
+    }
+    return DNX_OK;
+}
+
+// Idle workgroups at which a flow launch over `grid` workgroups should stop
+// so that the largest running instance can get a gang (0: none could).
+static int epoch_idle_trigger(const dnx_batch *b, const std::vector<int> &run, int grid)
+{
+    if (b->epoch_min_clauses <= 0 || b->gang_blocks <= 0 || b->cus % GANG_XCD != 0) return 0;
+    int mmax = 0;
+    for (int s : run) mmax = std::max(mmax, b->slots[(size_t)s].m);
+    if (mmax < b->epoch_min_clauses) return 0;
+    const int lo = std::max(1, (mmax + GANG_CPW - 1) / GANG_CPW);
+    if (lo > GANG_MAX) return 0;
+    // CUs spare once `need` more workgroups have no instance: a gang of lo
+    // takes lo - 1 of them besides its instance's own
+    const int have = b->cus - grid;
+    return std::max(1, lo - 1 - have);
+}
+
 int dnx_batch_run(dnx_batch *b, int steps_per_launch, double walltime_s, int *remaining)
 {
     if (!b || steps_per_launch <= 0) return DNX_E_ARG;
@@ -1445,9 +1657,24 @@ int dnx_batch_run(dnx_batch *b, int steps_per_launch, double walltime_s, int *re
             std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > walltime_s)
             break;
         long long clauses = 0;
+        bool all0 = true;
+        std::vector<int> run;
         for (int s = 0; s < b->nslots; s++)
-            if (b->slots[(size_t)s].used && known[s].status == DNX_RUNNING) clauses += b->slots[(size_t)s].m;
-        if (clauses <= threshold) {
+            if (b->slots[(size_t)s].used && known[s].status == DNX_RUNNING) {
+                clauses += b->slots[(size_t)s].m;
+                all0 &= b->slots[(size_t)s].mode == BM_LDS_ALL;
+                run.push_back(s);
+            }
+        // the elastic tail (epochs) replaces the clause threshold while every
+        // running instance is a mode-0 instance and the run uses flow launches
+        const bool elastic = b->epoch_min_clauses > 0 && b->gang_blocks > 0 && b->flow && walltime_s <= 0 &&
+                             !b->group_stop && all0 && b->cus % GANG_XCD == 0;
+        if (elastic) {
+            bool used = false;
+            if ((rc = batch_epoch_run(b, &used))) return rc;
+            if (used) continue;
+        }
+        if (!elastic && clauses <= threshold) {
             bool used = false;
             if ((rc = batch_gang_run(b, walltime_s, t0, &used))) return rc;
             if (!used && b->tail_split && !b->group_stop && walltime_s <= 0 &&
@@ -1465,16 +1692,15 @@ int dnx_batch_run(dnx_batch *b, int steps_per_launch, double walltime_s, int *re
         // to look at the statuses every steps_per_launch steps)
         BFlow *flow = nullptr;
         if (b->flow && walltime_s <= 0 && !b->group_stop) {
-            bool all0 = true;
-            for (int s = 0; s < b->nslots && all0; s++)
-                if (b->slots[(size_t)s].used && known[s].status == DNX_RUNNING && b->slots[(size_t)s].mode != BM_LDS_ALL)
-                    all0 = false;
             if (all0) {
                 if (!b->b_flow.ptr && (rc = b->b_flow.alloc(sizeof(BFlow)))) return rc;
                 BFlow f{};
                 f.live = clauses;
-                f.threshold = threshold;
+                f.threshold = elastic ? -1 : threshold;
                 f.stop = 0;
+                // elastic: stop once enough workgroups idle for the largest
+                // running instance to get a gang
+                f.idle_trigger = elastic ? epoch_idle_trigger(b, run, std::min(running, b->cus)) : 0;
                 hipError_t e;
                 if ((e = hipMemcpyAsync(b->b_flow.ptr, &f, sizeof f, hipMemcpyHostToDevice, b->ctx->stream)) !=
                         hipSuccess ||
@@ -1646,6 +1872,14 @@ int dnx_batch_set_tail_split(dnx_batch *b, int enable)
 {
     if (!b) return DNX_E_ARG;
     b->tail_split = enable != 0;
+    return DNX_OK;
+}
+
+int dnx_batch_set_epoch(dnx_batch *b, int64_t min_clauses, int steps)
+{
+    if (!b || min_clauses < 0 || steps <= 0) return DNX_E_ARG;
+    b->epoch_min_clauses = min_clauses;
+    b->epoch_steps = steps;
     return DNX_OK;
 }
 
@@ -2739,8 +2973,9 @@ static int tune_run(dnx_ctx *ctx, const std::vector<TuneInst> &inst, const std::
         dnx_batch *t;
         ~Close() { dnx_batch_close(t); }
     } closer{t};
+    // the fused kernel's shapes alone: no epochs
     if ((rc = dnx_batch_set_shape(t, BM_LDS_ALL, bs0, cap0)) || (rc = dnx_batch_set_shape(t, BM_LDS_V, bs1, 1 << 30)) ||
-        (rc = dnx_batch_set_wide(t, wide, 30)))
+        (rc = dnx_batch_set_wide(t, wide, 30)) || (rc = dnx_batch_set_epoch(t, 0, 256)))
         return rc;
     for (int i : pick) {
         dnx_params prm;
@@ -2781,7 +3016,7 @@ static int tune_gang_run(dnx_ctx *ctx, const std::vector<TuneInst> &inst, const 
         ~Close() { dnx_batch_close(t); }
     } closer{t};
     if ((rc = dnx_batch_set_gang(t, 256, cpw, steps)) ||
-        (rc = dnx_batch_set_wide(t, std::numeric_limits<int64_t>::max(), 30)))
+        (rc = dnx_batch_set_wide(t, std::numeric_limits<int64_t>::max(), 30)) || (rc = dnx_batch_set_epoch(t, 0, 256)))
         return rc;
     for (int i : pick) {
         dnx_params prm;

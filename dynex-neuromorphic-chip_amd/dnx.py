@@ -106,6 +106,7 @@ def lib():
     L.dnx_batch_set_tail_split.argtypes = [vp, ip]
     L.dnx_batch_set_order.argtypes = [vp, C.c_int64]
     L.dnx_batch_set_flow.argtypes = [vp, ip]
+    L.dnx_batch_set_epoch.argtypes = [vp, C.c_int64, ip]
     L.dnx_batch_tune.argtypes = [vp, ip, ip, dp, dp, ip, ip, C.c_uint64, vp, ip, C.POINTER(C.c_int)]
     L.dnx_batch_get_shape.argtypes = [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)]
     L.dnx_state_time.argtypes = [vp, ip, C.POINTER(C.c_double)]
@@ -476,16 +477,21 @@ class Batch:
     def set_flow(self, enable=True):
         check(lib().dnx_batch_set_flow(self.h, int(bool(enable))))
 
+    def set_epoch(self, min_clauses=10000, steps=1024):
+        """Elastic tail: spare CUs serve the largest running instances (at least min_clauses clauses) as
+        gangs beside the single-workgroup ones, in epochs of `steps` gang steps; 0 turns it off."""
+        check(lib().dnx_batch_set_epoch(self.h, int(min_clauses), int(steps)))
+
     def set_wide_fraction(self, fraction=0.0125, steps_per_sync=300):
         check(lib().dnx_batch_set_wide_fraction(self.h, float(fraction), int(steps_per_sync)))
 
     def timing(self, enable=-1, with_bytes=False):
-        """Summed device ms and launch counts per fused-kernel mode (0-2), wide mode (3), gang mode (4) and
-        the fused kernel's tail launch beside wide mode (5); with_bytes adds the compulsory bytes of the
-        steps those launches took."""
-        ms = np.zeros(6, np.float64)
-        nl = np.zeros(6, np.int64)
-        ab = np.zeros(6, np.float64)
+        """Summed device ms and launch counts per fused-kernel mode (0-2), wide mode (3), gang mode (4),
+        the fused kernel's tail launch beside wide mode (5) and epoch launches (6); with_bytes adds the
+        compulsory bytes of the steps those launches took."""
+        ms = np.zeros(7, np.float64)
+        nl = np.zeros(7, np.int64)
+        ab = np.zeros(7, np.float64)
         check(lib().dnx_batch_timing(self.h, int(enable), ms.ctypes.data_as(C.c_void_p), nl.ctypes.data_as(C.c_void_p),
                                      ab.ctypes.data_as(C.c_void_p)))
         return (ms, nl, ab) if with_bytes else (ms, nl)

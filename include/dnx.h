@@ -312,6 +312,25 @@ int dnx_batch_set_order(dnx_batch *b, int64_t large_clauses);
  * steps_per_launch steps.  Results are bit-identical either way. */
 int dnx_batch_set_flow(dnx_batch *b, int enable);
 
+/* The elastic tail (default off; min_clauses 10,000 with 1,024 steps is the
+ * measured setting): while every
+ * running instance is held in LDS (mode 0) and flow launches apply, the
+ * clause threshold below is not used.  Instead, once fewer instances run
+ * than the device has CUs, the spare CUs go to the largest running
+ * instances of at least min_clauses clauses as gangs (the workgroups their
+ * clauses need in LDS, grown towards clauses_per_workgroup of
+ * dnx_batch_set_gang while CUs are spare), in cooperative epoch launches
+ * that keep every other running instance on a workgroup of its own; each
+ * epoch ends once its gangs have taken `steps` steps, and the next is
+ * planned from the statuses.  The flow launch before the first epoch stops
+ * as soon as enough workgroups have run out of instances for the largest
+ * instance's gang.  min_clauses = 0 turns it off.  No counterpart in the
+ * reference; results are bit-identical either way.  On the 4,096-instance
+ * mixed workload the makespan equals the threshold tail's: a gang step
+ * beside single-workgroup instances is latency-bound at ~12 us, against
+ * ~17 us for the largest instances on one workgroup. */
+int dnx_batch_set_epoch(dnx_batch *b, int64_t min_clauses, int steps);
+
 /* The tail threshold relative to the run: once the running instances hold
  * at most `fraction` of the clauses that were running when dnx_batch_run
  * began -- and, with gang mode on, at most 75% of the clauses its workgroup
@@ -386,9 +405,9 @@ int dnx_batch_set_gang(dnx_batch *b, int max_blocks, int clauses_per_workgroup, 
 int dnx_batch_get_gang(dnx_batch *b, int *max_blocks, int *clauses_per_workgroup, int *steps_per_launch);
 
 /* Waits for queued work, then reports the summed device time (ms) and the
- * count of timed launches since the last reset, in six buckets: modes 0-2
- * of the fused kernel, wide mode, gang mode, and the fused kernel's tail
- * launch beside wide mode (arrays of 6).  alg_bytes (may be
+ * count of timed launches since the last reset, in seven buckets: modes 0-2
+ * of the fused kernel, wide mode, gang mode, the fused kernel's tail launch
+ * beside wide mode, and epoch launches (arrays of 7).  alg_bytes (may be
  * NULL) receives the compulsory bytes of the steps those launches took
  * inside dnx_batch_run: per instance step, 4 per literal + 16 per clause +
  * 8 per variable.  enable = 1/0 turns launch timing on/off and resets the

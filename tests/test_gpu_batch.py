@@ -58,7 +58,7 @@ def oracle_solve(n, cp, lits, x0, cap):
 
 
 def run_batch(ctx, inst, cap, spl, shapes=None, stop=True, wide=0, gang=None, split=False, flow=True, order=0,
-              many=False):
+              many=False, epoch=None):
     """gang = (max_blocks, clauses_per_workgroup, steps_per_launch) enables
     gang mode for the tail (the instances left once wide mode applies);
     split keeps the single-chunk LDS instances of the tail in the fused
@@ -66,7 +66,8 @@ def run_batch(ctx, inst, cap, spl, shapes=None, stop=True, wide=0, gang=None, sp
     threshold instead of spl steps per launch; order > 0 starts instances of
     at least that many clauses first and the others smallest first; many
     admits the whole set in one dnx_batch_admit_many call ("defaults": with
-    prms NULL)."""
+    prms NULL); epoch = (min_clauses, steps) turns the elastic tail on (off
+    otherwise)."""
     b = dnx.Batch(ctx, slots=len(inst) + 2, arena_bytes=1 << 30)
     b.set_wide(wide, 30)
     b.set_tail_split(split)
@@ -76,6 +77,7 @@ def run_batch(ctx, inst, cap, spl, shapes=None, stop=True, wide=0, gang=None, sp
         b.set_gang(*gang)
     else:
         b.set_gang(0)
+    b.set_epoch(*(epoch or (0, 256)))
     if shapes:
         for md, (bs, lds) in shapes.items():
             b.set_shape(md, bs, lds)
@@ -181,6 +183,35 @@ def test_gang_mode_gives_identical_bits(ctx, inst_set, oracle_results, wide, gan
         assert same_bits(r["x"], xo)
         if status == 1:
             assert r["unsat"] == 0
+
+
+
+@pytest.mark.parametrize("epoch", [(2000, 32), (6000, 7), (1, 256)])
+def test_epoch_gives_identical_bits(ctx, inst_set, oracle_results, epoch):
+    """The elastic tail: fewer instances than CUs run from the start, so
+    every epoch gives the spare CUs to the instances of at least min_clauses
+    clauses as gangs and keeps the others on single workgroups; gangs are
+    re-planned every `steps` gang steps (instances move between gangs of
+    other sizes and single workgroups).  Every instance computes the
+    oracle's bits."""
+    out = run_batch(ctx, inst_set, 1500, spl=37, gang=(256, 384, 512), flow=True, epoch=epoch)
+    for (n, cp, lits, x0), o, (steps, t, status, xo) in zip(inst_set, out, oracle_results):
+        assert o["status"] == status and o["steps"] == steps and o["t"] == t, (n, o["steps"], steps)
+        assert same_bits(o["x"], xo), n
+
+
+def test_epoch_after_flow_with_more_instances_than_cus(ctx):
+    """More instances than CUs: the first flow launch stops once enough of
+    its workgroups found the list empty for the largest instance's gang, and
+    epochs take over; the bits equal the oracle's for every instance."""
+    inst = make_set(13, 300, nmin=60, nmax=300, with_general=False)
+    big = make_set(17, 3, nmin=2300, nmax=2600, rmin=4.4, rmax=4.6, with_general=False)
+    inst = big[:1] + inst + big[1:]
+    ref = [oracle_solve(n, cp, lits, x0, 700) for n, cp, lits, x0 in inst]
+    out = run_batch(ctx, inst, 700, spl=37, gang=(256, 384, 512), flow=True, epoch=(5000, 64), order=5000)
+    for (n, cp, lits, x0), o, (steps, t, status, xo) in zip(inst, out, ref):
+        assert o["status"] == status and o["steps"] == steps and o["t"] == t, (n, o["steps"], steps)
+        assert same_bits(o["x"], xo), n
 
 
 @pytest.mark.parametrize("flow,order", [(True, 6000), (False, 1 << 40)])
