@@ -1,6 +1,7 @@
 // api.cpp -- the C ABI of libdnx (include/dnx.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <climits>
 #include <cstdio>
@@ -87,7 +88,7 @@ static int ensure_sliced(dnx_problem *p, int prec, bool *have)
         hipError_t e;
         if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "exchange tables sync");
         p->sx_nunits[i] = (int)pl.units.size() / 8;
-        p->sx_region[i] = pl.max_region;
+        p->sx_region[i] = pl.stride;
         p->sx_state[i] = 1;
     }
     *have = true;
@@ -595,10 +596,13 @@ int dnx_state_create(dnx_problem *p, int prec, const dnx_params *prm, const doub
     }
     DevBuf bx;
     // one entry past the end of Xs, Xl, e (and the stored C): the sliced
-    // clause pass sends the stores of lanes without a clause there
+    // clause pass sends the stores of lanes without a clause there; the
+    // sliced exchange's e holds every fold block's padded region
+    size_t elen = (size_t)h.nnz();
+    if (s->sliced) elen = std::max(elen, (size_t)((h.n + SX_F - 1) / SX_F) * (size_t)p->sx_region[prec == DNX_FP64 ? 0 : 1]);
     if ((rc = s->b_vd.alloc(2 * es * (size_t)(h.n > 0 ? h.n : 1))) ||
         (rc = s->b_xs.alloc(es * ((size_t)h.m + 1))) || (rc = s->b_xl.alloc(es * ((size_t)h.m + 1))) ||
-        (rc = s->b_e.alloc(es * ((size_t)h.nnz() + 1))) || (rc = s->b_ctl.alloc(sizeof(Ctl))) ||
+        (rc = s->b_e.alloc(es * (elen + 1))) || (rc = s->b_ctl.alloc(sizeof(Ctl))) ||
         (rc = bx.alloc(es * (N ? N : 1)))) {
         delete s;
         return rc;

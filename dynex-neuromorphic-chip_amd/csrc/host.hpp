@@ -95,6 +95,12 @@ bool grouped_tables(const HostProblem &p, int vb, std::vector<uint32_t> &gw, std
 struct SlicedPlan {
     int S = 0, SUB = 0, F = 0;
     int max_region = 0;             // slots of the largest fold block
+    // every fold block's region of e has the same length, stride (max_region
+    // rounded up to four): block b owns [b stride, (b + 1) stride), its
+    // padding's q entries name slot `stride` (a sink), and e holds
+    // e_len = nblocks x stride entries plus the clause pass's sink after them
+    int stride = 0;
+    long long e_len = 0;
     // 3 per clause, processing order: sign << 31 | LDS index << 15 | run << 7
     // | offset in the run (the run's index within its sub-batch)
     std::vector<uint32_t> gw;

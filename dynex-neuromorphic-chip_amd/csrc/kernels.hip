@@ -259,7 +259,7 @@ struct SXShape<double> {
 // it with their block index, or with the block's entry of a map over many
 // instances).
 template <typename T>
-__device__ __forceinline__ void sx_clause_body(int n, int vstride, int m, long long nnz,
+__device__ __forceinline__ void sx_clause_body(int n, int vstride, int m, long long esink,
                                                const uint32_t *__restrict__ sw, const int32_t *__restrict__ units,
                                                const int32_t *__restrict__ sbrun, const int2 *__restrict__ runs,
                                                PT<T> p, const Pair<T> *__restrict__ vd,
@@ -286,7 +286,8 @@ __device__ __forceinline__ void sx_clause_body(int n, int vstride, int m, long l
     const long long steps0 = ctl->steps;
     const int32_t *U = units + 8 * ub;
     const int c0 = U[0], c1 = U[1], nsl = U[2], sl0 = U[3], sl1 = U[4], sl2 = U[5], sb0 = U[6], run0 = U[7];
-    const int nsb = (c1 - c0 + SUB - 1) / SUB;
+    // sub-batches of equal size (the planner's sx_unit_sub), at most SUB
+    const int nsb = sx_unit_batches(c1 - c0, SUB), sub = sx_unit_sub(c1 - c0, SUB);
     // Every device-memory access of the sub-batch loop is made on every path
     // (indices clamped into the arrays, stores of lanes without a clause sent
     // to the sink entry one past the end of Xs, Xl, cst and e; the run and
@@ -319,6 +320,25 @@ __device__ __forceinline__ void sx_clause_body(int n, int vstride, int m, long l
     // the unit's sub-batch table (first run of each sub-batch), into LDS
     const int sbr = sbrun[sb0 + min(tid, nsb + 1)];
     const int nr0 = sbrun[sb0 + 1] - run0;
+    // V of the unit's slices at ladder level 0, requested before the control
+    // words are known (the level is 0 at almost every step; a step at another
+    // level requests its own level's candidates below): 16-byte words from
+    // the fold's candidates (each level's array padded to whole words)
+    typedef T T16 __attribute__((ext_vector_type(16 / sizeof(T))));
+    constexpr int WPS = S * (int)sizeof(T) / 16;  // 16-byte words per slice
+    constexpr int SR = (WPS + BS - 1) / BS;
+    const int wmax = vstride * (int)sizeof(T) / 16 - 1;
+    T16 x[3][SR];
+    auto stage_loads = [&](int lev) {
+        const T16 *src = (const T16 *)(vnext + (size_t)lev * vstride);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int base = (k == 0 ? sl0 : k == 1 ? sl1 : sl2);
+#pragma unroll
+            for (int u = 0; u < SR; u++) x[k][u] = src[min(max(base, 0) * WPS + tid + u * BS, wmax)];
+        }
+    };
+    stage_loads(0);
     if (status != DNX_RUNNING) return;
     if (done) {
         if (ub == 0 && tid == 0) ctl->status = done;
@@ -335,22 +355,10 @@ __device__ __forceinline__ void sx_clause_body(int n, int vstride, int m, long l
         ctl->steps += 1;
     }
     // V of this state over the unit's slices: every load of a thread's share
-    // requested before the first LDS store; 16-byte words from the fold's
-    // candidates (each level's array padded to whole words), the stored V
-    // at a launch's very first state
+    // requested before the first LDS store; the candidates of the pending
+    // step's level, or the stored V at a launch's very first state
     if (pend) {
-        typedef T T16 __attribute__((ext_vector_type(16 / sizeof(T))));
-        constexpr int WPS = S * (int)sizeof(T) / 16;  // 16-byte words per slice
-        constexpr int SR = (WPS + BS - 1) / BS;
-        const int wmax = vstride * (int)sizeof(T) / 16 - 1;
-        const T16 *src = (const T16 *)(vnext + (size_t)(plev < 3 ? plev : 3) * vstride);
-        T16 x[3][SR];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const int base = (k == 0 ? sl0 : k == 1 ? sl1 : sl2);
-#pragma unroll
-            for (int u = 0; u < SR; u++) x[k][u] = src[min(max(base, 0) * WPS + tid + u * BS, wmax)];
-        }
+        if (plev != 0) stage_loads(plev < 3 ? plev : 3);
 #pragma unroll
         for (int k = 0; k < 3; k++)
 #pragma unroll
@@ -359,20 +367,20 @@ __device__ __forceinline__ void sx_clause_body(int n, int vstride, int m, long l
                 if (k < nsl && i < WPS) *(T16 *)(sv + k * S + i * (16 / (int)sizeof(T))) = x[k][u];
             }
     } else {
-        constexpr int SR = (S + BS - 1) / BS;
-        T x[3][SR];
+        constexpr int SRV = (S + BS - 1) / BS;
+        T xv[3][SRV];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             const int base = (k == 0 ? sl0 : k == 1 ? sl1 : sl2) * S;
 #pragma unroll
-            for (int u = 0; u < SR; u++) x[k][u] = vd[min(max(base, 0) + tid + u * BS, n - 1)].v;
+            for (int u = 0; u < SRV; u++) xv[k][u] = vd[min(max(base, 0) + tid + u * BS, n - 1)].v;
         }
 #pragma unroll
         for (int k = 0; k < 3; k++)
 #pragma unroll
-            for (int u = 0; u < SR; u++) {
+            for (int u = 0; u < SRV; u++) {
                 const int i = tid + u * BS;
-                if (k < nsl && i < S) sv[k * S + i] = x[k][u];
+                if (k < nsl && i < S) sv[k * S + i] = xv[k][u];
             }
     }
     if (tid <= nsb + 1) s_sbr[tid] = sbr;
@@ -385,9 +393,9 @@ __device__ __forceinline__ void sx_clause_body(int n, int vstride, int m, long l
     for (int j = 0; j <= nsb; j++) {
         const int b = j & 1;
         Ops nxt;
-        load_ops(c0 + (j + 1) * SUB, s_sbr[min(j + 1, nsb)], nxt);
+        load_ops(c0 + (j + 1) * sub, s_sbr[min(j + 1, nsb)], nxt);
         if (j < nsb) {
-            const int cb = c0 + j * SUB, ce = min(c1, cb + SUB);
+            const int cb = c0 + j * sub, ce = min(c1, cb + sub);
             const int2 *R = sr[j % 3];
             const bool ok = cb + tid < ce;
             T xs = cur.xs, xl = cur.xl, C = T(0.0);
@@ -432,14 +440,14 @@ __device__ __forceinline__ void sx_clause_body(int n, int vstride, int m, long l
         }
         if (j >= 1) {
             // sub-batch j - 1: one contiguous run per fold block
-            const int cb = c0 + (j - 1) * SUB, cnt = 3 * (min(c1, cb + SUB) - cb);
+            const int cb = c0 + (j - 1) * sub, cnt = 3 * (min(c1, cb + sub) - cb);
             const int2 *R = sr[(j - 1) % 3];
 #pragma unroll
             for (int u = 0; u < 3; u++) {
                 const int k = tid + u * BS;
                 const int kk = k < cnt ? k : 0;
                 const int2 r = R[smap[b ^ 1][kk]];
-                e[k < cnt ? (long long)r.y + (kk - (r.x >> 16)) : nnz] = st[b ^ 1][kk];
+                e[k < cnt ? (long long)r.y + (kk - (r.x >> 16)) : esink] = st[b ^ 1][kk];
             }
         }
         // the runs of sub-batch j + 1 (sub-batch j - 1's table is free now
@@ -456,7 +464,7 @@ __device__ __forceinline__ void sx_clause_body(int n, int vstride, int m, long l
 }
 
 template <typename T>
-__global__ __launch_bounds__(SXShape<T>::BS) void k_sx_clause(int n, int vstride, int m, long long nnz,
+__global__ __launch_bounds__(SXShape<T>::BS) void k_sx_clause(int n, int vstride, int m, long long esink,
                                                              const uint32_t *__restrict__ sw,
                                                              const int32_t *__restrict__ units,
                                                              const int32_t *__restrict__ sbrun,
@@ -466,7 +474,7 @@ __global__ __launch_bounds__(SXShape<T>::BS) void k_sx_clause(int n, int vstride
                                                              T *__restrict__ Xl, T *__restrict__ cst,
                                                              T *__restrict__ e, Ctl *__restrict__ ctl, int slot)
 {
-    sx_clause_body<T>(n, vstride, m, nnz, sw, units, sbrun, runs, p, vd, vnext, Xs, Xl, cst, e, ctl, slot,
+    sx_clause_body<T>(n, vstride, m, esink, sw, units, sbrun, runs, p, vd, vnext, Xs, Xl, cst, e, ctl, slot,
                       (int)blockIdx.x);
 }
 
@@ -478,14 +486,18 @@ __global__ __launch_bounds__(SXShape<T>::BS) void k_sx_clause_multi(const SXDesc
 {
     const int2 mb = map[blockIdx.x];
     const SXDesc &D = d[mb.x];
-    sx_clause_body<T>(D.n, D.vstride, D.m, 3LL * D.m, D.sw, D.units, D.sbrun, (const int2 *)D.runs,
+    sx_clause_body<T>(D.n, D.vstride, D.m, (long long)((D.n + SX_F - 1) / SX_F) * D.stride, D.sw, D.units, D.sbrun,
+                      (const int2 *)D.runs,
                       to_pt<T>(D.prm), (const Pair<T> *)D.vd, (const T *)D.vnext, (T *)D.xs, (T *)D.xl,
                       (T *)D.cst, (T *)D.e, D.ctl, slot, mb.y);
 }
 
-// The variable pass of fold block `vb` of one instance.
+// The variable pass of fold block `vb` of one instance.  The block's region
+// of e is [vb stride, (vb + 1) stride): every region is padded to the same
+// length (its padding's slots name the sink slot `stride`), so the region's
+// first loads go out with the control words, before anything is known.
 template <typename T, bool SORT>
-__device__ __forceinline__ void sx_var_body(int n, int vstride, const int32_t *__restrict__ row_ptr,
+__device__ __forceinline__ void sx_var_body(int n, int vstride, int stride, const int32_t *__restrict__ row_ptr,
                                             const uint16_t *__restrict__ sq, const T *__restrict__ e,
                                             Pair<T> *__restrict__ vd, T *__restrict__ vnext,
                                             const int8_t *__restrict__ unit, Ctl *__restrict__ ctl, int slot, int vb)
@@ -494,18 +506,29 @@ __device__ __forceinline__ void sx_var_body(int n, int vstride, const int32_t *_
     T *sv = (T *)smem;
     __shared__ int sh[SX_F / 64];
     const int tid = threadIdx.x;
-    // everything that does not depend on another load is requested at once:
-    // the control words, the block's slot range, this thread's slot range
-    // and (V, dV) pair
+    // SU values per thread in flight (one round for the usual region sizes)
+    constexpr int SU = sizeof(T) == 4 ? 32 : 16;
+    const long long s0 = (long long)vb * stride;
+    T x[SU];
+    int q[SU];
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+        const int k = tid + u * SX_F;
+        q[u] = k < stride ? (int)sq[s0 + k] : stride;
+        x[u] = k < stride ? e[s0 + k] : T(0.0);
+    }
+    // everything else that does not depend on another load: the control
+    // words, the block's first slot, this thread's slot range and (V, dV)
+    // pair
     const int ps = (slot + 2) % 3;
     const int status = ctl->status, pdone = ctl->done[ps], stepped = ctl->stepped[ps];
     const int plev = dev_level(ctl, ps);
     const bool unsolved_any = any_flag(ctl->flags[slot][FG_UNSOLVED]);
     const int stop = ctl->stop_on_solve, track = ctl->track_loc;
     const long long steps = ctl->steps, cap = ctl->cap;
-    const int v0 = vb * SX_F, v1 = min(n, v0 + SX_F);
+    const int v0 = vb * SX_F;
     const int v = v0 + tid;
-    const int s0 = row_ptr[v0], s1 = row_ptr[v1];
+    const int rb0 = row_ptr[v0];
     const int rb = v < n ? row_ptr[v] : 0, re = v < n ? row_ptr[v + 1] : 0;
     Pair<T> a;
     a.v = a.d = T(0.0);
@@ -517,22 +540,20 @@ __device__ __forceinline__ void sx_var_body(int n, int vstride, const int32_t *_
     const int done = solved ? DNX_SAT : (steps >= cap ? DNX_UNKNOWN : 0);
     if (vb == 0 && tid < 64 && track) fold_loc(ctl, slot);
     if (!done) {
-        // SU values per thread in flight (one round for the usual block
-        // sizes), then their LDS stores
-        constexpr int SU = sizeof(T) == 4 ? 32 : 16;
-        const int cnt = s1 - s0;
-        for (int k0 = tid; k0 < cnt; k0 += SU * SX_F) {
-            T x[SU];
-            int q[SU];
+#pragma unroll
+        for (int u = 0; u < SU; u++)
+            if (tid + u * SX_F < stride) sv[q[u]] = x[u];
+        // regions longer than SU workgroups' worth: further rounds
+        for (int k0 = tid + SU * SX_F; k0 < stride; k0 += SU * SX_F) {
 #pragma unroll
             for (int u = 0; u < SU; u++) {
                 const int k = k0 + u * SX_F;
-                q[u] = k < cnt ? (int)sq[s0 + k] : 0;
-                x[u] = k < cnt ? e[s0 + k] : T(0.0);
+                q[u] = k < stride ? (int)sq[s0 + k] : stride;
+                x[u] = k < stride ? e[s0 + k] : T(0.0);
             }
 #pragma unroll
             for (int u = 0; u < SU; u++)
-                if (k0 + u * SX_F < cnt) sv[q[u]] = x[u];
+                if (k0 + u * SX_F < stride) sv[q[u]] = x[u];
         }
     }
     __syncthreads();
@@ -543,7 +564,7 @@ __device__ __forceinline__ void sx_var_body(int n, int vstride, const int32_t *_
         const T V = pend ? clampV(a.v + h * a.d) : a.v;
         T dV = T(0.0);
         if (!done) {
-            T *sg = sv + (rb - s0);
+            T *sg = sv + (rb - rb0);
             const int deg = re - rb;
             T acc = T(0.0);
             if (SORT) {
@@ -581,12 +602,12 @@ __device__ __forceinline__ void sx_var_body(int n, int vstride, const int32_t *_
 }
 
 template <typename T, bool SORT>
-__global__ __launch_bounds__(SX_F) void k_sx_var(int n, int vstride, const int32_t *__restrict__ row_ptr,
+__global__ __launch_bounds__(SX_F) void k_sx_var(int n, int vstride, int stride, const int32_t *__restrict__ row_ptr,
                                                  const uint16_t *__restrict__ sq, const T *__restrict__ e,
                                                  Pair<T> *__restrict__ vd, T *__restrict__ vnext,
                                                  const int8_t *__restrict__ unit, Ctl *__restrict__ ctl, int slot)
 {
-    sx_var_body<T, SORT>(n, vstride, row_ptr, sq, e, vd, vnext, unit, ctl, slot, (int)blockIdx.x);
+    sx_var_body<T, SORT>(n, vstride, stride, row_ptr, sq, e, vd, vnext, unit, ctl, slot, (int)blockIdx.x);
 }
 
 // Many instances in one launch: block b folds block map[b].y of instance
@@ -597,8 +618,8 @@ __global__ __launch_bounds__(SX_F) void k_sx_var_multi(const SXDesc *__restrict_
 {
     const int2 mb = map[blockIdx.x];
     const SXDesc &D = d[mb.x];
-    sx_var_body<T, SORT>(D.n, D.vstride, D.row_ptr, D.sq, (const T *)D.e, (Pair<T> *)D.vd, (T *)D.vnext, D.unit,
-                         D.ctl, slot, mb.y);
+    sx_var_body<T, SORT>(D.n, D.vstride, D.stride, D.row_ptr, D.sq, (const T *)D.e, (Pair<T> *)D.vd, (T *)D.vnext,
+                         D.unit, D.ctl, slot, mb.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -892,14 +913,18 @@ static void step_sliced_impl(const DevProb &P, const DevParams &prm, void *vd, v
                              void *vnext, void *cst, Ctl *ctl, int slot, hipStream_t st)
 {
     constexpr bool SORT = sizeof(T) == 8;
-    k_sx_clause<T><<<P.nunits, SXShape<T>::BS, 0, st>>>(P.n, sx_vstride(P.n), P.m, (long long)3 * P.m, P.sw,
+    // the clause pass's sink entry follows the fold blocks' padded regions
+    const long long esink = (long long)nblk(P.n > 0 ? P.n : 1, SX_F) * P.sx_region;
+    k_sx_clause<T><<<P.nunits, SXShape<T>::BS, 0, st>>>(P.n, sx_vstride(P.n), P.m, esink, P.sw,
                                                         P.units, P.sbrun, (const int2 *)P.runs,
                                                         to_pt<T>(prm), (const Pair<T> *)vd, (const T *)vnext,
                                                         (T *)xs, (T *)xl, (T *)cst, (T *)e, ctl, slot);
-    const int lds = (int)sizeof(T) * (P.sx_region > 0 ? P.sx_region : 1);
+    // the block's region and its sink slot
+    const int lds = (int)sizeof(T) * (P.sx_region + 1);
     static int lds_set[MAX_DEV_ATTR] = {};
     lds_attr((const void *)k_sx_var<T, SORT>, lds, lds_set);
-    k_sx_var<T, SORT><<<nblk(P.n > 0 ? P.n : 1, SX_F), SX_F, lds, st>>>(P.n, sx_vstride(P.n), P.row_ptr, P.sq,
+    k_sx_var<T, SORT><<<nblk(P.n > 0 ? P.n : 1, SX_F), SX_F, lds, st>>>(P.n, sx_vstride(P.n), P.sx_region,
+                                                                      P.row_ptr, P.sq,
                                                                       (const T *)e, (Pair<T> *)vd, (T *)vnext,
                                                                       P.unit, ctl, slot);
 }
@@ -925,7 +950,7 @@ void launch_step_sliced_multi(const SXDesc *d, const void *map_c, int nclause, c
     hipStream_t st = (hipStream_t)stream;
     if (nclause > 0)
         k_sx_clause_multi<float><<<nclause, SXShape<float>::BS, 0, st>>>(d, (const int2 *)map_c, slot);
-    const int lds = (int)sizeof(float) * (lds_region > 0 ? lds_region : 1);
+    const int lds = (int)sizeof(float) * (lds_region + 1);
     static int lds_set[MAX_DEV_ATTR] = {};
     lds_attr((const void *)k_sx_var_multi<float, false>, lds, lds_set);
     if (nvar > 0) k_sx_var_multi<float, false><<<nvar, SX_F, lds, st>>>(d, (const int2 *)map_v, slot);

@@ -41,7 +41,7 @@ struct DevProb {
     const uint16_t *sq;
     const int32_t *pidx;
     int nunits;
-    int sx_region;  // slots of the largest fold block
+    int sx_region;  // slots per fold block's region of e (every region padded to it)
 };
 
 // Sliced exchange shapes: variables per slice (three slices of V in LDS),
@@ -52,6 +52,16 @@ inline int sx_slice(int prec) { return prec == 0 ? 5000 : 10000; }
 inline int sx_sub(int prec) { return prec == 0 ? 512 : 1024; }
 inline int sx_region_cap(int prec) { return prec == 0 ? 20000 : 40000; }
 inline int sx_max_rounds(int) { return 2; }
+// A unit of `size` clauses is taken in sx_unit_batches sub-batches of equal
+// size (at most `sub_max`, the workgroup's threads), sx_unit_sub clauses each
+// except the last: one more sub-batch of a few clauses would cost a whole
+// iteration of the clause pass.  (constexpr: usable in the kernels too.)
+constexpr int sx_unit_batches(int size, int sub_max) { return (size + sub_max - 1) / sub_max; }
+constexpr int sx_unit_sub(int size, int sub_max)
+{
+    return sx_unit_batches(size, sub_max) > 0 ? (size + sx_unit_batches(size, sub_max) - 1) / sx_unit_batches(size, sub_max)
+                                              : sub_max;
+}
 // elements per ladder level in the fold's candidate array: whole 16-byte words
 inline int sx_vstride(int n) { return (n + 3) / 4 * 4; }
 
@@ -73,7 +83,7 @@ struct Ctl;
 // One instance of a sliced step that advances many instances in one launch
 // pair (the sliced tables and state of a resident instance; see DevProb).
 struct SXDesc {
-    int n, m, vstride, pad;
+    int n, m, vstride, stride;  // stride: the plan's slots per fold-block region of e
     const uint32_t *sw;
     const int32_t *units, *sbrun, *runs, *row_ptr;
     const uint16_t *sq;
@@ -156,7 +166,7 @@ void launch_step(int prec, const DevProb &P, const DevParams &prm, void *vd, voi
 // per ladder level) written by the fold, cst each clause's last C
 // FP32 sliced step of many instances: nclause blocks (map_c: instance, unit)
 // then nvar blocks (map_v: instance, fold block), at step slot `slot`;
-// lds_region the largest fold-block region among them
+// lds_region the largest region stride among them
 void launch_step_sliced_multi(const SXDesc *d, const void *map_c, int nclause, const void *map_v, int nvar,
                               int lds_region, int slot, void *stream);
 // copies the first `words` ints of each instance's control block to out

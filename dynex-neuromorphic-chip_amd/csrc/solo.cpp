@@ -37,6 +37,7 @@ SXDesc solo_desc(const HostProblem &h, const SlicedPlan &pl, const SoloLayout &L
     d.n = h.n;
     d.m = h.m;
     d.vstride = sx_vstride(h.n > 0 ? h.n : 1);
+    d.stride = pl.stride;
     d.sw = (const uint32_t *)(base + L.sw);
     d.units = (const int32_t *)(base + L.units);
     d.sbrun = (const int32_t *)(base + L.sbrun);
@@ -52,7 +53,6 @@ SXDesc solo_desc(const HostProblem &h, const SlicedPlan &pl, const SoloLayout &L
     d.e = base + L.e;
     d.ctl = (Ctl *)(base + L.ctl);
     d.prm = prm;
-    (void)pl;
     return d;
 }
 
@@ -84,7 +84,7 @@ SoloLayout solo_layout(const HostProblem &h, const SlicedPlan &pl)
     L.vd = take(2 * 4 * (n > 0 ? n : 1));
     L.xs = take(4 * (m + 1));
     L.xl = take(4 * (m + 1));
-    L.e = take(4 * (nnz + 1));
+    L.e = take(4 * ((size_t)std::max<long long>((long long)nnz, pl.e_len) + 1));
     L.vnext = take(4 * 4 * (size_t)sx_vstride(h.n > 0 ? h.n : 1));
     L.cst = take(4 * (m + 1));
     L.total = o;
@@ -136,7 +136,7 @@ DevProb solo_devprob(const HostProblem &h, const SlicedPlan &pl, const SoloLayou
     d.sq = (const uint16_t *)(base + L.sq);
     d.pidx = (const int32_t *)(base + L.pidx);
     d.nunits = (int)pl.units.size() / 8;
-    d.sx_region = pl.max_region;
+    d.sx_region = pl.stride;
     return d;
 }
 
