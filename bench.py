@@ -25,17 +25,18 @@ sys.path.insert(0, os.path.join(ROOT, "dynex-neuromorphic-chip_amd"))
 import dnx  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0
-# HBM bytes per launch of the dominant kernel of the default workload, from
-# two rocprofv3 --pmc runs of `bench.py --profile-pass` (FETCH_SIZE, then
-# WRITE_SIZE, per dispatch of the batched solve), each scaled by what the
-# counter reads on the copy kernel (256 MiB each way: FETCH_SIZE x 2.000,
-# WRITE_SIZE x 1.000).  Keyed by bucket; k_batch<mode 0>, the flow launch:
-# 484.51 GB read + 287.04 GB written.
-TRAFFIC = {0: 771.55e9}
+# HBM bytes per launch of each kernel of the default workload's batched
+# solve, from two rocprofv3 --pmc runs of `bench.py --profile-pass`
+# (FETCH_SIZE, then WRITE_SIZE, per dispatch of the solve), each scaled by
+# what the counter reads on the copy kernel (256 MiB each way: FETCH_SIZE x
+# 2.000, WRITE_SIZE x 1.000).  Keyed by bucket: k_batch<mode 0> (the flow
+# launch) 357.18 GB read + 185.97 GB written; k_epoch 6.29 GB read + 3.52 GB
+# written per launch (26 launches).
+TRAFFIC = {0: 543.15e9, 6: 9.81e9}
 # Bytes past L2 per step of the n=100k FP32 single-instance step with the sliced exchange, from two
 # rocprofv3 --pmc runs of `bench.py --profile-single` (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, calibrated on
-# the copy kernel): k_sx_clause 24.1 MB read + 20.5 MB written, k_sx_var 15.9 MB read + 2.4 MB written.
-SINGLE_TRAFFIC = dict(total=62.9e6, k_sx_clause=44.6e6, k_sx_var=18.3e6, vs_algorithmic=round(62.9 / 23.2, 2))
+# the copy kernel): k_sx_clause 24.3 MB read + 20.9 MB written, k_sx_var 16.4 MB read + 2.4 MB written.
+SINGLE_TRAFFIC = dict(total=64.0e6, k_sx_clause=45.2e6, k_sx_var=18.8e6, vs_algorithmic=round(64.0 / 23.2, 2))
 METRIC = "solved instances/s + p50/p99 time-to-solution (batched FP32 vs plain FP64, 1 GPU)"
 
 
@@ -70,8 +71,9 @@ def percentile_ms(vals, q):
 
 def apply_shape(b, shape):
     """Autotuned fused-kernel shapes and gang clauses per workgroup; the tail
-    goes to gangs (or wide mode) once 1.25% of the run's clauses are left
-    (the library default)."""
+    runs in the library's default elastic epochs (gangs for the largest
+    running instances on the CUs the flow leaves idle, beside
+    single-workgroup instances)."""
     if shape:
         for md in (0, 1):
             b.set_shape(md, shape[md][0], shape[md][1])
@@ -450,10 +452,11 @@ def main():
     tts = last["tts"][last["verified"]]
 
     # Dominant kernel: the execution bucket with the most device time (HIP
-    # events around every launch on the stream it runs on).  Its algorithmic
-    # bytes are the compulsory bytes (bytes_alg) of the instance steps its
-    # launches took, counted by the engine from the status snapshots around
-    # each launch; achieved = those bytes per launch / average launch time.
+    # events around every launch, on the stream it runs on); achieved = the
+    # compulsory bytes (bytes_alg) of the instance steps its launches took,
+    # counted by the engine from the status snapshots around each launch, per
+    # launch / average launch time.  The bucket that carries the most
+    # algorithmic bytes is reported beside it (max_bytes_kernel).
     ms_sum = np.sum([r["ms"] for r in runs], axis=0)
     nl_sum = np.sum([r["launches"] for r in runs], axis=0)
     ab_sum = np.sum([r["alg"] for r in runs], axis=0)
@@ -464,11 +467,20 @@ def main():
     avg_us = float(ms_sum[dom] / nl_sum[dom] * 1e3) if nl_sum[dom] else 0.0
     per_launch = float(ab_sum[dom] / nl_sum[dom]) if nl_sum[dom] else 0.0
     achieved = per_launch / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
+    lng = int(np.argmax(ab_sum))
+    lng_gbps = float(ab_sum[lng] / ms_sum[lng] / 1e6) if ms_sum[lng] else 0.0
     all_s = float(ms_sum.sum()) / 1e3
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBPS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBPS, 4),
                     traffic=TRAFFIC.get(dom) if (args.instances, args.seed) == (4096, 4096) else None,
-                    kernel=names[dom], alg_bytes_per_launch=round(per_launch), avg_launch_us=round(avg_us, 2),
+                    kernel=names[dom], dominant_by="device time",
+                    alg_bytes_share=round(float(ab_sum[dom] / ab_sum.sum()), 3) if ab_sum.sum() else None,
+                    alg_bytes_per_launch=round(per_launch), avg_launch_us=round(avg_us, 2),
+                    max_bytes_kernel=dict(kernel=names[lng], device_ms=round(float(ms_sum[lng]), 2),
+                                          alg_GB=round(float(ab_sum[lng]) / 1e9, 2),
+                                          achieved_GBps=round(lng_gbps, 1), frac=round(lng_gbps / HBM_PEAK_GBPS, 4),
+                                          traffic=TRAFFIC.get(lng) if (args.instances, args.seed) == (4096, 4096)
+                                          else None),
                     device_ms={names[i]: round(float(ms_sum[i]), 2) for i in range(len(names))},
                     launches={names[i]: int(nl_sum[i]) for i in range(len(names))},
                     achieved_GBps={names[i]: round(float(ab_sum[i] / ms_sum[i] / 1e6), 1) if ms_sum[i] else None
@@ -549,7 +561,9 @@ def main():
             "workload": "mixed planted 3-SAT, 4096 instances: n~LogU[100,5000], m=round(U[4.3,8]*n), "
                         "P(t)=.46/.46/.08, v~U[-1,1), Xs=0.5, Xl=1, step cap 100k, batched FP32",
             "instances": len(work),
-            "fused_launch": "flow: each instance on its workgroup until it ends or 1.25% of the clauses run",
+            "fused_launch": "flow: each instance on its workgroup until it ends, until enough workgroups idle "
+                            "for a gang; then epochs: gangs of >= 27 workgroups (768 clauses each) for the largest "
+                            "running instances (>= 12,000 clauses) beside single-workgroup instances",
             "order": (f"instances of >= {args.order} clauses first, largest first, then the others smallest first"
                       if args.order > 0 else "largest first"),
         },

@@ -1015,7 +1015,10 @@ __device__ __forceinline__ void gang_run(const BInst *__restrict__ insts, BDyn *
     const int Cb = (m + G - 1) / G;
     const int c0 = min(m, rank * Cb);
     const int cn = min(m, c0 + Cb) - c0;
-    const int Vb = ((n + G - 1) / G + GBS - 1) / GBS * GBS;
+    // variables per workgroup in whole waves (not whole workgroups: with
+    // n / G below the workgroup size, most of a large gang's workgroups
+    // would have no variable and a few would fold for all of them)
+    const int Vb = ((n + G - 1) / G + 63) / 64 * 64;
     const int vb = rank * Vb;
     if (!general)
         for (int i = tid; i < 3 * cn; i += GBS) {
@@ -1058,51 +1061,68 @@ __device__ __forceinline__ void gang_run(const BInst *__restrict__ insts, BDyn *
         // steady-state forms apply (V is clamped below) -- the same bits
         const bool steady = k >= 1 && steps >= 2;
         if (!general) {
-            // 3-SAT: the (V, dV) pairs of all of a thread's clauses (at most
-            // four: GANG_CPW = 4 GBS) are requested before any is evaluated
-            static_assert(GANG_CPW <= 4 * GBS, "one pass of four clauses per thread");
-            for (int ib = tid; ib < cn; ib += 4 * GBS) {
-                BF2 A[4][3];
+            // 3-SAT: the (V, dV) pairs of all of a thread's clauses (CPT =
+            // GANG_CPW / GBS of them) are requested before any is evaluated.
+            // Every clause slot is evaluated on every path (indices clamped
+            // into the range; a slot past the range computes the last clause
+            // again and stores nothing), and a thread's contributions are
+            // stored after all its clauses: the compiler's waits then count a
+            // fixed number of accesses, and no clause waits for the stores of
+            // the one before it
+            constexpr int CPT = GANG_CPW / GBS;
+            static_assert(CPT * GBS == GANG_CPW, "one pass of CPT clauses per thread");
+            if (tid < cn) {
+                BF2 A[CPT][3];
+                int ic[CPT];
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int i = ib + u * GBS;
-                    if (i < cn)
+                for (int u = 0; u < CPT; u++) {
+                    ic[u] = min(tid + u * GBS, cn - 1);
 #pragma unroll
-                        for (int j = 0; j < 3; j++) A[u][j] = vdr_l2(I, abs(s_lits[3 * i + j]) - 1);
+                    for (int j = 0; j < 3; j++) A[u][j] = vdr_l2(I, abs(s_lits[3 * ic[u] + j]) - 1);
                 }
+                float E[CPT][3];
+                int Q[CPT][3];
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int i = ib + u * GBS;
-                    if (i >= cn) break;
-                    const int32_t l0 = s_lits[3 * i], l1 = s_lits[3 * i + 1], l2 = s_lits[3 * i + 2];
+                for (int u = 0; u < CPT; u++) {
+                    const int c = ic[u];
+                    const bool ok = tid + u * GBS < cn;
+                    const int32_t l0 = s_lits[3 * c], l1 = s_lits[3 * c + 1], l2 = s_lits[3 * c + 2];
                     const BF2 a0 = A[u][0], a1 = A[u][1], a2 = A[u][2];
-                    float xs = s_xs[i], xl = s_xl[i];
-                    auto emit = [&](int j, float val) { I.e[s_gpos[3 * i + j]] = val; };
+                    float xs = s_xs[c], xl = s_xl[c];
+                    auto emit = [&](int j, float val) { E[u][j] = val; };
                     float C;
+                    bool sat = true;
                     if (steady) {
-                        memory_update_steady(p, s_cst[i], h, xs, xl);
-                        s_xs[i] = xs;
-                        s_xl[i] = xl;
+                        memory_update_steady(p, s_cst[c], h, xs, xl);
                         C = clause3_steady(p, l0, l1, l2, clampV(a0.v + h * a0.d), clampV(a1.v + h * a1.d),
                                            clampV(a2.v + h * a2.d), xs, xl, emit);
-                        unsolved |= !(C < 0.5f);
                     } else {
                         float r0 = a0.v, r1 = a1.v, r2 = a2.v;
                         if (pend) {
                             memory_update(p, clause3_C_vals<float>(l0, l1, l2, r0, r1, r2), h, xs, xl);
-                            s_xs[i] = xs;
-                            s_xl[i] = xl;
                             r0 = clampV(a0.v + h * a0.d);
                             r1 = clampV(a1.v + h * a1.d);
                             r2 = clampV(a2.v + h * a2.d);
                         }
-                        bool sat;
                         C = clause3_vals<float>(p, l0, l1, l2, r0, r1, r2, clamp01(xs), clampXl(xl, p.xlmax), emit,
                                                 sat);
+                    }
+#pragma unroll
+                    for (int j = 0; j < 3; j++) Q[u][j] = s_gpos[3 * c + j];
+                    if (ok) {
+                        if (steady || pend) {
+                            s_xs[c] = xs;
+                            s_xl[c] = xl;
+                        }
+                        s_cst[c] = C;
                         unsolved |= !(C < 0.5f) || !sat;
                     }
-                    s_cst[i] = C;
                 }
+#pragma unroll
+                for (int u = 0; u < CPT; u++)
+                    if (tid + u * GBS < cn)
+#pragma unroll
+                        for (int j = 0; j < 3; j++) I.e[Q[u][j]] = E[u][j];
             }
         }
         // any clause length (instances with general clauses)
@@ -1137,14 +1157,18 @@ __device__ __forceinline__ void gang_run(const BInst *__restrict__ insts, BDyn *
         // ---- variable pass --------------------------------------------------
         int fmax = 0;
         const int w = tid >> 6;
-        for (int v0 = vb + w * 64; v0 < vb + Vb; v0 += GBS) {
+        // every wave takes the same number of rounds (the fold stages through
+        // LDS behind a workgroup barrier); a wave past the range folds nothing
+        for (int r0 = 0; r0 < Vb; r0 += GBS) {
+            const int v0 = vb + r0 + w * 64;
+            const bool wok = r0 + w * 64 < Vb;
             const int v = v0 + (tid & 63);
-            const bool vok = v < n;
-            const bool first = v0 == vb + w * 64;
+            const bool vok = wok && v < n;
+            const bool first = r0 == 0;
             const int rb = first ? RB0 : vok ? I.row_ptr[v] : 0, re = first ? RE0 : vok ? I.row_ptr[v + 1] : 0;
             const BF2 a = first ? PA : vok ? vdr_l2(I, v) : BF2{0.0f, 0.0f};  // requested before the fold
             const float dVf = wave_fold_vm<CW>([&](int q) { return er_l2(I, q); }, rb, re,
-                                                  v0 < n ? min(63, n - 1 - v0) : 0, stage[w]);
+                                                  wok && v0 < n ? min(63, n - 1 - v0) : 0, stage[w]);
             if (vok) {
                 const float V = pend ? clampV(a.v + h * a.d) : a.v;
                 float dV = 0.0f;

@@ -206,10 +206,16 @@ struct dnx_batch {
     // instances than CUs), epoch launches give them to the largest running
     // instances as gangs while the others keep one workgroup each
     // (batch_epoch_run).  epoch_min_clauses: the smallest instance given a
-    // gang (0, the default, disables epochs); epoch_steps: gang steps per
-    // epoch launch.
-    long long epoch_min_clauses = 0;
+    // gang (0 disables epochs); epoch_steps: gang steps per epoch launch.
+    // (4,096 mixed instances: 338 ms per batch, against 372 ms with the
+    // threshold tail; 10,000 / 8,000 clauses: 341-343 ms.)
+    long long epoch_min_clauses = 12000;
     int epoch_steps = 1024;
+    // a gang forms only with at most this many clauses per workgroup: a
+    // smaller gang steps no faster than one workgroup (m = 20,000: ~18 us
+    // per step in 10 workgroups, ~9.5 us in 20, against ~17 us in one;
+    // 1,024 / 640: 347 / 370 ms per batch)
+    int epoch_cpw = 768;
     DevBuf b_egmap;  // GANG_XCD x GANG_MAX (slot, rank, G, gang), then GANG_XCD counts
     // admission staging: two halves of adm_chunk bytes, pinned on the host
     // (h_adm) and on the device (b_adm); an admission waits for its uploads
@@ -1466,7 +1472,7 @@ static bool epoch_plan(const dnx_batch *b, const std::vector<int> &run, EpochPla
     for (int s : order) {
         const int m = b->slots[(size_t)s].m;
         if (m < b->epoch_min_clauses) break;
-        const int lo = std::max(1, (m + GANG_CPW - 1) / GANG_CPW);
+        const int lo = std::max({1, (m + GANG_CPW - 1) / GANG_CPW, (m + b->epoch_cpw - 1) / b->epoch_cpw});
         if (lo > GANG_MAX || lo - 1 > spare) continue;
         int x = 0;
         for (int y = 1; y < GANG_XCD; y++)
@@ -1613,11 +1619,15 @@ static int batch_epoch_run(dnx_batch *b, bool *used)
 static int epoch_idle_trigger(const dnx_batch *b, const std::vector<int> &run, int grid)
 {
     if (b->epoch_min_clauses <= 0 || b->gang_blocks <= 0 || b->cus % GANG_XCD != 0) return 0;
-    int mmax = 0;
-    for (int s : run) mmax = std::max(mmax, b->slots[(size_t)s].m);
-    if (mmax < b->epoch_min_clauses) return 0;
-    const int lo = std::max(1, (mmax + GANG_CPW - 1) / GANG_CPW);
-    if (lo > GANG_MAX) return 0;
+    // the largest running instance that a gang can take
+    int lo = 0;
+    for (int s : run) {
+        const int m = b->slots[(size_t)s].m;
+        if (m < b->epoch_min_clauses) continue;
+        const int l = std::max({1, (m + GANG_CPW - 1) / GANG_CPW, (m + b->epoch_cpw - 1) / b->epoch_cpw});
+        if (l <= GANG_MAX) lo = std::max(lo, l);
+    }
+    if (lo == 0) return 0;
     // CUs spare once `need` more workgroups have no instance: a gang of lo
     // takes lo - 1 of them besides its instance's own
     const int have = b->cus - grid;
@@ -1875,11 +1885,12 @@ int dnx_batch_set_tail_split(dnx_batch *b, int enable)
     return DNX_OK;
 }
 
-int dnx_batch_set_epoch(dnx_batch *b, int64_t min_clauses, int steps)
+int dnx_batch_set_epoch(dnx_batch *b, int64_t min_clauses, int steps, int clauses_per_workgroup)
 {
-    if (!b || min_clauses < 0 || steps <= 0) return DNX_E_ARG;
+    if (!b || min_clauses < 0 || steps <= 0 || clauses_per_workgroup < 64) return DNX_E_ARG;
     b->epoch_min_clauses = min_clauses;
     b->epoch_steps = steps;
+    b->epoch_cpw = clauses_per_workgroup;
     return DNX_OK;
 }
 
@@ -2975,7 +2986,7 @@ static int tune_run(dnx_ctx *ctx, const std::vector<TuneInst> &inst, const std::
     } closer{t};
     // the fused kernel's shapes alone: no epochs
     if ((rc = dnx_batch_set_shape(t, BM_LDS_ALL, bs0, cap0)) || (rc = dnx_batch_set_shape(t, BM_LDS_V, bs1, 1 << 30)) ||
-        (rc = dnx_batch_set_wide(t, wide, 30)) || (rc = dnx_batch_set_epoch(t, 0, 256)))
+        (rc = dnx_batch_set_wide(t, wide, 30)) || (rc = dnx_batch_set_epoch(t, 0, 256, 1024)))
         return rc;
     for (int i : pick) {
         dnx_params prm;
@@ -3016,7 +3027,7 @@ static int tune_gang_run(dnx_ctx *ctx, const std::vector<TuneInst> &inst, const 
         ~Close() { dnx_batch_close(t); }
     } closer{t};
     if ((rc = dnx_batch_set_gang(t, 256, cpw, steps)) ||
-        (rc = dnx_batch_set_wide(t, std::numeric_limits<int64_t>::max(), 30)) || (rc = dnx_batch_set_epoch(t, 0, 256)))
+        (rc = dnx_batch_set_wide(t, std::numeric_limits<int64_t>::max(), 30)) || (rc = dnx_batch_set_epoch(t, 0, 256, 1024)))
         return rc;
     for (int i : pick) {
         dnx_params prm;

@@ -312,24 +312,26 @@ int dnx_batch_set_order(dnx_batch *b, int64_t large_clauses);
  * steps_per_launch steps.  Results are bit-identical either way. */
 int dnx_batch_set_flow(dnx_batch *b, int enable);
 
-/* The elastic tail (default off; min_clauses 10,000 with 1,024 steps is the
- * measured setting): while every
+/* The elastic tail (default on: min_clauses 12,000, steps 1,024,
+ * clauses_per_workgroup 768): while every
  * running instance is held in LDS (mode 0) and flow launches apply, the
  * clause threshold below is not used.  Instead, once fewer instances run
  * than the device has CUs, the spare CUs go to the largest running
- * instances of at least min_clauses clauses as gangs (the workgroups their
- * clauses need in LDS, grown towards clauses_per_workgroup of
- * dnx_batch_set_gang while CUs are spare), in cooperative epoch launches
+ * instances of at least min_clauses clauses as gangs (at least the
+ * workgroups that hold their clauses in LDS and one per
+ * clauses_per_workgroup clauses -- a smaller gang steps no faster than one
+ * workgroup -- grown towards dnx_batch_set_gang's clauses_per_workgroup
+ * while CUs are spare), in cooperative epoch launches
  * that keep every other running instance on a workgroup of its own; each
  * epoch ends once its gangs have taken `steps` steps, and the next is
  * planned from the statuses.  The flow launch before the first epoch stops
  * as soon as enough workgroups have run out of instances for the largest
- * instance's gang.  min_clauses = 0 turns it off.  No counterpart in the
+ * instance's gang.  min_clauses = 0 turns it off (the clause threshold of
+ * dnx_batch_set_wide_fraction then applies).  No counterpart in the
  * reference; results are bit-identical either way.  On the 4,096-instance
- * mixed workload the makespan equals the threshold tail's: a gang step
- * beside single-workgroup instances is latency-bound at ~12 us, against
- * ~17 us for the largest instances on one workgroup. */
-int dnx_batch_set_epoch(dnx_batch *b, int64_t min_clauses, int steps);
+ * mixed workload a batch takes 338 ms with it, 372 ms with the threshold
+ * tail. */
+int dnx_batch_set_epoch(dnx_batch *b, int64_t min_clauses, int steps, int clauses_per_workgroup);
 
 /* The tail threshold relative to the run: once the running instances hold
  * at most `fraction` of the clauses that were running when dnx_batch_run

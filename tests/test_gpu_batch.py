@@ -186,7 +186,7 @@ def test_gang_mode_gives_identical_bits(ctx, inst_set, oracle_results, wide, gan
 
 
 
-@pytest.mark.parametrize("epoch", [(2000, 32), (6000, 7), (1, 256)])
+@pytest.mark.parametrize("epoch", [(2000, 32), (6000, 7, 256), (1, 256, 4096)])
 def test_epoch_gives_identical_bits(ctx, inst_set, oracle_results, epoch):
     """The elastic tail: fewer instances than CUs run from the start, so
     every epoch gives the spare CUs to the instances of at least min_clauses
@@ -295,7 +295,14 @@ def test_full_mixed_workload(ctx):
         assert b.assignment(s, n)[1] == 0
     steps = res["steps"][slots]
     b.close()
+    # six short instances, and the two that ran longest (with the default
+    # elastic tail those end in gangs, beside single-workgroup instances)
     pick = [i for i in np.random.default_rng(1).permutation(count) if steps[i] < 1500][:6]
+    longest = [int(i) for i in np.argsort(steps)[-2:]]
+    for i in longest:
+        n, cp, lits, x0 = inst[i]
+        o_steps, o_t, o_status, _ = oracle_solve(n, cp, lits, x0, 100000)
+        assert o_status == 1 and o_steps == steps[i] and o_t == res["t"][slots][i], (n, o_steps, steps[i])
     for i in pick:
         n, cp, lits, x0 = inst[i]
         gp = ctx.problem_from_lits(n, cp, lits)
