@@ -30,9 +30,9 @@ HBM_PEAK_GBPS = 8000.0
 # (FETCH_SIZE, then WRITE_SIZE, per dispatch of the solve), each scaled by
 # what the counter reads on the copy kernel (256 MiB each way: FETCH_SIZE x
 # 2.000, WRITE_SIZE x 1.000).  Keyed by bucket: k_batch<mode 0> (the flow
-# launch) 357.18 GB read + 185.97 GB written; k_epoch 6.29 GB read + 3.52 GB
-# written per launch (26 launches).
-TRAFFIC = {0: 543.15e9, 6: 9.81e9}
+# launch) 355.37 GB read + 185.03 GB written; k_epoch 5.19 GB read + 3.36 GB
+# written per launch (25 launches).
+TRAFFIC = {0: 540.40e9, 6: 8.55e9}
 # Bytes past L2 per step of the n=100k FP32 single-instance step with the sliced exchange, from two
 # rocprofv3 --pmc runs of `bench.py --profile-single` (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, calibrated on
 # the copy kernel): k_sx_clause 24.3 MB read + 20.9 MB written, k_sx_var 16.4 MB read + 2.4 MB written.
@@ -462,7 +462,7 @@ def main():
     ab_sum = np.sum([r["alg"] for r in runs], axis=0)
     names = ["k_batch<mode 0>", "k_batch<mode 1>", "k_batch<mode 2>",
              "k_wide_clause+k_wide_var (one chunk of steps between status syncs)", "k_gang",
-             "k_batch<mode 0> tail launch beside wide mode", "k_epoch (gangs beside single-workgroup instances)"]
+             "k_batch<mode 0> tail launch beside wide mode", "k_epoch (regrouped gangs beside single-workgroup instances)"]
     dom = int(np.argmax(ms_sum))
     avg_us = float(ms_sum[dom] / nl_sum[dom] * 1e3) if nl_sum[dom] else 0.0
     per_launch = float(ab_sum[dom] / nl_sum[dom]) if nl_sum[dom] else 0.0
@@ -562,8 +562,9 @@ def main():
                         "P(t)=.46/.46/.08, v~U[-1,1), Xs=0.5, Xl=1, step cap 100k, batched FP32",
             "instances": len(work),
             "fused_launch": "flow: each instance on its workgroup until it ends, until enough workgroups idle "
-                            "for a gang; then epochs: gangs of >= 27 workgroups (768 clauses each) for the largest "
-                            "running instances (>= 12,000 clauses) beside single-workgroup instances",
+                            "for a gang; then epochs: gangs of at least m/1536 workgroups (contributions regrouped "
+                            "by folding workgroup) for the largest running instances (>= 12,000 clauses) beside "
+                            "single-workgroup instances",
             "order": (f"instances of >= {args.order} clauses first, largest first, then the others smallest first"
                       if args.order > 0 else "largest first"),
         },

@@ -1212,6 +1212,350 @@ __device__ __forceinline__ void gang_run(const BInst *__restrict__ insts, BDyn *
     }
 }
 
+// ---------------------------------------------------------------------------
+// Regrouped gang (3-SAT).  gang_run's steps and barriers -- workgroup r of G
+// evaluates clauses [r Cb, r Cb + cn), owns variables [r Vb, r Vb + vn) and
+// publishes their (V, dV) pairs in vd; a gang barrier follows each pass --
+// with the contributions regrouped: writer A stages its clauses'
+// contributions in LDS by owner B (region (A, B); inside it, in the order of
+// B's variable-major slots) and writes its area as one contiguous run, and
+// owner B reads its G regions (one contiguous run each) into LDS, where each
+// owned variable's contributions lie together in slot (clause) order: the
+// fold is the sum gang_run and the fused kernel form, in the same order.
+// gang_run writes each contribution to its own transpose slot instead (3m
+// scattered 4-byte stores per step, each its own L2 request) and gathers
+// them back through the transpose.  The layout is derived at each launch from
+// the transpose (eidx, row_ptr): owners count and rank their slots by writer
+// and publish the ranks and region sizes, and the launch's first gang barrier
+// makes them visible.
+// ---------------------------------------------------------------------------
+struct RxLds {
+    int32_t lits[3 * GANG_CPW];          // the workgroup's clauses' literals
+    uint16_t ldst[3 * GANG_CPW];         // each literal's place in the writer's area
+    uint16_t sdst[RX_SLOTS];             // owner: the slot (from its first) each gathered value folds as
+    float xs[GANG_CPW], xl[GANG_CPW], cst[GANG_CPW];
+    float sv[RX_VCAP], sd[RX_VCAP];      // owned variables' V and dV
+    int pre[GANG_MAX + 1];               // writer: start of region (this, B) in its area
+    int soff[GANG_MAX + 1];              // owner: start of region (A, this) in its gathered values
+    int rst[GANG_MAX];                   // owner: start of region (A, this) in the scratch
+    union {
+        int cnt[RX_NCAP + GANG_MAX];       // set-up: contributions per (writer, owned variable)
+        struct {
+            float stage[RX_SLOTS];         // clause pass: the area; variable pass: the gathered regions
+            int src[RX_SLOTS];             // owner: scratch index of each gathered value
+        } s;
+    } u;
+};
+constexpr int RX_LDS = (int)sizeof(RxLds);
+
+__device__ __forceinline__ float rx_ld(GLB const float *p)
+{
+    return __hip_atomic_load((float *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int rx_ldi(GLB const int *p)
+{
+    return __hip_atomic_load((int *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Workgroup `rank` of regrouped gang `gang` (G workgroups) advances instance
+// `slot` by up to K steps, with gang_run's bits.  Returns false, having
+// changed nothing, when the instance does not fit the tables (every
+// workgroup of the gang judges this alike, from row_ptr at the block
+// boundaries); the caller then runs gang_run.  Barrier epochs: base + 1 for
+// the set-up, base + 2 + 2k and base + 3 + 2k for step k.
+template <int GBS>
+__device__ __forceinline__ bool gang_rx(const BInst *__restrict__ insts, BDyn *__restrict__ dyn, int slot, int rank,
+                                        int G, int gang, unsigned long long *__restrict__ words,
+                                        unsigned long long base, int K, int *__restrict__ err,
+                                        unsigned char *__restrict__ scr, RxLds &L)
+{
+    constexpr int NW = GBS / 64;
+    constexpr int CPT = GANG_CPW / GBS;     // clauses per thread
+    constexpr int RV = RX_VCAP / GBS;       // owned variables per thread
+    constexpr int NR = RX_SLOTS / GBS;      // gathered values per thread
+    static_assert(CPT * GBS == GANG_CPW && RV * GBS == RX_VCAP && NR * GBS == RX_SLOTS, "whole rounds");
+    __shared__ int sh[NW];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const BDyn d = dyn[slot];
+    if (d.status != DNX_RUNNING) return true;
+    const GInst I = global_view(insts[slot]);
+    const int n = I.n, m = I.m;
+    if (I.flags & BF_GENERAL) return false;
+    const int Cb = (m + G - 1) / G, c0 = min(m, rank * Cb), cn = min(m, c0 + Cb) - c0;
+    const int Vb = rx_vblock(n, G), vb = min(n, rank * Vb), vn = min(n, vb + Vb) - vb;
+    {
+        int over = 0;
+        if (tid < G) {
+            const int b0 = min(n, tid * Vb), b1 = min(n, b0 + Vb);
+            over = I.row_ptr[b1] - I.row_ptr[b0] > RX_SLOTS;
+        }
+        if (__syncthreads_or(over) || Vb > RX_VCAP || n > RX_NCAP || Cb > GANG_CPW) return false;
+    }
+    const RxLayout lay = rx_layout(n, m, G);
+    GLB float *ew = (GLB float *)(scr + lay.ew);
+    GLB int *rankt = (GLB int *)(scr + lay.rank);
+    GLB int *szt = (GLB int *)(scr + lay.szt);
+    PT<float> p;
+    p.alpha = I.alpha;
+    p.beta = I.beta;
+    p.gamma = I.gamma;
+    p.delta = I.delta;
+    p.eps = I.eps;
+    p.zeta = I.zeta;
+    p.xlmax = I.xlmax;
+
+    // ---- set-up: the clauses, memories and owned (V, dV) into LDS ----------
+    for (int i = tid; i < 3 * cn; i += GBS) L.lits[i] = I.lits[3 * c0 + i];
+    for (int i = tid; i < cn; i += GBS) {
+        L.xs[i] = I.xs[c0 + i];
+        L.xl[i] = I.xl[c0 + i];
+    }
+    for (int i = tid; i < vn; i += GBS) {
+        const BF2 a = vdr(I, vb + i);
+        L.sv[i] = a.v;
+        L.sd[i] = a.d;
+    }
+    for (int i = tid; i < G * Vb; i += GBS) L.u.cnt[i] = 0;
+    const int qb = I.row_ptr[vb];
+    int RB[RV], RE[RV], PIN = 0;  // owned variables' slot ranges and unit pins
+#pragma unroll
+    for (int r = 0; r < RV; r++) {
+        const int vl = tid + r * GBS;
+        RB[r] = vl < vn ? I.row_ptr[vb + vl] : 0;
+        RE[r] = vl < vn ? I.row_ptr[vb + vl + 1] : 0;
+        PIN |= (vl < vn && (I.flags & BF_UNIT) && I.unit[vb + vl] != 0) << r;
+    }
+    __syncthreads();
+    // owner: contributions to each owned variable from each writer (a
+    // variable's slots run in clause order, so its writers do not decrease)
+#pragma unroll
+    for (int r = 0; r < RV; r++)
+        for (int q = RB[r]; q < RE[r]; q++) L.u.cnt[((I.eidx[q] / 3) / Cb) * Vb + tid + r * GBS]++;
+    __syncthreads();
+    // exclusive scan of each writer's row over the owned variables; the
+    // row's total is the size of region (writer, this owner)
+    for (int A = w; A < G; A += NW) {
+        int carry = 0;
+        for (int x0 = 0; x0 < Vb; x0 += 64) {
+            const int x = x0 + lane < Vb ? L.u.cnt[A * Vb + x0 + lane] : 0;
+            int inc = x;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(inc, o);
+                if (lane >= o) inc += y;
+            }
+            if (x0 + lane < Vb) L.u.cnt[A * Vb + x0 + lane] = carry + inc - x;
+            carry += __shfl(inc, 63);
+        }
+        if (lane == 0) {
+            L.soff[A] = carry;  // region size for now
+            szt[A * G + rank] = carry;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int s = 0;
+        for (int A = 0; A < G; A++) {
+            const int z = L.soff[A];
+            L.soff[A] = s;
+            s += z;
+        }
+        L.soff[G] = s;
+    }
+    __syncthreads();
+    // each owned slot's rank in its region (for its writer), and the slot
+    // each gathered value folds as
+#pragma unroll
+    for (int r = 0; r < RV; r++) {
+        int pa = -1, k = 0;
+        for (int q = RB[r]; q < RE[r]; q++) {
+            const int A = (I.eidx[q] / 3) / Cb;
+            k = A == pa ? k + 1 : 0;
+            pa = A;
+            const int rk = L.u.cnt[A * Vb + tid + r * GBS] + k;
+            rankt[q] = rk;
+            L.sdst[L.soff[A] + rk] = (uint16_t)(q - qb);
+        }
+    }
+    unsigned long long epoch = base;
+    int dummy;
+    if (!gang_barrier(words, gang, G, rank, ++epoch, 0, 0, dummy, err)) return true;
+    // writer: region starts in its area; each literal's place in it
+    if (tid == 0) {
+        int s = 0;
+        for (int B = 0; B < G; B++) {
+            L.pre[B] = s;
+            s += rx_ldi(szt + rank * G + B);
+        }
+        L.pre[G] = s;
+    }
+    // owner: where region (A, this) starts in the scratch
+    if (tid < G) {
+        int s = 0;
+        for (int B = 0; B < rank; B++) s += rx_ldi(szt + tid * G + B);
+        L.rst[tid] = tid * 3 * Cb + s;
+    }
+    __syncthreads();
+    if (L.pre[G] != 3 * cn) {
+        if (tid == 0) __hip_atomic_store(err, 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return true;
+    }
+    for (int i = tid; i < 3 * cn; i += GBS)
+        L.ldst[i] = (uint16_t)(L.pre[(abs(L.lits[i]) - 1) / Vb] + rx_ldi(rankt + I.gpos[3 * c0 + i]));
+    for (int A = w; A < G; A += NW)
+        for (int j = lane; j < L.soff[A + 1] - L.soff[A]; j += 64) L.u.s.src[L.soff[A] + j] = L.rst[A] + j;
+    const int totB = L.soff[G];
+    __syncthreads();
+
+    // ---- steps ------------------------------------------------------------
+    bool pend = d.plevel >= 0;
+    int plev = d.plevel;
+    long long steps = d.steps;
+    double t = d.t;
+    int done = 0;
+    for (int k = 0; k < K; k++) {
+        const float h = pend ? level_h<float>(plev) : 0.0f;
+        if (pend) {
+            steps += 1;
+            t += level_dt(plev);
+        }
+        // ---- clause pass (gang_run's arithmetic) ----------------------------
+        int unsolved = 0;
+        const bool steady = k >= 1 && steps >= 2;
+        if (tid < cn) {
+            BF2 A[CPT][3];
+            int ic[CPT];
+#pragma unroll
+            for (int u = 0; u < CPT; u++) {
+                ic[u] = min(tid + u * GBS, cn - 1);
+#pragma unroll
+                for (int j = 0; j < 3; j++) A[u][j] = vdr_l2(I, abs(L.lits[3 * ic[u] + j]) - 1);
+            }
+#pragma unroll
+            for (int u = 0; u < CPT; u++) {
+                const int c = ic[u];
+                const bool ok = tid + u * GBS < cn;
+                const int32_t l0 = L.lits[3 * c], l1 = L.lits[3 * c + 1], l2 = L.lits[3 * c + 2];
+                const BF2 a0 = A[u][0], a1 = A[u][1], a2 = A[u][2];
+                float xs = L.xs[c], xl = L.xl[c];
+                float E[3];
+                auto emit = [&](int j, float val) { E[j] = val; };
+                float C;
+                bool sat = true;
+                if (steady) {
+                    memory_update_steady(p, L.cst[c], h, xs, xl);
+                    C = clause3_steady(p, l0, l1, l2, clampV(a0.v + h * a0.d), clampV(a1.v + h * a1.d),
+                                       clampV(a2.v + h * a2.d), xs, xl, emit);
+                } else {
+                    float r0 = a0.v, r1 = a1.v, r2 = a2.v;
+                    if (pend) {
+                        memory_update(p, clause3_C_vals<float>(l0, l1, l2, r0, r1, r2), h, xs, xl);
+                        r0 = clampV(a0.v + h * a0.d);
+                        r1 = clampV(a1.v + h * a1.d);
+                        r2 = clampV(a2.v + h * a2.d);
+                    }
+                    C = clause3_vals<float>(p, l0, l1, l2, r0, r1, r2, clamp01(xs), clampXl(xl, p.xlmax), emit,
+                                            sat);
+                }
+                if (ok) {
+                    if (steady || pend) {
+                        L.xs[c] = xs;
+                        L.xl[c] = xl;
+                    }
+                    L.cst[c] = C;
+                    unsolved |= !(C < 0.5f) || !sat;
+#pragma unroll
+                    for (int j = 0; j < 3; j++) L.u.s.stage[L.ldst[3 * c + j]] = E[j];
+                }
+            }
+        }
+        const int wg_unsolved = __syncthreads_or(unsolved);
+        // the area, one contiguous run
+        for (int i = tid; i < 3 * cn; i += GBS) ew[(size_t)rank * 3 * Cb + i] = L.u.s.stage[i];
+        int any_unsolved = 0;
+        if (!gang_barrier(words, gang, G, rank, ++epoch, wg_unsolved, 0, any_unsolved, err)) return true;
+        if (d.stop_on_solve && !any_unsolved)
+            done = DNX_SAT;
+        else if (steps >= d.cap)
+            done = DNX_UNKNOWN;
+        // ---- variable pass --------------------------------------------------
+        {
+            float x[NR];
+#pragma unroll
+            for (int u = 0; u < NR; u++) {
+                const int i = tid + u * GBS;
+                x[u] = i < totB ? rx_ld(ew + L.u.s.src[i]) : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < NR; u++) {
+                const int i = tid + u * GBS;
+                if (i < totB) L.u.s.stage[L.sdst[i]] = x[u];
+            }
+        }
+        __syncthreads();
+        int fmax = 0;
+#pragma unroll
+        for (int r = 0; r < RV; r++) {
+            const int vl = tid + r * GBS;
+            if (vl < vn) {
+                // the variable's contributions lie together in slot order:
+                // eight loads at a time, summed in that order
+                float acc = 0.0f;
+                const float *sg = L.u.s.stage + (RB[r] - qb);
+                const int cnt = RE[r] - RB[r];
+                for (int j0 = 0; j0 < cnt; j0 += 8) {
+                    float y[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) y[u] = j0 + u < cnt ? sg[j0 + u] : 0.0f;
+#pragma unroll
+                    for (int u = 0; u < 8; u++)
+                        if (j0 + u < cnt) acc = acc + y[u];
+                }
+                const float V = pend ? clampV(L.sv[vl] + h * L.sd[vl]) : L.sv[vl];
+                float dV = 0.0f;
+                if (!done) {
+                    dV = acc;
+                    if ((PIN >> r) & 1) dV = 0.0f;  // dynex.cc:306
+                    fmax = max(fmax, fail_level(V, dV));
+                }
+                L.sv[vl] = V;
+                L.sd[vl] = dV;
+                BF2 o;
+                o.v = V;
+                o.d = dV;
+                vdw(I, vb + vl, o);
+            }
+        }
+        if (done) break;
+        const int bmx = block_max_int<GBS>(fmax, sh);
+        if (tid == 0) sh[0] = bmx;
+        __syncthreads();
+        const int wg_level = sh[0];
+        if (!gang_barrier(words, gang, G, rank, ++epoch, wg_level, 1, plev, err)) return true;
+        pend = true;
+    }
+    // ---- write-back ----------------------------------------------------------
+    __syncthreads();
+    for (int i = tid; i < cn; i += GBS) {
+        I.xs[c0 + i] = L.xs[i];
+        I.xl[c0 + i] = L.xl[i];
+    }
+    if (rank == 0 && tid == 0) {
+        BDyn o = d;
+        o.steps = steps;
+        o.t = t;
+        if (done) {
+            o.status = done;
+            o.plevel = -1;
+            o.t_done = wall_clock64();
+        } else {
+            o.plevel = pend ? plev : -1;
+        }
+        dyn[slot] = o;
+    }
+    return true;
+}
+
 // map[b] = (slot, rank, G, gang) for block b (slot -1: idle block).  Before
 // the first step every block checks that it runs on XCD b % 8; one that does
 // not sets *err = 2, and since every block passes a launch-wide arrival count
@@ -1267,7 +1611,9 @@ __global__ __launch_bounds__(EBS) void k_epoch(const BInst *__restrict__ insts, 
                                               const int4 *__restrict__ gmap, const int *__restrict__ gcount,
                                               unsigned long long *__restrict__ words, unsigned long long base, int K,
                                               int *__restrict__ err, const int *__restrict__ work, int nwork,
-                                              int *__restrict__ head, BFlow *__restrict__ flow)
+                                              int *__restrict__ head, BFlow *__restrict__ flow,
+                                              unsigned char *__restrict__ gscr,
+                                              const unsigned long long *__restrict__ goff)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_role;
@@ -1303,9 +1649,14 @@ __global__ __launch_bounds__(EBS) void k_epoch(const BInst *__restrict__ insts, 
     if (role >= 0) {
         const int4 gm = gmap[role];
         const int slot = __builtin_amdgcn_readfirstlane(gm.x), rank = __builtin_amdgcn_readfirstlane(gm.y);
-        const int G = __builtin_amdgcn_readfirstlane(gm.z), gang = __builtin_amdgcn_readfirstlane(gm.w);
-        if (slot >= 0)
+        const int gz = __builtin_amdgcn_readfirstlane(gm.z), gang = __builtin_amdgcn_readfirstlane(gm.w);
+        const int G = gz & 0xffff;
+        // a regrouped gang that does not fit its tables runs as gang_run
+        if (slot >= 0 && (!(gz >> 16) || !gang_rx<EBS>(insts, dyn, slot, rank, G, gang, words, base, K, err,
+                                                        gscr + goff[gang], *(RxLds *)smem))) {
+            __syncthreads();
             gang_run<EBS, ECAPW>(insts, dyn, slot, rank, G, gang, words, base, K, err, *(GangLds<EBS, ECAPW> *)smem);
+        }
     }
     __syncthreads();
     if (tid == 0 &&
@@ -1397,7 +1748,7 @@ int launch_gang(const BInst *insts, BDyn *dyn, const int *map, int nblk, unsigne
                                            (hipStream_t)stream);
 }
 
-int epoch_lds(int flow_lds) { return flow_lds > EGANG_LDS ? flow_lds : EGANG_LDS; }
+int epoch_lds(int flow_lds) { return max(flow_lds, max(EGANG_LDS, RX_LDS)); }
 
 int epoch_occupancy(int flow_lds)
 {
@@ -1411,14 +1762,15 @@ int epoch_occupancy(int flow_lds)
 
 int launch_epoch(const BInst *insts, BDyn *dyn, const int *gmap, const int *gcount, unsigned long long *words,
                  unsigned long long base, int K, int *err, const int *work, int nwork, int *head, BFlow *flow,
-                 int flow_lds, int grid, void *stream)
+                 unsigned char *gscr, const unsigned long long *goff, int flow_lds, int grid, void *stream)
 {
     static int set[MAX_DEV_ATTR] = {};
     const int lds = epoch_lds(flow_lds);
     lds_attr((const void *)k_epoch, lds, set);
     const int4 *gm = (const int4 *)gmap;
-    void *args[] = {(void *)&insts, (void *)&dyn, (void *)&gm,   (void *)&gcount, (void *)&words, (void *)&base,
-                    (void *)&K,     (void *)&err, (void *)&work, (void *)&nwork,  (void *)&head,  (void *)&flow};
+    void *args[] = {(void *)&insts, (void *)&dyn,  (void *)&gm,   (void *)&gcount, (void *)&words,
+                    (void *)&base,  (void *)&K,    (void *)&err,  (void *)&work,   (void *)&nwork,
+                    (void *)&head,  (void *)&flow, (void *)&gscr, (void *)&goff};
     return (int)hipLaunchCooperativeKernel((const void *)k_epoch, dim3(grid), dim3(EBS), args, lds,
                                            (hipStream_t)stream);
 }

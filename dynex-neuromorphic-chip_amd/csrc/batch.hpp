@@ -146,15 +146,42 @@ constexpr int GANG_XCD = 8;      // XCDs the dispatcher deals workgroups to, in 
 int launch_gang(const BInst *insts, BDyn *dyn, const int *map, int nblk, unsigned long long *words,
                 unsigned long long base, int K, int *err, int *arrive, int arrive_base, void *stream);
 
+// Regrouped gangs (3-SAT; see gang_rx in batch.hip): per-gang scratch.  A
+// workgroup evaluates at most GANG_CPW clauses, folds at most RX_SLOTS
+// contributions and owns at most RX_VCAP variables; the instance has at most
+// RX_NCAP variables.
+constexpr int RX_SLOTS = 3 * GANG_CPW;
+constexpr int RX_VCAP = 2048;
+constexpr int RX_NCAP = 8192;
+struct RxLayout {
+    unsigned long long ew, rank, szt, total;  // byte offsets in the gang's scratch, and its size
+};
+constexpr unsigned long long rx_align(unsigned long long x) { return (x + 255) / 256 * 256; }
+// ew: G writer areas of 3 Cb contributions (Cb = ceil(m / G) clauses per
+// workgroup); rank: 3m ints (each variable-major slot's place in its region);
+// szt: G x G ints (region sizes)
+constexpr RxLayout rx_layout(int n, int m, int G)
+{
+    const unsigned long long cb = (unsigned long long)((m + G - 1) / G);
+    const unsigned long long ew = 0, rk = ew + rx_align(4ull * (unsigned long long)G * 3ull * cb);
+    const unsigned long long sz = rk + rx_align(4ull * 3ull * (unsigned long long)m);
+    (void)n;
+    return RxLayout{ew, rk, sz, sz + rx_align(4ull * (unsigned long long)G * (unsigned long long)G)};
+}
+// variables a regrouped gang's workgroup owns
+constexpr int rx_vblock(int n, int G) { return (n + G - 1) / G; }
+
 // epoch launch (see batch.hip): `grid` workgroups (one per CU), gangs listed
-// per XCD in gmap (GANG_XCD x GANG_MAX entries of (slot, rank, G, gang)) with
-// gcount[x] members on XCD x, flow workers over `work`; flow->gangs holds the
-// total of gcount, and flow's counters start at zero.  *err: 1 a gang barrier
-// timed out, 3 not every gang member started (the gangs changed nothing).
-// Returns the hipError_t of the launch.
+// per XCD in gmap (GANG_XCD x GANG_MAX entries of (slot, rank, G | regrouped
+// << 16, gang)) with gcount[x] members on XCD x, flow workers over `work`;
+// flow->gangs holds the total of gcount, and flow's counters start at zero.
+// A regrouped gang uses the rx_layout scratch at gscr + goff[gang].  *err: 1 a
+// gang barrier timed out, 3 not every gang member started (the gangs changed
+// nothing), 4 a regrouped gang's tables disagreed.  Returns the hipError_t of
+// the launch.
 int epoch_occupancy(int flow_lds);
 int launch_epoch(const BInst *insts, BDyn *dyn, const int *gmap, const int *gcount, unsigned long long *words,
                  unsigned long long base, int K, int *err, const int *work, int nwork, int *head, BFlow *flow,
-                 int flow_lds, int grid, void *stream);
+                 unsigned char *gscr, const unsigned long long *goff, int flow_lds, int grid, void *stream);
 
 }  // namespace dnx

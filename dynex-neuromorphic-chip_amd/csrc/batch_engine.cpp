@@ -207,16 +207,21 @@ struct dnx_batch {
     // instances as gangs while the others keep one workgroup each
     // (batch_epoch_run).  epoch_min_clauses: the smallest instance given a
     // gang (0 disables epochs); epoch_steps: gang steps per epoch launch.
-    // (4,096 mixed instances: 338 ms per batch, against 372 ms with the
-    // threshold tail; 10,000 / 8,000 clauses: 341-343 ms.)
+    // (4,096 mixed instances with regrouped gangs: 298-311 ms per batch,
+    // against 372 ms with the threshold tail; 8,000 / 6,000 clauses, 512 /
+    // 2,048 steps: within 300-310 ms.)
     long long epoch_min_clauses = 12000;
     int epoch_steps = 1024;
-    // a gang forms only with at most this many clauses per workgroup: a
-    // smaller gang steps no faster than one workgroup (m = 20,000: ~18 us
-    // per step in 10 workgroups, ~9.5 us in 20, against ~17 us in one;
-    // 1,024 / 640: 347 / 370 ms per batch)
-    int epoch_cpw = 768;
-    DevBuf b_egmap;  // GANG_XCD x GANG_MAX (slot, rank, G, gang), then GANG_XCD counts
+    // a gang forms only with at most this many clauses per workgroup (a
+    // regrouped gang's step at m = 20,000: 10.8 / 8.2 / 7.1 us in 10 / 16 /
+    // 32 workgroups, against ~17 us in one; 768 / 1,024 / 2,048: 301-306 /
+    // 305-306 / 318-322 ms per batch)
+    int epoch_cpw = 1536;
+    // epoch gangs of 3-SAT instances regroup their contributions by the
+    // workgroup that folds them (gang_rx in batch.hip)
+    bool gang_regroup = true;
+    DevBuf b_egmap;  // GANG_XCD x GANG_MAX (slot, rank, G, gang), GANG_XCD counts, then per-gang scratch offsets
+    DevBuf b_gscr;   // regrouped gangs' scratch
     // admission staging: two halves of adm_chunk bytes, pinned on the host
     // (h_adm) and on the device (b_adm); an admission waits for its uploads
     // before it returns
@@ -1439,10 +1444,12 @@ static int batch_split_tail(dnx_batch *b, std::chrono::steady_clock::time_point 
 // statuses (more spare CUs as instances end: more or larger gangs).
 // ---------------------------------------------------------------------------
 struct EpochPlan {
-    std::vector<int> gmap;    // GANG_XCD x GANG_MAX entries of (slot, rank, G, gang)
+    std::vector<int> gmap;    // GANG_XCD x GANG_MAX entries of (slot, rank, G | regrouped << 16, gang)
     int gcount[GANG_XCD] = {};
     int members = 0;          // gang workgroups in total
     int gangs = 0;
+    std::vector<unsigned long long> goff;  // each gang's scratch offset (regrouped gangs)
+    unsigned long long scratch = 0;        // scratch bytes in total
     std::vector<int> flow;    // the other running instances, largest first
 };
 
@@ -1502,13 +1509,24 @@ static bool epoch_plan(const dnx_batch *b, const std::vector<int> &run, EpochPla
     for (int x = 0; x < GANG_XCD; x++) P.gcount[x] = 0;
     P.members = 0;
     P.gangs = (int)gs.size();
+    P.goff.assign(gs.size(), 0);
+    P.scratch = 0;
     for (size_t gi = 0; gi < gs.size(); gi++) {
         const G &g = gs[gi];
+        const SlotHost &sl = b->slots[(size_t)g.slot];
+        // regrouped form: 3-SAT, and the tables fit (the kernel also checks
+        // each workgroup's slot count and runs gang_run if one does not)
+        const bool rx = b->gang_regroup && !sl.general && sl.n <= RX_NCAP && rx_vblock(sl.n, g.size) <= RX_VCAP &&
+                        (sl.m + g.size - 1) / g.size <= GANG_CPW;
+        if (rx) {
+            P.goff[gi] = P.scratch;
+            P.scratch += rx_layout(sl.n, sl.m, g.size).total;
+        }
         for (int r = 0; r < g.size; r++) {
             int *e = &P.gmap[((size_t)g.x * GANG_MAX + (size_t)P.gcount[g.x]) * 4];
             e[0] = g.slot;
             e[1] = r;
-            e[2] = g.size;
+            e[2] = g.size | (rx ? 1 << 16 : 0);
             e[3] = (int)gi;
             P.gcount[g.x]++;
         }
@@ -1544,13 +1562,18 @@ static int batch_epoch_run(dnx_batch *b, bool *used)
     hipStream_t st = b->ctx->stream;
     hipError_t e;
     int rc;
-    // gang table and counts, the flow work list, the flow control block
-    const size_t gbytes = sizeof(int) * (P.gmap.size() + GANG_XCD);
+    // gang table and counts, the gangs' scratch offsets, the flow work list,
+    // the flow control block
+    const size_t mapw = P.gmap.size() + GANG_XCD;  // ints (even: the offsets that follow are 8-byte aligned)
+    const size_t gbytes = sizeof(int) * mapw + sizeof(unsigned long long) * (size_t)P.gangs;
     if (b->b_egmap.size < gbytes && (rc = b->b_egmap.alloc(gbytes))) return rc;
     std::vector<int> up(P.gmap);
     up.insert(up.end(), P.gcount, P.gcount + GANG_XCD);
+    up.resize(gbytes / sizeof(int));
+    std::memcpy(up.data() + mapw, P.goff.data(), sizeof(unsigned long long) * (size_t)P.gangs);
     if ((e = hipMemcpyAsync(b->b_egmap.ptr, up.data(), gbytes, hipMemcpyHostToDevice, st)) != hipSuccess)
         return hip_fail(e, "epoch map upload");
+    if (P.scratch > 0 && b->b_gscr.size < P.scratch && (rc = b->b_gscr.alloc(P.scratch))) return rc;
     int *hw = b->h_work + (size_t)b->buf * BM_COUNT * S;
     int *dw = (int *)b->b_work.ptr + (size_t)b->buf * BM_COUNT * S;
     int *dh = (int *)b->b_head.ptr + b->buf * BM_COUNT;
@@ -1588,7 +1611,8 @@ static int batch_epoch_run(dnx_batch *b, bool *used)
     const int le = launch_epoch((const BInst *)b->b_inst.ptr, (BDyn *)b->b_dyn.ptr, (const int *)b->b_egmap.ptr,
                                 (const int *)b->b_egmap.ptr + P.gmap.size(), (unsigned long long *)b->b_gwords.ptr,
                                 b->gang_epoch, K, (int *)b->b_gerr.ptr, dw, (int)P.flow.size(), dh,
-                                (BFlow *)b->b_flow.ptr, lds, b->cus, st);
+                                (BFlow *)b->b_flow.ptr, (unsigned char *)b->b_gscr.ptr,
+                                (const unsigned long long *)((const int *)b->b_egmap.ptr + mapw), lds, b->cus, st);
     b->gang_epoch += 2 * (unsigned long long)K + 2;
     if (le != 0) return hip_fail((hipError_t)le, "epoch launch");
     if (timed) {
@@ -1607,6 +1631,10 @@ static int batch_epoch_run(dnx_batch *b, bool *used)
         // workers' steps stand); epochs are switched off for this batch
         b->epoch_min_clauses = 0;
         if ((e = hipMemset(b->b_gerr.ptr, 0, sizeof(int))) != hipSuccess) return hip_fail(e, "gang flag");
+    } else if (err == 4) {
+        set_error("epoch launch: a regrouped gang's exchange tables disagreed; the gang's instances are in an "
+                  "undefined state");
+        return DNX_E_STATE;
     } else if (err) {
         set_error("epoch launch: a gang barrier wait timed out; the gang's instances are in an undefined state");
         return DNX_E_STATE;
@@ -1891,6 +1919,13 @@ int dnx_batch_set_epoch(dnx_batch *b, int64_t min_clauses, int steps, int clause
     b->epoch_min_clauses = min_clauses;
     b->epoch_steps = steps;
     b->epoch_cpw = clauses_per_workgroup;
+    return DNX_OK;
+}
+
+int dnx_batch_set_gang_regroup(dnx_batch *b, int enable)
+{
+    if (!b) return DNX_E_ARG;
+    b->gang_regroup = enable != 0;
     return DNX_OK;
 }
 

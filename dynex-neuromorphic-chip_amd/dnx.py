@@ -107,6 +107,7 @@ def lib():
     L.dnx_batch_set_order.argtypes = [vp, C.c_int64]
     L.dnx_batch_set_flow.argtypes = [vp, ip]
     L.dnx_batch_set_epoch.argtypes = [vp, C.c_int64, ip, ip]
+    L.dnx_batch_set_gang_regroup.argtypes = [vp, ip]
     L.dnx_batch_tune.argtypes = [vp, ip, ip, dp, dp, ip, ip, C.c_uint64, vp, ip, C.POINTER(C.c_int)]
     L.dnx_batch_get_shape.argtypes = [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)]
     L.dnx_state_time.argtypes = [vp, ip, C.POINTER(C.c_double)]
@@ -477,11 +478,16 @@ class Batch:
     def set_flow(self, enable=True):
         check(lib().dnx_batch_set_flow(self.h, int(bool(enable))))
 
-    def set_epoch(self, min_clauses=12000, steps=1024, clauses_per_workgroup=768):
+    def set_epoch(self, min_clauses=12000, steps=1024, clauses_per_workgroup=1536):
         """Elastic tail: spare CUs serve the largest running instances (at least min_clauses clauses) as
         gangs of at most clauses_per_workgroup clauses per workgroup beside the single-workgroup ones, in
         epochs of `steps` gang steps; min_clauses 0 turns it off."""
         check(lib().dnx_batch_set_epoch(self.h, int(min_clauses), int(steps), int(clauses_per_workgroup)))
+
+    def set_gang_regroup(self, enable=True):
+        """Epoch gangs of 3-SAT instances regroup their contributions by folding workgroup (True) or
+        exchange them through the transpose."""
+        check(lib().dnx_batch_set_gang_regroup(self.h, int(bool(enable))))
 
     def set_wide_fraction(self, fraction=0.0125, steps_per_sync=300):
         check(lib().dnx_batch_set_wide_fraction(self.h, float(fraction), int(steps_per_sync)))

@@ -313,7 +313,7 @@ int dnx_batch_set_order(dnx_batch *b, int64_t large_clauses);
 int dnx_batch_set_flow(dnx_batch *b, int enable);
 
 /* The elastic tail (default on: min_clauses 12,000, steps 1,024,
- * clauses_per_workgroup 768): while every
+ * clauses_per_workgroup 1,536): while every
  * running instance is held in LDS (mode 0) and flow launches apply, the
  * clause threshold below is not used.  Instead, once fewer instances run
  * than the device has CUs, the spare CUs go to the largest running
@@ -329,9 +329,19 @@ int dnx_batch_set_flow(dnx_batch *b, int enable);
  * instance's gang.  min_clauses = 0 turns it off (the clause threshold of
  * dnx_batch_set_wide_fraction then applies).  No counterpart in the
  * reference; results are bit-identical either way.  On the 4,096-instance
- * mixed workload a batch takes 338 ms with it, 372 ms with the threshold
- * tail. */
+ * mixed workload a batch takes ~300 ms with it (regrouped gangs), 372 ms
+ * with the threshold tail. */
 int dnx_batch_set_epoch(dnx_batch *b, int64_t min_clauses, int steps, int clauses_per_workgroup);
+
+/* How epoch gangs of 3-SAT instances exchange contributions (default 1).
+ * 1: regrouped -- each gang workgroup stages its clauses' contributions in
+ * LDS grouped by the workgroup that folds them and writes them as one
+ * contiguous run, and each folding workgroup reads its share as one run per
+ * writer; 0: every contribution goes to its own transpose slot and is
+ * gathered back through the transpose.  Instances with other clause lengths
+ * always take the transpose.  No counterpart in the reference; results are
+ * bit-identical either way. */
+int dnx_batch_set_gang_regroup(dnx_batch *b, int enable);
 
 /* The tail threshold relative to the run: once the running instances hold
  * at most `fraction` of the clauses that were running when dnx_batch_run

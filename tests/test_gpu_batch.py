@@ -58,7 +58,7 @@ def oracle_solve(n, cp, lits, x0, cap):
 
 
 def run_batch(ctx, inst, cap, spl, shapes=None, stop=True, wide=0, gang=None, split=False, flow=True, order=0,
-              many=False, epoch=None):
+              many=False, epoch=None, regroup=True):
     """gang = (max_blocks, clauses_per_workgroup, steps_per_launch) enables
     gang mode for the tail (the instances left once wide mode applies);
     split keeps the single-chunk LDS instances of the tail in the fused
@@ -67,7 +67,7 @@ def run_batch(ctx, inst, cap, spl, shapes=None, stop=True, wide=0, gang=None, sp
     at least that many clauses first and the others smallest first; many
     admits the whole set in one dnx_batch_admit_many call ("defaults": with
     prms NULL); epoch = (min_clauses, steps) turns the elastic tail on (off
-    otherwise)."""
+    otherwise); regroup picks the exchange of its 3-SAT gangs."""
     b = dnx.Batch(ctx, slots=len(inst) + 2, arena_bytes=1 << 30)
     b.set_wide(wide, 30)
     b.set_tail_split(split)
@@ -78,6 +78,7 @@ def run_batch(ctx, inst, cap, spl, shapes=None, stop=True, wide=0, gang=None, sp
     else:
         b.set_gang(0)
     b.set_epoch(*(epoch or (0, 256)))
+    b.set_gang_regroup(regroup)
     if shapes:
         for md, (bs, lds) in shapes.items():
             b.set_shape(md, bs, lds)
@@ -186,15 +187,18 @@ def test_gang_mode_gives_identical_bits(ctx, inst_set, oracle_results, wide, gan
 
 
 
+@pytest.mark.parametrize("regroup", [True, False])
 @pytest.mark.parametrize("epoch", [(2000, 32), (6000, 7, 256), (1, 256, 4096)])
-def test_epoch_gives_identical_bits(ctx, inst_set, oracle_results, epoch):
+def test_epoch_gives_identical_bits(ctx, inst_set, oracle_results, epoch, regroup):
     """The elastic tail: fewer instances than CUs run from the start, so
     every epoch gives the spare CUs to the instances of at least min_clauses
     clauses as gangs and keeps the others on single workgroups; gangs are
     re-planned every `steps` gang steps (instances move between gangs of
-    other sizes and single workgroups).  Every instance computes the
-    oracle's bits."""
-    out = run_batch(ctx, inst_set, 1500, spl=37, gang=(256, 384, 512), flow=True, epoch=epoch)
+    other sizes and single workgroups).  3-SAT gangs regroup their
+    contributions by folding workgroup or exchange them through the
+    transpose; the instances with other clause lengths always take the
+    transpose.  Every instance computes the oracle's bits."""
+    out = run_batch(ctx, inst_set, 1500, spl=37, gang=(256, 384, 512), flow=True, epoch=epoch, regroup=regroup)
     for (n, cp, lits, x0), o, (steps, t, status, xo) in zip(inst_set, out, oracle_results):
         assert o["status"] == status and o["steps"] == steps and o["t"] == t, (n, o["steps"], steps)
         assert same_bits(o["x"], xo), n
@@ -294,15 +298,21 @@ def test_full_mixed_workload(ctx):
     for (n, cp, lits, x0), s in zip(inst, slots):
         assert b.assignment(s, n)[1] == 0
     steps = res["steps"][slots]
-    b.close()
-    # six short instances, and the two that ran longest (with the default
-    # elastic tail those end in gangs, beside single-workgroup instances)
+    # six short instances; the two that ran longest (on single workgroups:
+    # below the elastic tail's 12,000 clauses); and the two longest of at
+    # least 12,000 clauses, which end in gangs beside single-workgroup
+    # instances -- these four also compare their final state bit for bit
     pick = [i for i in np.random.default_rng(1).permutation(count) if steps[i] < 1500][:6]
-    longest = [int(i) for i in np.argsort(steps)[-2:]]
+    order = np.argsort(steps)
+    longest = [int(i) for i in order[-2:]]
+    longest += [int(i) for i in order[::-1] if len(inst[i][1]) - 1 >= 12000][:2]
+    states = {i: b.read_state(slots[i], inst[i][0], len(inst[i][1]) - 1)["x"].astype(np.float32) for i in longest}
+    b.close()
     for i in longest:
         n, cp, lits, x0 = inst[i]
-        o_steps, o_t, o_status, _ = oracle_solve(n, cp, lits, x0, 100000)
+        o_steps, o_t, o_status, xo = oracle_solve(n, cp, lits, x0, 100000)
         assert o_status == 1 and o_steps == steps[i] and o_t == res["t"][slots][i], (n, o_steps, steps[i])
+        assert same_bits(states[i], xo), n
     for i in pick:
         n, cp, lits, x0 = inst[i]
         gp = ctx.problem_from_lits(n, cp, lits)
