@@ -930,7 +930,8 @@ __device__ __forceinline__ float er_l2(const GInst &I, int q)
 // timed out (or another workgroup's did); `res` is the gang-wide OR (op 0)
 // or max (op 1) of `value`.
 __device__ __forceinline__ bool gang_barrier(unsigned long long *words, int gang, int G, int rank,
-                                             unsigned long long epoch, int value, int op, int &res, int *err)
+                                             unsigned long long epoch, int value, int op, int &res, int *err,
+                                             const int *lvalue = nullptr)
 {
     __shared__ int s_res, s_ok;
     // this wave's stores of the step have reached the XCD's L2
@@ -938,9 +939,11 @@ __device__ __forceinline__ bool gang_barrier(unsigned long long *words, int gang
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     __syncthreads();
     const int s = (int)(epoch & 1);
+    // lvalue: the workgroup's value, gathered in LDS before this barrier
     if (threadIdx.x == 0)
-        __hip_atomic_store(gang_line(words, gang, s, rank), (epoch << 16) | (unsigned)(value & 0xffff),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(gang_line(words, gang, s, rank),
+                           (epoch << 16) | (unsigned)((lvalue ? *lvalue : value) & 0xffff), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
         bool ok = false;
@@ -1274,7 +1277,7 @@ __device__ __forceinline__ bool gang_rx(const BInst *__restrict__ insts, BDyn *_
     constexpr int RV = RX_VCAP / GBS;       // owned variables per thread
     constexpr int NR = RX_SLOTS / GBS;      // gathered values per thread
     static_assert(CPT * GBS == GANG_CPW && RV * GBS == RX_VCAP && NR * GBS == RX_SLOTS, "whole rounds");
-    __shared__ int sh[NW];
+    __shared__ int s_lv;  // the workgroup's largest ladder level of the step
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const BDyn d = dyn[slot];
     if (d.status != DNX_RUNNING) return true;
@@ -1423,16 +1426,21 @@ __device__ __forceinline__ bool gang_rx(const BInst *__restrict__ insts, BDyn *_
         int unsolved = 0;
         const bool steady = k >= 1 && steps >= 2;
         if (tid < cn) {
+            // rounds past the workgroup's clauses are skipped (the test is
+            // uniform); inside the last one a thread past them evaluates the
+            // last clause again and keeps nothing
             BF2 A[CPT][3];
             int ic[CPT];
 #pragma unroll
             for (int u = 0; u < CPT; u++) {
                 ic[u] = min(tid + u * GBS, cn - 1);
+                if (u * GBS < cn)
 #pragma unroll
-                for (int j = 0; j < 3; j++) A[u][j] = vdr_l2(I, abs(L.lits[3 * ic[u] + j]) - 1);
+                    for (int j = 0; j < 3; j++) A[u][j] = vdr_l2(I, abs(L.lits[3 * ic[u] + j]) - 1);
             }
 #pragma unroll
             for (int u = 0; u < CPT; u++) {
+                if (u * GBS >= cn) continue;
                 const int c = ic[u];
                 const bool ok = tid + u * GBS < cn;
                 const int32_t l0 = L.lits[3 * c], l1 = L.lits[3 * c + 1], l2 = L.lits[3 * c + 2];
@@ -1479,6 +1487,7 @@ __device__ __forceinline__ bool gang_rx(const BInst *__restrict__ insts, BDyn *_
         else if (steps >= d.cap)
             done = DNX_UNKNOWN;
         // ---- variable pass --------------------------------------------------
+        if (tid == 0) s_lv = 0;  // read only after barrier B's first workgroup barrier
         {
             float x[NR];
 #pragma unroll
@@ -1527,11 +1536,9 @@ __device__ __forceinline__ bool gang_rx(const BInst *__restrict__ insts, BDyn *_
             }
         }
         if (done) break;
-        const int bmx = block_max_int<GBS>(fmax, sh);
-        if (tid == 0) sh[0] = bmx;
-        __syncthreads();
-        const int wg_level = sh[0];
-        if (!gang_barrier(words, gang, G, rank, ++epoch, wg_level, 1, plev, err)) return true;
+        const int wm = wave_max(fmax);
+        if (lane == 0 && wm > 0) atomicMax(&s_lv, wm);
+        if (!gang_barrier(words, gang, G, rank, ++epoch, 0, 1, plev, err, &s_lv)) return true;
         pend = true;
     }
     // ---- write-back ----------------------------------------------------------
