@@ -329,10 +329,7 @@ __device__ __forceinline__ void flow_worker(const BInst *__restrict__ insts, BDy
                         else
                             load_group(I, g, c_lo, c_hi, G);
                         first = false;
-#pragma unroll
-                        for (int u = 0; u < 4; u++) {
-                            const int c = c4 + u;
-                            if (c < c_lo || c >= c_hi) continue;
+                        auto one = [&](int u) {
                             const int32_t l0 = (int32_t)G.W[u][0], l1 = (int32_t)G.W[u][1], l2 = (int32_t)G.W[u][2];
                             float xs = G.XS[u], xl = G.XL[u];
                             const float V0 = pvv(__builtin_amdgcn_ubfe(G.W[u][0], 16, 15));
@@ -345,6 +342,16 @@ __device__ __forceinline__ void flow_worker(const BInst *__restrict__ insts, BDy
                             const float C = clause3_steady(p, l0, l1, l2, V0, V1, V2, xs, xl, emit);
                             G.CX[u] = C;
                             cmax = dmax(cmax, C);
+                        };
+                        // 3-SAT chunks start at multiples of four clauses:
+                        // only an instance's last group can be partial
+                        if (c4 >= c_lo && c4 + 4 <= c_hi) {
+#pragma unroll
+                            for (int u = 0; u < 4; u++) one(u);
+                        } else {
+#pragma unroll
+                            for (int u = 0; u < 4; u++)
+                                if (c4 + u >= c_lo && c4 + u < c_hi) one(u);
                         }
                         store_group(I, g, c_lo, c_hi, G);
                     }

@@ -292,14 +292,20 @@ static bool plan_mode0(const dnx_batch *b, const HostProblem &h, Mode0Plan &P)
     if (pv >= cap) return false;
     const int64_t slots = (int64_t)((cap - pv) / 4);
     if (slots < DNX_MAX_LITS || slots * KC_MAX < nnz) return false;
+    // 3-SAT chunks start at multiples of four clauses, so the steady clause
+    // pass's groups of four never straddle two chunks (a variable's fold runs
+    // on across chunks in slot order, so the cuts do not change any sum)
     P.bound.push_back(0);
     int64_t used = 0, most = 0;
     for (int c = 0; c < h.m; c++) {
         const int k = h.cp[(size_t)c + 1] - h.cp[(size_t)c];
         if (used + k > slots) {
-            P.bound.push_back(c);
-            most = std::max(most, used);
-            used = 0;
+            int cut = c;
+            if (h.uniform3 && cut % 4 != 0 && cut - cut % 4 > P.bound.back()) cut -= cut % 4;
+            const int64_t moved = h.cp[(size_t)c] - h.cp[(size_t)cut];  // slots of the clauses moved into the new chunk
+            most = std::max(most, used - moved);
+            P.bound.push_back(cut);
+            used = moved;
         }
         used += k;
     }

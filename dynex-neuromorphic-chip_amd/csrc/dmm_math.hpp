@@ -282,7 +282,10 @@ __device__ __forceinline__ void memory_update_steady(const PT<float> &p, float C
     float dxs = p.beta * (xs + p.eps);
     dxs = dxs * (Cprev - p.gamma);
     float dxl = p.alpha * (Cprev - p.delta);
-    xs = clamp01(xs + h * dxs);
+    // clamp01 as one median of three: a memory that came out of an update is
+    // never -0 (x + y is -0 only for two -0 terms, and clamp01 maps any
+    // negative value to +0), the one input the two forms could tell apart
+    xs = __builtin_amdgcn_fmed3f(xs + h * dxs, 0.0f, 1.0f);
     xl = clampXl(xl + h * dxl, p.xlmax);
 }
 
