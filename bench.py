@@ -30,9 +30,9 @@ HBM_PEAK_GBPS = 8000.0
 # (FETCH_SIZE, then WRITE_SIZE, per dispatch of the solve), each scaled by
 # what the counter reads on the copy kernel (256 MiB each way: FETCH_SIZE x
 # 2.000, WRITE_SIZE x 1.000).  Keyed by bucket: k_batch<mode 0> (the flow
-# launch) 355.37 GB read + 185.03 GB written; k_epoch 5.19 GB read + 3.36 GB
-# written per launch (25 launches).
-TRAFFIC = {0: 540.40e9, 6: 8.55e9}
+# launch) 363.23 GB read + 187.89 GB written; k_epoch 5.14 GB read + 2.69 GB
+# written per launch (30-31 launches).
+TRAFFIC = {0: 551.13e9, 6: 7.82e9}
 # Bytes past L2 per step of the n=100k FP32 single-instance step with the sliced exchange, from two
 # rocprofv3 --pmc runs of `bench.py --profile-single` (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, calibrated on
 # the copy kernel): k_sx_clause 24.3 MB read + 20.9 MB written, k_sx_var 16.4 MB read + 2.4 MB written.
@@ -562,8 +562,8 @@ def main():
                         "P(t)=.46/.46/.08, v~U[-1,1), Xs=0.5, Xl=1, step cap 100k, batched FP32",
             "instances": len(work),
             "fused_launch": "flow: each instance on its workgroup until it ends, until enough workgroups idle "
-                            "for a gang; then epochs: gangs of at least m/1536 workgroups (contributions regrouped "
-                            "by folding workgroup) for the largest running instances (>= 12,000 clauses) beside "
+                            "for a gang; then epochs: gangs of at least m/768 workgroups (contributions regrouped "
+                            "by folding workgroup) for the largest running instances (>= 6,000 clauses) beside "
                             "single-workgroup instances",
             "order": (f"instances of >= {args.order} clauses first, largest first, then the others smallest first"
                       if args.order > 0 else "largest first"),

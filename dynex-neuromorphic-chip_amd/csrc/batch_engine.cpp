@@ -207,16 +207,15 @@ struct dnx_batch {
     // instances as gangs while the others keep one workgroup each
     // (batch_epoch_run).  epoch_min_clauses: the smallest instance given a
     // gang (0 disables epochs); epoch_steps: gang steps per epoch launch.
-    // (4,096 mixed instances with regrouped gangs: 298-311 ms per batch,
-    // against 372 ms with the threshold tail; 8,000 / 6,000 clauses, 512 /
-    // 2,048 steps: within 300-310 ms.)
-    long long epoch_min_clauses = 12000;
+    // (4,096 mixed instances: 273-278 ms per batch, against 372 ms with the
+    // threshold tail; 12,000 clauses with m / 1536 workgroups: 278-290 ms.)
+    long long epoch_min_clauses = 6000;
     int epoch_steps = 1024;
     // a gang forms only with at most this many clauses per workgroup (a
-    // regrouped gang's step at m = 20,000: 10.8 / 8.2 / 7.1 us in 10 / 16 /
-    // 32 workgroups, against ~17 us in one; 768 / 1,024 / 2,048: 301-306 /
-    // 305-306 / 318-322 ms per batch)
-    int epoch_cpw = 1536;
+    // regrouped gang's step at m = 20,000: 8.5-10 / 7.3-7.8 / 6.1 us in 10 /
+    // 16 / 32 workgroups, against ~17 us in one; at m = 6,547: 5.4 / 5.0 us
+    // in 8 / 16, against 7.7 in one)
+    int epoch_cpw = 768;
     // epoch gangs of 3-SAT instances regroup their contributions by the
     // workgroup that folds them (gang_rx in batch.hip)
     bool gang_regroup = true;

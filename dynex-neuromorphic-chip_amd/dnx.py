@@ -478,7 +478,7 @@ class Batch:
     def set_flow(self, enable=True):
         check(lib().dnx_batch_set_flow(self.h, int(bool(enable))))
 
-    def set_epoch(self, min_clauses=12000, steps=1024, clauses_per_workgroup=1536):
+    def set_epoch(self, min_clauses=6000, steps=1024, clauses_per_workgroup=768):
         """Elastic tail: spare CUs serve the largest running instances (at least min_clauses clauses) as
         gangs of at most clauses_per_workgroup clauses per workgroup beside the single-workgroup ones, in
         epochs of `steps` gang steps; min_clauses 0 turns it off."""

@@ -312,8 +312,8 @@ int dnx_batch_set_order(dnx_batch *b, int64_t large_clauses);
  * steps_per_launch steps.  Results are bit-identical either way. */
 int dnx_batch_set_flow(dnx_batch *b, int enable);
 
-/* The elastic tail (default on: min_clauses 12,000, steps 1,024,
- * clauses_per_workgroup 1,536): while every
+/* The elastic tail (default on: min_clauses 6,000, steps 1,024,
+ * clauses_per_workgroup 768): while every
  * running instance is held in LDS (mode 0) and flow launches apply, the
  * clause threshold below is not used.  Instead, once fewer instances run
  * than the device has CUs, the spare CUs go to the largest running
@@ -329,8 +329,8 @@ int dnx_batch_set_flow(dnx_batch *b, int enable);
  * instance's gang.  min_clauses = 0 turns it off (the clause threshold of
  * dnx_batch_set_wide_fraction then applies).  No counterpart in the
  * reference; results are bit-identical either way.  On the 4,096-instance
- * mixed workload a batch takes ~300 ms with it (regrouped gangs), 372 ms
- * with the threshold tail. */
+ * mixed workload a batch takes 273-278 ms with it (regrouped gangs), 372
+ * ms with the threshold tail. */
 int dnx_batch_set_epoch(dnx_batch *b, int64_t min_clauses, int steps, int clauses_per_workgroup);
 
 /* How epoch gangs of 3-SAT instances exchange contributions (default 1).
