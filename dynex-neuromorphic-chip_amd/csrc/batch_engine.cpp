@@ -219,7 +219,9 @@ struct dnx_batch {
     // epoch gangs of 3-SAT instances regroup their contributions by the
     // workgroup that folds them (gang_rx in batch.hip)
     bool gang_regroup = true;
-    DevBuf b_egmap;  // GANG_XCD x GANG_MAX (slot, rank, G, gang), GANG_XCD counts, then per-gang scratch offsets
+    // an epoch's upload (batch_epoch_run): pinned on the host, and its device copy
+    unsigned char *h_ep = nullptr;
+    DevBuf b_ep;
     DevBuf b_gscr;   // regrouped gangs' scratch
     // admission staging: two halves of adm_chunk bytes, pinned on the host
     // (h_adm) and on the device (b_adm); an admission waits for its uploads
@@ -234,6 +236,7 @@ struct dnx_batch {
         if (ustream) (void)hipStreamSynchronize(ustream);
         if (arena_base) (void)hipFree(arena_base);
         if (h_adm) (void)hipHostFree(h_adm);
+        if (h_ep) (void)hipHostFree(h_ep);
         for (auto &e : adm_ev)
             if (e) (void)hipEventDestroy(e);
         if (h_dyn) (void)hipHostFree(h_dyn);
@@ -1448,6 +1451,8 @@ static int batch_split_tail(dnx_batch *b, std::chrono::steady_clock::time_point 
 // gangs have taken epoch_steps steps, and the next epoch is planned from the
 // statuses (more spare CUs as instances end: more or larger gangs).
 // ---------------------------------------------------------------------------
+static size_t ep_align(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
 struct EpochPlan {
     std::vector<int> gmap;    // GANG_XCD x GANG_MAX entries of (slot, rank, G | regrouped << 16, gang)
     int gcount[GANG_XCD] = {};
@@ -1567,33 +1572,36 @@ static int batch_epoch_run(dnx_batch *b, bool *used)
     hipStream_t st = b->ctx->stream;
     hipError_t e;
     int rc;
-    // gang table and counts, the gangs' scratch offsets, the flow work list,
-    // the flow control block
-    const size_t mapw = P.gmap.size() + GANG_XCD;  // ints (even: the offsets that follow are 8-byte aligned)
-    const size_t gbytes = sizeof(int) * mapw + sizeof(unsigned long long) * (size_t)P.gangs;
-    if (b->b_egmap.size < gbytes && (rc = b->b_egmap.alloc(gbytes))) return rc;
-    std::vector<int> up(P.gmap);
-    up.insert(up.end(), P.gcount, P.gcount + GANG_XCD);
-    up.resize(gbytes / sizeof(int));
-    std::memcpy(up.data() + mapw, P.goff.data(), sizeof(unsigned long long) * (size_t)P.gangs);
-    if ((e = hipMemcpyAsync(b->b_egmap.ptr, up.data(), gbytes, hipMemcpyHostToDevice, st)) != hipSuccess)
-        return hip_fail(e, "epoch map upload");
-    if (P.scratch > 0 && b->b_gscr.size < P.scratch && (rc = b->b_gscr.alloc(P.scratch))) return rc;
-    int *hw = b->h_work + (size_t)b->buf * BM_COUNT * S;
-    int *dw = (int *)b->b_work.ptr + (size_t)b->buf * BM_COUNT * S;
-    int *dh = (int *)b->b_head.ptr + b->buf * BM_COUNT;
-    for (size_t i = 0; i < P.flow.size(); i++) hw[i] = P.flow[i];
-    if (!P.flow.empty() &&
-        (e = hipMemcpyAsync(dw, hw, sizeof(int) * P.flow.size(), hipMemcpyHostToDevice, st)) != hipSuccess)
-        return hip_fail(e, "epoch work list upload");
-    launch_batch_zero(dh, 1, st);
-    if (!b->b_flow.ptr && (rc = b->b_flow.alloc(sizeof(BFlow)))) return rc;
+    // everything the launch reads goes over in one copy from a pinned
+    // buffer: the gang table and counts, the gangs' scratch offsets, the
+    // flow control block, the work list's head and the work list.  The host
+    // rewrites the buffer only for the next epoch, after this one's statuses
+    // came back, so the copy has long run by then.
+    const size_t mapw = P.gmap.size() + GANG_XCD;  // ints
+    const size_t o_goff = ep_align(sizeof(int) * mapw, 8), o_flow = ep_align(o_goff + 8 * GANG_XCD * GANG_MAX, 128);
+    const size_t o_head = o_flow + sizeof(BFlow), o_work = o_head + 16;
+    const size_t ebytes = o_work + sizeof(int) * P.flow.size();
+    if (!b->h_ep) {
+        const size_t cap = o_work + sizeof(int) * (size_t)S;
+        if ((e = hipHostMalloc((void **)&b->h_ep, cap, hipHostMallocDefault)) != hipSuccess)
+            return hip_fail(e, "hipHostMalloc");
+        if ((rc = b->b_ep.alloc(cap))) return rc;
+    }
+    unsigned char *he = b->h_ep;
+    std::memcpy(he, P.gmap.data(), sizeof(int) * P.gmap.size());
+    std::memcpy(he + sizeof(int) * P.gmap.size(), P.gcount, sizeof(int) * GANG_XCD);
+    std::memcpy(he + o_goff, P.goff.data(), sizeof(unsigned long long) * (size_t)P.gangs);
     BFlow f{};
     f.live = LLONG_MAX / 2;
     f.threshold = -1;
     f.gangs = P.members;
-    if ((e = hipMemcpyAsync(b->b_flow.ptr, &f, sizeof f, hipMemcpyHostToDevice, st)) != hipSuccess)
-        return hip_fail(e, "epoch flow upload");
+    std::memcpy(he + o_flow, &f, sizeof f);
+    std::memset(he + o_head, 0, 16);
+    std::memcpy(he + o_work, P.flow.data(), sizeof(int) * P.flow.size());
+    if ((e = hipMemcpyAsync(b->b_ep.ptr, he, ebytes, hipMemcpyHostToDevice, st)) != hipSuccess)
+        return hip_fail(e, "epoch upload");
+    unsigned char *de = (unsigned char *)b->b_ep.ptr;
+    if (P.scratch > 0 && b->b_gscr.size < P.scratch && (rc = b->b_gscr.alloc(P.scratch))) return rc;
     const size_t wbytes = (size_t)P.gangs * 2 * GANG_MAX * GANG_LINE * sizeof(unsigned long long);
     if (b->b_gwords.size < wbytes) {
         // fresh lines are zero; epochs only grow, so lines never need clearing again
@@ -1605,19 +1613,17 @@ static int batch_epoch_run(dnx_batch *b, bool *used)
         if ((e = hipMemsetAsync(b->b_gerr.ptr, 0, 2 * sizeof(int), st)) != hipSuccess) return hip_fail(e, "gang flag");
         b->gang_arrive = 0;
     }
-    // the host buffers above must not change before the uploads ran
-    if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "epoch upload");
     std::vector<std::pair<int, long long>> before;
     if (b->timing)
         for (int s : run) before.push_back({s, b->h_dyn[s].steps});
     const int K = std::max(1, b->epoch_steps);
     const bool timed = b->timing && !b->t_pending[BK_EPOCH];
     if (timed && (e = hipEventRecord(b->t0[BK_EPOCH], st)) != hipSuccess) return hip_fail(e, "timing");
-    const int le = launch_epoch((const BInst *)b->b_inst.ptr, (BDyn *)b->b_dyn.ptr, (const int *)b->b_egmap.ptr,
-                                (const int *)b->b_egmap.ptr + P.gmap.size(), (unsigned long long *)b->b_gwords.ptr,
-                                b->gang_epoch, K, (int *)b->b_gerr.ptr, dw, (int)P.flow.size(), dh,
-                                (BFlow *)b->b_flow.ptr, (unsigned char *)b->b_gscr.ptr,
-                                (const unsigned long long *)((const int *)b->b_egmap.ptr + mapw), lds, b->cus, st);
+    const int le = launch_epoch((const BInst *)b->b_inst.ptr, (BDyn *)b->b_dyn.ptr, (const int *)de,
+                                (const int *)de + P.gmap.size(), (unsigned long long *)b->b_gwords.ptr, b->gang_epoch,
+                                K, (int *)b->b_gerr.ptr, (const int *)(de + o_work), (int)P.flow.size(),
+                                (int *)(de + o_head), (BFlow *)(de + o_flow), (unsigned char *)b->b_gscr.ptr,
+                                (const unsigned long long *)(de + o_goff), lds, b->cus, st);
     b->gang_epoch += 2 * (unsigned long long)K + 2;
     if (le != 0) return hip_fail((hipError_t)le, "epoch launch");
     if (timed) {
