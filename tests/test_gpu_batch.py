@@ -204,6 +204,26 @@ def test_epoch_gives_identical_bits(ctx, inst_set, oracle_results, epoch, regrou
         assert same_bits(o["x"], xo), n
 
 
+@pytest.mark.parametrize("regroup", [True, False])
+def test_epoch_gangs_wider_than_their_instances(ctx, regroup):
+    """Gangs of 32 workgroups over instances of 200-256 clauses and 46-60
+    variables: several workgroups of each gang evaluate no clause and own no
+    variable (their regions are empty); every instance computes the oracle's
+    bits for 300 steps."""
+    inst = []
+    for i, (n, m) in enumerate([(46, 200), (60, 256), (53, 231)]):
+        lits = dnx.generate_planted(777 + i, n, m)
+        inst.append((n, planted_cp(m), lits, dnx.init_uniform(777 + i, n, m)))
+    out = run_batch(ctx, inst, 300, spl=37, gang=(256, 1, 512), flow=True, epoch=(1, 64, 64), stop=False,
+                    regroup=regroup)
+    for (n, cp, lits, x0), o in zip(inst, out):
+        op = O.Problem(n, len(cp) - 1, cp, lits)
+        x = x0.astype(np.float32)
+        t, _, _ = O.trajectory(op, O.params(op.m), x, 300, prec=32)
+        assert o["status"] == dnx.UNKNOWN and o["steps"] == 300 and o["t"] == t, (n, o["steps"])
+        assert same_bits(o["x"], x), n
+
+
 def test_epoch_after_flow_with_more_instances_than_cus(ctx):
     """More instances than CUs: the first flow launch stops once enough of
     its workgroups found the list empty for the largest instance's gang, and
