@@ -316,7 +316,23 @@ __device__ __forceinline__ void flow_worker(const BInst *__restrict__ insts, BDy
                     // word per literal packs its sign, variable and LDS slot.
                     // (Buffer-resource addressing of the device arrays
                     // measured slower than these global accesses.)
-                    float cmax = 0.0f;  // C >= 1/2 anywhere means unsolved
+                    // C >= 1/2 anywhere means unsolved.  C lies in [+0, 1]
+                    // here, where the order of the bit patterns as unsigned
+                    // ints is that of the values: an integer max (a float max
+                    // would first quiet its running operand every clause).
+                    uint32_t cmaxb = 0u;
+                    // V's LDS base and each extracted variable as values the
+                    // compiler cannot fold, so that a literal's V address is
+                    // a bit-field extract and one shift-and-add (folded, the
+                    // compiler makes it a shift, a mask and an add)
+                    typedef __attribute__((address_space(3))) const float LdsF;
+                    uint32_t lvb = (uint32_t)(uintptr_t)(LdsF *)lv;
+                    asm volatile("" : "+s"(lvb));  // no instruction: hides the constant
+                    auto vat = [&](uint32_t w) {
+                        uint32_t v = __builtin_amdgcn_ubfe(w, 16, 15);
+                        asm volatile("" : "+v"(v));  // keeps the extract one instruction
+                        return *(LdsF *)(uintptr_t)(lvb + (v << 2));
+                    };
                     // Each thread takes four consecutive clauses 4g .. 4g+3 at
                     // a time (load_group / store_group).
                     const int g_end = (c_hi + 3) >> 2;
@@ -332,16 +348,14 @@ __device__ __forceinline__ void flow_worker(const BInst *__restrict__ insts, BDy
                         auto one = [&](int u) {
                             const int32_t l0 = (int32_t)G.W[u][0], l1 = (int32_t)G.W[u][1], l2 = (int32_t)G.W[u][2];
                             float xs = G.XS[u], xl = G.XL[u];
-                            const float V0 = pvv(__builtin_amdgcn_ubfe(G.W[u][0], 16, 15));
-                            const float V1 = pvv(__builtin_amdgcn_ubfe(G.W[u][1], 16, 15));
-                            const float V2 = pvv(__builtin_amdgcn_ubfe(G.W[u][2], 16, 15));
+                            const float V0 = vat(G.W[u][0]), V1 = vat(G.W[u][1]), V2 = vat(G.W[u][2]);
                             memory_update_steady(p, G.CX[u], h, xs, xl);
                             G.XS[u] = xs;
                             G.XL[u] = xl;
                             auto emit = [&](int j, float val) { se[G.W[u][j] & 0xffffu] = val; };
                             const float C = clause3_steady(p, l0, l1, l2, V0, V1, V2, xs, xl, emit);
                             G.CX[u] = C;
-                            cmax = dmax(cmax, C);
+                            cmaxb = max(cmaxb, __float_as_uint(C));
                         };
                         // 3-SAT chunks start at multiples of four clauses:
                         // only an instance's last group can be partial
@@ -368,7 +382,7 @@ __device__ __forceinline__ void flow_worker(const BInst *__restrict__ insts, BDy
                             have_pre = true;
                         }
                     }
-                    unsolved |= !(cmax < 0.5f);
+                    unsolved |= cmaxb >= 0x3f000000u;  // 0.5f
                 } else if (!general && steady) {
                     // the same for modes 1 and 2 (signed literals,
                     // contributions in device memory)
