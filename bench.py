@@ -30,9 +30,9 @@ HBM_PEAK_GBPS = 8000.0
 # (FETCH_SIZE, then WRITE_SIZE, per dispatch of the solve), each scaled by
 # what the counter reads on the copy kernel (256 MiB each way: FETCH_SIZE x
 # 2.000, WRITE_SIZE x 1.000).  Keyed by bucket: k_batch<mode 0> (the flow
-# launch) 363.23 GB read + 187.89 GB written; k_epoch 5.14 GB read + 2.69 GB
+# launch) 364.10 GB read + 187.85 GB written; k_epoch 4.89 GB read + 2.74 GB
 # written per launch (30-31 launches).
-TRAFFIC = {0: 551.13e9, 6: 7.82e9}
+TRAFFIC = {0: 551.94e9, 6: 7.64e9}
 # Bytes past L2 per step of the n=100k FP32 single-instance step with the sliced exchange, from two
 # rocprofv3 --pmc runs of `bench.py --profile-single` (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, calibrated on
 # the copy kernel): k_sx_clause 24.3 MB read + 20.9 MB written, k_sx_var 16.4 MB read + 2.4 MB written.
