@@ -1581,8 +1581,11 @@ static int batch_epoch_run(dnx_batch *b, bool *used)
     const size_t o_goff = ep_align(sizeof(int) * mapw, 8), o_flow = ep_align(o_goff + 8 * GANG_XCD * GANG_MAX, 128);
     const size_t o_head = o_flow + sizeof(BFlow), o_work = o_head + 16;
     const size_t ebytes = o_work + sizeof(int) * P.flow.size();
+    // the launch's error word comes back into the same pinned buffer, past
+    // the largest work list, with the statuses
+    const size_t o_err = ep_align(o_work + sizeof(int) * (size_t)S, 64);
     if (!b->h_ep) {
-        const size_t cap = o_work + sizeof(int) * (size_t)S;
+        const size_t cap = o_err + 64;
         if ((e = hipHostMalloc((void **)&b->h_ep, cap, hipHostMallocDefault)) != hipSuccess)
             return hip_fail(e, "hipHostMalloc");
         if ((rc = b->b_ep.alloc(cap))) return rc;
@@ -1601,8 +1604,12 @@ static int batch_epoch_run(dnx_batch *b, bool *used)
     if ((e = hipMemcpyAsync(b->b_ep.ptr, he, ebytes, hipMemcpyHostToDevice, st)) != hipSuccess)
         return hip_fail(e, "epoch upload");
     unsigned char *de = (unsigned char *)b->b_ep.ptr;
-    if (P.scratch > 0 && b->b_gscr.size < P.scratch && (rc = b->b_gscr.alloc(P.scratch))) return rc;
-    const size_t wbytes = (size_t)P.gangs * 2 * GANG_MAX * GANG_LINE * sizeof(unsigned long long);
+    // buffers grow at least twofold (a reallocation waits for the device)
+    if (P.scratch > 0 && b->b_gscr.size < P.scratch &&
+        (rc = b->b_gscr.alloc(std::max((size_t)P.scratch, 2 * b->b_gscr.size))))
+        return rc;
+    // barrier lines for as many gangs as an epoch can hold (one per gang workgroup)
+    const size_t wbytes = (size_t)GANG_XCD * GANG_MAX * 2 * GANG_MAX * GANG_LINE * sizeof(unsigned long long);
     if (b->b_gwords.size < wbytes) {
         // fresh lines are zero; epochs only grow, so lines never need clearing again
         if ((rc = b->b_gwords.alloc(wbytes))) return rc;
@@ -1630,13 +1637,14 @@ static int batch_epoch_run(dnx_batch *b, bool *used)
         if ((e = hipEventRecord(b->t1[BK_EPOCH], st)) != hipSuccess) return hip_fail(e, "timing");
         b->t_pending[BK_EPOCH] = true;
     }
+    if ((e = hipMemcpyAsync(he + o_err, b->b_gerr.ptr, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess)
+        return hip_fail(e, "epoch flag read");
     if ((rc = batch_snapshot(b, 0)) || (rc = batch_wait(b, 0))) return rc;
     batch_collect_timing(b);
     for (auto &q : before)
         b->t_alg[BK_EPOCH] += (double)(b->h_dyn[q.first].steps - q.second) * step_bytes(b->slots[(size_t)q.first]);
     int err = 0;
-    if ((e = hipMemcpy(&err, b->b_gerr.ptr, sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess)
-        return hip_fail(e, "epoch flag read");
+    std::memcpy(&err, he + o_err, sizeof(int));
     if (err == 3) {
         // not every gang member started: the gangs changed nothing (the flow
         // workers' steps stand); epochs are switched off for this batch
